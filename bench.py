@@ -1,0 +1,187 @@
+"""Benchmark of the particle-mesh step on one MI355X.
+
+Workload: 2D2V Landau damping with M6 (quintic spline) deposition and gather,
+grid 128 x 128 on [0, 4pi]^2, 1e7 particles, alpha = 0.01, k = 0.5,
+dt = 0.05, 400 steps; the FP32 cell-relative state and the FP64 state run
+side by side from the same quiet start.  A step is one iteration of the
+reference's time loop (examples/landau_damping_poisson_cic_m6.jl:70-83):
+Poisson solve, kick, drift, deposit, on device-resident particles.
+
+Prints one JSON line.  value = particle-steps/s of the FP32 path over the
+timed steps; roofline = the fused FP32 particle pass (the dominant kernel),
+algorithmic bytes (read 20 B + write 20 B per particle) over its average
+duration from HIP events on the context stream; cpu_baseline = the oracle's
+C restatement of the reference loop on a bounded sample of the same
+particles.
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_F32 = 40.0   # per particle per fused pass: cell, ox, oy, vx, vy read and written
+BYTES_F64 = 64.0   # x, y, vx, vy read and written
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_info():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return os.cpu_count(), model or platform.processor()
+
+
+def run_path(pic, mesh, p0, state, dt, steps, warmup, sort_every):
+    """Warmup, then the timed run from the initial state. Returns dict."""
+    L = pic._lib
+    sim = pic.Simulation(mesh, p0, pic.M6(), state=state)
+    sim.sort()
+    if warmup:
+        sim.run(dt, warmup, sort_every=sort_every)
+    # restart from the initial state for the timed (and fitted) run
+    sim.set_particles(p0, state=state)
+    sim.sort()
+    sim.set_timing(True)
+    slow0 = sim.slow_chunks()
+    sim.ctx.sync()
+    t0 = time.perf_counter()
+    e = sim.run(dt, steps, sort_every=sort_every)
+    sim.ctx.sync()
+    t1 = time.perf_counter()
+    pass_ms, passes, solve_ms = sim.timing()
+    out = {
+        "seconds": t1 - t0,
+        "energy": e,
+        "pass_ms_avg": pass_ms / max(passes, 1),
+        "solve_ms_avg": solve_ms / max(passes, 1),
+        "passes": passes,
+        "slow_chunks": sim.slow_chunks() - slow0,
+    }
+    sim.close()
+    del L
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--particles", type=int, default=10_000_000)
+    ap.add_argument("--sort-every", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["f32", "f64", "both"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   f"(got {args.gpus})"}))
+        sys.exit(2)
+
+    import __graft_entry__ as entry
+    pic = entry.load_package()
+    from picmi355x import diagnostics
+
+    nx = ny = 128
+    alpha, kx, dt = 0.01, 0.5, 0.05
+    mesh = pic.TwoDGrid(0.0, 2 * math.pi / kx, nx, 0.0, 2 * math.pi / kx, ny)
+    n = args.particles
+    log(f"bench: sampling {n} particles (Landau quiet start)")
+    p0 = pic.sampling.landau_damping(n, mesh, alpha, kx)
+
+    res = {}
+    for name, state in (("f32", 1), ("f64", 0)):
+        if args.only not in ("both", name):
+            continue
+        log(f"bench: {name} path, warmup {args.warmup}, timed {args.steps} steps")
+        res[name] = run_path(pic, mesh, p0, state, dt, args.steps, args.warmup, args.sort_every)
+        r = res[name]
+        log(f"bench: {name} {r['seconds']*1e3:.2f} ms total, pass {r['pass_ms_avg']*1e3:.1f} us,"
+            f" solve {r['solve_ms_avg']*1e3:.1f} us, slow chunks {r['slow_chunks']}")
+
+    t = np.arange(1, args.steps + 1) * dt
+    gam = {}
+    for name, r in res.items():
+        g, w, nm = diagnostics.damping_rate(t, np.sqrt(r["energy"][:, 0]), tmin=0.0)
+        gam[name] = (g, w, nm)
+        log(f"bench: {name} gamma {g:.5f} omega {w:.4f} from {nm} maxima")
+
+    head = "f32" if "f32" in res else "f64"
+    r = res[head]
+    value = n * args.steps / r["seconds"]
+    bytes_pp = BYTES_F32 if head == "f32" else BYTES_F64
+    achieved = bytes_pp * n / (r["pass_ms_avg"] * 1e-3) / 1e9
+    out = {
+        "metric": "particle-steps/s FP32 vs FP64 at <=1% damping-rate error; % HBM roofline",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["seconds"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head == "f32" else "f64",
+        "data": "synthetic (Landau quiet start, Sobol)",
+        "config": {"workload": "landau2d2v-m6-128x128-1e7-dt0.05-400steps", "grid": [nx, ny],
+                   "particles": n, "kernel": "M6", "alpha": alpha, "k": kx, "dt": dt,
+                   "state": "int32 cell + fp32 offset + fp32 velocity" if head == "f32" else
+                   "fp64 soa", "sort_every": args.sort_every, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_step_{head} (M6, kick+drift+deposit)",
+                     "bytes_per_particle": bytes_pp, "avg_us": r["pass_ms_avg"] * 1e3},
+    }
+    if "f32" in res and "f64" in res:
+        v64 = n * args.steps / res["f64"]["seconds"]
+        g32, g64 = gam["f32"][0], gam["f64"][0]
+        out["fp64"] = {"value": v64, "ms_per_step": res["f64"]["seconds"] * 1e3 / args.steps,
+                       "pass_us": res["f64"]["pass_ms_avg"] * 1e3,
+                       "achieved_GBs": BYTES_F64 * n / (res["f64"]["pass_ms_avg"] * 1e-3) / 1e9}
+        out["speedup_fp32_vs_fp64"] = value / v64
+        out["gamma"] = {"fp32": g32, "fp64": g64, "rel_err": abs(g32 - g64) / abs(g64),
+                        "analytic": -0.153359}
+    out["solve_us"] = r["solve_ms_avg"] * 1e3
+    out["slow_chunks"] = r["slow_chunks"]
+
+    if not args.no_cpu:
+        import __graft_entry__ as entry2
+        O = entry2.load_oracle()
+        steps_cpu = args.cpu_steps
+        log(f"bench: cpu baseline, oracle loop (M6), {n} particles, {steps_cpu} steps, "
+            f"{O.num_threads()} threads")
+        q = np.asfortranarray(p0.copy())
+        c0 = time.perf_counter()
+        O.landau_loop(q, "m6", nx, ny, mesh.dimx, mesh.dimy, dt, steps_cpu)
+        c1 = time.perf_counter()
+        cores, model = cpu_info()
+        out["cpu_baseline"] = {
+            "value": n * steps_cpu / (c1 - c0), "unit": "particle-steps/s",
+            "cores": O.num_threads(), "kind": "port",
+            "sample": f"same 1e7 particles, first {steps_cpu} steps of the reference loop "
+                      f"(oracle C restatement: serial deposit, OpenMP push), {model}, "
+                      f"nproc={cores}"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

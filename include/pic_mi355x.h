@@ -1,0 +1,258 @@
+/*
+ * pic_mi355x.h -- C ABI of the MI355X particle-mesh library (libpicmi355x.so).
+ *
+ * Drop-in for the 2D particle-mesh step of ParticleInCell.jl: charge
+ * deposition, field gather, the CIC/M6 pushers and the periodic FFT Poisson
+ * solve.  Every function returns 0 (PIC_OK) or a negative PIC_ERR_* code and
+ * leaves a message for pic_last_error().  Plain pointers and sizes only.
+ *
+ * Layouts are the reference's Julia layouts (column-major, x fastest):
+ *   particle group  p.array, 5 x n doubles: x, y, vx, vy, w per particle
+ *                   (GEMPIC ParticleGroup{2,2}, used at pushers_cic.jl:14-17)
+ *   grid            nx x ny doubles, element (i,j) at i + ld*j (0-based),
+ *                   ld = nx for the Julia methods, ld = nx+1 for the ghosted
+ *                   arrays of the Fortran twin and of MeshFields2D
+ *   MeshFields2D.e  2 x (nx+1) x (ny+1) doubles (component fastest),
+ *                   meshfields.jl:3-21
+ *   Particles.x/.e  2 x n doubles, particles.jl:12-35
+ *
+ * Particles and grids live on the GPU inside a context; calls on one context
+ * are ordered on the context's HIP stream and are not thread-safe.  Entry
+ * points taking host pointers copy through pageable memory and synchronise.
+ */
+#ifndef PIC_MI355X_H
+#define PIC_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pic_ctx pic_ctx;
+
+/* status codes */
+#define PIC_OK 0
+#define PIC_ERR_ARG -1        /* invalid argument (size, pointer, enum) */
+#define PIC_ERR_STATE -2      /* call not valid in the context's current state */
+#define PIC_ERR_HIP -3        /* HIP runtime error */
+#define PIC_ERR_FFT -4        /* rocFFT error */
+#define PIC_ERR_NOMEM -5      /* device allocation failed */
+
+/* particle shapes: the reference's dispatch tags CloudInCell
+ * (src/cloud_in_cell.jl:6) and M6 (src/pushers_m6.jl:3) */
+#define PIC_CIC 0
+#define PIC_M6 1
+
+/* particle state held on the device */
+#define PIC_STATE_F64 0       /* x, y, vx, vy (+w) FP64 structure of arrays */
+#define PIC_STATE_F32 1       /* int32 cell + FP32 in-cell offsets + FP32 velocity */
+
+/* grids of a context */
+#define PIC_GRID_RHO 0
+#define PIC_GRID_EX 1
+#define PIC_GRID_EY 2
+#define PIC_GRID_BZ 3
+#define PIC_GRID_PHI 4
+
+/* magnetic field seen by push_v */
+#define PIC_BZ_NONE 0         /* bz == 0 everywhere */
+#define PIC_BZ_UNIFORM 1      /* bz == constant */
+#define PIC_BZ_GRID 2         /* bz grid of the context */
+
+/* ------------------------------------------------------------ library */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *pic_last_error(void);
+
+/* Library version, e.g. 100 for 0.1.0. */
+int pic_version(void);
+
+/* Number of visible GPUs. */
+int pic_device_count(int *count);
+
+/* ------------------------------------------------------------ context */
+
+/* Creates a context for the periodic mesh TwoDGrid(xmin, xmax, nx, ymin,
+ * ymax, ny) (GEMPIC's TwoDGrid; dx = (xmax-xmin)/nx) on GPU `device`, with
+ * room for `capacity` particles.  Replaces the host allocations the reference
+ * makes per call (compute_rho's zeros(nx, ny), pushers_cic.jl:80) and the
+ * Poisson tables TwoDPoissonPeriodic(grid) (poisson_2d.jl:28-61) and
+ * Poisson2D(mesh) (meshfields.jl:27-73).  nx and ny must be powers of two
+ * >= 8 (every reference configuration is). */
+int pic_ctx_create(pic_ctx **out, int device, int nx, int ny, double xmin, double xmax,
+                   double ymin, double ymax, int64_t capacity);
+int pic_ctx_destroy(pic_ctx *ctx);
+
+/* Waits for all work queued on the context's stream. */
+int pic_synchronize(pic_ctx *ctx);
+
+/* The context's HIP stream (a hipStream_t), for callers that order their own
+ * work against it. */
+int pic_stream(pic_ctx *ctx, void **stream);
+
+/* --------------------------------------------------------- particles */
+
+/* Copies a host particle group p.array (5 x n) into the context's FP64 state.
+ * Sets the uniform-weight fast path when all w are equal. */
+int pic_set_particles(pic_ctx *ctx, const double *p, int64_t n);
+/* Copies the context's particles back into p.array (5 x n).  In FP32 state
+ * the positions are rebuilt as (cell + offset) * dx. */
+int pic_get_particles(pic_ctx *ctx, double *p);
+/* The same for a 5 x n particle array already in device memory. */
+int pic_set_particles_device(pic_ctx *ctx, const double *d_p, int64_t n);
+int pic_get_particles_device(pic_ctx *ctx, double *d_p);
+int pic_num_particles(pic_ctx *ctx, int64_t *n);
+
+/* Particles.x / Particles.v (2 x n each, particles.jl:12-35) and the scalar
+ * Particles.w: set or read the positions and velocities of the MeshFields
+ * API.  v may be NULL (velocities then 0 on set, skipped on get). */
+int pic_set_particles_xv(pic_ctx *ctx, const double *x, const double *v, double w, int64_t n);
+int pic_get_particles_xv(pic_ctx *ctx, double *x, double *v);
+/* Particles.e (2 x n): fields gathered by pic_interpol_eb_m6. */
+int pic_get_particle_fields(pic_ctx *ctx, double *e);
+
+/* Switches the device state between FP64 and FP32 cell-relative storage
+ * (conversion on the GPU). */
+int pic_set_state(pic_ctx *ctx, int state);
+/* Raw FP32 cell-relative state: cell (int32), ox, oy, vx, vy (float), n
+ * each, host memory. Any pointer may be NULL to skip it. */
+int pic_get_state_f32(pic_ctx *ctx, int32_t *cell, float *ox, float *oy, float *vx, float *vy);
+int pic_set_state_f32(pic_ctx *ctx, const int32_t *cell, const float *ox, const float *oy,
+                      const float *vx, const float *vy, double w, int64_t n);
+
+/* -------------------------------------------------------------- grids */
+
+/* Copies grid `which` (PIC_GRID_*) from/to host memory, nx x ny with leading
+ * dimension ld >= nx (element (i,j) at host[stride*(i + ld*j)], stride >= 1;
+ * stride 2 addresses one component of MeshFields2D.e).  On get with
+ * ld >= nx+1 and at least ny+1 columns (ghost = 1) the periodic ghost row and
+ * column are filled too, as meshfields.jl:100-103 and
+ * compute_rho_m6.jl:303-305 do. */
+int pic_set_grid(pic_ctx *ctx, int which, const double *host, int ld, int stride);
+int pic_get_grid(pic_ctx *ctx, int which, double *host, int ld, int stride, int ghost);
+/* Device pointer of a grid (nx*ny doubles, x fastest) for zero-copy use. */
+int pic_grid_device_ptr(pic_ctx *ctx, int which, double **d_ptr);
+/* Selects the magnetic field push_v sees: PIC_BZ_NONE, PIC_BZ_UNIFORM with
+ * value bz0, or PIC_BZ_GRID (the PIC_GRID_BZ grid). */
+int pic_set_bz(pic_ctx *ctx, int mode, double bz0);
+
+/* ---------------------------------------------- the reference's calls */
+
+/* compute_rho(p, kernel, mesh) -> rho (PIC_GRID_RHO).
+ * CIC: pushers_cic.jl:75-114 (normalised by dx*dy, mean removed).
+ * M6 : pushers_m6.jl:226-348 (normalised, rho_total/dimx/dimy removed).
+ * FP64 state. */
+int pic_compute_rho(pic_ctx *ctx, int kernel);
+
+/* push_v!(p, kernel, mesh, ex, ey, bz, dt): pushers_cic.jl:3-53 (CIC) and
+ * pushers_m6.jl:5-224 (M6), with the grids EX, EY and the field chosen by
+ * pic_set_bz.  FP64 state; CIC results are bit-identical to the reference's
+ * Fortran twin push_v (pic.f90:20-77). */
+int pic_push_v(pic_ctx *ctx, int kernel, double dt);
+
+/* push_x!(p, mesh, dt): pushers_cic.jl:55-72 (single conditional periodic
+ * wrap).  FP64 state. */
+int pic_push_x(pic_ctx *ctx, double dt);
+
+/* interpolate_cic!(epx, epy, mesh, p, ex, ey): interpolate_cic.jl:3-33.
+ * Writes n values each into host epx, epy (either may be NULL to keep the
+ * result on the device for pic_get_particle_fields). */
+int pic_interpolate_cic(pic_ctx *ctx, double *epx, double *epy);
+
+/* solve!(ex, ey, poisson, rho): poisson_2d.jl:97-113, RHO -> EX, EY. */
+int pic_solve_e(pic_ctx *ctx);
+/* solve!(phi, poisson, rho): poisson_2d.jl:78-84, RHO -> PHI. */
+int pic_solve_phi(pic_ctx *ctx);
+
+/* The hot-path driver's energy: out[0] = sum(ex.^2)*dx*dy, out[1] =
+ * sum(ey.^2)*dx*dy, FP64 (examples/landau_damping_poisson_cic_m6.jl:65 takes
+ * sqrt(out[0])). */
+int pic_field_energy(pic_ctx *ctx, double out[2]);
+
+/* compute_rho_m6!(fields, particles): compute_rho_m6.jl:181-316, positions
+ * from pic_set_particles_xv, result in RHO (read it with ghost = 1).
+ * compat = 1 reproduces the reference exactly: the swapped xmin/xmax
+ * (:190-191), so the rewritten positions land in [xmax, xmax+dimx), and its
+ * neighbour indices mod1(i+k, nx) on the 0-based cell (:216-227), which put
+ * every off-centre weight one node low.  compat = 0 uses the unswapped
+ * bounds and the neighbours of compute_rho(p, M6(), mesh). */
+int pic_compute_rho_m6_fields(pic_ctx *ctx, int compat);
+
+/* interpol_eb_m6!(particles, fields): interpolation_m6.jl:125-247, EX/EY ->
+ * Particles.e (pic_get_particle_fields); positions rewritten into
+ * [xmin, xmax). */
+int pic_interpol_eb_m6(pic_ctx *ctx);
+
+/* solve!(fields, poisson::Poisson2D): meshfields.jl:87-108, RHO -> EX, EY,
+ * returns sum(ex^2 + ey^2)*dx*dy over the (nx+1)(ny+1) ghosted nodes. */
+int pic_solve_fields(pic_ctx *ctx, double *energy);
+
+/* ------------------------------------------------ fused device loop */
+
+/* Stable counting sort of the particles by cell (0-based i + nx*j of the
+ * current position), reordering the device state.  perm (host, n entries,
+ * may be NULL) receives the source index of each sorted slot: the
+ * permutation std::stable_sort by cell gives.  No reference counterpart
+ * (build addition). */
+int pic_sort(pic_ctx *ctx, int32_t *perm);
+/* After pic_sort: exclusive offsets of each cell's run (nx*ny+1 entries). */
+int pic_cell_offsets(pic_ctx *ctx, int64_t *offsets);
+
+/* Runs `nsteps` iterations of the hot-path driver's time loop
+ * (examples/landau_damping_poisson_cic_m6.jl:70-83):
+ *   drift dt/2; rho = compute_rho(p, kernel); solve!(ex, ey, poisson, rho);
+ *   energy; push_v!(p, kernel, ex, ey, bz, dt); drift dt/2
+ * fused on the GPU into one particle kernel per step plus the FFT solve.
+ * Works in either state.  energy (host, may be NULL) receives per step
+ * out[2*s] = sum(ex^2)*dx*dy and out[2*s+1] = sum(ey^2)*dx*dy.
+ * sort_every > 0 re-sorts the particles by cell every that many steps
+ * (0 = never).  Leaves RHO, EX, EY of the last step in the context. */
+int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort_every, double *energy);
+
+/* One particle pass of pic_run with the fields currently in the context
+ * (EX, EY as last set or solved, and the pic_set_bz field), for callers that
+ * compose their own loop:
+ *   mode 0  drift dt/2, deposit                 (the loop's prologue)
+ *   mode 1  kick dt, drift dt (two half drifts in FP64 state), deposit
+ *   mode 2  kick dt, drift dt/2                 (the loop's last pass)
+ *   mode 3  deposit only
+ * The deposit is the raw charge sum of w times the shape weights (before the
+ * 1/(dx dy) normalisation and the mean removal of compute_rho); rho_raw
+ * (host, nx*ny, may be NULL) receives it. */
+int pic_step_pass(pic_ctx *ctx, int kernel, int mode, double dt, double *rho_raw);
+
+/* Kernel timing of pic_run: with timing enabled, pic_run records HIP events
+ * on the context's stream around every particle pass (kick, drift,
+ * deposit) and around every Poisson solve.  pic_get_timing returns the
+ * summed durations in milliseconds and the number of particle passes since
+ * timing was enabled (any pointer may be NULL). */
+int pic_set_timing(pic_ctx *ctx, int enable);
+int pic_get_timing(pic_ctx *ctx, double *pass_ms, int64_t *passes, double *solve_ms);
+
+/* Number of particle chunks that took the global-memory slow path in the
+ * fused kernels since the context was created (diagnostic). */
+int pic_slow_chunks(pic_ctx *ctx, uint64_t *count);
+
+/* ------------------------------------------ Fortran-twin drop-ins */
+
+/* The symbols of src/pic.f90, same arguments (scalars by reference, arrays
+ * column-major on the host), for F90.push_v!/push_x!/compute_current!
+ * (src/pic.jl:18-114): a ccall to (:push_v, piclib) becomes a ccall to
+ * (:pic_f90_push_v, libpicmi355x).  Fields are (nx+1) x (ny+1).  They use a
+ * private per-process context; not thread-safe.  Return PIC_OK or an error
+ * code (the Fortran originals return nothing). */
+int pic_f90_push_v(const int32_t *nbpart, double *p, const double *dt, const int32_t *nx,
+                   const int32_t *ny, const double *dx, const double *dy, const double *ex,
+                   const double *ey, const double *bz);
+int pic_f90_push_x(const int32_t *nbpart, const double *dimx, const double *dimy, double *p,
+                   const double *dt);
+int pic_f90_deposition(const int32_t *nbpart, const int32_t *nx, const int32_t *ny,
+                       const double *dx, const double *dy, const double *p, double *jx,
+                       double *jy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PIC_MI355X_H */

@@ -1,0 +1,306 @@
+"""ctypes front end of the CPU oracle (liboracle.so) and of the reference's
+Fortran twin (oracle/_ref/libpic.so, built from src/pic.f90).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The HIP library never calls it.
+
+Arrays follow the reference's Julia layouts, held in numpy with Fortran order:
+a particle group is ``p.array`` of shape (5, n) (rows x, y, vx, vy, w), grids
+are (nx, ny).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libpic.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+_dpc = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_cp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="F_CONTIGUOUS")
+_i64, _i32, _d, _f = C.c_int64, C.c_int, C.c_double, C.c_float
+
+
+def build():
+    """Compile liboracle.so (and _ref/libpic.so where the reference exists)."""
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "or_f_m6": (_d, [_d]),
+            "or_push_v_cic": (None, [_i64, _dp, _d, _i32, _i32, _d, _d, _dp, _dp, C.c_void_p, _i32]),
+            "or_push_x": (None, [_i64, _dp, _d, _d, _d]),
+            "or_compute_rho_cic": (None, [_i64, _dp, _i32, _i32, _d, _d, _dp]),
+            "or_interpolate_cic": (None, [_i64, _dp, _i32, _i32, _d, _d, _dp, _dp, _dpc, _dpc]),
+            "or_push_v_m6": (None, [_i64, _dp, _d, _i32, _i32, _d, _d, _dp, _dp, C.c_void_p, _i32]),
+            "or_compute_rho_m6": (None, [_i64, _dp, _i32, _i32, _d, _d, _d, _d, _dp]),
+            "or_compute_rho_m6_fields": (None, [_i64, _dp, _d, _i32, _i32, _d, _d, _d, _d, _i32, _dp]),
+            "or_interpol_eb_m6": (None, [_i64, _dp, _i32, _i32, _d, _d, _d, _d, _dp, _dp]),
+            "or_rfft2": (_i32, [_i32, _i32, _dp, _i32, _cp]),
+            "or_irfft2": (_i32, [_i32, _i32, _cp, _dp, _i32]),
+            "or_poisson_tables": (None, [_i32, _i32, _d, _d, _dp, _dp, _dp]),
+            "or_solve_phi": (_i32, [_i32, _i32, _d, _d, _dp, _dp]),
+            "or_solve_e": (_i32, [_i32, _i32, _d, _d, _dp, _dp, _dp]),
+            "or_solve_fields": (_i32, [_i32, _i32, _d, _d, _d, _d, _dp, _dp, C.POINTER(_d)]),
+            "or_energy": (_d, [_i32, _i32, _d, _d, _dp, C.c_void_p, C.c_void_p]),
+            "or_counting_sort": (None, [_i64, _ip, C.c_int32, _ip, _lp]),
+            "or_landau_loop": (_i32, [_i64, _dp, _i32, _i32, _d, _d, _d, _i32, _i32, _d, _dpc]),
+            "or_num_threads": (_i32, []),
+            "orf_m6_weights": (None, [_f, _fp]),
+            "orf_step": (None, [_i64, _ip, _fp, _fp, _fp, _fp, _i32, _i32, _i32, _fp, _i32, _f,
+                                C.c_void_p, _f, _f, _f, _f, _i32, _dp]),
+            "orf_deposit": (None, [_i64, _ip, _fp, _fp, _i32, _i32, _i32, _f, _dp]),
+            "orf_from_f64": (None, [_i64, _dp, _i32, _i32, _d, _d, _ip, _fp, _fp, _fp, _fp]),
+            "orf_to_f64": (None, [_i64, _ip, _fp, _fp, _fp, _fp, _i32, _d, _d, _d, _dp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr_or_none(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _F(a, dtype=np.float64):
+    return np.asfortranarray(a, dtype=dtype)
+
+
+# ------------------------------------------------------------------ CIC / M6
+
+def f_m6(q):
+    return lib().or_f_m6(float(q))
+
+
+def push_v(p, kernel, nx, ny, dx, dy, ex, ey, bz, dt):
+    """push_v!(p, kernel, mesh, ex, ey, bz, dt) on a (5, n) array, in place."""
+    ld = ex.shape[0]
+    assert p.flags.f_contiguous and p.dtype == np.float64
+    f = lib().or_push_v_cic if kernel == "cic" else lib().or_push_v_m6
+    bzf = None if bz is None else _F(bz)
+    f(p.shape[1], p, dt, nx, ny, dx, dy, _F(ex), _F(ey), _ptr_or_none(bzf), ld)
+
+
+def push_x(p, dimx, dimy, dt):
+    lib().or_push_x(p.shape[1], p, dt, dimx, dimy)
+
+
+def compute_rho(p, kernel, nx, ny, dx, dy, dimx=None, dimy=None):
+    rho = np.zeros((nx, ny), order="F")
+    if kernel == "cic":
+        lib().or_compute_rho_cic(p.shape[1], p, nx, ny, dx, dy, rho)
+    else:
+        lib().or_compute_rho_m6(p.shape[1], p, nx, ny, dx, dy, dimx, dimy, rho)
+    return rho
+
+
+def interpolate_cic(p, nx, ny, dx, dy, ex, ey):
+    n = p.shape[1]
+    epx, epy = np.zeros(n), np.zeros(n)
+    lib().or_interpolate_cic(n, p, nx, ny, dx, dy, _F(ex), _F(ey), epx, epy)
+    return epx, epy
+
+
+def compute_rho_m6_fields(x, w, nx, ny, xmin, xmax, ymin, ymax, compat=True):
+    """compute_rho_m6!(fields, particles): returns rho ((nx+1, ny+1)); x is
+    rewritten in place as the reference does."""
+    rho = np.zeros((nx + 1, ny + 1), order="F")
+    lib().or_compute_rho_m6_fields(x.shape[1], x, w, nx, ny, xmin, xmax, ymin, ymax,
+                                   1 if compat else 0, rho)
+    return rho
+
+
+def interpol_eb_m6(x, e_fields, nx, ny, xmin, xmax, ymin, ymax):
+    """interpol_eb_m6!(particles, fields): e_fields is (2, nx+1, ny+1);
+    returns particles.e (2, n); x is rewritten in place."""
+    n = x.shape[1]
+    ep = np.zeros((2, n), order="F")
+    lib().or_interpol_eb_m6(n, x, nx, ny, xmin, xmax, ymin, ymax, _F(e_fields), ep)
+    return ep
+
+
+# ------------------------------------------------------------------ Poisson
+
+def rfft2(a):
+    nx, ny = a.shape
+    out = np.zeros((nx // 2 + 1, ny), dtype=np.complex128, order="F")
+    rc = lib().or_rfft2(nx, ny, _F(a), nx, out)
+    assert rc == 0
+    return out
+
+
+def irfft2(y, nx):
+    ny = y.shape[1]
+    out = np.zeros((nx, ny), order="F")
+    rc = lib().or_irfft2(nx, ny, np.asfortranarray(y, dtype=np.complex128), out, nx)
+    assert rc == 0
+    return out
+
+
+def poisson_tables(nx, ny, dimx, dimy):
+    h = nx // 2 + 1
+    kx, ky, k2 = (np.zeros((h, ny), order="F") for _ in range(3))
+    lib().or_poisson_tables(nx, ny, dimx, dimy, kx, ky, k2)
+    return kx, ky, k2
+
+
+def solve_phi(rho, dimx, dimy):
+    nx, ny = rho.shape
+    phi = np.zeros((nx, ny), order="F")
+    assert lib().or_solve_phi(nx, ny, dimx, dimy, _F(rho), phi) == 0
+    return phi
+
+
+def solve_e(rho, dimx, dimy):
+    nx, ny = rho.shape
+    ex, ey = np.zeros((nx, ny), order="F"), np.zeros((nx, ny), order="F")
+    assert lib().or_solve_e(nx, ny, dimx, dimy, _F(rho), ex, ey) == 0
+    return ex, ey
+
+
+def solve_fields(rho_g, xmin, xmax, ymin, ymax):
+    """Poisson2D solve!(fields): rho_g is (nx+1, ny+1); returns (e, energy)
+    with e of shape (2, nx+1, ny+1)."""
+    nx, ny = rho_g.shape[0] - 1, rho_g.shape[1] - 1
+    e = np.zeros((2, nx + 1, ny + 1), order="F")
+    en = C.c_double(0.0)
+    assert lib().or_solve_fields(nx, ny, xmin, xmax, ymin, ymax, _F(rho_g), e, C.byref(en)) == 0
+    return e, en.value
+
+
+def energy(ex, dx, dy):
+    nx, ny = ex.shape
+    return lib().or_energy(nx, ny, dx, dy, _F(ex), None, None)
+
+
+def counting_sort(key, nkeys):
+    key = np.ascontiguousarray(key, dtype=np.int32)
+    perm = np.zeros(key.size, dtype=np.int32)
+    offs = np.zeros(nkeys + 1, dtype=np.int64)
+    lib().or_counting_sort(key.size, key, nkeys, perm, offs)
+    return perm, offs
+
+
+def landau_loop(p, kernel, nx, ny, dimx, dimy, dt, nsteps, bz0=0.0):
+    e = np.zeros(nsteps)
+    rc = lib().or_landau_loop(p.shape[1], p, nx, ny, dimx, dimy, dt, nsteps,
+                              0 if kernel == "cic" else 1, bz0, e)
+    assert rc == 0
+    return e
+
+
+def num_threads():
+    return lib().or_num_threads()
+
+
+# --------------------------------------------------------------- FP32 state
+
+def f32_m6_weights(t):
+    w = np.zeros(6, dtype=np.float32)
+    lib().orf_m6_weights(np.float32(t), w)
+    return w
+
+
+def f32_from_f64(p, nx, ny, dx, dy):
+    n = p.shape[1]
+    cell = np.zeros(n, np.int32)
+    ox, oy, vx, vy = (np.zeros(n, np.float32) for _ in range(4))
+    lib().orf_from_f64(n, p, nx, ny, dx, dy, cell, ox, oy, vx, vy)
+    return cell, ox, oy, vx, vy
+
+
+def f32_to_f64(state, nx, dx, dy, w):
+    cell, ox, oy, vx, vy = state
+    n = cell.size
+    p = np.zeros((5, n), order="F")
+    lib().orf_to_f64(n, cell, ox, oy, vx, vy, nx, dx, dy, w, p)
+    return p
+
+
+def f32_step(state, nx, ny, kernel, e32, hdt, dtdx, dtdy, w, bz_mode=0, bz0=0.0,
+             bz32=None, deposit=True):
+    """One fused FP32 step in place on state = (cell, ox, oy, vx, vy); returns
+    the FP64 accumulation of the deposited FP32 contributions (nx*ny, raw)."""
+    cell, ox, oy, vx, vy = state
+    rho = np.zeros((nx, ny), order="F")
+    b = None if bz32 is None else np.ascontiguousarray(bz32, dtype=np.float32)
+    lib().orf_step(cell.size, cell, ox, oy, vx, vy, nx, ny, 0 if kernel == "cic" else 1,
+                   np.ascontiguousarray(e32, dtype=np.float32), bz_mode, np.float32(bz0),
+                   _ptr_or_none(b), np.float32(hdt), np.float32(dtdx), np.float32(dtdy),
+                   np.float32(w), 1 if deposit else 0, rho)
+    return rho
+
+
+def f32_deposit(state, nx, ny, kernel, w):
+    cell, ox, oy = state[:3]
+    rho = np.zeros((nx, ny), order="F")
+    lib().orf_deposit(cell.size, cell, ox, oy, nx, ny, 0 if kernel == "cic" else 1,
+                      np.float32(w), rho)
+    return rho
+
+
+# ------------------------------------------------- reference Fortran twin
+
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    """src/pic.f90 (push_v, push_x, deposition) as built by the Makefile."""
+    global _ref
+    if _ref is None:
+        L = C.CDLL(REF_PATH)
+        pi, pd = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+        L.push_v.argtypes = [pi, _dp, pd, pi, pi, pd, pd, _dp, _dp, _dp]
+        L.push_x.argtypes = [pi, pd, pd, _dp, pd]
+        L.deposition.argtypes = [pi, pi, pi, pd, pd, _dp, _dp, _dp]
+        for f in (L.push_v, L.push_x, L.deposition):
+            f.restype = None
+        _ref = L
+    return _ref
+
+
+def _i(v):
+    return C.byref(C.c_int32(v))
+
+
+def _r(v):
+    return C.byref(C.c_double(v))
+
+
+def ref_push_v(p, nx, ny, dx, dy, ex, ey, bz, dt):
+    """F90.push_v! (src/pic.jl:79-114): fields carry the ghost row/column,
+    (nx+1, ny+1)."""
+    ref().push_v(_i(p.shape[1]), p, _r(dt), _i(nx), _i(ny), _r(dx), _r(dy), _F(ex), _F(ey), _F(bz))
+
+
+def ref_push_x(p, dimx, dimy, dt):
+    ref().push_x(_i(p.shape[1]), _r(dimx), _r(dimy), p, _r(dt))
+
+
+def ref_deposition(p, nx, ny, dx, dy):
+    jx = np.zeros((nx + 1, ny + 1), order="F")
+    jy = np.zeros((nx + 1, ny + 1), order="F")
+    ref().deposition(_i(p.shape[1]), _i(nx), _i(ny), _r(dx), _r(dy), p, jx, jy)
+    return jx, jy

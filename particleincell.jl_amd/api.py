@@ -1,0 +1,468 @@
+"""The reference's particle-mesh API, backed by libpicmi355x on the GPU.
+
+Mirrors ParticleInCell.jl's calls on the hot path with the same names,
+argument order and array layouts (a Julia `f!` is written `f_` here):
+
+    TwoDGrid(xmin, xmax, nx, ymin, ymax, ny) / TwoDGrid(dimx, nx, dimy, ny)
+    ParticleGroup(n, charge=1.0, mass=1.0, n_weights=1)   .array is (5, n)
+    CloudInCell(), M6()                                    kernel tags
+    compute_rho(p, kernel, mesh)            pushers_cic.jl:75, pushers_m6.jl:226
+    push_v_(p, kernel, mesh, ex, ey, bz, dt) pushers_cic.jl:3, pushers_m6.jl:5
+    push_x_(p, mesh, dt)                     pushers_cic.jl:55
+    interpolate_cic_(epx, epy, mesh, p, ex, ey)   interpolate_cic.jl:3
+    TwoDPoissonPeriodic(grid)                poisson_2d.jl:28
+    solve_(phi, poisson, rho)                poisson_2d.jl:78
+    solve_(ex, ey, poisson, rho)             poisson_2d.jl:97
+    MeshFields2D(mesh), Poisson2D(mesh), Particles(n, w)
+    solve_(fields, poisson2d) -> energy      meshfields.jl:87
+    compute_rho_m6_(fields, particles)       compute_rho_m6.jl:181
+    interpol_eb_m6_(particles, fields)       interpolation_m6.jl:125
+    F90.push_v_/push_x_/compute_current_     pic.jl:18-114 (Fortran-twin ABI)
+
+Arrays are numpy float64 in Fortran order, so the memory layout is the Julia
+one.  Each call copies its inputs to the GPU and its outputs back (the
+drop-in semantics of a host array API); `Simulation` keeps particles and
+fields resident for the time loop.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+
+
+# ------------------------------------------------------------------ types
+
+class TwoDGrid:
+    """GEMPIC's TwoDGrid: periodic nx x ny cells, dx = (xmax-xmin)/nx,
+    x = LinRange(xmin, xmax, nx+1)."""
+
+    def __init__(self, *args):
+        if len(args) == 4:  # TwoDGrid(dimx, nx, dimy, ny)
+            dimx, nx, dimy, ny = args
+            xmin, xmax, ymin, ymax = 0.0, float(dimx), 0.0, float(dimy)
+        elif len(args) == 6:  # TwoDGrid(xmin, xmax, nx, ymin, ymax, ny)
+            xmin, xmax, nx, ymin, ymax, ny = args
+        else:
+            raise TypeError("TwoDGrid(dimx, nx, dimy, ny) or TwoDGrid(xmin, xmax, nx, ymin, ymax, ny)")
+        self.xmin, self.xmax, self.nx = float(xmin), float(xmax), int(nx)
+        self.ymin, self.ymax, self.ny = float(ymin), float(ymax), int(ny)
+        self.dimx = self.xmax - self.xmin
+        self.dimy = self.ymax - self.ymin
+        self.dx = self.dimx / self.nx
+        self.dy = self.dimy / self.ny
+        self.x = np.linspace(self.xmin, self.xmax, self.nx + 1)
+        self.y = np.linspace(self.ymin, self.ymax, self.ny + 1)
+
+    def key(self):
+        return (self.nx, self.ny, self.xmin, self.xmax, self.ymin, self.ymax)
+
+
+class ParticleGroup:
+    """ParticleGroup{2,2}: .array is 5 x n (x, y, vx, vy, w), column-major."""
+
+    def __init__(self, n, charge=1.0, mass=1.0, n_weights=1):
+        self.n_particles = int(n)
+        self.charge, self.mass, self.n_weights = charge, mass, n_weights
+        self.array = np.zeros((5, self.n_particles), order="F")
+
+
+class CloudInCell:
+    code = L.PIC_CIC
+
+
+class M6:
+    code = L.PIC_M6
+
+
+def _kcode(kernel):
+    k = kernel if isinstance(kernel, type) else type(kernel)
+    if k is CloudInCell:
+        return L.PIC_CIC
+    if k is M6:
+        return L.PIC_M6
+    raise TypeError(f"unsupported kernel {kernel!r}: CloudInCell() or M6()")
+
+
+# --------------------------------------------------------- device context
+
+class Context:
+    """One libpicmi355x context: mesh, particles and grids on one GPU."""
+
+    def __init__(self, mesh, capacity=0, device=0):
+        self.mesh = mesh
+        h = C.c_void_p()
+        L.call("pic_ctx_create", C.byref(h), device, mesh.nx, mesh.ny, mesh.xmin, mesh.xmax,
+               mesh.ymin, mesh.ymax, int(capacity))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            L.load().pic_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # particles
+    def set_particles(self, arr):
+        a = np.asfortranarray(arr, dtype=np.float64)
+        assert a.ndim == 2 and a.shape[0] == 5
+        L.call("pic_set_particles", self.h, a.ctypes.data, a.shape[1])
+        return a
+
+    def get_particles(self, out):
+        assert out.flags.f_contiguous and out.dtype == np.float64
+        L.call("pic_get_particles", self.h, out.ctypes.data)
+        return out
+
+    def num_particles(self):
+        n = C.c_int64()
+        L.call("pic_num_particles", self.h, C.byref(n))
+        return n.value
+
+    def set_state(self, state):
+        L.call("pic_set_state", self.h, state)
+
+    def get_state_f32(self):
+        n = self.num_particles()
+        cell = np.zeros(n, np.int32)
+        ox, oy, vx, vy = (np.zeros(n, np.float32) for _ in range(4))
+        L.call("pic_get_state_f32", self.h, cell.ctypes.data, ox.ctypes.data, oy.ctypes.data,
+               vx.ctypes.data, vy.ctypes.data)
+        return cell, ox, oy, vx, vy
+
+    def set_state_f32(self, state, w):
+        cell, ox, oy, vx, vy = (np.ascontiguousarray(a) for a in state)
+        assert cell.dtype == np.int32 and ox.dtype == np.float32
+        L.call("pic_set_state_f32", self.h, cell.ctypes.data, ox.ctypes.data, oy.ctypes.data,
+               vx.ctypes.data, vy.ctypes.data, float(w), cell.size)
+
+    # grids
+    def set_grid(self, which, arr, stride=1):
+        a = np.asarray(arr, dtype=np.float64)
+        if stride == 1:
+            a = np.asfortranarray(a)
+            ld = a.shape[0]
+        else:
+            ld = a.shape[1]
+        L.call("pic_set_grid", self.h, which, a.ctypes.data, ld, stride)
+
+    def get_grid(self, which, ghost=False):
+        nx, ny = self.mesh.nx, self.mesh.ny
+        out = np.zeros((nx + 1, ny + 1) if ghost else (nx, ny), order="F")
+        L.call("pic_get_grid", self.h, which, out.ctypes.data, out.shape[0], 1, 1 if ghost else 0)
+        return out
+
+    def set_bz(self, mode, bz0=0.0):
+        L.call("pic_set_bz", self.h, mode, float(bz0))
+
+    def sync(self):
+        L.call("pic_synchronize", self.h)
+
+
+_ctx_cache = {}
+
+
+def _context(mesh):
+    """A cached context per mesh for the per-call API (plans are reused)."""
+    k = mesh.key()
+    c = _ctx_cache.get(k)
+    if c is None:
+        c = Context(mesh)
+        _ctx_cache[k] = c
+    return c
+
+
+def _fields_in(ctx, ex, ey, bz):
+    ctx.set_grid(L.PIC_GRID_EX, ex)
+    ctx.set_grid(L.PIC_GRID_EY, ey)
+    if bz is None:
+        ctx.set_bz(L.PIC_BZ_NONE)
+    else:
+        b = np.asarray(bz, dtype=np.float64)
+        if b.ndim == 0:
+            ctx.set_bz(L.PIC_BZ_UNIFORM, float(b))
+        else:
+            ctx.set_grid(L.PIC_GRID_BZ, b)
+            ctx.set_bz(L.PIC_BZ_GRID)
+
+
+def _check_grid(a, mesh, name):
+    a = np.asarray(a)
+    if a.shape[0] < mesh.nx or a.shape[1] < mesh.ny:
+        raise ValueError(f"{name} has shape {a.shape}, the mesh needs at least "
+                         f"({mesh.nx}, {mesh.ny})")
+
+
+# ------------------------------------------------------ the reference calls
+
+def compute_rho(p, kernel, mesh):
+    """compute_rho(p, kernel, mesh): returns the nx x ny charge density,
+    normalised and with its mean removed (pushers_cic.jl:75-114,
+    pushers_m6.jl:226-348)."""
+    ctx = _context(mesh)
+    ctx.set_particles(p.array)
+    L.call("pic_compute_rho", ctx.h, _kcode(kernel))
+    return ctx.get_grid(L.PIC_GRID_RHO)
+
+
+def push_v_(p, kernel, mesh, ex, ey, bz, dt):
+    """push_v!(p, kernel, mesh, ex, ey, bz, dt), in place on p.array.
+    bz may be an array, a scalar (uniform field) or None (no field)."""
+    for a, nm in ((ex, "ex"), (ey, "ey")):
+        _check_grid(a, mesh, nm)
+    ctx = _context(mesh)
+    ctx.set_particles(p.array)
+    _fields_in(ctx, ex, ey, bz)
+    L.call("pic_push_v", ctx.h, _kcode(kernel), float(dt))
+    ctx.get_particles(p.array)
+
+
+def push_x_(p, mesh, dt):
+    """push_x!(p, mesh, dt), in place on p.array."""
+    ctx = _context(mesh)
+    ctx.set_particles(p.array)
+    L.call("pic_push_x", ctx.h, float(dt))
+    ctx.get_particles(p.array)
+
+
+def interpolate_cic_(epx, epy, mesh, p, ex, ey):
+    """interpolate_cic!(epx, epy, mesh, p, ex, ey), fills epx, epy."""
+    ctx = _context(mesh)
+    ctx.set_particles(p.array)
+    _fields_in(ctx, ex, ey, None)
+    n = p.array.shape[1]
+    if epx.shape[0] < n or epy.shape[0] < n:
+        raise ValueError("epx/epy shorter than the particle count")
+    ax = np.zeros(n)
+    ay = np.zeros(n)
+    L.call("pic_interpolate_cic", ctx.h, ax.ctypes.data, ay.ctypes.data)
+    epx[:n] = ax
+    epy[:n] = ay
+
+
+class TwoDPoissonPeriodic:
+    """TwoDPoissonPeriodic(grid) (poisson_2d.jl:28-61): the tables live in the
+    device context of the grid."""
+
+    def __init__(self, grid):
+        self.grid = grid
+
+
+class Poisson2D:
+    """Poisson2D(mesh) (meshfields.jl:27-73)."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+
+
+class MeshFields2D:
+    """MeshFields2D(mesh): e is 2 x (nx+1) x (ny+1), rho (nx+1) x (ny+1)."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        self.e = np.zeros((2, mesh.nx + 1, mesh.ny + 1), order="F")
+        self.rho = np.zeros((mesh.nx + 1, mesh.ny + 1), order="F")
+
+
+class Particles:
+    """Particles(nbpart, w) (particles.jl:12-35): x, v, e are 2 x n."""
+
+    def __init__(self, nbpart, w):
+        self.nbpart = int(nbpart)
+        self.x = np.zeros((2, self.nbpart), order="F")
+        self.v = np.zeros((2, self.nbpart), order="F")
+        self.e = np.zeros((2, self.nbpart), order="F")
+        self.b = np.zeros(self.nbpart)
+        self.t = np.zeros(self.nbpart)
+        self.w = float(w)
+
+
+def solve_(*args):
+    """solve!(phi, poisson, rho); solve!(ex, ey, poisson, rho);
+    solve!(fields, poisson::Poisson2D) -> field energy."""
+    if len(args) == 2 and isinstance(args[1], Poisson2D):
+        fields, ps = args
+        ctx = _context(ps.mesh)
+        ctx.set_grid(L.PIC_GRID_RHO, fields.rho)
+        en = C.c_double()
+        L.call("pic_solve_fields", ctx.h, C.byref(en))
+        fields.e[0] = ctx.get_grid(L.PIC_GRID_EX, ghost=True)
+        fields.e[1] = ctx.get_grid(L.PIC_GRID_EY, ghost=True)
+        return en.value
+    if len(args) == 3:
+        phi, ps, rho = args
+        ctx = _context(ps.grid)
+        ctx.set_grid(L.PIC_GRID_RHO, rho)
+        L.call("pic_solve_phi", ctx.h)
+        phi[...] = ctx.get_grid(L.PIC_GRID_PHI)
+        return phi
+    if len(args) == 4:
+        ex, ey, ps, rho = args
+        ctx = _context(ps.grid)
+        ctx.set_grid(L.PIC_GRID_RHO, rho)
+        L.call("pic_solve_e", ctx.h)
+        ex[...] = ctx.get_grid(L.PIC_GRID_EX)
+        ey[...] = ctx.get_grid(L.PIC_GRID_EY)
+        return ex, ey
+    raise TypeError("solve_(phi, poisson, rho) | solve_(ex, ey, poisson, rho) | "
+                    "solve_(fields, poisson2d)")
+
+
+def _set_xv(ctx, particles):
+    x = np.asfortranarray(particles.x)
+    v = np.asfortranarray(particles.v)
+    L.call("pic_set_particles_xv", ctx.h, x.ctypes.data, v.ctypes.data, particles.w,
+           particles.nbpart)
+
+
+def compute_rho_m6_(fields, particles, compat=True):
+    """compute_rho_m6!(fields, particles) (compute_rho_m6.jl:181-316).
+    compat=True reproduces the reference's swapped bounds and neighbour
+    indices; compat=False the corrected stencil (see the C ABI header)."""
+    ctx = _context(fields.mesh)
+    _set_xv(ctx, particles)
+    L.call("pic_compute_rho_m6_fields", ctx.h, 1 if compat else 0)
+    fields.rho[...] = ctx.get_grid(L.PIC_GRID_RHO, ghost=True)
+    x = np.zeros((2, particles.nbpart), order="F")
+    L.call("pic_get_particles_xv", ctx.h, x.ctypes.data, None)
+    particles.x[...] = x
+
+
+def interpol_eb_m6_(particles, fields):
+    """interpol_eb_m6!(particles, fields) (interpolation_m6.jl:125-247)."""
+    ctx = _context(fields.mesh)
+    _set_xv(ctx, particles)
+    ctx.set_grid(L.PIC_GRID_EX, fields.e[0])
+    ctx.set_grid(L.PIC_GRID_EY, fields.e[1])
+    L.call("pic_interpol_eb_m6", ctx.h)
+    e = np.zeros((2, particles.nbpart), order="F")
+    L.call("pic_get_particle_fields", ctx.h, e.ctypes.data)
+    particles.e[...] = e
+    x = np.zeros((2, particles.nbpart), order="F")
+    L.call("pic_get_particles_xv", ctx.h, x.ctypes.data, None)
+    particles.x[...] = x
+
+
+class F90:
+    """ParticleInCell.F90 (src/pic.jl): the Fortran twin's entry points with its
+    exact argument conventions, served by pic_f90_* of the HIP library."""
+
+    @staticmethod
+    def push_v_(p, kernel, m, ex, ey, bz, dt):
+        _kcode(kernel)
+        a = p.array
+        n = C.c_int32(a.shape[1])
+        ex, ey, bz = (np.asfortranarray(g, dtype=np.float64) for g in (ex, ey, bz))
+        L.call("pic_f90_push_v", C.byref(n), a.ctypes.data, C.byref(C.c_double(dt)),
+               C.byref(C.c_int32(m.nx)), C.byref(C.c_int32(m.ny)), C.byref(C.c_double(m.dx)),
+               C.byref(C.c_double(m.dy)), ex.ctypes.data, ey.ctypes.data, bz.ctypes.data)
+
+    @staticmethod
+    def push_x_(p, mesh, dt):
+        a = p.array
+        L.call("pic_f90_push_x", C.byref(C.c_int32(a.shape[1])), C.byref(C.c_double(mesh.dimx)),
+               C.byref(C.c_double(mesh.dimy)), a.ctypes.data, C.byref(C.c_double(dt)))
+
+    @staticmethod
+    def compute_current_(jx, jy, m, kernel, p):
+        """F90.compute_current! (pic.jl:18-58): deposition, then the Julia-side
+        fold of the jx ghosts (pic.jl:49-56; jy is not folded there)."""
+        _kcode(kernel)
+        a = p.array
+        assert jx.flags.f_contiguous and jy.flags.f_contiguous
+        L.call("pic_f90_deposition", C.byref(C.c_int32(a.shape[1])), C.byref(C.c_int32(m.nx)),
+               C.byref(C.c_int32(m.ny)), C.byref(C.c_double(m.dx)), C.byref(C.c_double(m.dy)),
+               a.ctypes.data, jx.ctypes.data, jy.ctypes.data)
+        nx, ny = m.nx, m.ny
+        for i in range(nx + 1):
+            jx[i, 0] += jx[i, ny]
+            jx[i, ny] = jx[i, 0]
+        for j in range(ny + 1):
+            jx[0, j] += jx[nx, j]
+            jx[nx, j] = jx[0, j]
+
+
+# ------------------------------------------------- device-resident run
+
+class Simulation:
+    """Particles and fields resident on the GPU for the hot-path time loop
+    (examples/landau_damping_poisson_cic_m6.jl:70-83)."""
+
+    def __init__(self, mesh, p, kernel, state=L.PIC_STATE_F64, bz=None, device=0):
+        self.mesh = mesh
+        self.kernel = _kcode(kernel)
+        arr = p.array if isinstance(p, ParticleGroup) else p
+        self.ctx = Context(mesh, capacity=arr.shape[1], device=device)
+        self.ctx.set_particles(arr)
+        if bz is None:
+            self.ctx.set_bz(L.PIC_BZ_NONE)
+        elif np.ndim(bz) == 0:
+            self.ctx.set_bz(L.PIC_BZ_UNIFORM, float(bz))
+        else:
+            self.ctx.set_grid(L.PIC_GRID_BZ, bz)
+            self.ctx.set_bz(L.PIC_BZ_GRID)
+        self.ctx.set_state(state)
+        self.n = arr.shape[1]
+
+    def sort(self):
+        perm = np.zeros(self.n, np.int32)
+        L.call("pic_sort", self.ctx.h, perm.ctypes.data)
+        return perm
+
+    def cell_offsets(self):
+        off = np.zeros(self.mesh.nx * self.mesh.ny + 1, np.int64)
+        L.call("pic_cell_offsets", self.ctx.h, off.ctypes.data)
+        return off
+
+    def run(self, dt, nsteps, sort_every=0):
+        """Runs nsteps loop iterations; returns (sum ex^2 dxdy, sum ey^2 dxdy)
+        per step as an (nsteps, 2) array."""
+        e = np.zeros((nsteps, 2))
+        L.call("pic_run", self.ctx.h, self.kernel, float(dt), int(nsteps), int(sort_every),
+               e.ctypes.data)
+        return e
+
+    def particles(self):
+        out = np.zeros((5, self.n), order="F")
+        return self.ctx.get_particles(out)
+
+    def state_f32(self):
+        return self.ctx.get_state_f32()
+
+    def grid(self, which):
+        return self.ctx.get_grid(which)
+
+    def set_timing(self, on=True):
+        L.call("pic_set_timing", self.ctx.h, 1 if on else 0)
+
+    def timing(self):
+        """(pass_ms_total, passes, solve_ms_total) of pic_run since timing
+        was enabled (HIP events on the context stream)."""
+        a, b, n = C.c_double(), C.c_double(), C.c_int64()
+        L.call("pic_get_timing", self.ctx.h, C.byref(a), C.byref(n), C.byref(b))
+        return a.value, n.value, b.value
+
+    def set_particles(self, p, state=L.PIC_STATE_F64):
+        arr = p.array if isinstance(p, ParticleGroup) else p
+        self.ctx.set_particles(arr)
+        self.ctx.set_state(state)
+        self.n = arr.shape[1]
+
+    def slow_chunks(self):
+        v = C.c_uint64()
+        L.call("pic_slow_chunks", self.ctx.h, C.byref(v))
+        return v.value
+
+    def close(self):
+        self.ctx.close()
+
+
+def energy(ex, mesh):
+    """sqrt(sum(ex.^2) * dx * dy) (examples/landau_damping_poisson_cic_m6.jl:65)."""
+    return math.sqrt(float(np.sum(np.asarray(ex) ** 2)) * mesh.dx * mesh.dy)

@@ -1,0 +1,228 @@
+// pic_internal.h -- shared declarations of the MI355X particle-mesh library.
+//
+// One context (`pic_ctx`) owns, on one GPU and one HIP stream:
+//   * the mesh (nx x ny periodic cells on [xmin,xmax) x [ymin,ymax)),
+//   * the particles, device resident, structure-of-arrays:
+//       FP64 state  x, y, vx, vy (+ w when weights differ)       32-40 B each
+//       FP32 state  cell (int32), ox, oy, vx, vy (float)          20 B each
+//   * the grids rho, ex, ey, bz, phi (FP64, nx*ny, x fastest like Julia's
+//     column-major nx x ny arrays) and the FP32 interleaved (ex, ey) copy the
+//     FP32 gather reads,
+//   * rocFFT plans for the Poisson solve, sort scratch, reduction scratch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/pic_mi355x.h"
+
+namespace pic {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+int fail(int code, const char *fmt, ...);
+
+#define PIC_HIP(call)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return ::pic::fail(PIC_ERR_HIP, "%s failed: %s (%s:%d)", #call,              \
+                               hipGetErrorString(e_), __FILE__, __LINE__);               \
+    } while (0)
+
+#define PIC_CHECK(cond, code, ...)                                                       \
+    do {                                                                                 \
+        if (!(cond)) return ::pic::fail(code, __VA_ARGS__);                              \
+    } while (0)
+
+#define PIC_TRY(call)                                                                    \
+    do {                                                                                 \
+        int r_ = (call);                                                                 \
+        if (r_ != PIC_OK) return r_;                                                     \
+    } while (0)
+
+// ------------------------------------------------------------ the context
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // mesh (TwoDGrid: dx = dimx/nx as GEMPIC's grid defines it)
+    int nx = 0, ny = 0;
+    double xmin = 0, xmax = 0, ymin = 0, ymax = 0;
+    double dimx = 0, dimy = 0, dx = 0, dy = 0;
+
+    // particles
+    int64_t n = 0, cap = 0;
+    int state = PIC_STATE_F64;          // which representation is current
+    bool uniform_w = true;
+    double w0 = 0.0;                    // the common weight when uniform_w
+    double *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr, *w = nullptr;
+    double *epx = nullptr, *epy = nullptr;  // particle fields (gather results)
+    int32_t *cell = nullptr;
+    float *ox = nullptr, *oy = nullptr, *vx32 = nullptr, *vy32 = nullptr;
+    void *swap = nullptr;               // 40 B/particle reorder scratch
+    int64_t swap_bytes = 0;
+    // second set of state arrays: the sort gathers into these and swaps the
+    // pointers (keyed by the address of the Ctx member they pair with)
+    static constexpr int NALT = 10;
+    void **alt_of[NALT] = {};
+    void *alt[NALT] = {};
+
+    // grids, nx*ny each, FP64
+    double *rho = nullptr, *ex = nullptr, *ey = nullptr, *bz = nullptr, *phi = nullptr;
+    double *rho_acc = nullptr;          // raw deposit accumulator (not normalised)
+    float *e32 = nullptr;               // (ex, ey) interleaved FP32 per node
+    float *bz32 = nullptr;
+    int bz_mode = PIC_BZ_NONE;          // none / uniform / grid
+    double bz0 = 0.0;
+
+    // Poisson: tables (nx/2+1) x ny, spectra, rocFFT
+    double *kx_k2 = nullptr, *ky_k2 = nullptr, *k2 = nullptr;
+    double2 *rhat = nullptr, *ehat = nullptr;  // ehat holds 2 spectra (x, y)
+    rocfft_plan plan_r2c = nullptr, plan_c2r2 = nullptr, plan_c2r1 = nullptr, plan_r2c2 = nullptr;
+    rocfft_execution_info fft_info = nullptr;
+    void *fft_work = nullptr;
+    size_t fft_work_bytes = 0;
+
+    // reductions: partial sums and results (device), counters
+    double *red_part = nullptr;         // [RED_BLOCKS * 4]
+    unsigned *red_count = nullptr;      // arrival counter (reset by the last block)
+    double *energy_hist = nullptr;      // device energy history [energy_cap * 2]
+    int64_t energy_cap = 0;
+    int *step_counter = nullptr;        // device index into energy_hist
+
+    // sort scratch
+    uint32_t *sort_keys[2] = {nullptr, nullptr};
+    uint32_t *sort_vals[2] = {nullptr, nullptr};
+    uint32_t *sort_hist = nullptr;
+    int64_t sort_hist_cap = 0;
+    uint32_t *sort_blk = nullptr;
+    int64_t *cell_off = nullptr;        // ncells + 1 exclusive offsets after a sort
+    int64_t sort_cap = 0;
+
+    // fused-step tile statistics (diagnostic): chunks that took the slow path
+    unsigned long long *slow_chunks = nullptr;
+
+    // captured step graphs
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_key = -1;
+
+    // pic_run kernel timing (HIP events on the context stream)
+    bool timing = false;
+    double t_pass_ms = 0.0, t_solve_ms = 0.0;
+    int64_t t_passes = 0;
+};
+
+Ctx *ctx_of(pic_ctx *c);
+
+// ---------------------------------------------------------- allocation
+int ensure_particles(Ctx *c, int64_t n);
+int ensure_f32(Ctx *c);
+int ensure_swap(Ctx *c, int64_t bytes);
+int ensure_pfields(Ctx *c);
+int ensure_sort(Ctx *c);
+int ensure_energy(Ctx *c, int64_t n);
+
+// ------------------------------------------------------ module entry points
+int poisson_init(Ctx *c);
+void poisson_free(Ctx *c);
+int poisson_solve_e(Ctx *c, const double *rho_in, double scale, int make_e32, int energy_slot);
+int poisson_solve_phi(Ctx *c);
+int poisson_adjoint(Ctx *c, const double *lex, const double *ley, double *lrho, double scale);
+int finalize_rho(Ctx *c, int kernel);
+int field_energy(Ctx *c, double *out2);
+int field_energy_mf(Ctx *c, double *out);
+
+int sort_by_cell(Ctx *c, int32_t *perm_host);
+
+int launch_fill(Ctx *c, double *p, double v, int64_t n);
+
+// ------------------------------------------------------------ constants
+constexpr int RED_BLOCKS = 256;
+constexpr int RED_THREADS = 256;
+
+}  // namespace pic
+
+// ------------------------------------------------------- device helpers
+namespace picd {
+
+__host__ __device__ inline int wrapi(int i, int n) {
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+// Julia mod1(x, n) - 1, i.e. the 0-based periodic index of 1-based x.
+__host__ __device__ inline long long mod1z(long long x, long long n) {
+    long long r = (x - 1) % n;
+    return r < 0 ? r + n : r;
+}
+
+// Quintic B-spline (M6) weights of nodes i-2..i+3 at offset t in [0,1), FP32.
+// Expanded pieces of f_m6 (compute_rho_m6.jl:10-25), coefficients divided by
+// 120 and rounded to float; evaluated by Horner with fused multiply-adds.
+#define PIC_C120(v) ((float)((v) / 120.0))
+__device__ __forceinline__ float m6_p3f(float u) {
+    float r = PIC_C120(10.0);
+    r = __builtin_fmaf(r, u, PIC_C120(-20.0));
+    r = __builtin_fmaf(r, u, PIC_C120(-20.0));
+    r = __builtin_fmaf(r, u, PIC_C120(20.0));
+    r = __builtin_fmaf(r, u, PIC_C120(50.0));
+    r = __builtin_fmaf(r, u, PIC_C120(26.0));
+    return r;
+}
+__device__ __forceinline__ float m6_p4f(float u) {
+    float r = PIC_C120(-5.0);
+    r = __builtin_fmaf(r, u, PIC_C120(5.0));
+    r = __builtin_fmaf(r, u, PIC_C120(10.0));
+    r = __builtin_fmaf(r, u, PIC_C120(10.0));
+    r = __builtin_fmaf(r, u, PIC_C120(5.0));
+    r = __builtin_fmaf(r, u, PIC_C120(1.0));
+    return r;
+}
+__device__ __forceinline__ void m6_weights_f(float t, float w[6]) {
+    float s = 1.0f - t;
+    float t2 = t * t, s2 = s * s;
+    float t4 = t2 * t2, s4 = s2 * s2;
+    w[0] = (s4 * s) * PIC_C120(1.0);
+    w[1] = m6_p4f(s);
+    w[2] = m6_p3f(s);
+    w[3] = m6_p3f(t);
+    w[4] = m6_p4f(t);
+    w[5] = (t4 * t) * PIC_C120(1.0);
+}
+
+// The same six weights in FP64, exact integer coefficients, divided by 120 at
+// the end (Horner, fused multiply-adds).
+__device__ __forceinline__ double m6_p3d(double u) {
+    double r = 10.0;
+    r = __builtin_fma(r, u, -20.0);
+    r = __builtin_fma(r, u, -20.0);
+    r = __builtin_fma(r, u, 20.0);
+    r = __builtin_fma(r, u, 50.0);
+    r = __builtin_fma(r, u, 26.0);
+    return r;
+}
+__device__ __forceinline__ double m6_p4d(double u) {
+    double r = -5.0;
+    r = __builtin_fma(r, u, 5.0);
+    r = __builtin_fma(r, u, 10.0);
+    r = __builtin_fma(r, u, 10.0);
+    r = __builtin_fma(r, u, 5.0);
+    r = __builtin_fma(r, u, 1.0);
+    return r;
+}
+__device__ __forceinline__ void m6_weights_d(double t, double w[6]) {
+    const double c = 1.0 / 120.0;
+    double s = 1.0 - t;
+    double t2 = t * t, s2 = s * s;
+    w[0] = (s2 * s2 * s) * c;
+    w[1] = m6_p4d(s) * c;
+    w[2] = m6_p3d(s) * c;
+    w[3] = m6_p3d(t) * c;
+    w[4] = m6_p4d(t) * c;
+    w[5] = (t2 * t2 * t) * c;
+}
+
+}  // namespace picd

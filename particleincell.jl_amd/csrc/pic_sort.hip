@@ -1,0 +1,360 @@
+// pic_sort.hip -- stable sort of the particles by cell.
+//
+// LSD radix sort of (cell, index) pairs: ceil(B/8) passes over digits of at
+// most 8 bits, B = bits of the largest cell index.  Every pass is a stable
+// counting sort of one digit:
+//   k_digit_hist   per tile of 4096 keys, a histogram of the digit
+//                  (LDS counters, one atomic per run of equal digits)
+//   k_scan_*       exclusive scan of the histograms in digit-major order,
+//                  giving each (digit, tile) its first output slot
+//   k_digit_scatter  per tile, in item order: the rank of each key among the
+//                  equal digits before it (wave ballots for the lanes of one
+//                  wave, LDS counts for earlier waves and rounds), then the
+//                  scatter
+// Equal keys keep their input order in every pass, so the result is the
+// permutation std::stable_sort by cell produces.  The permutation is then
+// applied to the particle state by a gather into a second set of arrays
+// whose pointers are swapped in.
+#include <climits>
+#include <vector>
+
+#include "pic_internal.h"
+
+namespace pic {
+
+constexpr int SORT_T = 256;          // threads per tile
+constexpr int SORT_R = 16;           // rounds (items per thread)
+constexpr int SORT_TILE = SORT_T * SORT_R;
+
+// keys from the current state: cell index i + nx*j (FP64 state: the cell of
+// floor(x/dx), floor(y/dy), wrapped).
+__global__ void k_keys_f64(int64_t n, const double *x, const double *y, double dx, double dy,
+                           int nx, int ny, uint32_t *key, uint32_t *val) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int i = picd::wrapi((int)floor(x[k] / dx), nx);
+    int j = picd::wrapi((int)floor(y[k] / dy), ny);
+    key[k] = (uint32_t)(i + nx * j);
+    val[k] = (uint32_t)k;
+}
+
+__global__ void k_keys_f32(int64_t n, const int32_t *cell, uint32_t *key, uint32_t *val) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    key[k] = (uint32_t)cell[k];
+    val[k] = (uint32_t)k;
+}
+
+__global__ __launch_bounds__(SORT_T) void k_digit_hist(const uint32_t *__restrict__ key, int64_t n,
+                                                       int shift, int bits, int ntiles,
+                                                       uint32_t *__restrict__ hist) {
+    __shared__ uint32_t cnt[256];
+    const int R = 1 << bits;
+    const int tid = threadIdx.x;
+    for (int d = tid; d < R; d += SORT_T) cnt[d] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
+    uint32_t cur = 0xffffffffu, run = 0;
+    for (int r = 0; r < SORT_R; ++r) {
+        int64_t p = base + (int64_t)r * SORT_T + tid;
+        if (p >= n) break;
+        uint32_t d = (key[p] >> shift) & (uint32_t)(R - 1);
+        if (d == cur) {
+            ++run;
+        } else {
+            if (run) atomicAdd(&cnt[cur], run);
+            cur = d;
+            run = 1;
+        }
+    }
+    if (run) atomicAdd(&cnt[cur], run);
+    __syncthreads();
+    for (int d = tid; d < R; d += SORT_T) hist[(int64_t)d * ntiles + blockIdx.x] = cnt[d];
+}
+
+// --- exclusive scan of uint32 (3 phases, 2048 items per block)
+constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_B = SCAN_T * SCAN_I;
+
+__device__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh, uint32_t &total) {
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int o = 1; o < SCAN_T; o <<= 1) {
+        uint32_t t = tid >= o ? sh[tid - o] : 0u;
+        __syncthreads();
+        sh[tid] += t;
+        __syncthreads();
+    }
+    total = sh[SCAN_T - 1];
+    uint32_t excl = sh[tid] - v;
+    __syncthreads();
+    return excl;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const uint32_t *in, int64_t n,
+                                                        uint32_t *bsum) {
+    __shared__ uint32_t sh[SCAN_T];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_B;
+    uint32_t s = 0;
+    for (int i = 0; i < SCAN_I; ++i) {
+        int64_t p = base + (int64_t)threadIdx.x * SCAN_I + i;
+        if (p < n) s += in[p];
+    }
+    uint32_t tot;
+    block_excl_scan(s, sh, tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// single block: exclusive scan of nb block sums in place
+__global__ __launch_bounds__(SCAN_T) void k_scan_top(uint32_t *bsum, int nb) {
+    __shared__ uint32_t sh[SCAN_T];
+    uint32_t carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += SCAN_T) {
+        int i = b0 + threadIdx.x;
+        uint32_t v = i < nb ? bsum[i] : 0u;
+        uint32_t tot;
+        uint32_t e = block_excl_scan(v, sh, tot);
+        if (i < nb) bsum[i] = carry + e;
+        carry += tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_down(uint32_t *inout, int64_t n,
+                                                      const uint32_t *bsum) {
+    __shared__ uint32_t sh[SCAN_T];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_B;
+    uint32_t v[SCAN_I], s = 0;
+    for (int i = 0; i < SCAN_I; ++i) {
+        int64_t p = base + (int64_t)threadIdx.x * SCAN_I + i;
+        v[i] = p < n ? inout[p] : 0u;
+        s += v[i];
+    }
+    uint32_t tot;
+    uint32_t e = block_excl_scan(s, sh, tot) + bsum[blockIdx.x];
+    for (int i = 0; i < SCAN_I; ++i) {
+        int64_t p = base + (int64_t)threadIdx.x * SCAN_I + i;
+        if (p < n) inout[p] = e;
+        e += v[i];
+    }
+}
+
+__global__ __launch_bounds__(SORT_T) void k_digit_scatter(
+    const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, int64_t n, int shift,
+    int bits, int ntiles, const uint32_t *__restrict__ offs, uint32_t *__restrict__ kout,
+    uint32_t *__restrict__ vout) {
+    __shared__ uint32_t base_d[256];         // next slot of each digit in this tile
+    __shared__ uint32_t wcnt[SORT_T / 64][256];
+    const int R = 1 << bits;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int d = tid; d < R; d += SORT_T) {
+        base_d[d] = offs[(int64_t)d * ntiles + blockIdx.x];
+        for (int w = 0; w < SORT_T / 64; ++w) wcnt[w][d] = 0;
+    }
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int r = 0; r < SORT_R; ++r) {
+        const int64_t p = base + (int64_t)r * SORT_T + tid;
+        const bool valid = p < n;
+        uint32_t kk = valid ? kin[p] : 0u, vv = valid ? vin[p] : 0u;
+        uint32_t d = (kk >> shift) & (uint32_t)(R - 1);
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < bits; ++b) {
+            uint64_t bal = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        const int rank_w = __popcll(peers & lt);
+        const bool leader = valid && (peers & lt) == 0;
+        if (leader) wcnt[wv][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t before = 0;
+            for (int w = 0; w < wv; ++w) before += wcnt[w][d];
+            uint32_t dst = base_d[d] + before + (uint32_t)rank_w;
+            kout[dst] = kk;
+            vout[dst] = vv;
+        }
+        __syncthreads();
+        if (leader) {
+            atomicAdd(&base_d[d], wcnt[wv][d]);
+        }
+        __syncthreads();
+        if (leader) wcnt[wv][d] = 0;
+        // the next round reads wcnt only after its own __syncthreads
+    }
+}
+
+// Exclusive offsets of every cell's run in the sorted keys (ncells + 1).
+__global__ void k_cell_offsets(const uint32_t *__restrict__ key, int64_t n, int ncells,
+                               int64_t *__restrict__ off) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > n) return;
+    int lo = r == 0 ? 0 : (int)key[r - 1] + 1;
+    int hi = r == n ? ncells : (int)key[r];
+    for (int c = lo; c <= hi; ++c) off[c] = r;
+}
+
+template <typename T>
+__global__ void k_gather(int64_t n, const uint32_t *__restrict__ perm, const T *__restrict__ in,
+                         T *__restrict__ out) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) out[r] = in[perm[r]];
+}
+
+int ensure_sort(Ctx *c) {
+    if (c->sort_cap < c->cap || !c->sort_keys[0]) {
+        for (int k = 0; k < 2; ++k) {
+            if (c->sort_keys[k]) (void)hipFree(c->sort_keys[k]);
+            if (c->sort_vals[k]) (void)hipFree(c->sort_vals[k]);
+            c->sort_keys[k] = c->sort_vals[k] = nullptr;
+        }
+        if (c->sort_blk) (void)hipFree(c->sort_blk);
+        c->sort_blk = nullptr;
+        int64_t cap = c->cap > 0 ? c->cap : 1;
+        for (int k = 0; k < 2; ++k) {
+            PIC_HIP(hipMalloc(&c->sort_keys[k], cap * 4));
+            PIC_HIP(hipMalloc(&c->sort_vals[k], cap * 4));
+        }
+        c->sort_cap = cap;
+    }
+    const int64_t ntiles = (c->cap + SORT_TILE - 1) / SORT_TILE + 1;
+    const int64_t hcap = 256 * ntiles;
+    if (hcap > c->sort_hist_cap) {
+        if (c->sort_hist) (void)hipFree(c->sort_hist);
+        if (c->sort_blk) (void)hipFree(c->sort_blk);
+        PIC_HIP(hipMalloc(&c->sort_hist, hcap * 4));
+        PIC_HIP(hipMalloc(&c->sort_blk, ((hcap + SCAN_B - 1) / SCAN_B + 1) * 4));
+        c->sort_hist_cap = hcap;
+    }
+    if (!c->cell_off) PIC_HIP(hipMalloc(&c->cell_off, ((size_t)c->nx * c->ny + 1) * 8));
+    return PIC_OK;
+}
+
+static int scan(Ctx *c, uint32_t *a, int64_t n) {
+    const int nb = (int)((n + SCAN_B - 1) / SCAN_B);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, c->stream, a, n, c->sort_blk);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, c->stream, c->sort_blk, nb);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_T), 0, c->stream, a, n, c->sort_blk);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+// Sorts (sort_keys[0], sort_vals[0]) in place (ping-pong), stable.
+static int radix_sort_pairs(Ctx *c, int64_t n, int keybits) {
+    const int passes = (keybits + 7) / 8;
+    const int ntiles = (int)((n + SORT_TILE - 1) / SORT_TILE);
+    int cur = 0;
+    int shift = 0;
+    for (int ps = 0; ps < passes; ++ps) {
+        const int bits = (keybits - shift + (passes - ps) - 1) / (passes - ps);
+        hipLaunchKernelGGL(k_digit_hist, dim3(ntiles), dim3(SORT_T), 0, c->stream,
+                           c->sort_keys[cur], n, shift, bits, ntiles, c->sort_hist);
+        PIC_TRY(scan(c, c->sort_hist, (int64_t)(1 << bits) * ntiles));
+        hipLaunchKernelGGL(k_digit_scatter, dim3(ntiles), dim3(SORT_T), 0, c->stream,
+                           c->sort_keys[cur], c->sort_vals[cur], n, shift, bits, ntiles,
+                           c->sort_hist, c->sort_keys[1 - cur], c->sort_vals[1 - cur]);
+        PIC_HIP(hipGetLastError());
+        cur = 1 - cur;
+        shift += bits;
+    }
+    if (cur != 0) {
+        std::swap(c->sort_keys[0], c->sort_keys[1]);
+        std::swap(c->sort_vals[0], c->sort_vals[1]);
+    }
+    return PIC_OK;
+}
+
+// Gathers arr through the sort permutation into its alternate array and
+// swaps the two pointers.  Alternates are allocated once per array (same
+// capacity) and freed with the particles.
+template <typename T>
+static int permute(Ctx *c, T *&arr, int64_t n) {
+    void **key = reinterpret_cast<void **>(&arr);
+    int slot = -1;
+    for (int k = 0; k < Ctx::NALT; ++k)
+        if (c->alt_of[k] == key) slot = k;
+    if (slot < 0) {
+        for (int k = 0; k < Ctx::NALT && slot < 0; ++k)
+            if (!c->alt_of[k]) slot = k;
+        PIC_CHECK(slot >= 0, PIC_ERR_STATE, "no free alternate array slot");
+        void *q = nullptr;
+        PIC_HIP(hipMalloc(&q, (size_t)(c->cap > 0 ? c->cap : 1) * sizeof(T)));
+        c->alt_of[slot] = key;
+        c->alt[slot] = q;
+    }
+    T *alt = static_cast<T *>(c->alt[slot]);
+    hipLaunchKernelGGL(k_gather<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+                       c->sort_vals[0], arr, alt);
+    PIC_HIP(hipGetLastError());
+    c->alt[slot] = arr;
+    arr = alt;
+    return PIC_OK;
+}
+
+int sort_by_cell(Ctx *c, int32_t *perm_host) {
+    const int64_t n = c->n;
+    if (n == 0) return PIC_OK;
+    PIC_TRY(ensure_sort(c));
+    const int ncells = c->nx * c->ny;
+    int keybits = 1;
+    while ((1 << keybits) < ncells) ++keybits;
+    dim3 g((unsigned)((n + 255) / 256));
+    if (c->state == PIC_STATE_F32)
+        hipLaunchKernelGGL(k_keys_f32, g, dim3(256), 0, c->stream, n, c->cell, c->sort_keys[0],
+                           c->sort_vals[0]);
+    else
+        hipLaunchKernelGGL(k_keys_f64, g, dim3(256), 0, c->stream, n, c->x, c->y, c->dx, c->dy,
+                           c->nx, c->ny, c->sort_keys[0], c->sort_vals[0]);
+    PIC_HIP(hipGetLastError());
+    PIC_TRY(radix_sort_pairs(c, n, keybits));
+    hipLaunchKernelGGL(k_cell_offsets, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0,
+                       c->stream, c->sort_keys[0], n, ncells, c->cell_off);
+    PIC_HIP(hipGetLastError());
+    if (c->state == PIC_STATE_F32) {
+        PIC_TRY(permute(c, c->cell, n));
+        PIC_TRY(permute(c, c->ox, n));
+        PIC_TRY(permute(c, c->oy, n));
+        PIC_TRY(permute(c, c->vx32, n));
+        PIC_TRY(permute(c, c->vy32, n));
+    } else {
+        PIC_TRY(permute(c, c->x, n));
+        PIC_TRY(permute(c, c->y, n));
+        PIC_TRY(permute(c, c->vx, n));
+        PIC_TRY(permute(c, c->vy, n));
+        if (!c->uniform_w) PIC_TRY(permute(c, c->w, n));
+    }
+    if (perm_host) {
+        PIC_HIP(hipMemcpyAsync(perm_host, c->sort_vals[0], (size_t)n * 4, hipMemcpyDeviceToHost,
+                               c->stream));
+        PIC_HIP(hipStreamSynchronize(c->stream));
+    }
+    return PIC_OK;
+}
+
+}  // namespace pic
+
+using namespace pic;
+
+extern "C" {
+
+int pic_sort(pic_ctx *ctx, int32_t *perm) {
+    PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
+    Ctx *c = ctx_of(ctx);
+    PIC_HIP(hipSetDevice(c->device));
+    PIC_TRY(sort_by_cell(c, perm));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_cell_offsets(pic_ctx *ctx, int64_t *offsets) {
+    PIC_CHECK(ctx && offsets, PIC_ERR_ARG, "null pointer");
+    Ctx *c = ctx_of(ctx);
+    PIC_CHECK(c->cell_off, PIC_ERR_STATE, "no sort has run on this context");
+    PIC_HIP(hipMemcpyAsync(offsets, c->cell_off, ((size_t)c->nx * c->ny + 1) * 8,
+                           hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,460 @@
+"""Parity of the HIP library with the CPU oracle, per entry point, on the same
+seeded inputs.  Bars: bit-exact for cell indices, sort order and the CIC
+push (the reference's Fortran twin is bit-identical to Julia); 1e-12
+normwise relative for rho and E in FP64."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden, landau_particles, random_particles
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12  # FP64 rho and E, normwise relative
+
+
+def nrel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+# ------------------------------------------------------------------ push
+
+def test_push_v_push_x_cic_bitwise_f90(gpu, O):
+    pic = gpu
+    g = golden("f90_push.npz")
+    nx, ny = int(g["nx"]), int(g["ny"])
+    mesh = pic.TwoDGrid(float(g["dimx"]), nx, float(g["dimy"]), ny)
+    p = pic.ParticleGroup(g["p_in"].shape[1])
+    p.array[...] = g["p_in"]
+    pic.push_v_(p, pic.CloudInCell(), mesh, g["ex"][:nx, :ny], g["ey"][:nx, :ny],
+                g["bz"][:nx, :ny], float(g["dt"]))
+    assert np.array_equal(p.array, g["p_push_v"])
+    pic.push_x_(p, mesh, float(g["dt"]))
+    assert np.array_equal(p.array, g["p_push_x"])
+
+
+def test_f90_dropins_bitwise(gpu):
+    """ParticleInCell.F90's entry points (pic.jl:60-114) through pic_f90_*:
+    the fixture of pic.f90 and the test_push.jl 50-step trajectory."""
+    pic = gpu
+    g = golden("f90_push.npz")
+    nx, ny = int(g["nx"]), int(g["ny"])
+    mesh = pic.TwoDGrid(float(g["dimx"]), nx, float(g["dimy"]), ny)
+    p = pic.ParticleGroup(g["p_in"].shape[1])
+    p.array[...] = g["p_in"]
+    pic.F90.push_v_(p, pic.CloudInCell(), mesh, g["ex"], g["ey"], g["bz"], float(g["dt"]))
+    assert np.array_equal(p.array, g["p_push_v"])
+    pic.F90.push_x_(p, mesh, float(g["dt"]))
+    assert np.array_equal(p.array, g["p_push_x"])
+    traj = g["single_traj"]
+    m = pic.TwoDGrid(1.0, 100, 1.0, 100)
+    q = pic.ParticleGroup(1)
+    q.array[0, 0] = q.array[1, 0] = 0.1
+    q.array[4, 0] = 1.0
+    ones = np.ones((101, 101), order="F")
+    zeros = np.zeros((101, 101), order="F")
+    for s in range(50):
+        pic.F90.push_v_(q, pic.CloudInCell(), m, ones, zeros, ones, 0.1)
+        pic.F90.push_x_(q, m, 0.1)
+        assert np.array_equal(q.array[:, 0], traj[s]), s
+    assert q.array[0, 0] == 0.7723249868055071
+
+
+@pytest.mark.parametrize("bz", ["none", "uniform", "grid"])
+def test_push_v_m6(gpu, O, bz):
+    pic = gpu
+    rng = np.random.default_rng(31)
+    nx, ny = 32, 64
+    mesh = pic.TwoDGrid(4 * np.pi, nx, 2 * np.pi, ny)
+    p = pic.ParticleGroup(50000)
+    p.array[...] = random_particles(rng, 50000, mesh.dimx, mesh.dimy)
+    ex, ey = (np.asfortranarray(rng.normal(size=(nx, ny))) for _ in range(2))
+    bzg = np.asfortranarray(rng.normal(size=(nx, ny)))
+    bz_arg = {"none": None, "uniform": 1.7, "grid": bzg}[bz]
+    bz_o = {"none": None, "uniform": np.full((nx, ny), 1.7, order="F"), "grid": bzg}[bz]
+    q = p.array.copy(order="F")
+    pic.push_v_(p, pic.M6(), mesh, ex, ey, bz_arg, 0.05)
+    O.push_v(q, "m6", nx, ny, mesh.dx, mesh.dy, ex, ey, bz_o, 0.05)
+    assert np.array_equal(p.array[:2], q[:2])
+    assert np.max(np.abs(p.array[2:4] - q[2:4])) < 1e-14
+
+
+@pytest.mark.parametrize("bz", ["none", "uniform", "grid"])
+def test_push_v_cic_bitwise(gpu, O, bz):
+    pic = gpu
+    rng = np.random.default_rng(32)
+    nx, ny = 64, 32
+    mesh = pic.TwoDGrid(4 * np.pi, nx, 2.0, ny)
+    p = pic.ParticleGroup(50000)
+    p.array[...] = random_particles(rng, 50000, mesh.dimx, mesh.dimy)
+    ex, ey, bzg = (np.asfortranarray(rng.normal(size=(nx, ny))) for _ in range(3))
+    bz_arg = {"none": None, "uniform": -2.5, "grid": bzg}[bz]
+    bz_o = {"none": None, "uniform": np.full((nx, ny), -2.5, order="F"), "grid": bzg}[bz]
+    q = p.array.copy(order="F")
+    pic.push_v_(p, pic.CloudInCell(), mesh, ex, ey, bz_arg, 0.1)
+    O.push_v(q, "cic", nx, ny, mesh.dx, mesh.dy, ex, ey, bz_o, 0.1)
+    assert np.array_equal(p.array, q)
+
+
+def test_push_edge_positions(gpu, O):
+    """Particles on x = 0, x = dimx (kept by push_x!'s strict >) and just
+    below the upper bound: the cell wraps periodically on both sides."""
+    pic = gpu
+    nx, ny = 16, 16
+    mesh = pic.TwoDGrid(1.0, nx, 1.0, ny)
+    p = pic.ParticleGroup(4)
+    p.array[0] = [0.0, 1.0, np.nextafter(1.0, 0), 0.5]
+    p.array[1] = [1.0, 0.0, 0.5, np.nextafter(1.0, 0)]
+    p.array[2:4] = 0.3
+    p.array[4] = 0.25
+    ex = np.asfortranarray(np.arange(nx * ny, dtype=np.float64).reshape(nx, ny, order="F"))
+    ey = np.asfortranarray(-ex)
+    q = p.array.copy(order="F")
+    pic.push_v_(p, pic.CloudInCell(), mesh, ex, ey, None, 0.1)
+    O.push_v(q, "cic", nx, ny, mesh.dx, mesh.dy, ex, ey, None, 0.1)
+    assert np.array_equal(p.array, q)
+    rho = pic.compute_rho(p, pic.CloudInCell(), mesh)
+    rho_o = O.compute_rho(q, "cic", nx, ny, mesh.dx, mesh.dy)
+    assert nrel(rho, rho_o) < RTOL
+
+
+# ------------------------------------------------------------ deposits
+
+@pytest.mark.parametrize("kernel", ["cic", "m6"])
+@pytest.mark.parametrize("order", ["random", "sorted"])
+def test_compute_rho(gpu, O, kernel, order):
+    pic = gpu
+    rng = np.random.default_rng(33)
+    nx, ny, n = 128, 64, 200000
+    mesh = pic.TwoDGrid(4 * np.pi, nx, 4 * np.pi, ny)
+    p = pic.ParticleGroup(n)
+    p.array[...] = random_particles(rng, n, mesh.dimx, mesh.dimy,
+                                    w=rng.uniform(0.5, 1.5, n) * mesh.dimx * mesh.dimy / n)
+    if order == "sorted":
+        cell = (np.floor(p.array[0] / mesh.dx) + nx * np.floor(p.array[1] / mesh.dy)).astype(int)
+        p.array[...] = p.array[:, np.argsort(cell, kind="stable")]
+    k = pic.CloudInCell() if kernel == "cic" else pic.M6()
+    rho = pic.compute_rho(p, k, mesh)
+    rho_o = O.compute_rho(p.array, kernel, nx, ny, mesh.dx, mesh.dy, mesh.dimx, mesh.dimy)
+    assert nrel(rho, rho_o) < RTOL
+
+
+def test_compute_rho_uniform_and_empty(gpu, O):
+    pic = gpu
+    mesh, p = landau_particles(pic, 65536, 64, 64)
+    pg = pic.ParticleGroup(p.shape[1])
+    pg.array[...] = p
+    rho = pic.compute_rho(pg, pic.M6(), mesh)
+    rho_o = O.compute_rho(p, "m6", 64, 64, mesh.dx, mesh.dy, mesh.dimx, mesh.dimy)
+    assert nrel(rho, rho_o) < RTOL
+    empty = pic.ParticleGroup(0)
+    rho0 = pic.compute_rho(empty, pic.CloudInCell(), mesh)
+    assert np.all(rho0 == 0)
+
+
+def test_f90_deposition_total_current(gpu, O):
+    """test/test_deposition.jl: 128^2 on [0,4pi]^2, 1e6 particles around 2pi,
+    v = w = (4pi)^2/1e6: sum(jx) == sum(jy) ~ 2.5872575761191685."""
+    pic = gpu
+    rng = np.random.default_rng(1234)
+    n, nx = 1_000_000, 128
+    mesh = pic.TwoDGrid(4 * np.pi, nx, 4 * np.pi, nx)
+    p = pic.ParticleGroup(n)
+    p.array[0:2] = np.clip(rng.normal(size=(2, n)) + 2 * np.pi, 0, 4 * np.pi - 1e-9)
+    p.array[2:5] = (mesh.dimx * mesh.dimy) / n
+    jx = np.zeros((nx + 1, nx + 1), order="F")
+    jy = np.zeros((nx + 1, nx + 1), order="F")
+    pic.F90.compute_current_(jx, jy, mesh, pic.CloudInCell(), p)
+    assert math.isclose(jx.sum(), 2.5872575761191685, rel_tol=1.5e-8)
+    assert math.isclose(jy.sum(), 2.5872575761191685, rel_tol=1.5e-8)
+    if O.ref_available():
+        rx, ry = O.ref_deposition(p.array, nx, nx, mesh.dx, mesh.dy)
+        assert nrel(jy, ry) < 1e-13
+
+
+# ------------------------------------------------------------ gathers
+
+def test_interpolate_cic_bitwise(gpu, O):
+    pic = gpu
+    rng = np.random.default_rng(34)
+    nx, ny, n = 32, 32, 20000
+    mesh = pic.TwoDGrid(2.0, nx, 3.0, ny)
+    p = pic.ParticleGroup(n)
+    p.array[...] = random_particles(rng, n, mesh.dimx, mesh.dimy)
+    ex, ey = (np.asfortranarray(rng.normal(size=(nx, ny))) for _ in range(2))
+    epx, epy = np.zeros(n), np.zeros(n)
+    pic.interpolate_cic_(epx, epy, mesh, p, ex, ey)
+    ox, oy = O.interpolate_cic(p.array, nx, ny, mesh.dx, mesh.dy, ex, ey)
+    assert np.array_equal(epx, ox) and np.array_equal(epy, oy)
+
+
+@pytest.mark.parametrize("compat", [True, False])
+def test_meshfields_m6(gpu, O, compat):
+    """compute_rho_m6!(fields, particles) and interpol_eb_m6!(particles,
+    fields), MeshFields API (compute_rho_m6.jl:181, interpolation_m6.jl:125)."""
+    pic = gpu
+    rng = np.random.default_rng(35)
+    nx, ny, n = 64, 32, 30000
+    mesh = pic.TwoDGrid(0.0, 4 * np.pi, nx, 0.0, 2 * np.pi, ny)
+    w = mesh.dimx * mesh.dimy / n
+    parts = pic.Particles(n, w)
+    parts.x[0] = rng.uniform(-mesh.dimx, 2 * mesh.dimx, n)  # outside the box: wrapped
+    parts.x[1] = rng.uniform(0, mesh.dimy, n)
+    fields = pic.MeshFields2D(mesh)
+    xo = parts.x.copy(order="F")
+    pic.compute_rho_m6_(fields, parts, compat=compat)
+    rho_o = O.compute_rho_m6_fields(xo, w, nx, ny, 0.0, mesh.xmax, 0.0, mesh.ymax, compat=compat)
+    assert nrel(fields.rho, rho_o) < RTOL
+    assert np.array_equal(parts.x, xo)
+    fields.e[0] = rng.normal(size=(nx + 1, ny + 1))
+    fields.e[1] = rng.normal(size=(nx + 1, ny + 1))
+    for c in (0, 1):  # periodic ghosts as solve! leaves them
+        fields.e[c, nx, :] = fields.e[c, 0, :]
+        fields.e[c, :, ny] = fields.e[c, :, 0]
+    pic.interpol_eb_m6_(parts, fields)
+    ep = O.interpol_eb_m6(xo, fields.e, nx, ny, 0.0, mesh.xmax, 0.0, mesh.ymax)
+    assert np.max(np.abs(parts.e - ep)) < 1e-13 * np.abs(ep).max()
+    assert np.array_equal(parts.x, xo)
+
+
+# ------------------------------------------------------------ Poisson
+
+def test_solve_e_phi(gpu, O):
+    pic = gpu
+    rng = np.random.default_rng(36)
+    for nx, ny, dimx, dimy in ((64, 64, 4 * np.pi, 4 * np.pi), (128, 32, 1.0, 3.0),
+                               (16, 256, 2 * np.pi, 4 * np.pi)):
+        grid = pic.TwoDGrid(0.0, dimx, nx, 0.0, dimy, ny)
+        ps = pic.TwoDPoissonPeriodic(grid)
+        rho = np.asfortranarray(rng.normal(size=(nx, ny)))
+        ex, ey = np.zeros((nx, ny), order="F"), np.zeros((nx, ny), order="F")
+        pic.solve_(ex, ey, ps, rho)
+        eo, fo = O.solve_e(rho, dimx, dimy)
+        assert nrel(ex, eo) < RTOL and nrel(ey, fo) < RTOL
+        phi = np.zeros((nx, ny), order="F")
+        pic.solve_(phi, ps, rho)
+        assert nrel(phi, O.solve_phi(rho, dimx, dimy)) < RTOL
+
+
+def test_solve_fields_fixture(gpu, O):
+    """solve!(fields, Poisson2D) on the input of test/test2.dat: FFTW's output
+    and the energy of meshfields.jl:105-106."""
+    pic = gpu
+    g = golden("poisson_test2.npz")
+    nx, ny = int(g["nx"]), int(g["ny"])
+    mesh = pic.TwoDGrid(0.0, float(g["xmax"]), nx, 0.0, float(g["ymax"]), ny)
+    fields = pic.MeshFields2D(mesh)
+    fields.rho[...] = g["rho"]
+    en = pic.solve_(fields, pic.Poisson2D(mesh))
+    assert np.max(np.abs(fields.e[0] - g["ex"])) < 4e-15
+    assert np.max(np.abs(fields.e[1] - g["ey"])) < 4e-15
+    eo, en_o = O.solve_fields(np.asfortranarray(g["rho"]), 0.0, float(g["xmax"]), 0.0,
+                              float(g["ymax"]))
+    assert en == pytest.approx(en_o, rel=1e-13)
+
+
+def test_field_energy(gpu, O):
+    pic = gpu
+    rng = np.random.default_rng(37)
+    nx, ny = 64, 32
+    mesh = pic.TwoDGrid(3.0, nx, 2.0, ny)
+    ctx = pic.Context(mesh)
+    ex, ey = (np.asfortranarray(rng.normal(size=(nx, ny))) for _ in range(2))
+    ctx.set_grid(pic._lib.PIC_GRID_EX, ex)
+    ctx.set_grid(pic._lib.PIC_GRID_EY, ey)
+    out = (C.c_double * 2)()
+    pic._lib.call("pic_field_energy", ctx.h, out)
+    assert out[0] == pytest.approx(np.sum(ex ** 2) * mesh.dx * mesh.dy, rel=1e-13)
+    assert out[1] == pytest.approx(np.sum(ey ** 2) * mesh.dx * mesh.dy, rel=1e-13)
+    assert math.sqrt(out[0]) == pytest.approx(O.energy(ex, mesh.dx, mesh.dy), rel=1e-13)
+    ctx.close()
+
+
+# ------------------------------------------------------------ sort
+
+@pytest.mark.parametrize("state", ["f64", "f32"])
+@pytest.mark.parametrize("nx,ny,n", [(64, 64, 100000), (128, 128, 300001), (512, 512, 70000),
+                                     (8, 8, 5)])
+def test_sort_bitwise(gpu, O, state, nx, ny, n):
+    pic = gpu
+    rng = np.random.default_rng(nx + n)
+    mesh = pic.TwoDGrid(4 * np.pi, nx, 4 * np.pi, ny)
+    p = random_particles(rng, n, mesh.dimx, mesh.dimy)
+    sim = pic.Simulation(mesh, p, pic.CloudInCell(),
+                         state=pic._lib.PIC_STATE_F32 if state == "f32" else 0)
+    if state == "f32":
+        cell = sim.state_f32()[0]
+    else:
+        cell = (np.floor(p[0] / mesh.dx).astype(int) % nx
+                + nx * (np.floor(p[1] / mesh.dy).astype(int) % ny)).astype(np.int32)
+    perm = sim.sort()
+    want, off = O.counting_sort(cell, nx * ny)
+    assert np.array_equal(perm, want)
+    assert np.array_equal(sim.cell_offsets(), off)
+    if state == "f32":
+        assert np.array_equal(sim.state_f32()[0], cell[want])
+    else:
+        assert np.array_equal(sim.particles(), p[:, want])
+    sim.close()
+
+
+# ------------------------------------------------------------ FP32 state
+
+@pytest.mark.parametrize("kernel", ["cic", "m6"])
+def test_f32_step_bitwise(gpu, O, kernel):
+    """One fused FP32 pass (kick, drift, deposit) from the same state and the
+    same FP32 field: cells, offsets and velocities bit-identical to the
+    oracle; the compensated LDS tile sums within 1e-12 of the oracle's FP64
+    accumulation of the same FP32 contributions."""
+    pic = gpu
+    rng = np.random.default_rng(38)
+    nx, ny, n = 64, 64, 150000
+    mesh = pic.TwoDGrid(4 * np.pi, nx, 4 * np.pi, ny)
+    p = random_particles(rng, n, mesh.dimx, mesh.dimy)
+    cell = (np.floor(p[0] / mesh.dx) + nx * np.floor(p[1] / mesh.dy)).astype(int)
+    p = np.asfortranarray(p[:, np.argsort(cell, kind="stable")])
+    st = O.f32_from_f64(p, nx, ny, mesh.dx, mesh.dy)
+    w = np.float32(p[4, 0])
+    ex, ey = (np.asfortranarray(rng.normal(size=(nx, ny)) * 0.3) for _ in range(2))
+    ctx = pic.Context(mesh, capacity=n)
+    ctx.set_state_f32(st, float(w))
+    ctx.set_grid(pic._lib.PIC_GRID_EX, ex)
+    ctx.set_grid(pic._lib.PIC_GRID_EY, ey)
+    e32 = np.empty(2 * nx * ny, np.float32)
+    e32[0::2] = ex.ravel(order="F")
+    e32[1::2] = ey.ravel(order="F")
+    dt = 0.1
+    kc = pic._lib.PIC_CIC if kernel == "cic" else pic._lib.PIC_M6
+    ctx.set_bz(pic._lib.PIC_BZ_NONE)
+    raw = np.zeros((nx, ny), order="F")
+    pic._lib.call("pic_step_pass", ctx.h, kc, 1, dt, raw.ctypes.data)
+    got = ctx.get_state_f32()
+    acc = O.f32_step(st, nx, ny, kernel, e32, np.float32(dt / 2), np.float32(dt / mesh.dx),
+                     np.float32(dt / mesh.dy), w)
+    for a, b, name in zip(got, st, ("cell", "ox", "oy", "vx", "vy")):
+        assert np.array_equal(a, b), name
+    assert nrel(raw, acc) < 1e-12
+    # deposit-only pass of the new state
+    pic._lib.call("pic_step_pass", ctx.h, kc, 3, 0.0, raw.ctypes.data)
+    assert nrel(raw, O.f32_deposit(st, nx, ny, kernel, w)) < 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize("bz", ["none", "uniform", "grid"])
+@pytest.mark.parametrize("kernel", ["cic", "m6"])
+def test_f32_step_bz_bitwise(gpu, O, kernel, bz):
+    """The FP32 pass with a magnetic field (uniform, as the strong-B
+    configuration, or a grid): cells, offsets, velocities bit-identical."""
+    pic = gpu
+    rng = np.random.default_rng(39)
+    nx, ny, n = 32, 64, 60000
+    mesh = pic.TwoDGrid(4 * np.pi, nx, 4 * np.pi, ny)
+    p = random_particles(rng, n, mesh.dimx, mesh.dimy)
+    st = O.f32_from_f64(p, nx, ny, mesh.dx, mesh.dy)
+    w = np.float32(p[4, 0])
+    ex, ey, bzg = (np.asfortranarray(rng.normal(size=(nx, ny))) for _ in range(3))
+    ctx = pic.Context(mesh, capacity=n)
+    ctx.set_state_f32(st, float(w))
+    ctx.set_grid(pic._lib.PIC_GRID_EX, ex)
+    ctx.set_grid(pic._lib.PIC_GRID_EY, ey)
+    e32 = np.empty(2 * nx * ny, np.float32)
+    e32[0::2] = ex.ravel(order="F")
+    e32[1::2] = ey.ravel(order="F")
+    mode = {"none": 0, "uniform": 1, "grid": 2}[bz]
+    if bz == "grid":
+        ctx.set_grid(pic._lib.PIC_GRID_BZ, bzg)
+    ctx.set_bz(mode, 20.0)
+    dt = 0.0125
+    kc = pic._lib.PIC_CIC if kernel == "cic" else pic._lib.PIC_M6
+    raw = np.zeros((nx, ny), order="F")
+    pic._lib.call("pic_step_pass", ctx.h, kc, 1, dt, raw.ctypes.data)
+    got = ctx.get_state_f32()
+    acc = O.f32_step(st, nx, ny, kernel, e32, np.float32(dt / 2), np.float32(dt / mesh.dx),
+                     np.float32(dt / mesh.dy), w, bz_mode=mode, bz0=20.0,
+                     bz32=bzg.ravel(order="F").astype(np.float32))
+    for a, b, name in zip(got, st, ("cell", "ox", "oy", "vx", "vy")):
+        assert np.array_equal(a, b), name
+    assert nrel(raw, acc) < 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize("kernel", ["cic", "m6"])
+def test_f64_fused_pass(gpu, O, kernel):
+    """The fused FP64 pass = push_v!; push_x!(dt/2); push_x!(dt/2);
+    compute_rho (raw): positions and velocities bit-identical for CIC, within
+    rounding for M6; the deposit within 1e-12."""
+    pic = gpu
+    rng = np.random.default_rng(40)
+    nx, ny, n = 64, 64, 120000
+    mesh = pic.TwoDGrid(4 * np.pi, nx, 4 * np.pi, ny)
+    p = random_particles(rng, n, mesh.dimx, mesh.dimy)
+    ex, ey = (np.asfortranarray(rng.normal(size=(nx, ny))) for _ in range(2))
+    ctx = pic.Context(mesh, capacity=n)
+    ctx.set_particles(p)
+    ctx.set_grid(pic._lib.PIC_GRID_EX, ex)
+    ctx.set_grid(pic._lib.PIC_GRID_EY, ey)
+    ctx.set_bz(pic._lib.PIC_BZ_NONE)
+    kc = pic._lib.PIC_CIC if kernel == "cic" else pic._lib.PIC_M6
+    raw = np.zeros((nx, ny), order="F")
+    dt = 0.1
+    pic._lib.call("pic_step_pass", ctx.h, kc, 1, dt, raw.ctypes.data)
+    got = ctx.get_particles(np.zeros((5, n), order="F"))
+    q = p.copy(order="F")
+    O.push_v(q, kernel, nx, ny, mesh.dx, mesh.dy, ex, ey, None, dt)
+    O.push_x(q, mesh.dimx, mesh.dimy, dt / 2)
+    O.push_x(q, mesh.dimx, mesh.dimy, dt / 2)
+    if kernel == "cic":
+        assert np.array_equal(got, q)
+    else:
+        assert np.max(np.abs(got - q)) < 1e-13
+    rho_o = O.compute_rho(got, kernel, nx, ny, mesh.dx, mesh.dy, mesh.dimx, mesh.dimy)
+    rho = raw / (mesh.dx * mesh.dy)
+    rho -= rho.mean()
+    assert nrel(rho, rho_o) < RTOL
+    ctx.close()
+
+
+# ------------------------------------------------------------ time loop
+
+@pytest.mark.parametrize("kernel", ["cic", "m6"])
+def test_run_matches_reference_loop(gpu, O, kernel):
+    """pic_run against the oracle's restatement of the reference loop
+    (examples/landau_damping_poisson_cic_m6.jl:70-83), FP64, 10 steps: the
+    energies agree to 1e-10 (atomics reorder the deposit sums, rocFFT and
+    the oracle's FFT round differently, so trajectories are equal to
+    rounding, not bit for bit)."""
+    pic = gpu
+    mesh, p = landau_particles(pic, 100000, 64, 64)
+    sim = pic.Simulation(mesh, p, pic.CloudInCell() if kernel == "cic" else pic.M6())
+    e = sim.run(0.1, 10)
+    got = sim.particles()
+    sim.close()
+    q = p.copy(order="F")
+    eo = O.landau_loop(q, kernel, 64, 64, mesh.dimx, mesh.dimy, 0.1, 10)
+    assert np.max(np.abs(np.sqrt(e[:, 0]) - eo) / eo) < 1e-10
+    assert np.max(np.abs(got[:4] - q[:4])) < 1e-10
+
+
+def test_run_sorted_matches_unsorted(gpu):
+    """Sorting every step only permutes the particles: the energy history is
+    the same to rounding."""
+    pic = gpu
+    mesh, p = landau_particles(pic, 200000, 128, 128)
+    a = pic.Simulation(mesh, p, pic.M6())
+    ea = a.run(0.05, 12, sort_every=0)
+    a.close()
+    b = pic.Simulation(mesh, p, pic.M6())
+    eb = b.run(0.05, 12, sort_every=1)
+    b.close()
+    assert np.max(np.abs(ea - eb) / ea) < 1e-9
+
+
+def test_run_empty(gpu):
+    pic = gpu
+    mesh = pic.TwoDGrid(4 * np.pi, 16, 4 * np.pi, 16)
+    sim = pic.Simulation(mesh, np.zeros((5, 0), order="F"), pic.CloudInCell())
+    e = sim.run(0.1, 3)
+    assert np.all(e == 0)
+    sim.close()
