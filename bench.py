@@ -53,12 +53,13 @@ def run_path(pic, mesh, p0, state, dt, steps, warmup, sort_every):
     """Warmup, then the timed run from the initial state. Returns dict."""
     L = pic._lib
     sim = pic.Simulation(mesh, p0, pic.M6(), state=state)
-    sim.sort()
+    sim.sort(lanes=True)
     if warmup:
         sim.run(dt, warmup, sort_every=sort_every)
-    # restart from the initial state for the timed (and fitted) run
+    # restart from the initial state for the timed (and fitted) run; the
+    # initial ordering (sort by cell, lane spread) is part of the set-up
     sim.set_particles(p0, state=state)
-    sim.sort()
+    sim.sort(lanes=True)
     sim.set_timing(True)
     slow0 = sim.slow_chunks()
     sim.ctx.sync()
@@ -90,6 +91,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["f32", "f64", "both"], default="both")
+    ap.add_argument("--dump", default="", help="write the energy histories to this .npz")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -118,6 +120,8 @@ def main():
             f" solve {r['solve_ms_avg']*1e3:.1f} us, slow chunks {r['slow_chunks']}")
 
     t = np.arange(1, args.steps + 1) * dt
+    if args.dump:
+        np.savez(args.dump, t=t, **{k: v["energy"] for k, v in res.items()})
     gam = {}
     for name, r in res.items():
         g, w, nm = diagnostics.damping_rate(t, np.sqrt(r["energy"][:, 0]), tmin=0.0)

@@ -196,6 +196,13 @@ int pic_solve_fields(pic_ctx *ctx, double *energy);
  * permutation std::stable_sort by cell gives.  No reference counterpart
  * (build addition). */
 int pic_sort(pic_ctx *ctx, int32_t *perm);
+/* pic_sort, then within every segment of 64 consecutive cells (s = cell/64,
+ * l = cell%64) the particles are ordered by (rank k within their cell, l):
+ * round k holds the k-th particle of each cell of the segment that has more
+ * than k.  A wave of 64 consecutive particles then covers 64 different
+ * adjacent cells, the order pic_run keeps between its re-sorts.  perm
+ * receives the source index of each slot.  No reference counterpart. */
+int pic_sort_lanes(pic_ctx *ctx, int32_t *perm);
 /* After pic_sort: exclusive offsets of each cell's run (nx*ny+1 entries). */
 int pic_cell_offsets(pic_ctx *ctx, int64_t *offsets);
 

@@ -199,6 +199,20 @@ def counting_sort(key, nkeys):
     return perm, offs
 
 
+def lane_spread(cell, ncells):
+    """The lane-spread order of the fused kernels (build addition, no
+    reference counterpart): stable sort by cell, then within each segment of
+    64 consecutive cells order by (rank within the cell, cell).  Returns the
+    source index of each slot."""
+    cell = np.asarray(cell, dtype=np.int64)
+    perm = np.argsort(cell, kind="stable")
+    sc = cell[perm]
+    first = np.searchsorted(sc, np.arange(ncells + 1), side="left")
+    rank = np.arange(sc.size) - first[sc]
+    order = np.lexsort((sc % 64, rank, sc // 64))
+    return perm[order].astype(np.int32)
+
+
 def landau_loop(p, kernel, nx, ny, dimx, dimy, dt, nsteps, bz0=0.0):
     e = np.zeros(nsteps)
     rc = lib().or_landau_loop(p.shape[1], p, nx, ny, dimx, dimy, dt, nsteps,

@@ -55,6 +55,7 @@ SIGNATURES = {
     "pic_interpol_eb_m6": [_vp],
     "pic_solve_fields": [_vp, _pd],
     "pic_sort": [_vp, _vp],
+    "pic_sort_lanes": [_vp, _vp],
     "pic_cell_offsets": [_vp, _vp],
     "pic_run": [_vp, _i, _d, _i, _i, _vp],
     "pic_step_pass": [_vp, _i, _i, _d, _vp],

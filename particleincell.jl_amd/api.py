@@ -410,9 +410,11 @@ class Simulation:
         self.ctx.set_state(state)
         self.n = arr.shape[1]
 
-    def sort(self):
+    def sort(self, lanes=False):
+        """Stable sort by cell (pic_sort), or that followed by the lane-spread
+        order of the fused kernels (pic_sort_lanes); returns the permutation."""
         perm = np.zeros(self.n, np.int32)
-        L.call("pic_sort", self.ctx.h, perm.ctypes.data)
+        L.call("pic_sort_lanes" if lanes else "pic_sort", self.ctx.h, perm.ctypes.data)
         return perm
 
     def cell_offsets(self):

@@ -60,6 +60,7 @@ static void free_particles(Ctx *c) {
     }
     for (int k = 0; k < 2; ++k) { dfree(c->sort_keys[k]); dfree(c->sort_vals[k]); }
     dfree(c->sort_blk);
+    dfree(c->perm2);
     c->sort_cap = 0;
 }
 
@@ -217,6 +218,8 @@ int to_state(Ctx *c, int state) {
     if (state == PIC_STATE_F32) {
         PIC_CHECK(c->uniform_w, PIC_ERR_STATE,
                   "the FP32 cell-relative state needs uniform particle weights");
+        PIC_CHECK((c->nx & (c->nx - 1)) == 0, PIC_ERR_STATE,
+                  "the FP32 cell-relative state needs nx a power of two (nx=%d)", c->nx);
         PIC_TRY(ensure_f32(c));
         hipLaunchKernelGGL(k_f64_to_f32, grid1(c->n), dim3(256), 0, c->stream, c->n, c->x, c->y,
                            c->vx, c->vy, c->nx, c->ny, c->dx, c->dy, c->cell, c->ox, c->oy,
@@ -301,8 +304,7 @@ int pic_ctx_create(pic_ctx **out, int device, int nx, int ny, double xmin, doubl
                    double ymin, double ymax, int64_t capacity) {
     PIC_CHECK(out, PIC_ERR_ARG, "null context pointer");
     *out = nullptr;
-    auto pow2 = [](int v) { return v >= 8 && (v & (v - 1)) == 0; };
-    PIC_CHECK(pow2(nx) && pow2(ny), PIC_ERR_ARG, "nx=%d, ny=%d: need powers of two >= 8", nx, ny);
+    PIC_CHECK(nx >= 4 && ny >= 4, PIC_ERR_ARG, "nx=%d, ny=%d: need at least 4 cells each", nx, ny);
     PIC_CHECK((int64_t)nx * ny <= (1 << 24), PIC_ERR_ARG, "grid %dx%d too large", nx, ny);
     PIC_CHECK(xmax > xmin && ymax > ymin, PIC_ERR_ARG, "empty domain");
     PIC_CHECK(capacity >= 0 && capacity < (int64_t)INT32_MAX, PIC_ERR_ARG,
@@ -380,6 +382,7 @@ int pic_ctx_destroy(pic_ctx *ctx) {
     dfree(c->rho_acc); dfree(c->e32); dfree(c->bz32);
     dfree(c->red_part); dfree(c->red_count); dfree(c->energy_hist); dfree(c->step_counter);
     dfree(c->sort_hist); dfree(c->cell_off); dfree(c->slow_chunks);
+    dfree(c->seg_k); dfree(c->seg_tbl); dfree(c->seg_A); dfree(c->seg_M);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PIC_OK;
@@ -531,6 +534,8 @@ int pic_set_state_f32(pic_ctx *ctx, const int32_t *cell, const float *ox, const 
     PIC_CHECK(n == 0 || (cell && ox && oy && vx && vy), PIC_ERR_ARG, "null state pointer");
     Ctx *c = ctx_of(ctx);
     PIC_HIP(hipSetDevice(c->device));
+    PIC_CHECK((c->nx & (c->nx - 1)) == 0, PIC_ERR_STATE,
+              "the FP32 cell-relative state needs nx a power of two (nx=%d)", c->nx);
     int ncell = c->nx * c->ny;
     for (int64_t k = 0; k < n; ++k) {
         PIC_CHECK(cell[k] >= 0 && cell[k] < ncell, PIC_ERR_ARG, "cell[%lld] = %d out of range",

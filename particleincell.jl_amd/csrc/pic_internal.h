@@ -101,6 +101,11 @@ struct Ctx {
     uint32_t *sort_blk = nullptr;
     int64_t *cell_off = nullptr;        // ncells + 1 exclusive offsets after a sort
     int64_t sort_cap = 0;
+    // lane-spread order: per-segment max count, table offsets, tables, perm
+    uint32_t *seg_k = nullptr, *seg_tbl = nullptr, *seg_A = nullptr, *perm2 = nullptr;
+    unsigned long long *seg_M = nullptr;
+    int64_t seg_cap = 0;
+    bool lanes_order = false;           // last sort used the lane-spread order
 
     // fused-step tile statistics (diagnostic): chunks that took the slow path
     unsigned long long *slow_chunks = nullptr;
@@ -135,7 +140,7 @@ int finalize_rho(Ctx *c, int kernel);
 int field_energy(Ctx *c, double *out2);
 int field_energy_mf(Ctx *c, double *out);
 
-int sort_by_cell(Ctx *c, int32_t *perm_host);
+int sort_by_cell(Ctx *c, int32_t *perm_host, bool lanes);
 
 int launch_fill(Ctx *c, double *p, double v, int64_t n);
 

@@ -75,7 +75,7 @@ extern "C" int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort
         if (rc == PIC_OK) rc = step_pass(c, kernel, last ? 2 : 1, dt);
         if (rc == PIC_OK && c->timing) rc = mark();
         if (rc == PIC_OK && !last && sort_every > 0 && (s + 1) % sort_every == 0)
-            rc = sort_by_cell(c, nullptr);
+            rc = sort_by_cell(c, nullptr, true);
     }
     if (rc == PIC_OK && energy)
         rc = hipMemcpyAsync(energy, c->energy_hist, (size_t)nsteps * 2 * sizeof(double),

@@ -202,6 +202,77 @@ __global__ void k_gather(int64_t n, const uint32_t *__restrict__ perm, const T *
     if (r < n) out[r] = in[perm[r]];
 }
 
+// ---------------------------------------------------------- lane spread
+//
+// After the stable sort by cell, the particles of every segment of 64
+// consecutive cells (s = cell / 64, l = cell % 64) are re-ordered by
+// (rank k within the cell, l): round k holds the k-th particle of each cell of
+// the segment that has more than k, in cell order.  A wave of 64 consecutive
+// particles then spans 64 different, adjacent cells.  With A_s(k) =
+// sum_l min(count_l, k) and M_s(k) = {l : count_l > k}, the particle of rank
+// k in cell l goes to
+//     off[64 s] + A_s(k) + popcount(M_s(k) & ((1 << l) - 1)).
+
+// One wave per segment: the largest cell count of the segment.
+__global__ void k_seg_max(const int64_t *__restrict__ off, int ncells, int nseg,
+                          uint32_t *__restrict__ kmax) {
+    const int s = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int l = threadIdx.x & 63;
+    if (s >= nseg) return;
+    const int c = 64 * s + l;
+    int cnt = c < ncells ? (int)(off[c + 1] - off[c]) : 0;
+    for (int o = 32; o > 0; o >>= 1) cnt = max(cnt, __shfl_xor(cnt, o));
+    if (l == 0) kmax[s] = (uint32_t)cnt;
+}
+
+// One wave per segment: tables A_s(k), M_s(k) at tbl[s] + k.
+__global__ void k_seg_tables(const int64_t *__restrict__ off, int ncells, int nseg,
+                             const uint32_t *__restrict__ kmax, const uint32_t *__restrict__ tbl,
+                             uint32_t *__restrict__ A, unsigned long long *__restrict__ M) {
+    const int s = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int l = threadIdx.x & 63;
+    if (s >= nseg) return;
+    const int c = 64 * s + l;
+    const uint32_t cnt = c < ncells ? (uint32_t)(off[c + 1] - off[c]) : 0u;
+    const uint32_t K = kmax[s], base = tbl[s];
+    uint32_t acc = 0;
+    for (uint32_t k0 = 0; k0 < K; k0 += 64) {
+        // lane j writes entry k0 + j: gather the 64 masks of this block of k
+        unsigned long long mine = 0;
+        uint32_t acc_mine = 0;
+        const uint32_t kend = min(K - k0, 64u);
+        for (uint32_t j = 0; j < kend; ++j) {
+            unsigned long long m = __ballot(cnt > k0 + j);
+            if ((uint32_t)l == j) {
+                mine = m;
+                acc_mine = acc;
+            }
+            acc += (uint32_t)__popcll(m);
+        }
+        if ((uint32_t)l < kend) {
+            A[base + k0 + l] = acc_mine;
+            M[base + k0 + l] = mine;
+        }
+    }
+}
+
+// Destination of every sorted slot r; writes the lane-spread permutation.
+__global__ void k_lane_dest(const uint32_t *__restrict__ key, const uint32_t *__restrict__ val,
+                            int64_t n, const int64_t *__restrict__ off,
+                            const uint32_t *__restrict__ tbl, const uint32_t *__restrict__ A,
+                            const unsigned long long *__restrict__ M,
+                            uint32_t *__restrict__ perm2) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t c = key[r];
+    const uint32_t s = c >> 6, l = c & 63u;
+    const uint32_t k = (uint32_t)(r - off[c]);
+    const uint32_t t = tbl[s] + k;
+    const unsigned long long below = l ? (M[t] & (~0ull >> (64 - l))) : 0ull;
+    const int64_t dest = off[64 * (int64_t)s] + A[t] + __popcll(below);
+    perm2[dest] = val[r];
+}
+
 int ensure_sort(Ctx *c) {
     if (c->sort_cap < c->cap || !c->sort_keys[0]) {
         for (int k = 0; k < 2; ++k) {
@@ -269,7 +340,7 @@ static int radix_sort_pairs(Ctx *c, int64_t n, int keybits) {
 // swaps the two pointers.  Alternates are allocated once per array (same
 // capacity) and freed with the particles.
 template <typename T>
-static int permute(Ctx *c, T *&arr, int64_t n) {
+static int permute(Ctx *c, T *&arr, int64_t n, const uint32_t *perm) {
     void **key = reinterpret_cast<void **>(&arr);
     int slot = -1;
     for (int k = 0; k < Ctx::NALT; ++k)
@@ -285,15 +356,56 @@ static int permute(Ctx *c, T *&arr, int64_t n) {
     }
     T *alt = static_cast<T *>(c->alt[slot]);
     hipLaunchKernelGGL(k_gather<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
-                       c->sort_vals[0], arr, alt);
+                       perm, arr, alt);
     PIC_HIP(hipGetLastError());
     c->alt[slot] = arr;
     arr = alt;
     return PIC_OK;
 }
 
-int sort_by_cell(Ctx *c, int32_t *perm_host) {
+// Lane-spread permutation of the cell-sorted slots into c->perm2.
+static int lane_spread(Ctx *c) {
     const int64_t n = c->n;
+    const int ncells = c->nx * c->ny;
+    const int nseg = (ncells + 63) / 64;
+    if (!c->seg_k) {
+        PIC_HIP(hipMalloc(&c->seg_k, (size_t)nseg * 4));
+        PIC_HIP(hipMalloc(&c->seg_tbl, (size_t)(nseg + 1) * 4));
+    }
+    if (!c->perm2) PIC_HIP(hipMalloc(&c->perm2, (size_t)(c->cap > 0 ? c->cap : 1) * 4));
+    const int wpb = 4;  // waves (segments) per block
+    dim3 gs((unsigned)((nseg + wpb - 1) / wpb)), bs(64 * wpb);
+    hipLaunchKernelGGL(k_seg_max, gs, bs, 0, c->stream, c->cell_off, ncells, nseg, c->seg_k);
+    PIC_HIP(hipMemcpyAsync(c->seg_tbl, c->seg_k, (size_t)nseg * 4, hipMemcpyDeviceToDevice,
+                           c->stream));
+    PIC_TRY(scan(c, c->seg_tbl, nseg));
+    uint32_t last[2] = {0, 0};
+    PIC_HIP(hipMemcpyAsync(&last[0], c->seg_tbl + nseg - 1, 4, hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipMemcpyAsync(&last[1], c->seg_k + nseg - 1, 4, hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    const int64_t total = (int64_t)last[0] + last[1];
+    if (total > c->seg_cap) {
+        if (c->seg_A) (void)hipFree(c->seg_A);
+        if (c->seg_M) (void)hipFree(c->seg_M);
+        c->seg_A = nullptr;
+        c->seg_M = nullptr;
+        int64_t cap = total + total / 4 + 64;
+        PIC_HIP(hipMalloc(&c->seg_A, (size_t)cap * 4));
+        PIC_HIP(hipMalloc(&c->seg_M, (size_t)cap * 8));
+        c->seg_cap = cap;
+    }
+    hipLaunchKernelGGL(k_seg_tables, gs, bs, 0, c->stream, c->cell_off, ncells, nseg, c->seg_k,
+                       c->seg_tbl, c->seg_A, c->seg_M);
+    hipLaunchKernelGGL(k_lane_dest, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       c->sort_keys[0], c->sort_vals[0], n, c->cell_off, c->seg_tbl, c->seg_A,
+                       c->seg_M, c->perm2);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+int sort_by_cell(Ctx *c, int32_t *perm_host, bool lanes) {
+    const int64_t n = c->n;
+    c->lanes_order = lanes;
     if (n == 0) return PIC_OK;
     PIC_TRY(ensure_sort(c));
     const int ncells = c->nx * c->ny;
@@ -311,22 +423,26 @@ int sort_by_cell(Ctx *c, int32_t *perm_host) {
     hipLaunchKernelGGL(k_cell_offsets, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0,
                        c->stream, c->sort_keys[0], n, ncells, c->cell_off);
     PIC_HIP(hipGetLastError());
+    const uint32_t *perm = c->sort_vals[0];
+    if (lanes) {
+        PIC_TRY(lane_spread(c));
+        perm = c->perm2;
+    }
     if (c->state == PIC_STATE_F32) {
-        PIC_TRY(permute(c, c->cell, n));
-        PIC_TRY(permute(c, c->ox, n));
-        PIC_TRY(permute(c, c->oy, n));
-        PIC_TRY(permute(c, c->vx32, n));
-        PIC_TRY(permute(c, c->vy32, n));
+        PIC_TRY(permute(c, c->cell, n, perm));
+        PIC_TRY(permute(c, c->ox, n, perm));
+        PIC_TRY(permute(c, c->oy, n, perm));
+        PIC_TRY(permute(c, c->vx32, n, perm));
+        PIC_TRY(permute(c, c->vy32, n, perm));
     } else {
-        PIC_TRY(permute(c, c->x, n));
-        PIC_TRY(permute(c, c->y, n));
-        PIC_TRY(permute(c, c->vx, n));
-        PIC_TRY(permute(c, c->vy, n));
-        if (!c->uniform_w) PIC_TRY(permute(c, c->w, n));
+        PIC_TRY(permute(c, c->x, n, perm));
+        PIC_TRY(permute(c, c->y, n, perm));
+        PIC_TRY(permute(c, c->vx, n, perm));
+        PIC_TRY(permute(c, c->vy, n, perm));
+        if (!c->uniform_w) PIC_TRY(permute(c, c->w, n, perm));
     }
     if (perm_host) {
-        PIC_HIP(hipMemcpyAsync(perm_host, c->sort_vals[0], (size_t)n * 4, hipMemcpyDeviceToHost,
-                               c->stream));
+        PIC_HIP(hipMemcpyAsync(perm_host, perm, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
         PIC_HIP(hipStreamSynchronize(c->stream));
     }
     return PIC_OK;
@@ -342,7 +458,16 @@ int pic_sort(pic_ctx *ctx, int32_t *perm) {
     PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
     Ctx *c = ctx_of(ctx);
     PIC_HIP(hipSetDevice(c->device));
-    PIC_TRY(sort_by_cell(c, perm));
+    PIC_TRY(sort_by_cell(c, perm, false));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_sort_lanes(pic_ctx *ctx, int32_t *perm) {
+    PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
+    Ctx *c = ctx_of(ctx);
+    PIC_HIP(hipSetDevice(c->device));
+    PIC_TRY(sort_by_cell(c, perm, true));
     PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
 }

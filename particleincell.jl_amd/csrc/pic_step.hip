@@ -65,7 +65,7 @@ struct ArgsF32 {
     const float2 *e32;
     const float *bz;
     float bz0;
-    int nx, ny;
+    int nx, ny, lognx;  // nx is a power of two in FP32 state
     float hdt;        // dt/2 (kick halves)
     float dtdx, dtdy; // drift length / cell size
     double *acc;
@@ -83,9 +83,23 @@ struct Stencil<PIC_M6> {
     static constexpr int S = 6, HL = 2, HR = 3;
 };
 
-// Wrapped difference of cell indices into [-n/2, n/2) (n a power of two).
+// Wrapped difference of cell indices i, iref in [0, n) into [-n/2, n/2).
 __device__ __forceinline__ int rel(int i, int iref, int n) {
-    return ((i - iref + n / 2) & (n - 1)) - n / 2;
+    int d = i - iref;
+    if (d >= n - n / 2) d -= n;
+    else if (d < -(n / 2)) d += n;
+    return d;
+}
+
+// i mod n for i in [-n, 2n) (stencil neighbours of a cell in [0, n)).
+__device__ __forceinline__ int wrap1(int i, int n) {
+    return i < 0 ? i + n : (i >= n ? i - n : i);
+}
+
+// i mod n for any i.
+__device__ __forceinline__ int wrapm(int i, int n) {
+    int r = i % n;
+    return r < 0 ? r + n : r;
 }
 
 struct Box {
@@ -139,8 +153,8 @@ __device__ __forceinline__ void init_box(int *sbox) {
 // Global node of tile node (a, b).
 __device__ __forceinline__ int64_t tile_node(const Box &b, int a, int bb, int iref, int jref,
                                              int nx, int ny) {
-    int gi = (iref + b.x0 + a) & (nx - 1);
-    int gj = (jref + b.y0 + bb) & (ny - 1);
+    int gi = wrapm(iref + b.x0 + a, nx);
+    int gj = wrapm(jref + b.y0 + bb, ny);
     return (int64_t)gi + (int64_t)nx * gj;
 }
 
@@ -231,14 +245,14 @@ __global__ __launch_bounds__(ST) void k_step_f64(ArgsF64 a) {
             int lb = rel(cj[k], jref, ny) - (b.y0 + St::HL);
             auto node_e = [&](int u, int v) -> double2 {
                 if (b.fits) return te[(la + u) + b.tw * (lb + v)];
-                int64_t g = (int64_t)((ci[k] - St::HL + u) & (nx - 1)) +
-                            (int64_t)nx * ((cj[k] - St::HL + v) & (ny - 1));
+                int64_t g = (int64_t)wrap1(ci[k] - St::HL + u, nx) +
+                            (int64_t)nx * wrap1(cj[k] - St::HL + v, ny);
                 return make_double2(a.ex[g], a.ey[g]);
             };
             auto node_b = [&](int u, int v) -> double {
                 if (b.fits) return tb[(la + u) + b.tw * (lb + v)];
-                int64_t g = (int64_t)((ci[k] - St::HL + u) & (nx - 1)) +
-                            (int64_t)nx * ((cj[k] - St::HL + v) & (ny - 1));
+                int64_t g = (int64_t)wrap1(ci[k] - St::HL + u, nx) +
+                            (int64_t)nx * wrap1(cj[k] - St::HL + v, ny);
                 return a.bz[g];
             };
             if (K == PIC_CIC) {
@@ -372,8 +386,8 @@ __global__ __launch_bounds__(ST) void k_step_f64(ArgsF64 a) {
                 __hip_atomic_fetch_add(&tr[(la + u) + b.tw * (lb + v)], q, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
             } else {
-                int64_t g = (int64_t)((ci[k] - St::HL + u) & (nx - 1)) +
-                            (int64_t)nx * ((cj[k] - St::HL + v) & (ny - 1));
+                int64_t g = (int64_t)wrap1(ci[k] - St::HL + u, nx) +
+                            (int64_t)nx * wrap1(cj[k] - St::HL + v, ny);
                 unsafeAtomicAdd(&a.acc[g], q);
             }
         };
@@ -456,11 +470,11 @@ __global__ __launch_bounds__(ST) void k_step_f32(ArgsF32 a) {
             }
         }
         ci[k] = c & (nx - 1);
-        cj[k] = c / nx;
+        cj[k] = c >> a.lognx;
     }
     init_box(sbox);
     const int c0 = a.cell[base];
-    const int iref = c0 & (nx - 1), jref = c0 / nx;
+    const int iref = c0 & (nx - 1), jref = c0 >> a.lognx;
     __syncthreads();
 
     if (GATHER) {
@@ -496,14 +510,14 @@ __global__ __launch_bounds__(ST) void k_step_f32(ArgsF32 a) {
             int lb = rel(cj[k], jref, ny) - (b.y0 + St::HL);
             auto node_e = [&](int u, int v) -> float2 {
                 if (b.fits) return te[(la + u) + b.tw * (lb + v)];
-                int64_t g = (int64_t)((ci[k] - St::HL + u) & (nx - 1)) +
-                            (int64_t)nx * ((cj[k] - St::HL + v) & (ny - 1));
+                int64_t g = (int64_t)wrap1(ci[k] - St::HL + u, nx) +
+                            (int64_t)nx * wrap1(cj[k] - St::HL + v, ny);
                 return a.e32[g];
             };
             auto node_b = [&](int u, int v) -> float {
                 if (b.fits) return tb[(la + u) + b.tw * (lb + v)];
-                int64_t g = (int64_t)((ci[k] - St::HL + u) & (nx - 1)) +
-                            (int64_t)nx * ((cj[k] - St::HL + v) & (ny - 1));
+                int64_t g = (int64_t)wrap1(ci[k] - St::HL + u, nx) +
+                            (int64_t)nx * wrap1(cj[k] - St::HL + v, ny);
                 return a.bz[g];
             };
             float e1, e2, b3 = 0.0f;
@@ -630,8 +644,8 @@ __global__ __launch_bounds__(ST) void k_step_f32(ArgsF32 a) {
             if (b.fits) {
                 kahan_add(hi, lo, (la + u) + b.tw * (lb + v), q);
             } else {
-                int64_t g = (int64_t)((ci[k] - St::HL + u) & (nx - 1)) +
-                            (int64_t)nx * ((cj[k] - St::HL + v) & (ny - 1));
+                int64_t g = (int64_t)wrap1(ci[k] - St::HL + u, nx) +
+                            (int64_t)nx * wrap1(cj[k] - St::HL + v, ny);
                 unsafeAtomicAdd(&a.acc[g], (double)q);
             }
         };
@@ -766,6 +780,8 @@ int launch_step32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     a.bz = c->bz32;
     a.bz0 = (float)c->bz0;
     a.nx = c->nx; a.ny = c->ny;
+    a.lognx = 0;
+    while ((1 << a.lognx) < c->nx) ++a.lognx;
     a.hdt = (float)(0.5 * dt);
     a.dtdx = (float)(dtd / c->dx);
     a.dtdy = (float)(dtd / c->dy);

@@ -369,6 +369,20 @@ def test_counting_sort_is_stable_sort(O):
     assert np.array_equal(np.diff(off), np.bincount(key, minlength=300))
 
 
+def test_lane_spread_definition(O):
+    """Worked example: cells [0,0,1,0,1,2] -> ranks in cell 0: particles
+    0,1,3; cell 1: 2,4; cell 2: 5 -> round 0 (0,2,5), round 1 (1,4),
+    round 2 (3); a second segment (cells 64..) follows the first."""
+    assert O.lane_spread([0, 0, 1, 0, 1, 2], 128).tolist() == [0, 2, 5, 1, 4, 3]
+    assert O.lane_spread([64, 1, 64, 1, 0], 128).tolist() == [4, 1, 3, 0, 2]
+    rng = np.random.default_rng(5)
+    cell = rng.integers(0, 300, 5000)
+    perm = O.lane_spread(cell, 300)
+    assert np.array_equal(np.sort(perm), np.arange(5000))
+    sc = cell[perm]
+    assert np.all(np.diff(sc // 64) >= 0)  # segments stay in order
+
+
 # ------------------------------------------------------------ FP32 state
 
 def test_f32_weights(O):
