@@ -124,7 +124,10 @@ def main():
         np.savez(args.dump, t=t, **{k: v["energy"] for k, v in res.items()})
     gam = {}
     for name, r in res.items():
-        g, w, nm = diagnostics.damping_rate(t, np.sqrt(r["energy"][:, 0]), tmin=0.0)
+        # ey carries no signal in this problem: its level is the noise floor
+        noise = float(np.median(np.sqrt(r["energy"][:, 1])))
+        g, w, nm = diagnostics.damping_rate(t, np.sqrt(r["energy"][:, 0]), tmin=0.5,
+                                            noise=noise)
         gam[name] = (g, w, nm)
         log(f"bench: {name} gamma {g:.5f} omega {w:.4f} from {nm} maxima")
 

@@ -217,6 +217,11 @@ int pic_cell_offsets(pic_ctx *ctx, int64_t *offsets);
  * (0 = never).  Leaves RHO, EX, EY of the last step in the context. */
 int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort_every, double *energy);
 
+/* Charge accumulation of pic_run's FP32 passes in their LDS windows:
+ * 0 (default) compensated FP32 (hi, lo) pairs, 1 FP64 accumulation.  The
+ * grid totals are FP64 either way.  No reference counterpart. */
+int pic_set_f32_accumulation(pic_ctx *ctx, int mode);
+
 /* One particle pass of pic_run with the fields currently in the context
  * (EX, EY as last set or solved, and the pic_set_bz field), for callers that
  * compose their own loop:
@@ -226,7 +231,9 @@ int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort_every, dou
  *   mode 3  deposit only
  * The deposit is the raw charge sum of w times the shape weights (before the
  * 1/(dx dy) normalisation and the mean removal of compute_rho); rho_raw
- * (host, nx*ny, may be NULL) receives it. */
+ * (host, nx*ny, may be NULL) receives it.  After pic_sort_lanes (or
+ * pic_run) the pass runs pic_run's segment kernels, otherwise kernels that
+ * accept any particle order. */
 int pic_step_pass(pic_ctx *ctx, int kernel, int mode, double dt, double *rho_raw);
 
 /* Kernel timing of pic_run: with timing enabled, pic_run records HIP events

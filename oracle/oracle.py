@@ -67,6 +67,11 @@ def lib():
             "orf_deposit": (None, [_i64, _ip, _fp, _fp, _i32, _i32, _i32, _f, _dp]),
             "orf_from_f64": (None, [_i64, _dp, _i32, _i32, _d, _d, _ip, _fp, _fp, _fp, _fp]),
             "orf_to_f64": (None, [_i64, _ip, _fp, _fp, _fp, _fp, _i32, _d, _d, _d, _dp]),
+            "or_poisson_adjoint": (_i32, [_i32, _i32, _d, _d, _dp, _dp, _dp]),
+            "or_adjoint": (_i32, [_i64, _dp, _i32, _i32, _d, _d, _d, _i32, _i32, _i32,
+                                  C.POINTER(_d), _dp, _dp, C.c_void_p]),
+            "or_energy_objective": (_i32, [_i64, _dp, _i32, _i32, _d, _d, _d, _i32, _i32,
+                                           C.POINTER(_d)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -269,6 +274,37 @@ def f32_deposit(state, nx, ny, kernel, w):
     lib().orf_deposit(cell.size, cell, ox, oy, nx, ny, 0 if kernel == "cic" else 1,
                       np.float32(w), rho)
     return rho
+
+
+# ---------------------------------------------------------------- adjoint
+
+def poisson_adjoint(mux, muy, dimx, dimy):
+    nx, ny = mux.shape
+    out = np.zeros((nx, ny), order="F")
+    assert lib().or_poisson_adjoint(nx, ny, dimx, dimy, _F(mux), _F(muy), out) == 0
+    return out
+
+
+def adjoint(p0, kernel, nx, ny, dimx, dimy, dt, nsteps, ckpt):
+    """Forward loop + reverse sweep: returns (J, gx (2, n), gv (2, n), e_hist)
+    with J = sum(ex^2+ey^2) dx dy of the last step's field."""
+    n = p0.shape[1]
+    J = C.c_double()
+    gx = np.zeros((2, n), order="F")
+    gv = np.zeros((2, n), order="F")
+    eh = np.zeros(nsteps)
+    rc = lib().or_adjoint(n, _F(p0), nx, ny, dimx, dimy, dt, nsteps, 0 if kernel == "cic" else 1,
+                          ckpt, C.byref(J), gx, gv, eh.ctypes.data_as(C.c_void_p))
+    assert rc == 0, rc
+    return J.value, gx, gv, eh
+
+
+def energy_objective(p0, kernel, nx, ny, dimx, dimy, dt, nsteps):
+    J = C.c_double()
+    rc = lib().or_energy_objective(p0.shape[1], _F(p0), nx, ny, dimx, dimy, dt, nsteps,
+                                   0 if kernel == "cic" else 1, C.byref(J))
+    assert rc == 0
+    return J.value
 
 
 # ------------------------------------------------- reference Fortran twin

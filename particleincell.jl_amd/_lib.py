@@ -60,6 +60,7 @@ SIGNATURES = {
     "pic_run": [_vp, _i, _d, _i, _i, _vp],
     "pic_step_pass": [_vp, _i, _i, _d, _vp],
     "pic_set_timing": [_vp, _i],
+    "pic_set_f32_accumulation": [_vp, _i],
     "pic_get_timing": [_vp, _pd, C.POINTER(C.c_int64), _pd],
     "pic_slow_chunks": [_vp, C.POINTER(C.c_uint64)],
     "pic_f90_push_v": [_pi32, _vp, _pd, _pi32, _pi32, _pd, _pd, _vp, _vp, _vp],

@@ -238,6 +238,7 @@ static int set_particles_impl(Ctx *c, const double *d_p, int64_t n) {
     PIC_TRY(ensure_particles(c, n));
     c->n = n;
     c->state = PIC_STATE_F64;
+    c->seg_valid = false;
     if (n == 0) {
         c->uniform_w = true;
         c->w0 = 0.0;
@@ -382,7 +383,7 @@ int pic_ctx_destroy(pic_ctx *ctx) {
     dfree(c->rho_acc); dfree(c->e32); dfree(c->bz32);
     dfree(c->red_part); dfree(c->red_count); dfree(c->energy_hist); dfree(c->step_counter);
     dfree(c->sort_hist); dfree(c->cell_off); dfree(c->slow_chunks);
-    dfree(c->seg_k); dfree(c->seg_tbl); dfree(c->seg_A); dfree(c->seg_M);
+    dfree(c->seg_k); dfree(c->seg_tbl); dfree(c->seg_A); dfree(c->seg_M); dfree(c->seg_len);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PIC_OK;
@@ -556,6 +557,7 @@ int pic_set_state_f32(pic_ctx *ctx, const int32_t *cell, const float *ox, const 
     PIC_HIP(hipMemcpyAsync(c->vy32, vy, b, hipMemcpyHostToDevice, c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
     c->state = PIC_STATE_F32;
+    c->seg_valid = false;
     return PIC_OK;
 }
 
@@ -715,3 +717,10 @@ int pic_f90_push_x(const int32_t *nbpart, const double *dimx, const double *dimy
 }
 
 }  // extern "C"
+
+extern "C" int pic_set_f32_accumulation(pic_ctx *ctx, int mode) {
+    PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
+    PIC_CHECK(mode == 0 || mode == 1, PIC_ERR_ARG, "accumulation mode %d", mode);
+    ctx_of(ctx)->f32_acc = mode;
+    return PIC_OK;
+}

@@ -78,10 +78,10 @@ struct Ctx {
     int bz_mode = PIC_BZ_NONE;          // none / uniform / grid
     double bz0 = 0.0;
 
-    // Poisson: tables (nx/2+1) x ny, spectra, rocFFT
+    // Poisson: tables (nx/2+1) x ny, complex work grids nx x ny, rocFFT
     double *kx_k2 = nullptr, *ky_k2 = nullptr, *k2 = nullptr;
-    double2 *rhat = nullptr, *ehat = nullptr;  // ehat holds 2 spectra (x, y)
-    rocfft_plan plan_r2c = nullptr, plan_c2r2 = nullptr, plan_c2r1 = nullptr, plan_r2c2 = nullptr;
+    double2 *zf = nullptr, *zg = nullptr;
+    rocfft_plan plan_fwd = nullptr, plan_inv = nullptr;
     rocfft_execution_info fft_info = nullptr;
     void *fft_work = nullptr;
     size_t fft_work_bytes = 0;
@@ -106,6 +106,11 @@ struct Ctx {
     unsigned long long *seg_M = nullptr;
     int64_t seg_cap = 0;
     bool lanes_order = false;           // last sort used the lane-spread order
+    // after a lane sort, segment s occupies slots [cell_off[64 s],
+    // cell_off[min(64 s + 64, ncells)]) until the next sort
+    uint32_t *seg_len = nullptr;        // particles per segment (nseg)
+    int64_t max_seg_len = 0;            // largest segment after the last lane sort
+    bool seg_valid = false;             // cell_off describes the current slots
 
     // fused-step tile statistics (diagnostic): chunks that took the slow path
     unsigned long long *slow_chunks = nullptr;
@@ -113,6 +118,10 @@ struct Ctx {
     // captured step graphs
     hipGraphExec_t graph_exec = nullptr;
     int graph_key = -1;
+
+    // FP32 charge accumulation in the LDS windows: 0 compensated FP32
+    // (hi, lo) pairs, 1 FP64
+    int f32_acc = 0;
 
     // pic_run kernel timing (HIP events on the context stream)
     bool timing = false;
@@ -156,6 +165,15 @@ namespace picd {
 __host__ __device__ inline int wrapi(int i, int n) {
     int r = i % n;
     return r < 0 ? r + n : r;
+}
+
+// i mod n for i in [-2n, 3n) with two conditional corrections.
+__host__ __device__ inline int wrap2(int i, int n) {
+    if (i < 0) i += n;
+    if (i < 0) i += n;
+    if (i >= n) i -= n;
+    if (i >= n) i -= n;
+    return i;
 }
 
 // Julia mod1(x, n) - 1, i.e. the 0-based periodic index of 1-based x.

@@ -2,16 +2,20 @@
 // spectral kernel, and the FP64 grid reductions (charge, field energy).
 //
 // E = -grad(phi), -lap(phi) = rho (poisson_2d.jl:97-113):
-//   rhat = rfft(rho)                      rocFFT r2c, lengths {nx, ny}
-//   ehat_x = -i kx/k2 rhat, ehat_y = -i ky/k2 rhat, rhat[0,0] = 0
-//   ex, ey = irfft(ehat)                  one batched rocFFT c2r of 2
-// The spectral kernel also folds in the 1/(nx*ny) normalisation and the
-// charge normalisation 1/(dx*dy) of the raw deposit, and replaces the columns
-// ik = 0 and ik = nx/2 (self-conjugate in x) by their Hermitian part
-// (F[jk] + conj(F[-jk]))/2.  Multiplying a Hermitian column by -i*k makes it
-// anti-Hermitian there; FFTW's c2r (and numpy's) keep only the real part of
-// the inverse along y for those columns, which is exactly this projection, so
-// rocFFT receives data it is defined on and returns what FFTW returns.
+//   R = fft2(rho)                           rocFFT complex-to-complex forward
+//   half spectrum (ik <= nx/2): Fx = -i kx/k2 R, Fy = -i ky/k2 R, R[0,0] = 0
+//   (ex, ey) = (Re, Im) ifft2(Fx + i Fy)    rocFFT complex-to-complex inverse
+// The spectral kernel extends the half spectrum of the reference's rfft to
+// the full grid by Hermitian symmetry, F(ik, jk) = conj(F(nx-ik, -jk)) for
+// ik > nx/2, which is the array irfft inverts; because both extended spectra
+// are Hermitian, ifft2(Fx + i Fy) carries ex in its real and ey in its
+// imaginary part.  On the columns ik = 0 and ik = nx/2 (self-conjugate in x)
+// the half spectrum is first replaced by its Hermitian part
+// (F[jk] + conj(F[-jk]))/2: multiplying a Hermitian column by -i*k makes it
+// anti-Hermitian there, and FFTW's c2r keeps only the real part of the
+// inverse along y for those columns, which is this projection.  The kernel
+// also folds in the 1/(nx*ny) normalisation and, in the time loop, the
+// 1/(dx*dy) charge normalisation of the raw deposit.
 #include <cstdio>
 #include <cmath>
 #include <vector>
@@ -29,10 +33,10 @@ namespace pic {
 
 static bool g_rocfft_ready = false;
 
-static int make_plan(rocfft_plan *plan, rocfft_transform_type t, int nx, int ny, int batch) {
+static int make_plan(rocfft_plan *plan, rocfft_transform_type t, int nx, int ny) {
     size_t lengths[2] = {(size_t)nx, (size_t)ny};  // fastest dimension first
     PIC_FFT(rocfft_plan_create(plan, rocfft_placement_notinplace, t, rocfft_precision_double, 2,
-                               lengths, (size_t)batch, nullptr));
+                               lengths, 1, nullptr));
     return PIC_OK;
 }
 
@@ -42,7 +46,7 @@ int poisson_init(Ctx *c) {
         g_rocfft_ready = true;
     }
     const int nx = c->nx, ny = c->ny, h = nx / 2 + 1;
-    const size_t m = (size_t)h * ny;
+    const size_t m = (size_t)h * ny, mm = (size_t)nx * ny;
     // TwoDPoissonPeriodic tables (poisson_2d.jl:28-61), same arithmetic as the
     // reference: kx1 = (ik-1)*kx0, negative ky from jk = ny/2+1, kx[1,1] = 1.
     std::vector<double> kx(m), ky(m), k2(m);
@@ -64,17 +68,15 @@ int poisson_init(Ctx *c) {
     PIC_HIP(hipMalloc(&c->kx_k2, m * 8));
     PIC_HIP(hipMalloc(&c->ky_k2, m * 8));
     PIC_HIP(hipMalloc(&c->k2, m * 8));
-    PIC_HIP(hipMalloc(&c->rhat, 2 * m * sizeof(double2)));
-    PIC_HIP(hipMalloc(&c->ehat, 2 * m * sizeof(double2)));
+    PIC_HIP(hipMalloc(&c->zf, mm * sizeof(double2)));
+    PIC_HIP(hipMalloc(&c->zg, mm * sizeof(double2)));
     PIC_HIP(hipMemcpy(c->kx_k2, kx.data(), m * 8, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(c->ky_k2, ky.data(), m * 8, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(c->k2, k2.data(), m * 8, hipMemcpyHostToDevice));
-    PIC_TRY(make_plan(&c->plan_r2c, rocfft_transform_type_real_forward, nx, ny, 1));
-    PIC_TRY(make_plan(&c->plan_r2c2, rocfft_transform_type_real_forward, nx, ny, 2));
-    PIC_TRY(make_plan(&c->plan_c2r2, rocfft_transform_type_real_inverse, nx, ny, 2));
-    PIC_TRY(make_plan(&c->plan_c2r1, rocfft_transform_type_real_inverse, nx, ny, 1));
+    PIC_TRY(make_plan(&c->plan_fwd, rocfft_transform_type_complex_forward, nx, ny));
+    PIC_TRY(make_plan(&c->plan_inv, rocfft_transform_type_complex_inverse, nx, ny));
     size_t wb = 0;
-    for (rocfft_plan p : {c->plan_r2c, c->plan_r2c2, c->plan_c2r2, c->plan_c2r1}) {
+    for (rocfft_plan p : {c->plan_fwd, c->plan_inv}) {
         size_t s = 0;
         PIC_FFT(rocfft_plan_get_work_buffer_size(p, &s));
         if (s > wb) wb = s;
@@ -88,15 +90,15 @@ int poisson_init(Ctx *c) {
 }
 
 void poisson_free(Ctx *c) {
-    for (rocfft_plan *p : {&c->plan_r2c, &c->plan_r2c2, &c->plan_c2r2, &c->plan_c2r1})
+    for (rocfft_plan *p : {&c->plan_fwd, &c->plan_inv})
         if (*p) {
             rocfft_plan_destroy(*p);
             *p = nullptr;
         }
     if (c->fft_info) rocfft_execution_info_destroy(c->fft_info);
     c->fft_info = nullptr;
-    for (void **q : {(void **)&c->kx_k2, (void **)&c->ky_k2, (void **)&c->k2, (void **)&c->rhat,
-                     (void **)&c->ehat, &c->fft_work})
+    for (void **q : {(void **)&c->kx_k2, (void **)&c->ky_k2, (void **)&c->k2, (void **)&c->zf,
+                     (void **)&c->zg, &c->fft_work})
         if (*q) {
             (void)hipFree(*q);
             *q = nullptr;
@@ -109,75 +111,93 @@ static int exec(Ctx *c, rocfft_plan plan, void *in, void *out) {
     return PIC_OK;
 }
 
+// z = (a, b) (b may be NULL: imaginary part 0).
+__global__ void k_pack(const double *__restrict__ a, const double *__restrict__ b, int64_t m,
+                       double2 *__restrict__ z) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) z[k] = make_double2(a[k], b ? b[k] : 0.0);
+}
+
+__global__ void k_real_part(const double2 *__restrict__ z, int64_t m, double *__restrict__ out) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) out[k] = z[k].x;
+}
+
 // -i * m * v * scale for complex v = (a, b): (m*b*scale, -m*a*scale).
 __device__ __forceinline__ double2 mul_mi(double m, double2 v, double scale) {
     return make_double2(m * v.y * scale, -(m * v.x) * scale);
 }
 
-// mode 0: E spectra (x into out[0..m), y into out[m..2m)), DC zeroed.
-// mode 1: phi spectrum rhat/k2 into out[0..m), DC passed through.
-// sign multiplies the result (-1 for the adjoint: the E operator is a real
-// convolution with a purely imaginary, odd symbol, so its transpose is its
-// negative).
-__global__ void k_spectral(const double2 *__restrict__ rhat, double2 *__restrict__ out,
-                           const double *__restrict__ kxk2, const double *__restrict__ kyk2,
-                           const double *__restrict__ k2, int nx, int ny, double scale,
-                           int mode) {
+__device__ __forceinline__ double2 cconj(double2 v) { return make_double2(v.x, -v.y); }
+
+constexpr int SPEC_E = 0, SPEC_PHI = 1, SPEC_ADJ = 2;
+
+// Spectrum multiplier on the half spectrum index (ih <= nx/2, jh) of the
+// full forward transform S (nx x ny, x fastest); tables are (nx/2+1) x ny.
+//   SPEC_E:   (Fx, Fy) = -i (kx, ky)/k2 R, R[0,0] = 0
+//   SPEC_PHI:  Fphi = R / k2                         (no DC zeroing)
+//   SPEC_ADJ:  -(-i kx/k2 Mx - i ky/k2 My) with S = fft(mux + i muy),
+//              Mx = (S(k) + conj S(-k))/2, My = (S(k) - conj S(-k))/(2i)
+template <int MODE>
+__device__ __forceinline__ void half_value(const double2 *__restrict__ S, int nx, int ny, int ih,
+                                           int jh, const double *__restrict__ kxk2,
+                                           const double *__restrict__ kyk2,
+                                           const double *__restrict__ k2, double scale,
+                                           double2 &f, double2 &g) {
     const int h = nx / 2 + 1;
-    const int64_t m = (int64_t)h * ny;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m) return;
-    const int ik = (int)(t % h), jk = (int)(t / h);
-    const bool selfconj = (ik == 0) || (ik == nx / 2);
-    const int jm = (ny - jk) % ny;
-    const int64_t tm = (int64_t)ik + (int64_t)h * jm;
-    double2 r = rhat[t];
-    double2 rm = rhat[tm];
-    if (mode == 0) {
-        if (t == 0) r = make_double2(0.0, 0.0);
-        if (tm == 0) rm = make_double2(0.0, 0.0);
-        double2 fx = mul_mi(kxk2[t], r, scale), fy = mul_mi(kyk2[t], r, scale);
-        if (selfconj) {
-            double2 gx = mul_mi(kxk2[tm], rm, scale), gy = mul_mi(kyk2[tm], rm, scale);
-            fx = make_double2(0.5 * (fx.x + gx.x), 0.5 * (fx.y - gx.y));
-            fy = make_double2(0.5 * (fy.x + gy.x), 0.5 * (fy.y - gy.y));
+    const int64_t q = (int64_t)ih + (int64_t)h * jh;
+    const double2 R = S[(int64_t)ih + (int64_t)nx * jh];
+    if (MODE == SPEC_E) {
+        if (q == 0) {
+            f = g = make_double2(0.0, 0.0);
+        } else {
+            f = mul_mi(kxk2[q], R, scale);
+            g = mul_mi(kyk2[q], R, scale);
         }
-        out[t] = fx;
-        out[m + t] = fy;
+    } else if (MODE == SPEC_PHI) {
+        f = make_double2(R.x / k2[q] * scale, R.y / k2[q] * scale);
+        g = make_double2(0.0, 0.0);
     } else {
-        double2 f = make_double2(r.x / k2[t] * scale, r.y / k2[t] * scale);
-        if (selfconj) {
-            double2 g = make_double2(rm.x / k2[tm] * scale, rm.y / k2[tm] * scale);
-            f = make_double2(0.5 * (f.x + g.x), 0.5 * (f.y - g.y));
+        const int im = (nx - ih) % nx, jm = (ny - jh) % ny;
+        const double2 Rm = cconj(S[(int64_t)im + (int64_t)nx * jm]);
+        const double2 mx = make_double2(0.5 * (R.x + Rm.x), 0.5 * (R.y + Rm.y));
+        // (R - Rm) / (2i) = (Im, -Re) / 2 of the difference
+        const double2 my = make_double2(0.5 * (R.y - Rm.y), -0.5 * (R.x - Rm.x));
+        if (q == 0) {
+            f = make_double2(0.0, 0.0);
+        } else {
+            const double2 a = mul_mi(kxk2[q], mx, scale), b = mul_mi(kyk2[q], my, scale);
+            f = make_double2(-(a.x + b.x), -(a.y + b.y));
         }
-        out[t] = f;
+        g = make_double2(0.0, 0.0);
     }
 }
 
-// Adjoint spectral step: out = -(mx * X + my * Y) with X, Y the spectra of the
-// two field adjoints (in[0..m), in[m..2m)), projected like k_spectral.
-__global__ void k_spectral_adj(const double2 *__restrict__ in, double2 *__restrict__ out,
-                               const double *__restrict__ kxk2, const double *__restrict__ kyk2,
-                               int nx, int ny, double scale) {
-    const int h = nx / 2 + 1;
-    const int64_t m = (int64_t)h * ny;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m) return;
-    const int ik = (int)(t % h), jk = (int)(t / h);
-    const bool selfconj = (ik == 0) || (ik == nx / 2);
-    const int jm = (ny - jk) % ny;
-    const int64_t tm = (int64_t)ik + (int64_t)h * jm;
-    auto val = [&](int64_t q) {
-        if (q == 0) return make_double2(0.0, 0.0);
-        double2 a = mul_mi(kxk2[q], in[q], scale), b = mul_mi(kyk2[q], in[m + q], scale);
-        return make_double2(-(a.x + b.x), -(a.y + b.y));
-    };
-    double2 f = val(t);
-    if (selfconj) {
-        double2 g = val(tm);
-        f = make_double2(0.5 * (f.x + g.x), 0.5 * (f.y - g.y));
+// Full-grid spectrum Z for the inverse transform (see the file comment).
+template <int MODE>
+__global__ void k_spectral(const double2 *__restrict__ S, double2 *__restrict__ Z,
+                           const double *__restrict__ kxk2, const double *__restrict__ kyk2,
+                           const double *__restrict__ k2, int nx, int ny, double scale) {
+    const int64_t mm = (int64_t)nx * ny;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= mm) return;
+    const int ik = (int)(t % nx), jk = (int)(t / nx);
+    const bool mirror = ik > nx / 2;
+    const int ih = mirror ? nx - ik : ik;
+    const int jh = mirror ? (ny - jk) % ny : jk;
+    double2 f, g;
+    half_value<MODE>(S, nx, ny, ih, jh, kxk2, kyk2, k2, scale, f, g);
+    if (ih == 0 || 2 * ih == nx) {  // project the self-conjugate column
+        double2 fm, gm;
+        half_value<MODE>(S, nx, ny, ih, (ny - jh) % ny, kxk2, kyk2, k2, scale, fm, gm);
+        f = make_double2(0.5 * (f.x + fm.x), 0.5 * (f.y - fm.y));
+        g = make_double2(0.5 * (g.x + gm.x), 0.5 * (g.y - gm.y));
     }
-    out[t] = f;
+    if (mirror) {
+        f = cconj(f);
+        g = cconj(g);
+    }
+    Z[t] = make_double2(f.x - g.y, f.y + g.x);  // f + i g
 }
 
 // Block-wide sum of two doubles in a fixed order (deterministic).
@@ -199,15 +219,17 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *sh) {
     __syncthreads();
 }
 
-// After the c2r: optional FP32 interleaved copy of (ex, ey), optional zeroing
-// of the raw deposit accumulator for the next step, and the energy sums
-// sum(ex^2)*dxdy, sum(ey^2)*dxdy written by the last block to arrive into
-// hist[2*slot] (slot >= 0), hist[2*(*counter)] with *counter incremented
-// (slot == -2) or nowhere (slot == -1).  Partial sums are per block over a
-// fixed node subset and combined in block order, so the result is the same
-// on every run.
-__global__ __launch_bounds__(RED_THREADS) void k_post(const double *__restrict__ ex,
-                                                      const double *__restrict__ ey, int64_t m,
+// After the inverse transform: (ex, ey) from z (real, imaginary parts) when z
+// is given, else read from ex, ey; optional FP32 interleaved copy, optional
+// zeroing of the raw deposit accumulator for the next step, and the energy
+// sums sum(ex^2)*dxdy, sum(ey^2)*dxdy written by the last block to arrive
+// into hist[2*slot] (slot >= 0), hist[2*(*counter)] with *counter
+// incremented (slot == -2) or nowhere (slot == -1).  Partial sums are per
+// block over a fixed node subset and combined in block order, so the result
+// is the same on every run.
+__global__ __launch_bounds__(RED_THREADS) void k_post(const double2 *__restrict__ z,
+                                                      double *__restrict__ ex,
+                                                      double *__restrict__ ey, int64_t m,
                                                       float2 *__restrict__ e32,
                                                       double *__restrict__ zero_acc,
                                                       double *part, unsigned *count, double *hist,
@@ -217,7 +239,17 @@ __global__ __launch_bounds__(RED_THREADS) void k_post(const double *__restrict__
     double sx = 0.0, sy = 0.0;
     for (int64_t k = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; k < m;
          k += (int64_t)gridDim.x * RED_THREADS) {
-        double a = ex[k], b = ey[k];
+        double a, b;
+        if (z) {
+            const double2 v = z[k];
+            a = v.x;
+            b = v.y;
+            ex[k] = a;
+            ey[k] = b;
+        } else {
+            a = ex[k];
+            b = ey[k];
+        }
         sx += a * a;
         sy += b * b;
         if (e32) e32[k] = make_float2((float)a, (float)b);
@@ -253,52 +285,62 @@ __global__ __launch_bounds__(RED_THREADS) void k_post(const double *__restrict__
     }
 }
 
-// post: bit 0 writes the FP32 interleaved field copy, bit 1 clears the raw
-// deposit accumulator (after the r2c has read it).
-int poisson_solve_e(Ctx *c, const double *rho_in, double scale, int post, int energy_slot) {
-    const int h = c->nx / 2 + 1;
-    const int64_t m = (int64_t)h * c->ny;
+static inline dim3 grid_of(int64_t m) { return dim3((unsigned)((m + 255) / 256)); }
+
+template <int MODE>
+static int spectral(Ctx *c, double scale) {
     const int64_t mm = (int64_t)c->nx * c->ny;
-    PIC_TRY(exec(c, c->plan_r2c, (void *)rho_in, c->rhat));
-    hipLaunchKernelGGL(k_spectral, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->stream,
-                       c->rhat, c->ehat, c->kx_k2, c->ky_k2, c->k2, c->nx, c->ny, scale, 0);
+    hipLaunchKernelGGL(k_spectral<MODE>, grid_of(mm), dim3(256), 0, c->stream, c->zg, c->zf,
+                       c->kx_k2, c->ky_k2, c->k2, c->nx, c->ny, scale);
     PIC_HIP(hipGetLastError());
-    PIC_TRY(exec(c, c->plan_c2r2, c->ehat, c->ex));
-    if (post || energy_slot != -1) {
-        int blocks = (int)((mm + RED_THREADS - 1) / RED_THREADS);
-        if (blocks > RED_BLOCKS) blocks = RED_BLOCKS;
-        hipLaunchKernelGGL(k_post, dim3(blocks), dim3(RED_THREADS), 0, c->stream, c->ex, c->ey, mm,
-                           (post & 1) ? (float2 *)c->e32 : nullptr,
-                           (post & 2) ? c->rho_acc : nullptr, c->red_part, c->red_count,
-                           c->energy_hist, energy_slot, c->step_counter, c->dx * c->dy);
-        PIC_HIP(hipGetLastError());
-    }
+    return PIC_OK;
+}
+
+// rho_in (nx*ny real) -> ex, ey.  post: bit 0 writes the FP32 interleaved
+// field copy, bit 1 clears the raw deposit accumulator (after the forward
+// transform has read it).  energy_slot as in k_post.
+int poisson_solve_e(Ctx *c, const double *rho_in, double scale, int post, int energy_slot) {
+    const int64_t mm = (int64_t)c->nx * c->ny;
+    hipLaunchKernelGGL(k_pack, grid_of(mm), dim3(256), 0, c->stream, rho_in, (const double *)nullptr,
+                       mm, c->zf);
+    PIC_HIP(hipGetLastError());
+    PIC_TRY(exec(c, c->plan_fwd, c->zf, c->zg));
+    PIC_TRY(spectral<SPEC_E>(c, scale));
+    PIC_TRY(exec(c, c->plan_inv, c->zf, c->zg));
+    int blocks = (int)((mm + RED_THREADS - 1) / RED_THREADS);
+    if (blocks > RED_BLOCKS) blocks = RED_BLOCKS;
+    hipLaunchKernelGGL(k_post, dim3(blocks), dim3(RED_THREADS), 0, c->stream, c->zg, c->ex, c->ey,
+                       mm, (post & 1) ? (float2 *)c->e32 : nullptr,
+                       (post & 2) ? c->rho_acc : nullptr, c->red_part, c->red_count,
+                       c->energy_hist, energy_slot, c->step_counter, c->dx * c->dy);
+    PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
 
 int poisson_solve_phi(Ctx *c) {
-    const int h = c->nx / 2 + 1;
-    const int64_t m = (int64_t)h * c->ny;
-    PIC_TRY(exec(c, c->plan_r2c, c->rho, c->rhat));
-    double scale = 1.0 / ((double)c->nx * (double)c->ny);
-    hipLaunchKernelGGL(k_spectral, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->stream,
-                       c->rhat, c->ehat, c->kx_k2, c->ky_k2, c->k2, c->nx, c->ny, scale, 1);
+    const int64_t mm = (int64_t)c->nx * c->ny;
+    hipLaunchKernelGGL(k_pack, grid_of(mm), dim3(256), 0, c->stream, (const double *)c->rho,
+                       (const double *)nullptr, mm, c->zf);
     PIC_HIP(hipGetLastError());
-    PIC_TRY(exec(c, c->plan_c2r1, c->ehat, c->phi));
+    PIC_TRY(exec(c, c->plan_fwd, c->zf, c->zg));
+    PIC_TRY(spectral<SPEC_PHI>(c, 1.0 / ((double)c->nx * (double)c->ny)));
+    PIC_TRY(exec(c, c->plan_inv, c->zf, c->zg));
+    hipLaunchKernelGGL(k_real_part, grid_of(mm), dim3(256), 0, c->stream, c->zg, mm, c->phi);
+    PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
 
-// lrho = E^T (lex, ley): two r2c, the transposed spectral symbol, one c2r.
-// lex and ley must be contiguous (ley = lex + nx*ny).
+// lrho = E^T (lex, ley) = -(P_x lex + P_y ley): one forward transform of
+// lex + i ley, the transposed symbol, one inverse transform.
 int poisson_adjoint(Ctx *c, const double *lex, const double *ley, double *lrho, double scale) {
-    const int h = c->nx / 2 + 1;
-    const int64_t m = (int64_t)h * c->ny;
-    PIC_CHECK(ley == lex + (int64_t)c->nx * c->ny, PIC_ERR_ARG, "adjoint fields not contiguous");
-    PIC_TRY(exec(c, c->plan_r2c2, (void *)lex, c->ehat));
-    hipLaunchKernelGGL(k_spectral_adj, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->stream,
-                       c->ehat, c->rhat, c->kx_k2, c->ky_k2, c->nx, c->ny, scale);
+    const int64_t mm = (int64_t)c->nx * c->ny;
+    hipLaunchKernelGGL(k_pack, grid_of(mm), dim3(256), 0, c->stream, lex, ley, mm, c->zf);
     PIC_HIP(hipGetLastError());
-    PIC_TRY(exec(c, c->plan_c2r1, c->rhat, lrho));
+    PIC_TRY(exec(c, c->plan_fwd, c->zf, c->zg));
+    PIC_TRY(spectral<SPEC_ADJ>(c, scale));
+    PIC_TRY(exec(c, c->plan_inv, c->zf, c->zg));
+    hipLaunchKernelGGL(k_real_part, grid_of(mm), dim3(256), 0, c->stream, c->zg, mm, lrho);
+    PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
 
@@ -368,9 +410,9 @@ int field_energy(Ctx *c, double *out2) {
     const int64_t mm = (int64_t)c->nx * c->ny;
     int blocks = (int)((mm + RED_THREADS - 1) / RED_THREADS);
     if (blocks > RED_BLOCKS) blocks = RED_BLOCKS;
-    hipLaunchKernelGGL(k_post, dim3(blocks), dim3(RED_THREADS), 0, c->stream, c->ex, c->ey, mm,
-                       nullptr, nullptr, c->red_part, c->red_count, c->energy_hist, 0,
-                       c->step_counter, c->dx * c->dy);
+    hipLaunchKernelGGL(k_post, dim3(blocks), dim3(RED_THREADS), 0, c->stream,
+                       (const double2 *)nullptr, c->ex, c->ey, mm, nullptr, nullptr, c->red_part,
+                       c->red_count, c->energy_hist, 0, c->step_counter, c->dx * c->dy);
     PIC_HIP(hipGetLastError());
     PIC_HIP(hipMemcpyAsync(out2, c->energy_hist, 2 * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));

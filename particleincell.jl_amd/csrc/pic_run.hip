@@ -19,7 +19,19 @@ namespace pic {
 
 int launch_step64(Ctx *c, int kernel, int mode, double dt, double dtd);
 int launch_step32(Ctx *c, int kernel, int mode, double dt, double dtd);
+int launch_seg64(Ctx *c, int kernel, int mode, double dt, double dtd);
+int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd);
 
+// A pass of pic_run: segment-persistent kernels over the lane-sorted slots.
+static int seg_pass(Ctx *c, int kernel, int mode, double dt) {
+    if (mode == 3)
+        return c->state == PIC_STATE_F64 ? launch_seg64(c, kernel, 3, 0.0, 0.0)
+                                         : launch_seg32(c, kernel, 3, 0.0, 0.0);
+    if (c->state == PIC_STATE_F64) return launch_seg64(c, kernel, mode, dt, 0.5 * dt);
+    return launch_seg32(c, kernel, mode, dt, mode == 1 ? dt : 0.5 * dt);
+}
+
+// A pass of pic_step_pass: chunk kernels, any particle order.
 static int step_pass(Ctx *c, int kernel, int mode, double dt) {
     if (mode == 3) {
         return c->state == PIC_STATE_F64 ? launch_step64(c, kernel, 3, 0.0, 0.0)
@@ -51,8 +63,10 @@ extern "C" int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort
     const int64_t m = (int64_t)c->nx * c->ny;
     const double scale = 1.0 / ((double)c->nx * (double)c->ny * c->dx * c->dy);
     const int f32 = c->state == PIC_STATE_F32;
+    // the passes work segment by segment on lane-sorted slots
+    PIC_TRY(sort_by_cell(c, nullptr, true));
     PIC_TRY(launch_fill(c, c->rho_acc, 0.0, m));
-    PIC_TRY(step_pass(c, kernel, 0, dt));
+    PIC_TRY(seg_pass(c, kernel, 0, dt));
     // timing: events around every solve and every particle pass
     std::vector<hipEvent_t> ev;
     auto mark = [&]() -> int {
@@ -72,7 +86,7 @@ extern "C" int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort
         if (rc == PIC_OK && c->timing) rc = mark();
         if (rc == PIC_OK) rc = poisson_solve_e(c, c->rho_acc, scale, post, -2);
         if (rc == PIC_OK && c->timing) rc = mark();
-        if (rc == PIC_OK) rc = step_pass(c, kernel, last ? 2 : 1, dt);
+        if (rc == PIC_OK) rc = seg_pass(c, kernel, last ? 2 : 1, dt);
         if (rc == PIC_OK && c->timing) rc = mark();
         if (rc == PIC_OK && !last && sort_every > 0 && (s + 1) % sort_every == 0)
             rc = sort_by_cell(c, nullptr, true);
@@ -126,7 +140,12 @@ extern "C" int pic_step_pass(pic_ctx *ctx, int kernel, int mode, double dt, doub
     PIC_CHECK(c->state == PIC_STATE_F64 || c->uniform_w, PIC_ERR_STATE, "FP32 state needs uniform w");
     const int64_t m = (int64_t)c->nx * c->ny;
     PIC_TRY(launch_fill(c, c->rho_acc, 0.0, m));
-    PIC_TRY(step_pass(c, kernel, mode, dt));
+    // particles in the lane order of a pic_sort_lanes: the segment kernels of
+    // pic_run; any other order: the chunk kernels
+    if (c->seg_valid)
+        PIC_TRY(seg_pass(c, kernel, mode, dt));
+    else
+        PIC_TRY(step_pass(c, kernel, mode, dt));
     if (rho_raw)
         PIC_HIP(hipMemcpyAsync(rho_raw, c->rho_acc, (size_t)m * sizeof(double),
                                hipMemcpyDeviceToHost, c->stream));

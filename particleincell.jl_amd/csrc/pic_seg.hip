@@ -1,0 +1,675 @@
+// pic_seg.hip -- segment-persistent particle passes of the device time loop.
+//
+// After a lane sort (pic_sort_lanes) segment s of 64 consecutive cells owns
+// the slots [off[64 s], off[min(64 s + 64, ncells)]) until the next sort, and
+// its particles stay within a few cells of the segment's home cells.  Each
+// block takes a contiguous slice of one segment and:
+//   1. stages the field window (home cells + stencil halo + drift margin,
+//      periodic) into LDS and clears the charge window;
+//   2. streams its particles (coalesced, 64 distinct cells per wave in the
+//      lane-spread order): gather from the LDS window, kick, drift, deposit
+//      into the LDS window;
+//   3. flushes the non-zero charge nodes with FP64 global atomics, once.
+// No block-wide synchronisation inside the particle loop.  A particle whose
+// stencil leaves the window (drifted further than the margin since the sort)
+// reads and writes global memory instead and is counted.
+//
+// Arithmetic per particle is the same as in pic_step.hip's kernels (FP64:
+// the reference's operations; FP32: the cell-relative step), so results are
+// bit-identical to them up to the order of the charge sums.
+//
+// FP32 charge accumulation: compensated (hi, lo) FP32 tile pairs.  Every
+// contribution q is split as q_hi = (q + C) - C, q_lo = q - q_hi, with
+// C = 1.5 * 2^23 * u for a power-of-two quantum u: q_hi is a multiple of u,
+// so the hi tile sums exactly while below 2^24 u (u is chosen from the
+// particle count per block), and q_lo, below u/2, carries the rest.  Both
+// adds are non-returning LDS atomics.  hi + lo is flushed in FP64.
+#include <climits>
+#include <cmath>
+
+#include "pic_internal.h"
+
+namespace pic {
+
+constexpr int SEG_T = 512;  // threads per block
+
+template <int K>
+struct SegStencil;
+template <>
+struct SegStencil<PIC_CIC> {
+    static constexpr int S = 2, HL = 0;
+};
+template <>
+struct SegStencil<PIC_M6> {
+    static constexpr int S = 6, HL = 2;
+};
+
+struct Win {
+    int x0, y0, tw, th;  // global index of window node (0,0) (may be negative), size
+};
+
+// Window of segment s: its home cells widened by the stencil and a drift
+// margin, the margin reduced until the window fits `cap` nodes.  A window as
+// wide as the periodic grid plus the stencil covers every cell of that
+// direction (x0 = -HL then), duplicating S-1 columns.
+template <int S, int HL>
+__device__ Win seg_window(int s, int nx, int ny, int ncells, int margin, int cap) {
+    const int cf = 64 * s, cl = min(64 * s + 63, ncells - 1);
+    const int jh0 = cf / nx, jh1 = cl / nx;
+    int ih0, ih1;
+    if (jh0 == jh1) {
+        ih0 = cf - jh0 * nx;
+        ih1 = cl - jh0 * nx;
+    } else {
+        ih0 = 0;
+        ih1 = nx - 1;
+    }
+    const int hw = ih1 - ih0 + 1, hh = jh1 - jh0 + 1;
+    int M = margin;
+    int tw = 0, th = 0;
+    for (;; --M) {
+        tw = min(hw + S - 1 + 2 * M, nx + S - 1);
+        th = min(hh + S - 1 + 2 * M, ny + S - 1);
+        if (tw * th <= cap || M == 0) break;
+    }
+    Win w;
+    w.tw = tw;
+    w.th = th;
+    w.x0 = tw == nx + S - 1 ? -HL : ih0 - HL - M;
+    w.y0 = th == ny + S - 1 ? -HL : jh0 - HL - M;
+    return w;
+}
+
+__device__ __forceinline__ void count_out(bool out, unsigned long long *slow) {
+    unsigned long long b = __ballot(out);
+    if (b && (threadIdx.x & 63) == __ffsll((long long)b) - 1)
+        atomicAdd(slow, (unsigned long long)__popcll(b));
+}
+
+// ============================================================ FP32 state
+
+struct SegF32 {
+    int32_t *cell;
+    float *ox, *oy, *vx, *vy;
+    float w;
+    const float2 *e32;
+    const float *bz;
+    float bz0;
+    int nx, ny, lognx, ncells;
+    float hdt, dtdx, dtdy;
+    double *acc;
+    const int64_t *off;
+    int bps, margin, cap;
+    float cq;  // 1.5 * 2^23 * quantum
+    unsigned long long *slow;
+};
+
+__device__ __forceinline__ int drift1s(float o, float v, float dtd, float &onew) {
+    float t = __builtin_fmaf(v, dtd, o);
+    float s = floorf(t);
+    float r = t - s;
+    if (r >= 1.0f) {
+        r = 0.0f;
+        s = s + 1.0f;
+    }
+    onew = r;
+    return (int)s;
+}
+
+template <int K, bool GATHER, bool DRIFT, bool DEPOSIT, int BZ, int ACC>
+__global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
+    using St = SegStencil<K>;
+    constexpr int S = St::S, HL = St::HL;
+    extern __shared__ __align__(16) char smem_s32[];
+    const int tid = threadIdx.x;
+    const int nx = a.nx, ny = a.ny;
+    const int s = blockIdx.x / a.bps, bi = blockIdx.x - s * a.bps;
+    const int ce = min(64 * s + 64, a.ncells);
+    const int64_t s0 = a.off[64 * s], s1 = a.off[ce];
+    const int64_t len = s1 - s0;
+    const int64_t p0 = s0 + len * bi / a.bps, p1 = s0 + len * (bi + 1) / a.bps;
+    const Win W = seg_window<S, HL>(s, nx, ny, a.ncells, a.margin, a.cap);
+    const bool fits = W.tw * W.th <= a.cap;
+    const int cap = a.cap;
+    float2 *te = reinterpret_cast<float2 *>(smem_s32);
+    float *tb = reinterpret_cast<float *>(smem_s32 + 8 * (size_t)cap);
+    char *dep = smem_s32 + (GATHER ? (BZ == PIC_BZ_GRID ? 12 : 8) * (size_t)cap : 0);
+    float *hi = reinterpret_cast<float *>(dep), *lo = hi + cap;
+    double *dacc = reinterpret_cast<double *>(dep);
+    if (fits) {
+        const int nt = W.tw * W.th;
+        for (int t = tid; t < nt; t += SEG_T) {
+            const int u = t % W.tw, v = t / W.tw;
+            const int64_t g = (int64_t)picd::wrap2(W.x0 + u, nx) +
+                              (int64_t)nx * picd::wrap2(W.y0 + v, ny);
+            if (GATHER) {
+                te[t] = a.e32[g];
+                if (BZ == PIC_BZ_GRID) tb[t] = a.bz[g];
+            }
+            if (DEPOSIT) {
+                if (ACC == 0) {
+                    hi[t] = 0.0f;
+                    lo[t] = 0.0f;
+                } else {
+                    dacc[t] = 0.0;
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    for (int64_t p = p0 + tid; p < p1; p += SEG_T) {
+        const int c = a.cell[p];
+        int ci = c & (nx - 1), cj = c >> a.lognx;
+        float fx = a.ox[p], fy = a.oy[p];
+        float ux = 0.0f, uy = 0.0f;
+        if (GATHER || DRIFT) {
+            ux = a.vx[p];
+            uy = a.vy[p];
+        }
+        if (GATHER) {
+            const int lx = picd::wrap2(ci - HL - W.x0, nx), ly = picd::wrap2(cj - HL - W.y0, ny);
+            const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
+            count_out(!in, a.slow);
+            auto node_e = [&](int u, int v) -> float2 {
+                if (in) return te[(lx + u) + W.tw * (ly + v)];
+                return a.e32[(int64_t)picd::wrap2(ci - HL + u, nx) +
+                             (int64_t)nx * picd::wrap2(cj - HL + v, ny)];
+            };
+            auto node_b = [&](int u, int v) -> float {
+                if (in) return tb[(lx + u) + W.tw * (ly + v)];
+                return a.bz[(int64_t)picd::wrap2(ci - HL + u, nx) +
+                            (int64_t)nx * picd::wrap2(cj - HL + v, ny)];
+            };
+            float e1, e2, b3 = 0.0f;
+            if (K == PIC_CIC) {
+                float sx = 1.0f - fx, sy = 1.0f - fy;
+                float a0 = sx * sy, a1 = fx * sy, a2 = fx * fy, a3 = sx * fy;
+                float2 E00 = node_e(0, 0), E10 = node_e(1, 0), E11 = node_e(1, 1),
+                       E01 = node_e(0, 1);
+                e1 = a0 * E00.x;
+                e1 = __builtin_fmaf(a1, E10.x, e1);
+                e1 = __builtin_fmaf(a2, E11.x, e1);
+                e1 = __builtin_fmaf(a3, E01.x, e1);
+                e2 = a0 * E00.y;
+                e2 = __builtin_fmaf(a1, E10.y, e2);
+                e2 = __builtin_fmaf(a2, E11.y, e2);
+                e2 = __builtin_fmaf(a3, E01.y, e2);
+                if (BZ == PIC_BZ_GRID) {
+                    b3 = a0 * node_b(0, 0);
+                    b3 = __builtin_fmaf(a1, node_b(1, 0), b3);
+                    b3 = __builtin_fmaf(a2, node_b(1, 1), b3);
+                    b3 = __builtin_fmaf(a3, node_b(0, 1), b3);
+                }
+            } else {
+                float wx[6], wy[6];
+                picd::m6_weights_f(fx, wx);
+                picd::m6_weights_f(fy, wy);
+                e1 = 0.0f;
+                e2 = 0.0f;
+#pragma unroll
+                for (int u = 0; u < 6; ++u) {
+                    float2 E0 = node_e(u, 0);
+                    float t1 = wy[0] * E0.x, t2 = wy[0] * E0.y;
+#pragma unroll
+                    for (int v = 1; v < 6; ++v) {
+                        float2 E = node_e(u, v);
+                        t1 = __builtin_fmaf(wy[v], E.x, t1);
+                        t2 = __builtin_fmaf(wy[v], E.y, t2);
+                    }
+                    if (u == 0) {
+                        e1 = wx[0] * t1;
+                        e2 = wx[0] * t2;
+                    } else {
+                        e1 = __builtin_fmaf(wx[u], t1, e1);
+                        e2 = __builtin_fmaf(wx[u], t2, e2);
+                    }
+                }
+                if (BZ == PIC_BZ_GRID) {
+#pragma unroll
+                    for (int u = 0; u < 6; ++u) {
+                        float t3 = wy[0] * node_b(u, 0);
+#pragma unroll
+                        for (int v = 1; v < 6; ++v) t3 = __builtin_fmaf(wy[v], node_b(u, v), t3);
+                        b3 = u == 0 ? wx[0] * t3 : __builtin_fmaf(wx[u], t3, b3);
+                    }
+                }
+            }
+            if (BZ == PIC_BZ_UNIFORM) b3 = a.bz0;
+            float v1 = __builtin_fmaf(a.hdt, e1, ux);
+            float v2 = __builtin_fmaf(a.hdt, e2, uy);
+            if (BZ != PIC_BZ_NONE) {
+                float tn = a.hdt * b3;
+                float sn = (2.0f * tn) / __builtin_fmaf(tn, tn, 1.0f);
+                v1 = __builtin_fmaf(v2, tn, v1);
+                v2 = __builtin_fmaf(-v1, sn, v2);
+                v1 = __builtin_fmaf(v2, tn, v1);
+            }
+            ux = __builtin_fmaf(a.hdt, e1, v1);
+            uy = __builtin_fmaf(a.hdt, e2, v2);
+            a.vx[p] = ux;
+            a.vy[p] = uy;
+        }
+        if (DRIFT) {
+            float nfx, nfy;
+            const int si = drift1s(fx, ux, a.dtdx, nfx);
+            const int sj = drift1s(fy, uy, a.dtdy, nfy);
+            ci = picd::wrapi(ci + si, nx);
+            cj = picd::wrapi(cj + sj, ny);
+            fx = nfx;
+            fy = nfy;
+            a.cell[p] = ci + nx * cj;
+            a.ox[p] = nfx;
+            a.oy[p] = nfy;
+        }
+        if (DEPOSIT) {
+            const int lx = picd::wrap2(ci - HL - W.x0, nx), ly = picd::wrap2(cj - HL - W.y0, ny);
+            const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
+            count_out(!in, a.slow);
+            auto add = [&](int u, int v, float q) {
+                if (in) {
+                    const int t = (lx + u) + W.tw * (ly + v);
+                    if (ACC == 0) {
+                        const float qh = (q + a.cq) - a.cq;
+                        const float ql = q - qh;
+                        __hip_atomic_fetch_add(&hi[t], qh, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_fetch_add(&lo[t], ql, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    } else {
+                        __hip_atomic_fetch_add(&dacc[t], (double)q, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                } else {
+                    unsafeAtomicAdd(&a.acc[(int64_t)picd::wrap2(ci - HL + u, nx) +
+                                           (int64_t)nx * picd::wrap2(cj - HL + v, ny)],
+                                    (double)q);
+                }
+            };
+            if (K == PIC_CIC) {
+                float sx = 1.0f - fx, sy = 1.0f - fy;
+                add(0, 0, (sx * sy) * a.w);
+                add(1, 0, (fx * sy) * a.w);
+                add(1, 1, (fx * fy) * a.w);
+                add(0, 1, (sx * fy) * a.w);
+            } else {
+                float wx[6], wy[6];
+                picd::m6_weights_f(fx, wx);
+                picd::m6_weights_f(fy, wy);
+#pragma unroll
+                for (int v = 0; v < 6; ++v) wy[v] = wy[v] * a.w;
+#pragma unroll
+                for (int u = 0; u < 6; ++u)
+#pragma unroll
+                    for (int v = 0; v < 6; ++v) add(u, v, wx[u] * wy[v]);
+            }
+        }
+    }
+    if (!DEPOSIT || !fits) return;
+    __syncthreads();
+    const int nt = W.tw * W.th;
+    for (int t = tid; t < nt; t += SEG_T) {
+        const double q = ACC == 0 ? (double)hi[t] + (double)lo[t] : dacc[t];
+        if (q != 0.0) {
+            const int u = t % W.tw, v = t / W.tw;
+            unsafeAtomicAdd(&a.acc[(int64_t)picd::wrap2(W.x0 + u, nx) +
+                                   (int64_t)nx * picd::wrap2(W.y0 + v, ny)],
+                            q);
+        }
+    }
+}
+
+// ============================================================ FP64 state
+
+struct SegF64 {
+    double *x, *y, *vx, *vy;
+    const double *w;
+    double w0;
+    const double *ex, *ey, *bz;
+    double bz0;
+    int nx, ny, ncells;
+    double dx, dy, dimx, dimy, dt, dtd;
+    double *acc;
+    const int64_t *off;
+    int bps, margin, cap;
+    unsigned long long *slow;
+};
+
+// The reference's cell and in-cell offset (pushers_cic.jl:18-29): i =
+// floor(xp)+1 (trunc in compute_rho CIC, pushers_cic.jl:86-87, equal for
+// x >= 0), d = (xp - i) + 1; returns the 0-based wrapped cell.
+__device__ __forceinline__ int ref_cell_s(double x, double dx, int n, bool trunc_, double &d) {
+    double xp = x / dx;
+    long long i1 = (trunc_ ? (long long)xp : (long long)floor(xp)) + 1;
+    d = (xp - (double)i1) + 1;
+    return (int)picd::mod1z(i1, n);
+}
+
+template <int K, bool GATHER, int NDRIFT, bool DEPOSIT, int BZ>
+__global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
+    using St = SegStencil<K>;
+    constexpr int S = St::S, HL = St::HL;
+    extern __shared__ __align__(16) char smem_s64[];
+    const int tid = threadIdx.x;
+    const int nx = a.nx, ny = a.ny;
+    const int s = blockIdx.x / a.bps, bi = blockIdx.x - s * a.bps;
+    const int ce = min(64 * s + 64, a.ncells);
+    const int64_t s0 = a.off[64 * s], s1 = a.off[ce];
+    const int64_t len = s1 - s0;
+    const int64_t p0 = s0 + len * bi / a.bps, p1 = s0 + len * (bi + 1) / a.bps;
+    const Win W = seg_window<S, HL>(s, nx, ny, a.ncells, a.margin, a.cap);
+    const bool fits = W.tw * W.th <= a.cap;
+    const int cap = a.cap;
+    double2 *te = reinterpret_cast<double2 *>(smem_s64);
+    double *tb = reinterpret_cast<double *>(smem_s64 + 16 * (size_t)cap);
+    double *dacc = reinterpret_cast<double *>(
+        smem_s64 + (GATHER ? (BZ == PIC_BZ_GRID ? 24 : 16) * (size_t)cap : 0));
+    if (fits) {
+        const int nt = W.tw * W.th;
+        for (int t = tid; t < nt; t += SEG_T) {
+            const int u = t % W.tw, v = t / W.tw;
+            const int64_t g = (int64_t)picd::wrap2(W.x0 + u, nx) +
+                              (int64_t)nx * picd::wrap2(W.y0 + v, ny);
+            if (GATHER) {
+                te[t] = make_double2(a.ex[g], a.ey[g]);
+                if (BZ == PIC_BZ_GRID) tb[t] = a.bz[g];
+            }
+            if (DEPOSIT) dacc[t] = 0.0;
+        }
+    }
+    __syncthreads();
+
+    for (int64_t p = p0 + tid; p < p1; p += SEG_T) {
+        double px = a.x[p], py = a.y[p];
+        double pvx = 0.0, pvy = 0.0;
+        if (GATHER || NDRIFT > 0) {
+            pvx = a.vx[p];
+            pvy = a.vy[p];
+        }
+        if (GATHER) {
+            double fx, fy;
+            const int ci = ref_cell_s(px, a.dx, nx, false, fx);
+            const int cj = ref_cell_s(py, a.dy, ny, false, fy);
+            const int lx = picd::wrap2(ci - HL - W.x0, nx), ly = picd::wrap2(cj - HL - W.y0, ny);
+            const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
+            count_out(!in, a.slow);
+            auto node_e = [&](int u, int v) -> double2 {
+                if (in) return te[(lx + u) + W.tw * (ly + v)];
+                const int64_t g = (int64_t)picd::wrap2(ci - HL + u, nx) +
+                                  (int64_t)nx * picd::wrap2(cj - HL + v, ny);
+                return make_double2(a.ex[g], a.ey[g]);
+            };
+            auto node_b = [&](int u, int v) -> double {
+                if (in) return tb[(lx + u) + W.tw * (ly + v)];
+                return a.bz[(int64_t)picd::wrap2(ci - HL + u, nx) +
+                            (int64_t)nx * picd::wrap2(cj - HL + v, ny)];
+            };
+            double e1, e2, b3 = 0.0;
+            if (K == PIC_CIC) {
+                double a1 = (1 - fx) * (1 - fy);
+                double a2 = fx * (1 - fy);
+                double a3 = fx * fy;
+                double a4 = (1 - fx) * fy;
+                double2 E00 = node_e(0, 0), E10 = node_e(1, 0), E11 = node_e(1, 1),
+                        E01 = node_e(0, 1);
+                e1 = a1 * E00.x + a2 * E10.x + a3 * E11.x + a4 * E01.x;
+                e2 = a1 * E00.y + a2 * E10.y + a3 * E11.y + a4 * E01.y;
+                if (BZ == PIC_BZ_GRID)
+                    b3 = a1 * node_b(0, 0) + a2 * node_b(1, 0) + a3 * node_b(1, 1) +
+                         a4 * node_b(0, 1);
+            } else {
+                double wx[6], wy[6];
+                picd::m6_weights_d(fx, wx);
+                picd::m6_weights_d(fy, wy);
+                e1 = 0.0;
+                e2 = 0.0;
+#pragma unroll
+                for (int u = 0; u < 6; ++u) {
+                    double2 E0 = node_e(u, 0);
+                    double t1 = wy[0] * E0.x, t2 = wy[0] * E0.y;
+#pragma unroll
+                    for (int v = 1; v < 6; ++v) {
+                        double2 E = node_e(u, v);
+                        t1 = __builtin_fma(wy[v], E.x, t1);
+                        t2 = __builtin_fma(wy[v], E.y, t2);
+                    }
+                    e1 = __builtin_fma(wx[u], t1, e1);
+                    e2 = __builtin_fma(wx[u], t2, e2);
+                }
+                if (BZ == PIC_BZ_GRID) {
+#pragma unroll
+                    for (int u = 0; u < 6; ++u) {
+                        double t3 = wy[0] * node_b(u, 0);
+#pragma unroll
+                        for (int v = 1; v < 6; ++v) t3 = __builtin_fma(wy[v], node_b(u, v), t3);
+                        b3 = __builtin_fma(wx[u], t3, b3);
+                    }
+                }
+            }
+            if (BZ == PIC_BZ_UNIFORM) b3 = a.bz0;
+            const double dt = a.dt;
+            double v1 = pvx, v2 = pvy;
+            v1 += 0.5 * dt * e1;
+            v2 += 0.5 * dt * e2;
+            if (BZ != PIC_BZ_NONE) {
+                double tantheta = 0.5 * dt * b3;
+                double sintheta = 2 * tantheta / (1 + tantheta * tantheta);
+                v1 += v2 * tantheta;
+                v2 += -v1 * sintheta;
+                v1 += v2 * tantheta;
+            }
+            pvx = v1 + 0.5 * dt * e1;
+            pvy = v2 + 0.5 * dt * e2;
+            a.vx[p] = pvx;
+            a.vy[p] = pvy;
+        }
+#pragma unroll
+        for (int r = 0; r < NDRIFT; ++r) {
+            double p1 = px + a.dtd * pvx;
+            double p2 = py + a.dtd * pvy;
+            if (p1 > a.dimx) p1 -= a.dimx;
+            if (p2 > a.dimy) p2 -= a.dimy;
+            if (p1 < 0.0) p1 += a.dimx;
+            if (p2 < 0.0) p2 += a.dimy;
+            px = p1;
+            py = p2;
+        }
+        if (NDRIFT > 0) {
+            a.x[p] = px;
+            a.y[p] = py;
+        }
+        if (DEPOSIT) {
+            double fx, fy;
+            const int ci = ref_cell_s(px, a.dx, nx, K == PIC_CIC, fx);
+            const int cj = ref_cell_s(py, a.dy, ny, K == PIC_CIC, fy);
+            const int lx = picd::wrap2(ci - HL - W.x0, nx), ly = picd::wrap2(cj - HL - W.y0, ny);
+            const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
+            count_out(!in, a.slow);
+            const double wgt = a.w ? a.w[p] : a.w0;
+            auto add = [&](int u, int v, double q) {
+                if (in)
+                    __hip_atomic_fetch_add(&dacc[(lx + u) + W.tw * (ly + v)], q, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                else
+                    unsafeAtomicAdd(&a.acc[(int64_t)picd::wrap2(ci - HL + u, nx) +
+                                           (int64_t)nx * picd::wrap2(cj - HL + v, ny)],
+                                    q);
+            };
+            if (K == PIC_CIC) {
+                double a1 = (1 - fx) * (1 - fy);
+                double a2 = fx * (1 - fy);
+                double a3 = fx * fy;
+                double a4 = (1 - fx) * fy;
+                add(0, 0, a1 * wgt);
+                add(1, 0, a2 * wgt);
+                add(1, 1, a3 * wgt);
+                add(0, 1, a4 * wgt);
+            } else {
+                double wx[6], wy[6];
+                picd::m6_weights_d(fx, wx);
+                picd::m6_weights_d(fy, wy);
+#pragma unroll
+                for (int u = 0; u < 6; ++u)
+#pragma unroll
+                    for (int v = 0; v < 6; ++v) add(u, v, wx[u] * wy[v] * wgt);
+            }
+        }
+    }
+    if (!DEPOSIT || !fits) return;
+    __syncthreads();
+    const int nt = W.tw * W.th;
+    for (int t = tid; t < nt; t += SEG_T) {
+        const double q = dacc[t];
+        if (q != 0.0) {
+            const int u = t % W.tw, v = t / W.tw;
+            unsafeAtomicAdd(&a.acc[(int64_t)picd::wrap2(W.x0 + u, nx) +
+                                   (int64_t)nx * picd::wrap2(W.y0 + v, ny)],
+                            q);
+        }
+    }
+}
+
+// ============================================================== dispatch
+
+// LDS per block for the window: FP32 8 B field (+4 B bz) + 8 B charge pair,
+// FP64 16 B field (+8 B bz) + 8 B charge.
+constexpr size_t SEG_LDS_BYTES = 48 * 1024;
+
+static int seg_bps(const Ctx *c) {
+    const int nseg = (c->nx * c->ny + 63) / 64;
+    int bps = (3 * 256 + nseg - 1) / nseg;
+    if (bps < 1) bps = 1;
+    if (bps > 64) bps = 64;
+    return bps;
+}
+
+template <int K, bool G, bool DR, bool D, int ACC>
+static void seg32_bz(const Ctx *c, const SegF32 &a, int bz, dim3 grid, size_t sm) {
+    switch (bz) {
+        case PIC_BZ_NONE:
+            hipLaunchKernelGGL((k_seg_f32<K, G, DR, D, PIC_BZ_NONE, ACC>), grid, dim3(SEG_T), sm,
+                               c->stream, a);
+            break;
+        case PIC_BZ_UNIFORM:
+            hipLaunchKernelGGL((k_seg_f32<K, G, DR, D, PIC_BZ_UNIFORM, ACC>), grid, dim3(SEG_T), sm,
+                               c->stream, a);
+            break;
+        default:
+            hipLaunchKernelGGL((k_seg_f32<K, G, DR, D, PIC_BZ_GRID, ACC>), grid, dim3(SEG_T), sm,
+                               c->stream, a);
+    }
+}
+
+template <int K, int ACC>
+static int seg32_mode(const Ctx *c, const SegF32 &a, int mode, int bz, dim3 grid, size_t sm) {
+    switch (mode) {
+        case 0: seg32_bz<K, false, true, true, ACC>(c, a, PIC_BZ_NONE, grid, sm); break;
+        case 1: seg32_bz<K, true, true, true, ACC>(c, a, bz, grid, sm); break;
+        case 2: seg32_bz<K, true, true, false, ACC>(c, a, bz, grid, sm); break;
+        case 3: seg32_bz<K, false, false, true, ACC>(c, a, PIC_BZ_NONE, grid, sm); break;
+        default: return fail(PIC_ERR_ARG, "pass mode %d", mode);
+    }
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+// One segment-persistent pass in FP32 state (the particles must be in the
+// lane order of the last sort_by_cell(..., lanes = true)).
+int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
+    if (c->n == 0) return PIC_OK;
+    PIC_CHECK(c->seg_valid, PIC_ERR_STATE, "segment passes need a lane sort first");
+    SegF32 a;
+    a.cell = c->cell; a.ox = c->ox; a.oy = c->oy; a.vx = c->vx32; a.vy = c->vy32;
+    a.w = (float)c->w0;
+    a.e32 = reinterpret_cast<const float2 *>(c->e32);
+    a.bz = c->bz32;
+    a.bz0 = (float)c->bz0;
+    a.nx = c->nx; a.ny = c->ny;
+    a.lognx = 0;
+    while ((1 << a.lognx) < c->nx) ++a.lognx;
+    a.ncells = c->nx * c->ny;
+    a.hdt = (float)(0.5 * dt);
+    a.dtdx = (float)(dtd / c->dx);
+    a.dtdy = (float)(dtd / c->dy);
+    a.acc = c->rho_acc;
+    a.off = c->cell_off;
+    a.bps = seg_bps(c);
+    a.margin = 32;
+    const bool gather = mode == 1 || mode == 2;
+    const size_t per_node =
+        (gather ? (c->bz_mode == PIC_BZ_GRID ? 12 : 8) : 0) + (mode == 2 ? 0 : 8);
+    a.cap = (int)(SEG_LDS_BYTES / (per_node ? per_node : 8)) & ~15;
+    // quantum u: the hi tile of a node sums at most (particles per block) *
+    // max(w * w_x * w_y) <= 0.31 w * count; keep that below 2^24 u (and every
+    // single contribution below 2^22 u, where the split is exact)
+    const double bound = 0.31 * std::abs(c->w0) * (double)(c->max_seg_len / a.bps + 5) + 1e-300;
+    const int e = (int)std::ceil(std::log2(bound)) - 24;
+    a.cq = (float)(1.5 * std::ldexp(1.0, e + 23));
+    a.slow = c->slow_chunks;
+    dim3 grid((unsigned)(((c->nx * c->ny + 63) / 64) * a.bps));
+    const size_t sm = (size_t)a.cap * (per_node ? per_node : 8);
+    const int acc = c->f32_acc;
+    if (kernel == PIC_CIC)
+        return acc == 0 ? seg32_mode<PIC_CIC, 0>(c, a, mode, c->bz_mode, grid, sm)
+                        : seg32_mode<PIC_CIC, 1>(c, a, mode, c->bz_mode, grid, sm);
+    return acc == 0 ? seg32_mode<PIC_M6, 0>(c, a, mode, c->bz_mode, grid, sm)
+                    : seg32_mode<PIC_M6, 1>(c, a, mode, c->bz_mode, grid, sm);
+}
+
+template <int K, bool G, int ND, bool D>
+static void seg64_bz(const Ctx *c, const SegF64 &a, int bz, dim3 grid, size_t sm) {
+    switch (bz) {
+        case PIC_BZ_NONE:
+            hipLaunchKernelGGL((k_seg_f64<K, G, ND, D, PIC_BZ_NONE>), grid, dim3(SEG_T), sm,
+                               c->stream, a);
+            break;
+        case PIC_BZ_UNIFORM:
+            hipLaunchKernelGGL((k_seg_f64<K, G, ND, D, PIC_BZ_UNIFORM>), grid, dim3(SEG_T), sm,
+                               c->stream, a);
+            break;
+        default:
+            hipLaunchKernelGGL((k_seg_f64<K, G, ND, D, PIC_BZ_GRID>), grid, dim3(SEG_T), sm,
+                               c->stream, a);
+    }
+}
+
+template <int K>
+static int seg64_mode(const Ctx *c, const SegF64 &a, int mode, int bz, dim3 grid, size_t sm) {
+    switch (mode) {
+        case 0: seg64_bz<K, false, 1, true>(c, a, PIC_BZ_NONE, grid, sm); break;
+        case 1: seg64_bz<K, true, 2, true>(c, a, bz, grid, sm); break;
+        case 2: seg64_bz<K, true, 1, false>(c, a, bz, grid, sm); break;
+        case 3: seg64_bz<K, false, 0, true>(c, a, PIC_BZ_NONE, grid, sm); break;
+        default: return fail(PIC_ERR_ARG, "pass mode %d", mode);
+    }
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+int launch_seg64(Ctx *c, int kernel, int mode, double dt, double dtd) {
+    if (c->n == 0) return PIC_OK;
+    PIC_CHECK(c->seg_valid, PIC_ERR_STATE, "segment passes need a lane sort first");
+    SegF64 a;
+    a.x = c->x; a.y = c->y; a.vx = c->vx; a.vy = c->vy;
+    a.w = c->uniform_w ? nullptr : c->w;
+    a.w0 = c->w0;
+    a.ex = c->ex; a.ey = c->ey; a.bz = c->bz; a.bz0 = c->bz0;
+    a.nx = c->nx; a.ny = c->ny; a.ncells = c->nx * c->ny;
+    a.dx = c->dx; a.dy = c->dy; a.dimx = c->dimx; a.dimy = c->dimy;
+    a.dt = dt; a.dtd = dtd;
+    a.acc = c->rho_acc;
+    a.off = c->cell_off;
+    a.bps = seg_bps(c);
+    a.margin = 32;
+    const bool gather = mode == 1 || mode == 2;
+    const size_t per_node =
+        (gather ? (c->bz_mode == PIC_BZ_GRID ? 24 : 16) : 0) + (mode == 2 ? 0 : 8);
+    a.cap = (int)(SEG_LDS_BYTES / (per_node ? per_node : 8)) & ~15;
+    a.slow = c->slow_chunks;
+    dim3 grid((unsigned)(((c->nx * c->ny + 63) / 64) * a.bps));
+    const size_t sm = (size_t)a.cap * (per_node ? per_node : 8);
+    return kernel == PIC_CIC ? seg64_mode<PIC_CIC>(c, a, mode, c->bz_mode, grid, sm)
+                             : seg64_mode<PIC_M6>(c, a, mode, c->bz_mode, grid, sm);
+}
+
+}  // namespace pic

@@ -15,6 +15,7 @@
 // permutation std::stable_sort by cell produces.  The permutation is then
 // applied to the particle state by a gather into a second set of arrays
 // whose pointers are swapped in.
+#include <algorithm>
 #include <climits>
 #include <vector>
 
@@ -215,14 +216,18 @@ __global__ void k_gather(int64_t n, const uint32_t *__restrict__ perm, const T *
 
 // One wave per segment: the largest cell count of the segment.
 __global__ void k_seg_max(const int64_t *__restrict__ off, int ncells, int nseg,
-                          uint32_t *__restrict__ kmax) {
+                          uint32_t *__restrict__ kmax, uint32_t *__restrict__ len) {
     const int s = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     const int l = threadIdx.x & 63;
     if (s >= nseg) return;
     const int c = 64 * s + l;
     int cnt = c < ncells ? (int)(off[c + 1] - off[c]) : 0;
     for (int o = 32; o > 0; o >>= 1) cnt = max(cnt, __shfl_xor(cnt, o));
-    if (l == 0) kmax[s] = (uint32_t)cnt;
+    if (l == 0) {
+        kmax[s] = (uint32_t)cnt;
+        const int ce = min(64 * s + 64, ncells);
+        len[s] = (uint32_t)(off[ce] - off[64 * s]);
+    }
 }
 
 // One wave per segment: tables A_s(k), M_s(k) at tbl[s] + k.
@@ -371,18 +376,25 @@ static int lane_spread(Ctx *c) {
     if (!c->seg_k) {
         PIC_HIP(hipMalloc(&c->seg_k, (size_t)nseg * 4));
         PIC_HIP(hipMalloc(&c->seg_tbl, (size_t)(nseg + 1) * 4));
+        PIC_HIP(hipMalloc(&c->seg_len, (size_t)nseg * 4));
     }
     if (!c->perm2) PIC_HIP(hipMalloc(&c->perm2, (size_t)(c->cap > 0 ? c->cap : 1) * 4));
     const int wpb = 4;  // waves (segments) per block
     dim3 gs((unsigned)((nseg + wpb - 1) / wpb)), bs(64 * wpb);
-    hipLaunchKernelGGL(k_seg_max, gs, bs, 0, c->stream, c->cell_off, ncells, nseg, c->seg_k);
+    hipLaunchKernelGGL(k_seg_max, gs, bs, 0, c->stream, c->cell_off, ncells, nseg, c->seg_k,
+                       c->seg_len);
     PIC_HIP(hipMemcpyAsync(c->seg_tbl, c->seg_k, (size_t)nseg * 4, hipMemcpyDeviceToDevice,
                            c->stream));
     PIC_TRY(scan(c, c->seg_tbl, nseg));
     uint32_t last[2] = {0, 0};
+    std::vector<uint32_t> lens((size_t)nseg);
     PIC_HIP(hipMemcpyAsync(&last[0], c->seg_tbl + nseg - 1, 4, hipMemcpyDeviceToHost, c->stream));
     PIC_HIP(hipMemcpyAsync(&last[1], c->seg_k + nseg - 1, 4, hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipMemcpyAsync(lens.data(), c->seg_len, (size_t)nseg * 4, hipMemcpyDeviceToHost,
+                           c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
+    c->max_seg_len = 0;
+    for (uint32_t v : lens) c->max_seg_len = std::max<int64_t>(c->max_seg_len, v);
     const int64_t total = (int64_t)last[0] + last[1];
     if (total > c->seg_cap) {
         if (c->seg_A) (void)hipFree(c->seg_A);
@@ -406,6 +418,7 @@ static int lane_spread(Ctx *c) {
 int sort_by_cell(Ctx *c, int32_t *perm_host, bool lanes) {
     const int64_t n = c->n;
     c->lanes_order = lanes;
+    c->seg_valid = false;
     if (n == 0) return PIC_OK;
     PIC_TRY(ensure_sort(c));
     const int ncells = c->nx * c->ny;
@@ -427,6 +440,7 @@ int sort_by_cell(Ctx *c, int32_t *perm_host, bool lanes) {
     if (lanes) {
         PIC_TRY(lane_spread(c));
         perm = c->perm2;
+        c->seg_valid = true;
     }
     if (c->state == PIC_STATE_F32) {
         PIC_TRY(permute(c, c->cell, n, perm));

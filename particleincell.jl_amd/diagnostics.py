@@ -14,21 +14,35 @@ import numpy as np
 LANDAU_K05 = complex(1.415662, -0.153359)
 
 
-def local_maxima(t, e):
-    t = np.asarray(t)
-    e = np.asarray(e)
+def local_maxima(t, e, min_sep=0.0):
+    """Local maxima of e(t), at least min_sep apart in t (the larger of two
+    close maxima is kept)."""
+    t = np.asarray(t, dtype=np.float64)
+    e = np.asarray(e, dtype=np.float64)
     idx = [i for i in range(1, len(e) - 1) if e[i] > e[i - 1] and e[i] >= e[i + 1]]
-    return t[idx], e[idx]
+    kept = []
+    for i in sorted(idx, key=lambda k: -e[k]):
+        if all(abs(t[i] - t[j]) >= min_sep for j in kept):
+            kept.append(i)
+    kept.sort()
+    return t[kept], e[kept]
 
 
-def damping_rate(t, e, tmin=0.0, tmax=None):
-    """(gamma, omega, n_maxima): fit of ln(local maxima) against t in
-    [tmin, tmax]."""
-    tm, em = local_maxima(t, e)
+def damping_rate(t, e, tmin=0.0, tmax=None, min_sep=1.5, noise=None, snr=3.0):
+    """(gamma, omega, n_maxima): least-squares fit of ln(local maxima) against
+    t over [tmin, tmax], maxima at least min_sep apart.  With `noise` (the
+    level of a pure-noise series such as sqrt(sum(ey^2) dx dy)), the fit stops
+    at the first maximum below snr * noise."""
+    tm, em = local_maxima(t, e, min_sep)
     sel = tm >= tmin
     if tmax is not None:
         sel &= tm <= tmax
     tm, em = tm[sel], em[sel]
+    if noise is not None:
+        keep = em >= snr * noise
+        if not keep.all():
+            stop = int(np.argmin(keep))
+            tm, em = tm[:stop], em[:stop]
     if len(tm) < 2:
         return float("nan"), float("nan"), len(tm)
     gamma = np.polyfit(tm, np.log(em), 1)[0]

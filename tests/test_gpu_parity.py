@@ -372,6 +372,86 @@ def test_f32_step_bitwise(gpu, O, kernel):
     ctx.close()
 
 
+@pytest.mark.parametrize("acc", [0, 1])
+@pytest.mark.parametrize("kernel", ["cic", "m6"])
+@pytest.mark.parametrize("nx,ny", [(128, 128), (64, 32), (32, 64)])
+def test_f32_segment_pass_bitwise(gpu, O, kernel, acc, nx, ny):
+    """The segment-persistent FP32 pass of pic_run (after pic_sort_lanes):
+    cells, offsets, velocities bit-identical to the oracle; the charge of the
+    compensated FP32 pairs (acc 0) or FP64 windows (acc 1) within 1e-12.
+    Particles are spread wide enough that some leave their segment window."""
+    pic = gpu
+    rng = np.random.default_rng(41 + nx)
+    n = 200000
+    mesh = pic.TwoDGrid(4 * np.pi, nx, 4 * np.pi, ny)
+    p = random_particles(rng, n, mesh.dimx, mesh.dimy)
+    p[2:4] *= 3.0
+    st0 = O.f32_from_f64(p, nx, ny, mesh.dx, mesh.dy)
+    w = np.float32(p[4, 0])
+    ex, ey = (np.asfortranarray(rng.normal(size=(nx, ny)) * 0.3) for _ in range(2))
+    ctx = pic.Context(mesh, capacity=n)
+    ctx.set_state_f32(st0, float(w))
+    perm = np.zeros(n, np.int32)
+    pic._lib.call("pic_sort_lanes", ctx.h, perm.ctypes.data)
+    st = tuple(np.ascontiguousarray(a[perm]) for a in st0)
+    ctx.set_grid(pic._lib.PIC_GRID_EX, ex)
+    ctx.set_grid(pic._lib.PIC_GRID_EY, ey)
+    ctx.set_bz(pic._lib.PIC_BZ_NONE)
+    pic._lib.call("pic_set_f32_accumulation", ctx.h, acc)
+    e32 = np.empty(2 * nx * ny, np.float32)
+    e32[0::2] = ex.ravel(order="F")
+    e32[1::2] = ey.ravel(order="F")
+    kc = pic._lib.PIC_CIC if kernel == "cic" else pic._lib.PIC_M6
+    raw = np.zeros((nx, ny), order="F")
+    dt = 0.5
+    for _ in range(3):  # the slots stay; particles drift out of their windows
+        pic._lib.call("pic_step_pass", ctx.h, kc, 1, dt, raw.ctypes.data)
+        acc_o = O.f32_step(st, nx, ny, kernel, e32, np.float32(dt / 2),
+                           np.float32(dt / mesh.dx), np.float32(dt / mesh.dy), w)
+        got = ctx.get_state_f32()
+        for a, b, name in zip(got, st, ("cell", "ox", "oy", "vx", "vy")):
+            assert np.array_equal(a, b), name
+        assert nrel(raw, acc_o) < 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize("kernel", ["cic", "m6"])
+def test_f64_segment_pass(gpu, O, kernel):
+    """The segment-persistent FP64 pass after pic_sort_lanes: CIC bit-identical
+    to the reference sequence, M6 within rounding; deposit within 1e-12."""
+    pic = gpu
+    rng = np.random.default_rng(42)
+    nx, ny, n = 128, 64, 150000
+    mesh = pic.TwoDGrid(4 * np.pi, nx, 4 * np.pi, ny)
+    p = random_particles(rng, n, mesh.dimx, mesh.dimy)
+    ex, ey = (np.asfortranarray(rng.normal(size=(nx, ny))) for _ in range(2))
+    ctx = pic.Context(mesh, capacity=n)
+    ctx.set_particles(p)
+    perm = np.zeros(n, np.int32)
+    pic._lib.call("pic_sort_lanes", ctx.h, perm.ctypes.data)
+    q = np.asfortranarray(p[:, perm])
+    ctx.set_grid(pic._lib.PIC_GRID_EX, ex)
+    ctx.set_grid(pic._lib.PIC_GRID_EY, ey)
+    ctx.set_bz(pic._lib.PIC_BZ_NONE)
+    kc = pic._lib.PIC_CIC if kernel == "cic" else pic._lib.PIC_M6
+    raw = np.zeros((nx, ny), order="F")
+    dt = 0.2
+    pic._lib.call("pic_step_pass", ctx.h, kc, 1, dt, raw.ctypes.data)
+    got = ctx.get_particles(np.zeros((5, n), order="F"))
+    O.push_v(q, kernel, nx, ny, mesh.dx, mesh.dy, ex, ey, None, dt)
+    O.push_x(q, mesh.dimx, mesh.dimy, dt / 2)
+    O.push_x(q, mesh.dimx, mesh.dimy, dt / 2)
+    if kernel == "cic":
+        assert np.array_equal(got, q)
+    else:
+        assert np.max(np.abs(got - q)) < 1e-13
+    rho_o = O.compute_rho(got, kernel, nx, ny, mesh.dx, mesh.dy, mesh.dimx, mesh.dimy)
+    rho = raw / (mesh.dx * mesh.dy)
+    rho -= rho.mean()
+    assert nrel(rho, rho_o) < RTOL
+    ctx.close()
+
+
 @pytest.mark.parametrize("bz", ["none", "uniform", "grid"])
 @pytest.mark.parametrize("kernel", ["cic", "m6"])
 def test_f32_step_bz_bitwise(gpu, O, kernel, bz):
