@@ -244,6 +244,19 @@ int pic_step_pass(pic_ctx *ctx, int kernel, int mode, double dt, double *rho_raw
 int pic_set_timing(pic_ctx *ctx, int enable);
 int pic_get_timing(pic_ctx *ctx, double *pass_ms, int64_t *passes, double *solve_ms);
 
+/* Gradient of the field energy after `nsteps` iterations of pic_run's loop,
+ * J = sum(ex^2 + ey^2) dx dy of the last step's field (the energy the loop
+ * records at t = nsteps*dt), with respect to the initial positions and
+ * velocities of every particle: the exact discrete adjoint of the loop,
+ * bz = 0, FP64 state.  The forward pass keeps the solved field of every step
+ * and the particle state every `ckpt` steps; the reverse sweep recomputes
+ * each segment from its checkpoint.  gx, gv (host, 2 x n, may be NULL)
+ * receive dJ/d(x0, y0) and dJ/d(vx0, vy0) in the context's particle order;
+ * the particles are left as they were.  No reference counterpart (the Julia
+ * rrule of the drop-in wraps this call). */
+int pic_adjoint(pic_ctx *ctx, int kernel, double dt, int nsteps, int ckpt, double *J, double *gx,
+                double *gv);
+
 /* Number of particle chunks that took the global-memory slow path in the
  * fused kernels since the context was created (diagnostic). */
 int pic_slow_chunks(pic_ctx *ctx, uint64_t *count);

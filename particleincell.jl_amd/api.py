@@ -440,6 +440,17 @@ class Simulation:
     def grid(self, which):
         return self.ctx.get_grid(which)
 
+    def adjoint(self, dt, nsteps, ckpt):
+        """(J, gx, gv): J = sum(ex^2+ey^2) dx dy of the last step's field after
+        nsteps loop iterations, gx = dJ/d(x0, y0), gv = dJ/d(vx0, vy0), each
+        2 x n in the particle order of the context (pic_adjoint)."""
+        J = C.c_double()
+        gx = np.zeros((2, self.n), order="F")
+        gv = np.zeros((2, self.n), order="F")
+        L.call("pic_adjoint", self.ctx.h, self.kernel, float(dt), int(nsteps), int(ckpt),
+               C.byref(J), gx.ctypes.data, gv.ctypes.data)
+        return J.value, gx, gv
+
     def set_timing(self, on=True):
         L.call("pic_set_timing", self.ctx.h, 1 if on else 0)
 

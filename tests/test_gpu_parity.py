@@ -544,7 +544,13 @@ def test_run_matches_reference_loop(gpu, O, kernel):
     q = p.copy(order="F")
     eo = O.landau_loop(q, kernel, 64, 64, mesh.dimx, mesh.dimy, 0.1, 10)
     assert np.max(np.abs(np.sqrt(e[:, 0]) - eo) / eo) < 1e-10
-    assert np.max(np.abs(got[:4] - q[:4])) < 1e-10
+    # pic_run sorts the particles by cell, so match them as a set: every
+    # GPU particle's nearest oracle particle in (x, y, vx, vy) is within
+    # 1e-10 and the matching is one to one
+    from scipy.spatial import cKDTree
+    dist, idx = cKDTree(q[:4].T).query(got[:4].T)
+    assert np.max(dist) < 1e-10
+    assert np.unique(idx).size == q.shape[1]
 
 
 def test_run_sorted_matches_unsorted(gpu):
