@@ -49,10 +49,29 @@ def cpu_info():
     return os.cpu_count(), model or platform.processor()
 
 
-def run_path(pic, mesh, p0, state, dt, steps, warmup, sort_every):
+def copy_probe(mib, reps=5):
+    """Plain device copies of `mib` MiB (torch's elementwise copy kernel), the
+    calibration workload for the HBM byte counters: a copy of B bytes reads B
+    and writes B."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4,
+                      "GBs": 2 * n * 4 / dt / 1e9}), flush=True)
+
+
+def run_path(pic, mesh, p0, state, dt, steps, warmup, sort_every, f32_acc=1):
     """Warmup, then the timed run from the initial state. Returns dict."""
     L = pic._lib
     sim = pic.Simulation(mesh, p0, pic.M6(), state=state)
+    sim.set_f32_accumulation(f32_acc)
     sim.sort(lanes=True)
     if warmup:
         sim.run(dt, warmup, sort_every=sort_every)
@@ -91,12 +110,20 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["f32", "f64", "both"], default="both")
+    ap.add_argument("--f32-acc", type=int, choices=[0, 1], default=1,
+                    help="FP32 charge tiles: 0 compensated FP32 pairs, 1 FP64 sums")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run plain device copies of MIB MiB (byte-counter calibration)")
     ap.add_argument("--dump", default="", help="write the energy histories to this .npz")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    f"(got {args.gpus})"}))
         sys.exit(2)
+
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     import __graft_entry__ as entry
     pic = entry.load_package()
@@ -114,7 +141,8 @@ def main():
         if args.only not in ("both", name):
             continue
         log(f"bench: {name} path, warmup {args.warmup}, timed {args.steps} steps")
-        res[name] = run_path(pic, mesh, p0, state, dt, args.steps, args.warmup, args.sort_every)
+        res[name] = run_path(pic, mesh, p0, state, dt, args.steps, args.warmup, args.sort_every,
+                             args.f32_acc)
         r = res[name]
         log(f"bench: {name} {r['seconds']*1e3:.2f} ms total, pass {r['pass_ms_avg']*1e3:.1f} us,"
             f" solve {r['solve_ms_avg']*1e3:.1f} us, slow chunks {r['slow_chunks']}")
@@ -155,7 +183,9 @@ def main():
                    "fp64 soa", "sort_every": args.sort_every, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_step_{head} (M6, kick+drift+deposit)",
+                     "kernel": f"k_seg_{head}<M6> gather+kick+drift+deposit pass"
+                               + (" (FP64 LDS tiles)" if head == "f32" and args.f32_acc == 1
+                                  else ""),
                      "bytes_per_particle": bytes_pp, "avg_us": r["pass_ms_avg"] * 1e3},
     }
     if "f32" in res and "f64" in res:

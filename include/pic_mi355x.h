@@ -113,7 +113,8 @@ int pic_get_particles_xv(pic_ctx *ctx, double *x, double *v);
 int pic_get_particle_fields(pic_ctx *ctx, double *e);
 
 /* Switches the device state between FP64 and FP32 cell-relative storage
- * (conversion on the GPU). */
+ * (conversion on the GPU).  The FP32 state needs uniform particle weights
+ * and nx, ny powers of two (PIC_ERR_STATE otherwise). */
 int pic_set_state(pic_ctx *ctx, int state);
 /* Raw FP32 cell-relative state: cell (int32), ox, oy, vx, vy (float), n
  * each, host memory. Any pointer may be NULL to skip it. */
@@ -218,8 +219,9 @@ int pic_cell_offsets(pic_ctx *ctx, int64_t *offsets);
 int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort_every, double *energy);
 
 /* Charge accumulation of pic_run's FP32 passes in their LDS windows:
- * 0 (default) compensated FP32 (hi, lo) pairs, 1 FP64 accumulation.  The
- * grid totals are FP64 either way.  No reference counterpart. */
+ * 0 compensated FP32 (hi, lo) pairs, 1 (default) FP64 accumulation of the
+ * FP32 contributions.  The grid totals are FP64 either way.  No reference
+ * counterpart. */
 int pic_set_f32_accumulation(pic_ctx *ctx, int mode);
 
 /* One particle pass of pic_run with the fields currently in the context

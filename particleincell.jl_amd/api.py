@@ -454,6 +454,11 @@ class Simulation:
     def set_timing(self, on=True):
         L.call("pic_set_timing", self.ctx.h, 1 if on else 0)
 
+    def set_f32_accumulation(self, mode):
+        """Charge accumulation of the FP32 state: 0 = compensated FP32
+        (hi, lo) tile pairs, 1 = FP64 tile sums (pic_set_f32_accumulation)."""
+        L.call("pic_set_f32_accumulation", self.ctx.h, int(mode))
+
     def timing(self):
         """(pass_ms_total, passes, solve_ms_total) of pic_run since timing
         was enabled (HIP events on the context stream)."""

@@ -52,6 +52,7 @@ static void free_particles(Ctx *c) {
     dfree(c->epx); dfree(c->epy);
     dfree(c->cell); dfree(c->ox); dfree(c->oy); dfree(c->vx32); dfree(c->vy32);
     dfree(c->swap);
+    dfree(c->defer);
     c->swap_bytes = 0;
     for (int k = 0; k < Ctx::NALT; ++k) {
         if (c->alt[k]) (void)hipFree(c->alt[k]);
@@ -98,6 +99,8 @@ int ensure_swap(Ctx *c, int64_t bytes) {
     }
     return PIC_OK;
 }
+
+int ensure_defer(Ctx *c) { return dalloc(&c->defer, c->cap); }
 
 int ensure_pfields(Ctx *c) {
     PIC_TRY(dalloc(&c->epx, c->cap));
@@ -218,8 +221,9 @@ int to_state(Ctx *c, int state) {
     if (state == PIC_STATE_F32) {
         PIC_CHECK(c->uniform_w, PIC_ERR_STATE,
                   "the FP32 cell-relative state needs uniform particle weights");
-        PIC_CHECK((c->nx & (c->nx - 1)) == 0, PIC_ERR_STATE,
-                  "the FP32 cell-relative state needs nx a power of two (nx=%d)", c->nx);
+        PIC_CHECK((c->nx & (c->nx - 1)) == 0 && (c->ny & (c->ny - 1)) == 0, PIC_ERR_STATE,
+                  "the FP32 cell-relative state needs nx and ny powers of two (%dx%d)", c->nx,
+                  c->ny);
         PIC_TRY(ensure_f32(c));
         hipLaunchKernelGGL(k_f64_to_f32, grid1(c->n), dim3(256), 0, c->stream, c->n, c->x, c->y,
                            c->vx, c->vy, c->nx, c->ny, c->dx, c->dy, c->cell, c->ox, c->oy,
@@ -316,6 +320,10 @@ int pic_ctx_create(pic_ctx **out, int device, int nx, int ny, double xmin, doubl
     PIC_HIP(hipSetDevice(device));
     Ctx *c = new Ctx();
     c->device = device;
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) !=
+            hipSuccess ||
+        c->ncu < 1)
+        c->ncu = 1;
     c->nx = nx;
     c->ny = ny;
     c->xmin = xmin;
@@ -346,6 +354,7 @@ int pic_ctx_create(pic_ctx **out, int device, int nx, int ny, double xmin, doubl
     chk(dalloc(&c->red_count, 4));
     chk(dalloc(&c->step_counter, 4));
     chk(dalloc(&c->slow_chunks, 1));
+    chk(dalloc(&c->defer_count, 2));
     if (rc == PIC_OK && capacity > 0) chk(ensure_particles(c, capacity));
     if (rc == PIC_OK) chk(poisson_init(c));
     if (rc == PIC_OK) {
@@ -353,6 +362,7 @@ int pic_ctx_create(pic_ctx **out, int device, int nx, int ny, double xmin, doubl
         if (e == hipSuccess) e = hipMemsetAsync(c->step_counter, 0, 4 * sizeof(int), c->stream);
         if (e == hipSuccess)
             e = hipMemsetAsync(c->slow_chunks, 0, sizeof(unsigned long long), c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(c->defer_count, 0, 2 * sizeof(unsigned), c->stream);
         if (e == hipSuccess) e = hipMemsetAsync(c->rho, 0, m * 8, c->stream);
         if (e == hipSuccess) e = hipMemsetAsync(c->ex, 0, 2 * m * 8, c->stream);
         if (e == hipSuccess) e = hipMemsetAsync(c->bz, 0, m * 8, c->stream);
@@ -382,8 +392,8 @@ int pic_ctx_destroy(pic_ctx *ctx) {
     dfree(c->rho); dfree(c->ex); c->ey = nullptr; dfree(c->bz); dfree(c->phi);
     dfree(c->rho_acc); dfree(c->e32); dfree(c->bz32);
     dfree(c->red_part); dfree(c->red_count); dfree(c->energy_hist); dfree(c->step_counter);
-    dfree(c->sort_hist); dfree(c->cell_off); dfree(c->slow_chunks);
-    dfree(c->seg_k); dfree(c->seg_tbl); dfree(c->seg_A); dfree(c->seg_M); dfree(c->seg_len);
+    dfree(c->sort_hist); dfree(c->cell_off); dfree(c->slow_chunks); dfree(c->defer_count);
+    dfree(c->seg_k); dfree(c->seg_tbl); dfree(c->seg_A); dfree(c->seg_M);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PIC_OK;
@@ -535,8 +545,8 @@ int pic_set_state_f32(pic_ctx *ctx, const int32_t *cell, const float *ox, const 
     PIC_CHECK(n == 0 || (cell && ox && oy && vx && vy), PIC_ERR_ARG, "null state pointer");
     Ctx *c = ctx_of(ctx);
     PIC_HIP(hipSetDevice(c->device));
-    PIC_CHECK((c->nx & (c->nx - 1)) == 0, PIC_ERR_STATE,
-              "the FP32 cell-relative state needs nx a power of two (nx=%d)", c->nx);
+    PIC_CHECK((c->nx & (c->nx - 1)) == 0 && (c->ny & (c->ny - 1)) == 0, PIC_ERR_STATE,
+              "the FP32 cell-relative state needs nx and ny powers of two (%dx%d)", c->nx, c->ny);
     int ncell = c->nx * c->ny;
     for (int64_t k = 0; k < n; ++k) {
         PIC_CHECK(cell[k] >= 0 && cell[k] < ncell, PIC_ERR_ARG, "cell[%lld] = %d out of range",

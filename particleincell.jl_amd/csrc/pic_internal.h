@@ -46,6 +46,7 @@ int fail(int code, const char *fmt, ...);
 // ------------------------------------------------------------ the context
 struct Ctx {
     int device = 0;
+    int ncu = 1;                        // compute units of the device
     hipStream_t stream = nullptr;
 
     // mesh (TwoDGrid: dx = dimx/nx as GEMPIC's grid defines it)
@@ -108,20 +109,23 @@ struct Ctx {
     bool lanes_order = false;           // last sort used the lane-spread order
     // after a lane sort, segment s occupies slots [cell_off[64 s],
     // cell_off[min(64 s + 64, ncells)]) until the next sort
-    uint32_t *seg_len = nullptr;        // particles per segment (nseg)
-    int64_t max_seg_len = 0;            // largest segment after the last lane sort
     bool seg_valid = false;             // cell_off describes the current slots
 
     // fused-step tile statistics (diagnostic): chunks that took the slow path
     unsigned long long *slow_chunks = nullptr;
+    // particles that the segment passes hand to their global-memory fallback
+    // kernel: entry list (capacity entries) and {count, arrivals}
+    uint32_t *defer = nullptr;
+    unsigned *defer_count = nullptr;
 
     // captured step graphs
     hipGraphExec_t graph_exec = nullptr;
     int graph_key = -1;
 
     // FP32 charge accumulation in the LDS windows: 0 compensated FP32
-    // (hi, lo) pairs, 1 FP64
-    int f32_acc = 0;
+    // (hi, lo) pairs, 1 FP64 (the default: gfx950 executes ds_add_f64 some
+    // twenty times faster than ds_add_f32)
+    int f32_acc = 1;
 
     // pic_run kernel timing (HIP events on the context stream)
     bool timing = false;
@@ -138,6 +142,7 @@ int ensure_swap(Ctx *c, int64_t bytes);
 int ensure_pfields(Ctx *c);
 int ensure_sort(Ctx *c);
 int ensure_energy(Ctx *c, int64_t n);
+int ensure_defer(Ctx *c);
 
 // ------------------------------------------------------ module entry points
 int poisson_init(Ctx *c);

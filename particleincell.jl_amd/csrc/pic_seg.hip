@@ -12,18 +12,28 @@
 //   3. flushes the non-zero charge nodes with FP64 global atomics, once.
 // No block-wide synchronisation inside the particle loop.  A particle whose
 // stencil leaves the window (drifted further than the margin since the sort)
-// reads and writes global memory instead and is counted.
+// is appended to a deferred list instead, whole or for its deposit only; a
+// small kernel launched after the pass completes those from global memory
+// with the same arithmetic and counts them.  Keeping the global-memory path
+// out of the pass kernel keeps its register count, and so its occupancy, at
+// what the LDS path needs.
 //
 // Arithmetic per particle is the same as in pic_step.hip's kernels (FP64:
 // the reference's operations; FP32: the cell-relative step), so results are
 // bit-identical to them up to the order of the charge sums.
 //
-// FP32 charge accumulation: compensated (hi, lo) FP32 tile pairs.  Every
-// contribution q is split as q_hi = (q + C) - C, q_lo = q - q_hi, with
-// C = 1.5 * 2^23 * u for a power-of-two quantum u: q_hi is a multiple of u,
-// so the hi tile sums exactly while below 2^24 u (u is chosen from the
-// particle count per block), and q_lo, below u/2, carries the rest.  Both
-// adds are non-returning LDS atomics.  hi + lo is flushed in FP64.
+// FP32 charge accumulation, two modes (Ctx::f32_acc):
+//  1 (default) FP64 LDS tiles: every FP32 contribution is added exactly
+//    converted to FP64 (ds_add_f64);
+//  0 compensated (hi, lo) FP32 tile pairs.  Every contribution q is split as
+//    q_hi = (q + C) - C, q_lo = q - q_hi, with C = 1.5 * 2^23 * u for a
+//    power-of-two quantum u: q_hi is a multiple of u, so the hi tile sums
+//    exactly while below 2^24 u (u is chosen from the particle count per
+//    block), and q_lo, below u/2, carries the rest.  Both adds are
+//    non-returning LDS atomics (ds_add_f32).  hi + lo is flushed in FP64.
+// Measured on MI355X, ds_add_f32 retires about 0.33 lanes per clock per CU
+// against 7.3 for ds_add_f64 and 13 for ds_add_u32, so mode 1 is both the
+// more accurate and the faster one there.
 #include <climits>
 #include <cmath>
 
@@ -80,10 +90,19 @@ __device__ Win seg_window(int s, int nx, int ny, int ncells, int margin, int cap
     return w;
 }
 
-__device__ __forceinline__ void count_out(bool out, unsigned long long *slow) {
-    unsigned long long b = __ballot(out);
-    if (b && (threadIdx.x & 63) == __ffsll((long long)b) - 1)
-        atomicAdd(slow, (unsigned long long)__popcll(b));
+// End of a deferred-list kernel: block 0 adds the list length to the
+// slow-path counter; the last block to arrive (every block has read the
+// count by then) clears count and arrivals for the next pass.
+__device__ __forceinline__ void defer_finish(unsigned cnt, unsigned *dcount,
+                                             unsigned long long *slow) {
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    if (blockIdx.x == 0 && cnt) atomicAdd(slow, (unsigned long long)cnt);
+    const unsigned arrived = atomicAdd(&dcount[1], 1u);
+    if (arrived == gridDim.x - 1) {
+        __hip_atomic_store(&dcount[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&dcount[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // ============================================================ FP32 state
@@ -100,9 +119,22 @@ struct SegF32 {
     double *acc;
     const int64_t *off;
     int bps, margin, cap;
-    float cq;  // 1.5 * 2^23 * quantum
     unsigned long long *slow;
+    uint32_t *dlist;  // deferred particles and their count
+    unsigned *dcount;
 };
+
+// C = 1.5 * 2^23 * u of the (hi, lo) split for a block of `count`
+// particles: the hi tile of a node sums at most count * max(w * w_x * w_y)
+// <= 0.31 w count, which stays below 2^24 u for the power-of-two quantum
+// u = 2^(ceil(log2(0.31 w (count + 5))) - 24).
+__device__ __forceinline__ float split_constant(float w, int64_t count) {
+    const double bound = 0.31 * fabs((double)w) * (double)(count + 5) + 1e-300;
+    int ex;
+    const double m = frexp(bound, &ex);  // bound = m 2^ex, m in [0.5, 1)
+    const int e = (m == 0.5 ? ex - 1 : ex) - 24;
+    return (float)ldexp(1.5, e + 23);
+}
 
 __device__ __forceinline__ int drift1s(float o, float v, float dtd, float &onew) {
     float t = __builtin_fmaf(v, dtd, o);
@@ -116,6 +148,130 @@ __device__ __forceinline__ int drift1s(float o, float v, float dtd, float &onew)
     return (int)s;
 }
 
+// Field at a particle from the nodes (u, v) of its stencil, u, v in [0, S):
+// CIC bilinear or M6 tensor-product weights, FP32, in a fixed operation order.
+template <int K, int BZ, class FE, class FB>
+__device__ __forceinline__ void interp32(float fx, float fy, FE node_e, FB node_b, float &e1,
+                                         float &e2, float &b3) {
+    if (K == PIC_CIC) {
+        float sx = 1.0f - fx, sy = 1.0f - fy;
+        float a0 = sx * sy, a1 = fx * sy, a2 = fx * fy, a3 = sx * fy;
+        float2 E00 = node_e(0, 0), E10 = node_e(1, 0), E11 = node_e(1, 1),
+               E01 = node_e(0, 1);
+        e1 = a0 * E00.x;
+        e1 = __builtin_fmaf(a1, E10.x, e1);
+        e1 = __builtin_fmaf(a2, E11.x, e1);
+        e1 = __builtin_fmaf(a3, E01.x, e1);
+        e2 = a0 * E00.y;
+        e2 = __builtin_fmaf(a1, E10.y, e2);
+        e2 = __builtin_fmaf(a2, E11.y, e2);
+        e2 = __builtin_fmaf(a3, E01.y, e2);
+        if (BZ == PIC_BZ_GRID) {
+            b3 = a0 * node_b(0, 0);
+            b3 = __builtin_fmaf(a1, node_b(1, 0), b3);
+            b3 = __builtin_fmaf(a2, node_b(1, 1), b3);
+            b3 = __builtin_fmaf(a3, node_b(0, 1), b3);
+        }
+    } else {
+        float wx[6], wy[6];
+        picd::m6_weights_f(fx, wx);
+        picd::m6_weights_f(fy, wy);
+        e1 = 0.0f;
+        e2 = 0.0f;
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+            float2 E0 = node_e(u, 0);
+            float t1 = wy[0] * E0.x, t2 = wy[0] * E0.y;
+#pragma unroll
+            for (int v = 1; v < 6; ++v) {
+                float2 E = node_e(u, v);
+                t1 = __builtin_fmaf(wy[v], E.x, t1);
+                t2 = __builtin_fmaf(wy[v], E.y, t2);
+            }
+            if (u == 0) {
+                e1 = wx[0] * t1;
+                e2 = wx[0] * t2;
+            } else {
+                e1 = __builtin_fmaf(wx[u], t1, e1);
+                e2 = __builtin_fmaf(wx[u], t2, e2);
+            }
+        }
+        if (BZ == PIC_BZ_GRID) {
+#pragma unroll
+            for (int u = 0; u < 6; ++u) {
+                float t3 = wy[0] * node_b(u, 0);
+#pragma unroll
+                for (int v = 1; v < 6; ++v) t3 = __builtin_fmaf(wy[v], node_b(u, v), t3);
+                b3 = u == 0 ? wx[0] * t3 : __builtin_fmaf(wx[u], t3, b3);
+            }
+        }
+    }
+}
+
+// Velocity update of the FP32 step from the gathered field: half kick,
+// rotation by bz, half kick (pushers_cic.jl:38-49 in FP32 with FMAs).
+template <int BZ>
+__device__ __forceinline__ void kick32(const SegF32 &a, float e1, float e2, float b3, float &ux,
+                                       float &uy) {
+    if (BZ == PIC_BZ_UNIFORM) b3 = a.bz0;
+    float v1 = __builtin_fmaf(a.hdt, e1, ux);
+    float v2 = __builtin_fmaf(a.hdt, e2, uy);
+    if (BZ != PIC_BZ_NONE) {
+        float tn = a.hdt * b3;
+        float sn = (2.0f * tn) / __builtin_fmaf(tn, tn, 1.0f);
+        v1 = __builtin_fmaf(v2, tn, v1);
+        v2 = __builtin_fmaf(-v1, sn, v2);
+        v1 = __builtin_fmaf(v2, tn, v1);
+    }
+    ux = __builtin_fmaf(a.hdt, e1, v1);
+    uy = __builtin_fmaf(a.hdt, e2, v2);
+}
+
+// Calls add(u, v, q) with the FP32 charge q of stencil node (u, v) of a
+// particle at offset (fx, fy), in a fixed order.
+template <int K, class F>
+__device__ __forceinline__ void charge32(float fx, float fy, float w, F &&add) {
+    if (K == PIC_CIC) {
+        float sx = 1.0f - fx, sy = 1.0f - fy;
+        add(0, 0, (sx * sy) * w);
+        add(1, 0, (fx * sy) * w);
+        add(1, 1, (fx * fy) * w);
+        add(0, 1, (sx * fy) * w);
+    } else {
+        float wx[6], wy[6];
+        picd::m6_weights_f(fx, wx);
+        picd::m6_weights_f(fy, wy);
+#pragma unroll
+        for (int v = 0; v < 6; ++v) wy[v] = wy[v] * w;
+#pragma unroll
+        for (int u = 0; u < 6; ++u)
+#pragma unroll
+            for (int v = 0; v < 6; ++v) add(u, v, wx[u] * wy[v]);
+    }
+}
+
+// Appends `entry` for every lane with `want` set (one atomic per wave).  The
+// leader's slot is broadcast with v_readlane, not a shuffle: a shuffle is an
+// LDS instruction, and waiting for its result would also wait for every LDS
+// atomic the wave still has in flight.
+__device__ __forceinline__ void defer_push(bool want, uint32_t entry, uint32_t *list,
+                                           unsigned *count) {
+    const unsigned long long b = __ballot(want);
+    if (!b) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)b) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(count, (unsigned)__popcll(b));
+    base = (unsigned)__builtin_amdgcn_readlane((int)base, leader);
+    if (want) list[base + (unsigned)__popcll(b & ((1ull << lane) - 1ull))] = entry;
+}
+
+constexpr uint32_t DEFER_DEPOSIT_ONLY = 0x80000000u;
+
+// The particles of one segment slice.  A particle whose stencil lies outside
+// the block's LDS window is not touched here: it goes to the deferred list
+// whole (gather outside) or for its deposit only (drifted outside), and
+// k_defer_f32 completes it from global memory after this kernel.
 template <int K, bool GATHER, bool DRIFT, bool DEPOSIT, int BZ, int ACC>
 __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
     using St = SegStencil<K>;
@@ -123,6 +279,7 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
     extern __shared__ __align__(16) char smem_s32[];
     const int tid = threadIdx.x;
     const int nx = a.nx, ny = a.ny;
+    const int mx = nx - 1, my = ny - 1;  // nx, ny are powers of two
     const int s = blockIdx.x / a.bps, bi = blockIdx.x - s * a.bps;
     const int ce = min(64 * s + 64, a.ncells);
     const int64_t s0 = a.off[64 * s], s1 = a.off[ce];
@@ -131,6 +288,7 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
     const Win W = seg_window<S, HL>(s, nx, ny, a.ncells, a.margin, a.cap);
     const bool fits = W.tw * W.th <= a.cap;
     const int cap = a.cap;
+    const float cq = ACC == 0 ? split_constant(a.w, p1 - p0) : 0.0f;
     float2 *te = reinterpret_cast<float2 *>(smem_s32);
     float *tb = reinterpret_cast<float *>(smem_s32 + 8 * (size_t)cap);
     char *dep = smem_s32 + (GATHER ? (BZ == PIC_BZ_GRID ? 12 : 8) * (size_t)cap : 0);
@@ -140,8 +298,7 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
         const int nt = W.tw * W.th;
         for (int t = tid; t < nt; t += SEG_T) {
             const int u = t % W.tw, v = t / W.tw;
-            const int64_t g = (int64_t)picd::wrap2(W.x0 + u, nx) +
-                              (int64_t)nx * picd::wrap2(W.y0 + v, ny);
+            const int64_t g = (int64_t)((W.x0 + u) & mx) + (int64_t)nx * ((W.y0 + v) & my);
             if (GATHER) {
                 te[t] = a.e32[g];
                 if (BZ == PIC_BZ_GRID) tb[t] = a.bz[g];
@@ -160,102 +317,33 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
 
     for (int64_t p = p0 + tid; p < p1; p += SEG_T) {
         const int c = a.cell[p];
-        int ci = c & (nx - 1), cj = c >> a.lognx;
+        int ci = c & mx, cj = c >> a.lognx;
         float fx = a.ox[p], fy = a.oy[p];
         float ux = 0.0f, uy = 0.0f;
-        if (GATHER || DRIFT) {
+        if (GATHER) {
+            const int lx = (ci - HL - W.x0) & mx, ly = (cj - HL - W.y0) & my;
+            const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
+            defer_push(!in, (uint32_t)p, a.dlist, a.dcount);
+            if (!in) continue;
             ux = a.vx[p];
             uy = a.vy[p];
-        }
-        if (GATHER) {
-            const int lx = picd::wrap2(ci - HL - W.x0, nx), ly = picd::wrap2(cj - HL - W.y0, ny);
-            const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
-            count_out(!in, a.slow);
-            auto node_e = [&](int u, int v) -> float2 {
-                if (in) return te[(lx + u) + W.tw * (ly + v)];
-                return a.e32[(int64_t)picd::wrap2(ci - HL + u, nx) +
-                             (int64_t)nx * picd::wrap2(cj - HL + v, ny)];
-            };
-            auto node_b = [&](int u, int v) -> float {
-                if (in) return tb[(lx + u) + W.tw * (ly + v)];
-                return a.bz[(int64_t)picd::wrap2(ci - HL + u, nx) +
-                            (int64_t)nx * picd::wrap2(cj - HL + v, ny)];
-            };
             float e1, e2, b3 = 0.0f;
-            if (K == PIC_CIC) {
-                float sx = 1.0f - fx, sy = 1.0f - fy;
-                float a0 = sx * sy, a1 = fx * sy, a2 = fx * fy, a3 = sx * fy;
-                float2 E00 = node_e(0, 0), E10 = node_e(1, 0), E11 = node_e(1, 1),
-                       E01 = node_e(0, 1);
-                e1 = a0 * E00.x;
-                e1 = __builtin_fmaf(a1, E10.x, e1);
-                e1 = __builtin_fmaf(a2, E11.x, e1);
-                e1 = __builtin_fmaf(a3, E01.x, e1);
-                e2 = a0 * E00.y;
-                e2 = __builtin_fmaf(a1, E10.y, e2);
-                e2 = __builtin_fmaf(a2, E11.y, e2);
-                e2 = __builtin_fmaf(a3, E01.y, e2);
-                if (BZ == PIC_BZ_GRID) {
-                    b3 = a0 * node_b(0, 0);
-                    b3 = __builtin_fmaf(a1, node_b(1, 0), b3);
-                    b3 = __builtin_fmaf(a2, node_b(1, 1), b3);
-                    b3 = __builtin_fmaf(a3, node_b(0, 1), b3);
-                }
-            } else {
-                float wx[6], wy[6];
-                picd::m6_weights_f(fx, wx);
-                picd::m6_weights_f(fy, wy);
-                e1 = 0.0f;
-                e2 = 0.0f;
-#pragma unroll
-                for (int u = 0; u < 6; ++u) {
-                    float2 E0 = node_e(u, 0);
-                    float t1 = wy[0] * E0.x, t2 = wy[0] * E0.y;
-#pragma unroll
-                    for (int v = 1; v < 6; ++v) {
-                        float2 E = node_e(u, v);
-                        t1 = __builtin_fmaf(wy[v], E.x, t1);
-                        t2 = __builtin_fmaf(wy[v], E.y, t2);
-                    }
-                    if (u == 0) {
-                        e1 = wx[0] * t1;
-                        e2 = wx[0] * t2;
-                    } else {
-                        e1 = __builtin_fmaf(wx[u], t1, e1);
-                        e2 = __builtin_fmaf(wx[u], t2, e2);
-                    }
-                }
-                if (BZ == PIC_BZ_GRID) {
-#pragma unroll
-                    for (int u = 0; u < 6; ++u) {
-                        float t3 = wy[0] * node_b(u, 0);
-#pragma unroll
-                        for (int v = 1; v < 6; ++v) t3 = __builtin_fmaf(wy[v], node_b(u, v), t3);
-                        b3 = u == 0 ? wx[0] * t3 : __builtin_fmaf(wx[u], t3, b3);
-                    }
-                }
-            }
-            if (BZ == PIC_BZ_UNIFORM) b3 = a.bz0;
-            float v1 = __builtin_fmaf(a.hdt, e1, ux);
-            float v2 = __builtin_fmaf(a.hdt, e2, uy);
-            if (BZ != PIC_BZ_NONE) {
-                float tn = a.hdt * b3;
-                float sn = (2.0f * tn) / __builtin_fmaf(tn, tn, 1.0f);
-                v1 = __builtin_fmaf(v2, tn, v1);
-                v2 = __builtin_fmaf(-v1, sn, v2);
-                v1 = __builtin_fmaf(v2, tn, v1);
-            }
-            ux = __builtin_fmaf(a.hdt, e1, v1);
-            uy = __builtin_fmaf(a.hdt, e2, v2);
+            interp32<K, BZ>(
+                fx, fy, [&](int u, int v) { return te[(lx + u) + W.tw * (ly + v)]; },
+                [&](int u, int v) { return tb[(lx + u) + W.tw * (ly + v)]; }, e1, e2, b3);
+            kick32<BZ>(a, e1, e2, b3, ux, uy);
             a.vx[p] = ux;
             a.vy[p] = uy;
+        } else if (DRIFT) {
+            ux = a.vx[p];
+            uy = a.vy[p];
         }
         if (DRIFT) {
             float nfx, nfy;
             const int si = drift1s(fx, ux, a.dtdx, nfx);
             const int sj = drift1s(fy, uy, a.dtdy, nfy);
-            ci = picd::wrapi(ci + si, nx);
-            cj = picd::wrapi(cj + sj, ny);
+            ci = (ci + si) & mx;
+            cj = (cj + sj) & my;
             fx = nfx;
             fy = nfy;
             a.cell[p] = ci + nx * cj;
@@ -263,46 +351,24 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
             a.oy[p] = nfy;
         }
         if (DEPOSIT) {
-            const int lx = picd::wrap2(ci - HL - W.x0, nx), ly = picd::wrap2(cj - HL - W.y0, ny);
+            const int lx = (ci - HL - W.x0) & mx, ly = (cj - HL - W.y0) & my;
             const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
-            count_out(!in, a.slow);
-            auto add = [&](int u, int v, float q) {
-                if (in) {
-                    const int t = (lx + u) + W.tw * (ly + v);
-                    if (ACC == 0) {
-                        const float qh = (q + a.cq) - a.cq;
-                        const float ql = q - qh;
-                        __hip_atomic_fetch_add(&hi[t], qh, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                        __hip_atomic_fetch_add(&lo[t], ql, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                    } else {
-                        __hip_atomic_fetch_add(&dacc[t], (double)q, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                    }
+            defer_push(!in, (uint32_t)p | DEFER_DEPOSIT_ONLY, a.dlist, a.dcount);
+            if (!in) continue;
+            charge32<K>(fx, fy, a.w, [&](int u, int v, float q) {
+                const int t = (lx + u) + W.tw * (ly + v);
+                if (ACC == 0) {
+                    const float qh = (q + cq) - cq;
+                    const float ql = q - qh;
+                    __hip_atomic_fetch_add(&hi[t], qh, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(&lo[t], ql, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
                 } else {
-                    unsafeAtomicAdd(&a.acc[(int64_t)picd::wrap2(ci - HL + u, nx) +
-                                           (int64_t)nx * picd::wrap2(cj - HL + v, ny)],
-                                    (double)q);
+                    __hip_atomic_fetch_add(&dacc[t], (double)q, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
-            };
-            if (K == PIC_CIC) {
-                float sx = 1.0f - fx, sy = 1.0f - fy;
-                add(0, 0, (sx * sy) * a.w);
-                add(1, 0, (fx * sy) * a.w);
-                add(1, 1, (fx * fy) * a.w);
-                add(0, 1, (sx * fy) * a.w);
-            } else {
-                float wx[6], wy[6];
-                picd::m6_weights_f(fx, wx);
-                picd::m6_weights_f(fy, wy);
-#pragma unroll
-                for (int v = 0; v < 6; ++v) wy[v] = wy[v] * a.w;
-#pragma unroll
-                for (int u = 0; u < 6; ++u)
-#pragma unroll
-                    for (int v = 0; v < 6; ++v) add(u, v, wx[u] * wy[v]);
-            }
+            });
         }
     }
     if (!DEPOSIT || !fits) return;
@@ -312,11 +378,73 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
         const double q = ACC == 0 ? (double)hi[t] + (double)lo[t] : dacc[t];
         if (q != 0.0) {
             const int u = t % W.tw, v = t / W.tw;
-            unsafeAtomicAdd(&a.acc[(int64_t)picd::wrap2(W.x0 + u, nx) +
-                                   (int64_t)nx * picd::wrap2(W.y0 + v, ny)],
+            unsafeAtomicAdd(&a.acc[(int64_t)((W.x0 + u) & mx) + (int64_t)nx * ((W.y0 + v) & my)],
                             q);
         }
     }
+}
+
+constexpr int DEFER_T = 256, DEFER_BLOCKS = 256;
+
+// Completes the particles k_seg_f32 deferred, from global memory (same
+// arithmetic), adds their number to the slow-path counter, and the last
+// block to finish resets the list for the next pass.
+template <int K, bool GATHER, bool DRIFT, bool DEPOSIT, int BZ>
+__global__ __launch_bounds__(DEFER_T) void k_defer_f32(SegF32 a) {
+    constexpr int HL = SegStencil<K>::HL;
+    const unsigned cnt = __hip_atomic_load(a.dcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int nx = a.nx, mx = a.nx - 1, my = a.ny - 1;
+    for (unsigned i = blockIdx.x * DEFER_T + threadIdx.x; i < cnt; i += gridDim.x * DEFER_T) {
+        const uint32_t e = a.dlist[i];
+        const int64_t p = e & ~DEFER_DEPOSIT_ONLY;
+        const bool whole = (e & DEFER_DEPOSIT_ONLY) == 0;
+        const int c = a.cell[p];
+        int ci = c & mx, cj = c >> a.lognx;
+        float fx = a.ox[p], fy = a.oy[p];
+        if (whole) {
+            float ux = 0.0f, uy = 0.0f;
+            if (GATHER || DRIFT) {
+                ux = a.vx[p];
+                uy = a.vy[p];
+            }
+            if (GATHER) {
+                float e1, e2, b3 = 0.0f;
+                interp32<K, BZ>(
+                    fx, fy,
+                    [&](int u, int v) {
+                        return a.e32[(int64_t)((ci - HL + u) & mx) +
+                                     (int64_t)nx * ((cj - HL + v) & my)];
+                    },
+                    [&](int u, int v) {
+                        return a.bz[(int64_t)((ci - HL + u) & mx) +
+                                    (int64_t)nx * ((cj - HL + v) & my)];
+                    },
+                    e1, e2, b3);
+                kick32<BZ>(a, e1, e2, b3, ux, uy);
+                a.vx[p] = ux;
+                a.vy[p] = uy;
+            }
+            if (DRIFT) {
+                float nfx, nfy;
+                const int si = drift1s(fx, ux, a.dtdx, nfx);
+                const int sj = drift1s(fy, uy, a.dtdy, nfy);
+                ci = (ci + si) & mx;
+                cj = (cj + sj) & my;
+                fx = nfx;
+                fy = nfy;
+                a.cell[p] = ci + nx * cj;
+                a.ox[p] = nfx;
+                a.oy[p] = nfy;
+            }
+        }
+        if (DEPOSIT)
+            charge32<K>(fx, fy, a.w, [&](int u, int v, float q) {
+                unsafeAtomicAdd(
+                    &a.acc[(int64_t)((ci - HL + u) & mx) + (int64_t)nx * ((cj - HL + v) & my)],
+                    (double)q);
+            });
+    }
+    defer_finish(cnt, a.dcount, a.slow);
 }
 
 // ============================================================ FP64 state
@@ -333,6 +461,8 @@ struct SegF64 {
     const int64_t *off;
     int bps, margin, cap;
     unsigned long long *slow;
+    uint32_t *dlist;  // deferred particles and their count
+    unsigned *dcount;
 };
 
 // The reference's cell and in-cell offset (pushers_cic.jl:18-29): i =
@@ -345,6 +475,118 @@ __device__ __forceinline__ int ref_cell_s(double x, double dx, int n, bool trunc
     return (int)picd::mod1z(i1, n);
 }
 
+// Field at a particle from the nodes (u, v) of its stencil, FP64, with the
+// reference's operation order (pushers_cic.jl:31-44, pushers_m6.jl:63-83).
+template <int K, int BZ, class FE, class FB>
+__device__ __forceinline__ void interp64(double fx, double fy, FE node_e, FB node_b, double &e1,
+                                         double &e2, double &b3) {
+    if (K == PIC_CIC) {
+        double a1 = (1 - fx) * (1 - fy);
+        double a2 = fx * (1 - fy);
+        double a3 = fx * fy;
+        double a4 = (1 - fx) * fy;
+        double2 E00 = node_e(0, 0), E10 = node_e(1, 0), E11 = node_e(1, 1),
+                E01 = node_e(0, 1);
+        e1 = a1 * E00.x + a2 * E10.x + a3 * E11.x + a4 * E01.x;
+        e2 = a1 * E00.y + a2 * E10.y + a3 * E11.y + a4 * E01.y;
+        if (BZ == PIC_BZ_GRID)
+            b3 = a1 * node_b(0, 0) + a2 * node_b(1, 0) + a3 * node_b(1, 1) +
+                 a4 * node_b(0, 1);
+    } else {
+        double wx[6], wy[6];
+        picd::m6_weights_d(fx, wx);
+        picd::m6_weights_d(fy, wy);
+        e1 = 0.0;
+        e2 = 0.0;
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+            double2 E0 = node_e(u, 0);
+            double t1 = wy[0] * E0.x, t2 = wy[0] * E0.y;
+#pragma unroll
+            for (int v = 1; v < 6; ++v) {
+                double2 E = node_e(u, v);
+                t1 = __builtin_fma(wy[v], E.x, t1);
+                t2 = __builtin_fma(wy[v], E.y, t2);
+            }
+            e1 = __builtin_fma(wx[u], t1, e1);
+            e2 = __builtin_fma(wx[u], t2, e2);
+        }
+        if (BZ == PIC_BZ_GRID) {
+#pragma unroll
+            for (int u = 0; u < 6; ++u) {
+                double t3 = wy[0] * node_b(u, 0);
+#pragma unroll
+                for (int v = 1; v < 6; ++v) t3 = __builtin_fma(wy[v], node_b(u, v), t3);
+                b3 = __builtin_fma(wx[u], t3, b3);
+            }
+        }
+    }
+}
+
+// Half kick, rotation by bz, half kick with the reference's operations
+// (pushers_cic.jl:38-49).
+template <int BZ>
+__device__ __forceinline__ void kick64(const SegF64 &a, double e1, double e2, double b3,
+                                       double &pvx, double &pvy) {
+    if (BZ == PIC_BZ_UNIFORM) b3 = a.bz0;
+    const double dt = a.dt;
+    double v1 = pvx, v2 = pvy;
+    v1 += 0.5 * dt * e1;
+    v2 += 0.5 * dt * e2;
+    if (BZ != PIC_BZ_NONE) {
+        double tantheta = 0.5 * dt * b3;
+        double sintheta = 2 * tantheta / (1 + tantheta * tantheta);
+        v1 += v2 * tantheta;
+        v2 += -v1 * sintheta;
+        v1 += v2 * tantheta;
+    }
+    pvx = v1 + 0.5 * dt * e1;
+    pvy = v2 + 0.5 * dt * e2;
+}
+
+// NDRIFT drifts by dtd with the single conditional wrap of push_x!
+// (pushers_cic.jl:55-72).
+template <int NDRIFT>
+__device__ __forceinline__ void drift64(const SegF64 &a, double pvx, double pvy, double &px,
+                                        double &py) {
+#pragma unroll
+    for (int r = 0; r < NDRIFT; ++r) {
+        double p1 = px + a.dtd * pvx;
+        double p2 = py + a.dtd * pvy;
+        if (p1 > a.dimx) p1 -= a.dimx;
+        if (p2 > a.dimy) p2 -= a.dimy;
+        if (p1 < 0.0) p1 += a.dimx;
+        if (p2 < 0.0) p2 += a.dimy;
+        px = p1;
+        py = p2;
+    }
+}
+
+// Calls add(u, v, q) with the charge of stencil node (u, v)
+// (pushers_cic.jl:91-94, pushers_m6.jl:20-27).
+template <int K, class F>
+__device__ __forceinline__ void charge64(double fx, double fy, double wgt, F &&add) {
+    if (K == PIC_CIC) {
+        double a1 = (1 - fx) * (1 - fy);
+        double a2 = fx * (1 - fy);
+        double a3 = fx * fy;
+        double a4 = (1 - fx) * fy;
+        add(0, 0, a1 * wgt);
+        add(1, 0, a2 * wgt);
+        add(1, 1, a3 * wgt);
+        add(0, 1, a4 * wgt);
+    } else {
+        double wx[6], wy[6];
+        picd::m6_weights_d(fx, wx);
+        picd::m6_weights_d(fy, wy);
+#pragma unroll
+        for (int u = 0; u < 6; ++u)
+#pragma unroll
+            for (int v = 0; v < 6; ++v) add(u, v, wx[u] * wy[v] * wgt);
+    }
+}
+
+// FP64 counterpart of k_seg_f32; out-of-window particles go to k_defer_f64.
 template <int K, bool GATHER, int NDRIFT, bool DEPOSIT, int BZ>
 __global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
     using St = SegStencil<K>;
@@ -382,99 +624,29 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
     for (int64_t p = p0 + tid; p < p1; p += SEG_T) {
         double px = a.x[p], py = a.y[p];
         double pvx = 0.0, pvy = 0.0;
-        if (GATHER || NDRIFT > 0) {
-            pvx = a.vx[p];
-            pvy = a.vy[p];
-        }
         if (GATHER) {
             double fx, fy;
             const int ci = ref_cell_s(px, a.dx, nx, false, fx);
             const int cj = ref_cell_s(py, a.dy, ny, false, fy);
             const int lx = picd::wrap2(ci - HL - W.x0, nx), ly = picd::wrap2(cj - HL - W.y0, ny);
             const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
-            count_out(!in, a.slow);
-            auto node_e = [&](int u, int v) -> double2 {
-                if (in) return te[(lx + u) + W.tw * (ly + v)];
-                const int64_t g = (int64_t)picd::wrap2(ci - HL + u, nx) +
-                                  (int64_t)nx * picd::wrap2(cj - HL + v, ny);
-                return make_double2(a.ex[g], a.ey[g]);
-            };
-            auto node_b = [&](int u, int v) -> double {
-                if (in) return tb[(lx + u) + W.tw * (ly + v)];
-                return a.bz[(int64_t)picd::wrap2(ci - HL + u, nx) +
-                            (int64_t)nx * picd::wrap2(cj - HL + v, ny)];
-            };
+            defer_push(!in, (uint32_t)p, a.dlist, a.dcount);
+            if (!in) continue;
+            pvx = a.vx[p];
+            pvy = a.vy[p];
             double e1, e2, b3 = 0.0;
-            if (K == PIC_CIC) {
-                double a1 = (1 - fx) * (1 - fy);
-                double a2 = fx * (1 - fy);
-                double a3 = fx * fy;
-                double a4 = (1 - fx) * fy;
-                double2 E00 = node_e(0, 0), E10 = node_e(1, 0), E11 = node_e(1, 1),
-                        E01 = node_e(0, 1);
-                e1 = a1 * E00.x + a2 * E10.x + a3 * E11.x + a4 * E01.x;
-                e2 = a1 * E00.y + a2 * E10.y + a3 * E11.y + a4 * E01.y;
-                if (BZ == PIC_BZ_GRID)
-                    b3 = a1 * node_b(0, 0) + a2 * node_b(1, 0) + a3 * node_b(1, 1) +
-                         a4 * node_b(0, 1);
-            } else {
-                double wx[6], wy[6];
-                picd::m6_weights_d(fx, wx);
-                picd::m6_weights_d(fy, wy);
-                e1 = 0.0;
-                e2 = 0.0;
-#pragma unroll
-                for (int u = 0; u < 6; ++u) {
-                    double2 E0 = node_e(u, 0);
-                    double t1 = wy[0] * E0.x, t2 = wy[0] * E0.y;
-#pragma unroll
-                    for (int v = 1; v < 6; ++v) {
-                        double2 E = node_e(u, v);
-                        t1 = __builtin_fma(wy[v], E.x, t1);
-                        t2 = __builtin_fma(wy[v], E.y, t2);
-                    }
-                    e1 = __builtin_fma(wx[u], t1, e1);
-                    e2 = __builtin_fma(wx[u], t2, e2);
-                }
-                if (BZ == PIC_BZ_GRID) {
-#pragma unroll
-                    for (int u = 0; u < 6; ++u) {
-                        double t3 = wy[0] * node_b(u, 0);
-#pragma unroll
-                        for (int v = 1; v < 6; ++v) t3 = __builtin_fma(wy[v], node_b(u, v), t3);
-                        b3 = __builtin_fma(wx[u], t3, b3);
-                    }
-                }
-            }
-            if (BZ == PIC_BZ_UNIFORM) b3 = a.bz0;
-            const double dt = a.dt;
-            double v1 = pvx, v2 = pvy;
-            v1 += 0.5 * dt * e1;
-            v2 += 0.5 * dt * e2;
-            if (BZ != PIC_BZ_NONE) {
-                double tantheta = 0.5 * dt * b3;
-                double sintheta = 2 * tantheta / (1 + tantheta * tantheta);
-                v1 += v2 * tantheta;
-                v2 += -v1 * sintheta;
-                v1 += v2 * tantheta;
-            }
-            pvx = v1 + 0.5 * dt * e1;
-            pvy = v2 + 0.5 * dt * e2;
+            interp64<K, BZ>(
+                fx, fy, [&](int u, int v) { return te[(lx + u) + W.tw * (ly + v)]; },
+                [&](int u, int v) { return tb[(lx + u) + W.tw * (ly + v)]; }, e1, e2, b3);
+            kick64<BZ>(a, e1, e2, b3, pvx, pvy);
             a.vx[p] = pvx;
             a.vy[p] = pvy;
-        }
-#pragma unroll
-        for (int r = 0; r < NDRIFT; ++r) {
-            double p1 = px + a.dtd * pvx;
-            double p2 = py + a.dtd * pvy;
-            if (p1 > a.dimx) p1 -= a.dimx;
-            if (p2 > a.dimy) p2 -= a.dimy;
-            if (p1 < 0.0) p1 += a.dimx;
-            if (p2 < 0.0) p2 += a.dimy;
-            px = p1;
-            py = p2;
+        } else if (NDRIFT > 0) {
+            pvx = a.vx[p];
+            pvy = a.vy[p];
         }
         if (NDRIFT > 0) {
+            drift64<NDRIFT>(a, pvx, pvy, px, py);
             a.x[p] = px;
             a.y[p] = py;
         }
@@ -484,35 +656,12 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
             const int cj = ref_cell_s(py, a.dy, ny, K == PIC_CIC, fy);
             const int lx = picd::wrap2(ci - HL - W.x0, nx), ly = picd::wrap2(cj - HL - W.y0, ny);
             const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
-            count_out(!in, a.slow);
-            const double wgt = a.w ? a.w[p] : a.w0;
-            auto add = [&](int u, int v, double q) {
-                if (in)
-                    __hip_atomic_fetch_add(&dacc[(lx + u) + W.tw * (ly + v)], q, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                else
-                    unsafeAtomicAdd(&a.acc[(int64_t)picd::wrap2(ci - HL + u, nx) +
-                                           (int64_t)nx * picd::wrap2(cj - HL + v, ny)],
-                                    q);
-            };
-            if (K == PIC_CIC) {
-                double a1 = (1 - fx) * (1 - fy);
-                double a2 = fx * (1 - fy);
-                double a3 = fx * fy;
-                double a4 = (1 - fx) * fy;
-                add(0, 0, a1 * wgt);
-                add(1, 0, a2 * wgt);
-                add(1, 1, a3 * wgt);
-                add(0, 1, a4 * wgt);
-            } else {
-                double wx[6], wy[6];
-                picd::m6_weights_d(fx, wx);
-                picd::m6_weights_d(fy, wy);
-#pragma unroll
-                for (int u = 0; u < 6; ++u)
-#pragma unroll
-                    for (int v = 0; v < 6; ++v) add(u, v, wx[u] * wy[v] * wgt);
-            }
+            defer_push(!in, (uint32_t)p | DEFER_DEPOSIT_ONLY, a.dlist, a.dcount);
+            if (!in) continue;
+            charge64<K>(fx, fy, a.w ? a.w[p] : a.w0, [&](int u, int v, double q) {
+                __hip_atomic_fetch_add(&dacc[(lx + u) + W.tw * (ly + v)], q, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            });
         }
     }
     if (!DEPOSIT || !fits) return;
@@ -529,44 +678,122 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
     }
 }
 
+// Completes the particles k_seg_f64 deferred, from global memory.
+template <int K, bool GATHER, int NDRIFT, bool DEPOSIT, int BZ>
+__global__ __launch_bounds__(DEFER_T) void k_defer_f64(SegF64 a) {
+    constexpr int HL = SegStencil<K>::HL;
+    const unsigned cnt = __hip_atomic_load(a.dcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int nx = a.nx, ny = a.ny;
+    for (unsigned i = blockIdx.x * DEFER_T + threadIdx.x; i < cnt; i += gridDim.x * DEFER_T) {
+        const uint32_t e = a.dlist[i];
+        const int64_t p = e & ~DEFER_DEPOSIT_ONLY;
+        double px = a.x[p], py = a.y[p];
+        if ((e & DEFER_DEPOSIT_ONLY) == 0) {
+            double pvx = 0.0, pvy = 0.0;
+            if (GATHER || NDRIFT > 0) {
+                pvx = a.vx[p];
+                pvy = a.vy[p];
+            }
+            if (GATHER) {
+                double fx, fy;
+                const int ci = ref_cell_s(px, a.dx, nx, false, fx);
+                const int cj = ref_cell_s(py, a.dy, ny, false, fy);
+                double e1, e2, b3 = 0.0;
+                interp64<K, BZ>(
+                    fx, fy,
+                    [&](int u, int v) {
+                        const int64_t g = (int64_t)picd::wrap2(ci - HL + u, nx) +
+                                          (int64_t)nx * picd::wrap2(cj - HL + v, ny);
+                        return make_double2(a.ex[g], a.ey[g]);
+                    },
+                    [&](int u, int v) {
+                        return a.bz[(int64_t)picd::wrap2(ci - HL + u, nx) +
+                                    (int64_t)nx * picd::wrap2(cj - HL + v, ny)];
+                    },
+                    e1, e2, b3);
+                kick64<BZ>(a, e1, e2, b3, pvx, pvy);
+                a.vx[p] = pvx;
+                a.vy[p] = pvy;
+            }
+            if (NDRIFT > 0) {
+                drift64<NDRIFT>(a, pvx, pvy, px, py);
+                a.x[p] = px;
+                a.y[p] = py;
+            }
+        }
+        if (DEPOSIT) {
+            double fx, fy;
+            const int ci = ref_cell_s(px, a.dx, nx, K == PIC_CIC, fx);
+            const int cj = ref_cell_s(py, a.dy, ny, K == PIC_CIC, fy);
+            charge64<K>(fx, fy, a.w ? a.w[p] : a.w0, [&](int u, int v, double q) {
+                unsafeAtomicAdd(&a.acc[(int64_t)picd::wrap2(ci - HL + u, nx) +
+                                       (int64_t)nx * picd::wrap2(cj - HL + v, ny)],
+                                q);
+            });
+        }
+    }
+    defer_finish(cnt, a.dcount, a.slow);
+}
+
 // ============================================================== dispatch
 
-// LDS per block for the window: FP32 8 B field (+4 B bz) + 8 B charge pair,
-// FP64 16 B field (+8 B bz) + 8 B charge.
-constexpr size_t SEG_LDS_BYTES = 48 * 1024;
+// LDS per block for the window: FP32 8 B field (+4 B bz) + 8 B charge
+// (pair or FP64), FP64 16 B field (+8 B bz) + 8 B charge.  40 KiB lets four
+// 512-thread blocks share a CU's 160 KiB; an M6 window of a 64-cell row
+// segment then keeps a drift margin of 11 cells (FP32) or 7 cells (FP64).
+constexpr size_t SEG_LDS_BYTES = 40 * 1024;
 
-static int seg_bps(const Ctx *c) {
+// Blocks per segment: as many as keep every CU busy in one round of
+// resident blocks (the occupancy of this kernel instantiation at this LDS
+// size), at least one.  A larger grid than one round leaves a partial last
+// round; a grid with more segments than resident blocks takes several full
+// rounds.
+static int seg_bps(const Ctx *c, const void *kernel, size_t sm) {
     const int nseg = (c->nx * c->ny + 63) / 64;
-    int bps = (3 * 256 + nseg - 1) / nseg;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, SEG_T, sm) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    int bps = c->ncu * per_cu / nseg;
     if (bps < 1) bps = 1;
     if (bps > 64) bps = 64;
     return bps;
 }
 
+// Sets the blocks per segment and the FP32 split quantum for kernel `k`,
+// then launches it over every segment.
+template <class Kern, class Defer>
+static void seg32_launch(const Ctx *c, SegF32 a, Kern k, Defer kd, size_t sm) {
+    a.bps = seg_bps(c, reinterpret_cast<const void *>(k), sm);
+    dim3 grid((unsigned)(((c->nx * c->ny + 63) / 64) * a.bps));
+    hipLaunchKernelGGL(k, grid, dim3(SEG_T), sm, c->stream, a);
+    hipLaunchKernelGGL(kd, dim3(DEFER_BLOCKS), dim3(DEFER_T), 0, c->stream, a);
+}
+
 template <int K, bool G, bool DR, bool D, int ACC>
-static void seg32_bz(const Ctx *c, const SegF32 &a, int bz, dim3 grid, size_t sm) {
+static void seg32_bz(const Ctx *c, const SegF32 &a, int bz, size_t sm) {
     switch (bz) {
         case PIC_BZ_NONE:
-            hipLaunchKernelGGL((k_seg_f32<K, G, DR, D, PIC_BZ_NONE, ACC>), grid, dim3(SEG_T), sm,
-                               c->stream, a);
+            seg32_launch(c, a, k_seg_f32<K, G, DR, D, PIC_BZ_NONE, ACC>,
+                         k_defer_f32<K, G, DR, D, PIC_BZ_NONE>, sm);
             break;
         case PIC_BZ_UNIFORM:
-            hipLaunchKernelGGL((k_seg_f32<K, G, DR, D, PIC_BZ_UNIFORM, ACC>), grid, dim3(SEG_T), sm,
-                               c->stream, a);
+            seg32_launch(c, a, k_seg_f32<K, G, DR, D, PIC_BZ_UNIFORM, ACC>,
+                         k_defer_f32<K, G, DR, D, PIC_BZ_UNIFORM>, sm);
             break;
         default:
-            hipLaunchKernelGGL((k_seg_f32<K, G, DR, D, PIC_BZ_GRID, ACC>), grid, dim3(SEG_T), sm,
-                               c->stream, a);
+            seg32_launch(c, a, k_seg_f32<K, G, DR, D, PIC_BZ_GRID, ACC>,
+                         k_defer_f32<K, G, DR, D, PIC_BZ_GRID>, sm);
     }
 }
 
 template <int K, int ACC>
-static int seg32_mode(const Ctx *c, const SegF32 &a, int mode, int bz, dim3 grid, size_t sm) {
+static int seg32_mode(const Ctx *c, const SegF32 &a, int mode, int bz, size_t sm) {
     switch (mode) {
-        case 0: seg32_bz<K, false, true, true, ACC>(c, a, PIC_BZ_NONE, grid, sm); break;
-        case 1: seg32_bz<K, true, true, true, ACC>(c, a, bz, grid, sm); break;
-        case 2: seg32_bz<K, true, true, false, ACC>(c, a, bz, grid, sm); break;
-        case 3: seg32_bz<K, false, false, true, ACC>(c, a, PIC_BZ_NONE, grid, sm); break;
+        case 0: seg32_bz<K, false, true, true, ACC>(c, a, PIC_BZ_NONE, sm); break;
+        case 1: seg32_bz<K, true, true, true, ACC>(c, a, bz, sm); break;
+        case 2: seg32_bz<K, true, true, false, ACC>(c, a, bz, sm); break;
+        case 3: seg32_bz<K, false, false, true, ACC>(c, a, PIC_BZ_NONE, sm); break;
         default: return fail(PIC_ERR_ARG, "pass mode %d", mode);
     }
     PIC_HIP(hipGetLastError());
@@ -593,53 +820,57 @@ int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     a.dtdy = (float)(dtd / c->dy);
     a.acc = c->rho_acc;
     a.off = c->cell_off;
-    a.bps = seg_bps(c);
+    a.bps = 1;
     a.margin = 32;
     const bool gather = mode == 1 || mode == 2;
-    const size_t per_node =
-        (gather ? (c->bz_mode == PIC_BZ_GRID ? 12 : 8) : 0) + (mode == 2 ? 0 : 8);
-    a.cap = (int)(SEG_LDS_BYTES / (per_node ? per_node : 8)) & ~15;
-    // quantum u: the hi tile of a node sums at most (particles per block) *
-    // max(w * w_x * w_y) <= 0.31 w * count; keep that below 2^24 u (and every
-    // single contribution below 2^22 u, where the split is exact)
-    const double bound = 0.31 * std::abs(c->w0) * (double)(c->max_seg_len / a.bps + 5) + 1e-300;
-    const int e = (int)std::ceil(std::log2(bound)) - 24;
-    a.cq = (float)(1.5 * std::ldexp(1.0, e + 23));
-    a.slow = c->slow_chunks;
-    dim3 grid((unsigned)(((c->nx * c->ny + 63) / 64) * a.bps));
-    const size_t sm = (size_t)a.cap * (per_node ? per_node : 8);
     const int acc = c->f32_acc;
+    const size_t per_node = (gather ? (c->bz_mode == PIC_BZ_GRID ? 12 : 8) : 0) +
+                            (mode == 2 ? 0 : 8);
+    a.cap = (int)(SEG_LDS_BYTES / (per_node ? per_node : 8)) & ~15;
+    a.slow = c->slow_chunks;
+    PIC_TRY(ensure_defer(c));
+    a.dlist = c->defer;
+    a.dcount = c->defer_count;
+    const size_t sm = (size_t)a.cap * (per_node ? per_node : 8);
     if (kernel == PIC_CIC)
-        return acc == 0 ? seg32_mode<PIC_CIC, 0>(c, a, mode, c->bz_mode, grid, sm)
-                        : seg32_mode<PIC_CIC, 1>(c, a, mode, c->bz_mode, grid, sm);
-    return acc == 0 ? seg32_mode<PIC_M6, 0>(c, a, mode, c->bz_mode, grid, sm)
-                    : seg32_mode<PIC_M6, 1>(c, a, mode, c->bz_mode, grid, sm);
+        return acc == 0 ? seg32_mode<PIC_CIC, 0>(c, a, mode, c->bz_mode, sm)
+                        : seg32_mode<PIC_CIC, 1>(c, a, mode, c->bz_mode, sm);
+    return acc == 0 ? seg32_mode<PIC_M6, 0>(c, a, mode, c->bz_mode, sm)
+                    : seg32_mode<PIC_M6, 1>(c, a, mode, c->bz_mode, sm);
+}
+
+template <class Kern, class Defer>
+static void seg64_launch(const Ctx *c, SegF64 a, Kern k, Defer kd, size_t sm) {
+    a.bps = seg_bps(c, reinterpret_cast<const void *>(k), sm);
+    dim3 grid((unsigned)(((c->nx * c->ny + 63) / 64) * a.bps));
+    hipLaunchKernelGGL(k, grid, dim3(SEG_T), sm, c->stream, a);
+    hipLaunchKernelGGL(kd, dim3(DEFER_BLOCKS), dim3(DEFER_T), 0, c->stream, a);
 }
 
 template <int K, bool G, int ND, bool D>
-static void seg64_bz(const Ctx *c, const SegF64 &a, int bz, dim3 grid, size_t sm) {
+static void seg64_bz(const Ctx *c, const SegF64 &a, int bz, size_t sm) {
     switch (bz) {
         case PIC_BZ_NONE:
-            hipLaunchKernelGGL((k_seg_f64<K, G, ND, D, PIC_BZ_NONE>), grid, dim3(SEG_T), sm,
-                               c->stream, a);
+            seg64_launch(c, a, k_seg_f64<K, G, ND, D, PIC_BZ_NONE>,
+                         k_defer_f64<K, G, ND, D, PIC_BZ_NONE>, sm);
             break;
         case PIC_BZ_UNIFORM:
-            hipLaunchKernelGGL((k_seg_f64<K, G, ND, D, PIC_BZ_UNIFORM>), grid, dim3(SEG_T), sm,
-                               c->stream, a);
+            seg64_launch(c, a, k_seg_f64<K, G, ND, D, PIC_BZ_UNIFORM>,
+                         k_defer_f64<K, G, ND, D, PIC_BZ_UNIFORM>, sm);
             break;
         default:
-            hipLaunchKernelGGL((k_seg_f64<K, G, ND, D, PIC_BZ_GRID>), grid, dim3(SEG_T), sm,
-                               c->stream, a);
+            seg64_launch(c, a, k_seg_f64<K, G, ND, D, PIC_BZ_GRID>,
+                         k_defer_f64<K, G, ND, D, PIC_BZ_GRID>, sm);
     }
 }
 
 template <int K>
-static int seg64_mode(const Ctx *c, const SegF64 &a, int mode, int bz, dim3 grid, size_t sm) {
+static int seg64_mode(const Ctx *c, const SegF64 &a, int mode, int bz, size_t sm) {
     switch (mode) {
-        case 0: seg64_bz<K, false, 1, true>(c, a, PIC_BZ_NONE, grid, sm); break;
-        case 1: seg64_bz<K, true, 2, true>(c, a, bz, grid, sm); break;
-        case 2: seg64_bz<K, true, 1, false>(c, a, bz, grid, sm); break;
-        case 3: seg64_bz<K, false, 0, true>(c, a, PIC_BZ_NONE, grid, sm); break;
+        case 0: seg64_bz<K, false, 1, true>(c, a, PIC_BZ_NONE, sm); break;
+        case 1: seg64_bz<K, true, 2, true>(c, a, bz, sm); break;
+        case 2: seg64_bz<K, true, 1, false>(c, a, bz, sm); break;
+        case 3: seg64_bz<K, false, 0, true>(c, a, PIC_BZ_NONE, sm); break;
         default: return fail(PIC_ERR_ARG, "pass mode %d", mode);
     }
     PIC_HIP(hipGetLastError());
@@ -659,17 +890,19 @@ int launch_seg64(Ctx *c, int kernel, int mode, double dt, double dtd) {
     a.dt = dt; a.dtd = dtd;
     a.acc = c->rho_acc;
     a.off = c->cell_off;
-    a.bps = seg_bps(c);
+    a.bps = 1;
     a.margin = 32;
     const bool gather = mode == 1 || mode == 2;
     const size_t per_node =
         (gather ? (c->bz_mode == PIC_BZ_GRID ? 24 : 16) : 0) + (mode == 2 ? 0 : 8);
     a.cap = (int)(SEG_LDS_BYTES / (per_node ? per_node : 8)) & ~15;
     a.slow = c->slow_chunks;
-    dim3 grid((unsigned)(((c->nx * c->ny + 63) / 64) * a.bps));
+    PIC_TRY(ensure_defer(c));
+    a.dlist = c->defer;
+    a.dcount = c->defer_count;
     const size_t sm = (size_t)a.cap * (per_node ? per_node : 8);
-    return kernel == PIC_CIC ? seg64_mode<PIC_CIC>(c, a, mode, c->bz_mode, grid, sm)
-                             : seg64_mode<PIC_M6>(c, a, mode, c->bz_mode, grid, sm);
+    return kernel == PIC_CIC ? seg64_mode<PIC_CIC>(c, a, mode, c->bz_mode, sm)
+                             : seg64_mode<PIC_M6>(c, a, mode, c->bz_mode, sm);
 }
 
 }  // namespace pic

@@ -13,8 +13,8 @@
 //                  scatter
 // Equal keys keep their input order in every pass, so the result is the
 // permutation std::stable_sort by cell produces.  The permutation is then
-// applied to the particle state by a gather into a second set of arrays
-// whose pointers are swapped in.
+// applied to the whole particle state by one gather kernel into a second set
+// of arrays whose pointers are swapped in.
 #include <algorithm>
 #include <climits>
 #include <vector>
@@ -196,13 +196,6 @@ __global__ void k_cell_offsets(const uint32_t *__restrict__ key, int64_t n, int 
     for (int c = lo; c <= hi; ++c) off[c] = r;
 }
 
-template <typename T>
-__global__ void k_gather(int64_t n, const uint32_t *__restrict__ perm, const T *__restrict__ in,
-                         T *__restrict__ out) {
-    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n) out[r] = in[perm[r]];
-}
-
 // ---------------------------------------------------------- lane spread
 //
 // After the stable sort by cell, the particles of every segment of 64
@@ -216,18 +209,14 @@ __global__ void k_gather(int64_t n, const uint32_t *__restrict__ perm, const T *
 
 // One wave per segment: the largest cell count of the segment.
 __global__ void k_seg_max(const int64_t *__restrict__ off, int ncells, int nseg,
-                          uint32_t *__restrict__ kmax, uint32_t *__restrict__ len) {
+                          uint32_t *__restrict__ kmax) {
     const int s = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     const int l = threadIdx.x & 63;
     if (s >= nseg) return;
     const int c = 64 * s + l;
     int cnt = c < ncells ? (int)(off[c + 1] - off[c]) : 0;
     for (int o = 32; o > 0; o >>= 1) cnt = max(cnt, __shfl_xor(cnt, o));
-    if (l == 0) {
-        kmax[s] = (uint32_t)cnt;
-        const int ce = min(64 * s + 64, ncells);
-        len[s] = (uint32_t)(off[ce] - off[64 * s]);
-    }
+    if (l == 0) kmax[s] = (uint32_t)cnt;
 }
 
 // One wave per segment: tables A_s(k), M_s(k) at tbl[s] + k.
@@ -341,11 +330,11 @@ static int radix_sort_pairs(Ctx *c, int64_t n, int keybits) {
     return PIC_OK;
 }
 
-// Gathers arr through the sort permutation into its alternate array and
-// swaps the two pointers.  Alternates are allocated once per array (same
-// capacity) and freed with the particles.
+// The alternate array of state array `arr` (allocated once, same capacity,
+// freed with the particles): the sort gathers into it and then swaps the two
+// pointers.
 template <typename T>
-static int permute(Ctx *c, T *&arr, int64_t n, const uint32_t *perm) {
+static int alt_slot(Ctx *c, T *&arr, int *slot_out) {
     void **key = reinterpret_cast<void **>(&arr);
     int slot = -1;
     for (int k = 0; k < Ctx::NALT; ++k)
@@ -359,16 +348,90 @@ static int permute(Ctx *c, T *&arr, int64_t n, const uint32_t *perm) {
         c->alt_of[slot] = key;
         c->alt[slot] = q;
     }
-    T *alt = static_cast<T *>(c->alt[slot]);
-    hipLaunchKernelGGL(k_gather<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
-                       perm, arr, alt);
-    PIC_HIP(hipGetLastError());
-    c->alt[slot] = arr;
-    arr = alt;
+    *slot_out = slot;
     return PIC_OK;
 }
 
-// Lane-spread permutation of the cell-sorted slots into c->perm2.
+template <typename T>
+static void alt_swap(Ctx *c, T *&arr, int slot) {
+    T *alt = static_cast<T *>(c->alt[slot]);
+    c->alt[slot] = arr;
+    arr = alt;
+}
+
+// out_q[r] = in_q[perm[r]] for the state arrays of one particle
+// representation, in one pass (the permutation is read once).  N4 arrays of
+// 4-byte words, N8 of 8-byte words; a null input is skipped.
+struct StateGather {
+    const uint32_t *in4[5];
+    uint32_t *out4[5];
+    const uint64_t *in8[5];
+    uint64_t *out8[5];
+};
+
+template <int N4, int N8>
+__global__ void k_gather_state(int64_t n, const uint32_t *__restrict__ perm, StateGather g) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t src = perm[r];
+#pragma unroll
+    for (int q = 0; q < N4; ++q) g.out4[q][r] = g.in4[q][src];
+#pragma unroll
+    for (int q = 0; q < N8; ++q)
+        if (g.in8[q]) g.out8[q][r] = g.in8[q][src];
+}
+
+// Applies `perm` to the current state (FP32: cell, ox, oy, vx, vy; FP64: x,
+// y, vx, vy and w when weights differ).
+static int permute_state(Ctx *c, int64_t n, const uint32_t *perm) {
+    StateGather g = {};
+    dim3 grid((unsigned)((n + 255) / 256));
+    if (c->state == PIC_STATE_F32) {
+        int sl[5];
+        PIC_TRY(alt_slot(c, c->cell, &sl[0]));
+        PIC_TRY(alt_slot(c, c->ox, &sl[1]));
+        PIC_TRY(alt_slot(c, c->oy, &sl[2]));
+        PIC_TRY(alt_slot(c, c->vx32, &sl[3]));
+        PIC_TRY(alt_slot(c, c->vy32, &sl[4]));
+        void *src[5] = {c->cell, c->ox, c->oy, c->vx32, c->vy32};
+        for (int q = 0; q < 5; ++q) {
+            g.in4[q] = static_cast<const uint32_t *>(src[q]);
+            g.out4[q] = static_cast<uint32_t *>(c->alt[sl[q]]);
+        }
+        hipLaunchKernelGGL((k_gather_state<5, 0>), grid, dim3(256), 0, c->stream, n, perm, g);
+        PIC_HIP(hipGetLastError());
+        alt_swap(c, c->cell, sl[0]);
+        alt_swap(c, c->ox, sl[1]);
+        alt_swap(c, c->oy, sl[2]);
+        alt_swap(c, c->vx32, sl[3]);
+        alt_swap(c, c->vy32, sl[4]);
+    } else {
+        const bool w = !c->uniform_w;
+        int sl[5] = {0, 0, 0, 0, 0};
+        PIC_TRY(alt_slot(c, c->x, &sl[0]));
+        PIC_TRY(alt_slot(c, c->y, &sl[1]));
+        PIC_TRY(alt_slot(c, c->vx, &sl[2]));
+        PIC_TRY(alt_slot(c, c->vy, &sl[3]));
+        if (w) PIC_TRY(alt_slot(c, c->w, &sl[4]));
+        double *src[5] = {c->x, c->y, c->vx, c->vy, w ? c->w : nullptr};
+        for (int q = 0; q < 5; ++q) {
+            g.in8[q] = reinterpret_cast<const uint64_t *>(src[q]);
+            g.out8[q] = src[q] ? static_cast<uint64_t *>(c->alt[sl[q]]) : nullptr;
+        }
+        hipLaunchKernelGGL((k_gather_state<0, 5>), grid, dim3(256), 0, c->stream, n, perm, g);
+        PIC_HIP(hipGetLastError());
+        alt_swap(c, c->x, sl[0]);
+        alt_swap(c, c->y, sl[1]);
+        alt_swap(c, c->vx, sl[2]);
+        alt_swap(c, c->vy, sl[3]);
+        if (w) alt_swap(c, c->w, sl[4]);
+    }
+    return PIC_OK;
+}
+
+// Lane-spread permutation of the cell-sorted slots into c->perm2.  No host
+// synchronisation: the tables hold sum_s max_l count_l <= n entries, so
+// capacity n + 64 always suffices.
 static int lane_spread(Ctx *c) {
     const int64_t n = c->n;
     const int ncells = c->nx * c->ny;
@@ -376,36 +439,24 @@ static int lane_spread(Ctx *c) {
     if (!c->seg_k) {
         PIC_HIP(hipMalloc(&c->seg_k, (size_t)nseg * 4));
         PIC_HIP(hipMalloc(&c->seg_tbl, (size_t)(nseg + 1) * 4));
-        PIC_HIP(hipMalloc(&c->seg_len, (size_t)nseg * 4));
     }
     if (!c->perm2) PIC_HIP(hipMalloc(&c->perm2, (size_t)(c->cap > 0 ? c->cap : 1) * 4));
-    const int wpb = 4;  // waves (segments) per block
-    dim3 gs((unsigned)((nseg + wpb - 1) / wpb)), bs(64 * wpb);
-    hipLaunchKernelGGL(k_seg_max, gs, bs, 0, c->stream, c->cell_off, ncells, nseg, c->seg_k,
-                       c->seg_len);
-    PIC_HIP(hipMemcpyAsync(c->seg_tbl, c->seg_k, (size_t)nseg * 4, hipMemcpyDeviceToDevice,
-                           c->stream));
-    PIC_TRY(scan(c, c->seg_tbl, nseg));
-    uint32_t last[2] = {0, 0};
-    std::vector<uint32_t> lens((size_t)nseg);
-    PIC_HIP(hipMemcpyAsync(&last[0], c->seg_tbl + nseg - 1, 4, hipMemcpyDeviceToHost, c->stream));
-    PIC_HIP(hipMemcpyAsync(&last[1], c->seg_k + nseg - 1, 4, hipMemcpyDeviceToHost, c->stream));
-    PIC_HIP(hipMemcpyAsync(lens.data(), c->seg_len, (size_t)nseg * 4, hipMemcpyDeviceToHost,
-                           c->stream));
-    PIC_HIP(hipStreamSynchronize(c->stream));
-    c->max_seg_len = 0;
-    for (uint32_t v : lens) c->max_seg_len = std::max<int64_t>(c->max_seg_len, v);
-    const int64_t total = (int64_t)last[0] + last[1];
-    if (total > c->seg_cap) {
+    const int64_t need = (c->cap > 0 ? c->cap : 1) + 64;
+    if (need > c->seg_cap) {
         if (c->seg_A) (void)hipFree(c->seg_A);
         if (c->seg_M) (void)hipFree(c->seg_M);
         c->seg_A = nullptr;
         c->seg_M = nullptr;
-        int64_t cap = total + total / 4 + 64;
-        PIC_HIP(hipMalloc(&c->seg_A, (size_t)cap * 4));
-        PIC_HIP(hipMalloc(&c->seg_M, (size_t)cap * 8));
-        c->seg_cap = cap;
+        PIC_HIP(hipMalloc(&c->seg_A, (size_t)need * 4));
+        PIC_HIP(hipMalloc(&c->seg_M, (size_t)need * 8));
+        c->seg_cap = need;
     }
+    const int wpb = 4;  // waves (segments) per block
+    dim3 gs((unsigned)((nseg + wpb - 1) / wpb)), bs(64 * wpb);
+    hipLaunchKernelGGL(k_seg_max, gs, bs, 0, c->stream, c->cell_off, ncells, nseg, c->seg_k);
+    PIC_HIP(hipMemcpyAsync(c->seg_tbl, c->seg_k, (size_t)nseg * 4, hipMemcpyDeviceToDevice,
+                           c->stream));
+    PIC_TRY(scan(c, c->seg_tbl, nseg));
     hipLaunchKernelGGL(k_seg_tables, gs, bs, 0, c->stream, c->cell_off, ncells, nseg, c->seg_k,
                        c->seg_tbl, c->seg_A, c->seg_M);
     hipLaunchKernelGGL(k_lane_dest, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
@@ -442,19 +493,7 @@ int sort_by_cell(Ctx *c, int32_t *perm_host, bool lanes) {
         perm = c->perm2;
         c->seg_valid = true;
     }
-    if (c->state == PIC_STATE_F32) {
-        PIC_TRY(permute(c, c->cell, n, perm));
-        PIC_TRY(permute(c, c->ox, n, perm));
-        PIC_TRY(permute(c, c->oy, n, perm));
-        PIC_TRY(permute(c, c->vx32, n, perm));
-        PIC_TRY(permute(c, c->vy32, n, perm));
-    } else {
-        PIC_TRY(permute(c, c->x, n, perm));
-        PIC_TRY(permute(c, c->y, n, perm));
-        PIC_TRY(permute(c, c->vx, n, perm));
-        PIC_TRY(permute(c, c->vy, n, perm));
-        if (!c->uniform_w) PIC_TRY(permute(c, c->w, n, perm));
-    }
+    PIC_TRY(permute_state(c, n, perm));
     if (perm_host) {
         PIC_HIP(hipMemcpyAsync(perm_host, perm, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
         PIC_HIP(hipStreamSynchronize(c->stream));

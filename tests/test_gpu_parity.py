@@ -15,6 +15,13 @@ pytestmark = pytest.mark.gpu
 RTOL = 1e-12  # FP64 rho and E, normwise relative
 
 
+def slow_count(pic, ctx):
+    """Particles the segment passes completed on their global-memory path."""
+    v = C.c_uint64()
+    pic._lib.call("pic_slow_chunks", ctx.h, C.byref(v))
+    return v.value
+
+
 def nrel(a, b):
     return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
 
@@ -404,6 +411,7 @@ def test_f32_segment_pass_bitwise(gpu, O, kernel, acc, nx, ny):
     kc = pic._lib.PIC_CIC if kernel == "cic" else pic._lib.PIC_M6
     raw = np.zeros((nx, ny), order="F")
     dt = 0.5
+    slow0 = slow_count(pic, ctx)
     for _ in range(3):  # the slots stay; particles drift out of their windows
         pic._lib.call("pic_step_pass", ctx.h, kc, 1, dt, raw.ctypes.data)
         acc_o = O.f32_step(st, nx, ny, kernel, e32, np.float32(dt / 2),
@@ -412,6 +420,10 @@ def test_f32_segment_pass_bitwise(gpu, O, kernel, acc, nx, ny):
         for a, b, name in zip(got, st, ("cell", "ox", "oy", "vx", "vy")):
             assert np.array_equal(a, b), name
         assert nrel(raw, acc_o) < 1e-12
+    if nx * ny > 64 * 64:
+        # windows smaller than the grid: the global-memory fallback of
+        # out-of-window particles ran
+        assert slow_count(pic, ctx) > slow0
     ctx.close()
 
 
@@ -436,7 +448,10 @@ def test_f64_segment_pass(gpu, O, kernel):
     kc = pic._lib.PIC_CIC if kernel == "cic" else pic._lib.PIC_M6
     raw = np.zeros((nx, ny), order="F")
     dt = 0.2
+    slow0 = slow_count(pic, ctx)
     pic._lib.call("pic_step_pass", ctx.h, kc, 1, dt, raw.ctypes.data)
+    if kernel == "m6":
+        assert slow_count(pic, ctx) > slow0  # some deposits left their windows
     got = ctx.get_particles(np.zeros((5, n), order="F"))
     O.push_v(q, kernel, nx, ny, mesh.dx, mesh.dy, ex, ey, None, dt)
     O.push_x(q, mesh.dimx, mesh.dimy, dt / 2)
