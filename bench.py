@@ -49,6 +49,21 @@ def cpu_info():
     return os.cpu_count(), model or platform.processor()
 
 
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed byte-counter profile
+    of this workload (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, each scaled
+    by the same counter on a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_landau_m6_128x128_1e7.txt")
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) >= 3 and f[0] == kernel:
+                return float(f[1]) + float(f[2]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None
+
+
 def copy_probe(mib, reps=5):
     """Plain device copies of `mib` MiB (torch's elementwise copy kernel), the
     calibration workload for the HBM byte counters: a copy of B bytes reads B
@@ -67,7 +82,7 @@ def copy_probe(mib, reps=5):
                       "GBs": 2 * n * 4 / dt / 1e9}), flush=True)
 
 
-def run_path(pic, mesh, p0, state, dt, steps, warmup, sort_every, f32_acc=1):
+def run_path(pic, mesh, p0, state, dt, steps, warmup, sort_every, f32_acc=2):
     """Warmup, then the timed run from the initial state. Returns dict."""
     L = pic._lib
     sim = pic.Simulation(mesh, p0, pic.M6(), state=state)
@@ -106,12 +121,15 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--particles", type=int, default=10_000_000)
-    ap.add_argument("--sort-every", type=int, default=4)
+    ap.add_argument("--sort-every-f32", type=int, default=8,
+                    help="steps between lane sorts on the FP32 path")
+    ap.add_argument("--sort-every-f64", type=int, default=4,
+                    help="steps between lane sorts on the FP64 path")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["f32", "f64", "both"], default="both")
-    ap.add_argument("--f32-acc", type=int, choices=[0, 1], default=1,
-                    help="FP32 charge tiles: 0 compensated FP32 pairs, 1 FP64 sums")
+    ap.add_argument("--f32-acc", type=int, choices=[0, 1, 2], default=2,
+                    help="FP32 charge tiles: 0 compensated FP32 pairs, 1 FP64 sums, 2 int32 fixed point")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run plain device copies of MIB MiB (byte-counter calibration)")
     ap.add_argument("--dump", default="", help="write the energy histories to this .npz")
@@ -141,7 +159,8 @@ def main():
         if args.only not in ("both", name):
             continue
         log(f"bench: {name} path, warmup {args.warmup}, timed {args.steps} steps")
-        res[name] = run_path(pic, mesh, p0, state, dt, args.steps, args.warmup, args.sort_every,
+        sort_every = args.sort_every_f32 if name == "f32" else args.sort_every_f64
+        res[name] = run_path(pic, mesh, p0, state, dt, args.steps, args.warmup, sort_every,
                              args.f32_acc)
         r = res[name]
         log(f"bench: {name} {r['seconds']*1e3:.2f} ms total, pass {r['pass_ms_avg']*1e3:.1f} us,"
@@ -180,12 +199,14 @@ def main():
         "config": {"workload": "landau2d2v-m6-128x128-1e7-dt0.05-400steps", "grid": [nx, ny],
                    "particles": n, "kernel": "M6", "alpha": alpha, "k": kx, "dt": dt,
                    "state": "int32 cell + fp32 offset + fp32 velocity" if head == "f32" else
-                   "fp64 soa", "sort_every": args.sort_every, "parallelism": "single-gpu"},
+                   "fp64 soa", "sort_every": args.sort_every_f32 if head == "f32" else
+                   args.sort_every_f64, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": f"k_seg_{head}<M6> gather+kick+drift+deposit pass"
-                               + (" (FP64 LDS tiles)" if head == "f32" and args.f32_acc == 1
-                                  else ""),
+                               + ({0: " (FP32 pair tiles)", 1: " (FP64 tiles)",
+                                   2: " (int32 fixed-point tiles)"}[args.f32_acc]
+                                  if head == "f32" else " (FP64 tiles)"),
                      "bytes_per_particle": bytes_pp, "avg_us": r["pass_ms_avg"] * 1e3},
     }
     if "f32" in res and "f64" in res:
@@ -193,12 +214,18 @@ def main():
         g32, g64 = gam["f32"][0], gam["f64"][0]
         out["fp64"] = {"value": v64, "ms_per_step": res["f64"]["seconds"] * 1e3 / args.steps,
                        "pass_us": res["f64"]["pass_ms_avg"] * 1e3,
-                       "achieved_GBs": BYTES_F64 * n / (res["f64"]["pass_ms_avg"] * 1e-3) / 1e9}
+                       "achieved_GBs": BYTES_F64 * n / (res["f64"]["pass_ms_avg"] * 1e-3) / 1e9,
+                       "sort_every": args.sort_every_f64,
+                       "deferred_particles": res["f64"]["slow_chunks"]}
         out["speedup_fp32_vs_fp64"] = value / v64
         out["gamma"] = {"fp32": g32, "fp64": g64, "rel_err": abs(g32 - g64) / abs(g64),
                         "analytic": -0.153359}
     out["solve_us"] = r["solve_ms_avg"] * 1e3
-    out["slow_chunks"] = r["slow_chunks"]
+    out["deferred_particles"] = r["slow_chunks"]
+    traffic = pmc_traffic(f"k_seg_{head}")
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic[0]
+        out["roofline"]["traffic_source"] = traffic[1]
 
     if not args.no_cpu:
         import __graft_entry__ as entry2

@@ -219,9 +219,13 @@ int pic_cell_offsets(pic_ctx *ctx, int64_t *offsets);
 int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort_every, double *energy);
 
 /* Charge accumulation of pic_run's FP32 passes in their LDS windows:
- * 0 compensated FP32 (hi, lo) pairs, 1 (default) FP64 accumulation of the
- * FP32 contributions.  The grid totals are FP64 either way.  No reference
- * counterpart. */
+ * 0 compensated FP32 (hi, lo) pairs;
+ * 1 FP64 accumulation of the FP32 contributions;
+ * 2 (default) fixed point: every contribution rounded to a multiple of a
+ *   power-of-two quantum u chosen per block (2^-30 of the block's largest
+ *   possible node sum, error <= u/2 per contribution) and summed exactly as
+ *   int32.
+ * The grid totals are FP64 in every mode.  No reference counterpart. */
 int pic_set_f32_accumulation(pic_ctx *ctx, int mode);
 
 /* One particle pass of pic_run with the fields currently in the context

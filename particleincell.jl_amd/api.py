@@ -456,7 +456,8 @@ class Simulation:
 
     def set_f32_accumulation(self, mode):
         """Charge accumulation of the FP32 state: 0 = compensated FP32
-        (hi, lo) tile pairs, 1 = FP64 tile sums (pic_set_f32_accumulation)."""
+        (hi, lo) tile pairs, 1 = FP64 tile sums, 2 = int32 fixed-point tile
+        sums (pic_set_f32_accumulation)."""
         L.call("pic_set_f32_accumulation", self.ctx.h, int(mode))
 
     def timing(self):

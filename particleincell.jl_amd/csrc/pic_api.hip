@@ -730,7 +730,7 @@ int pic_f90_push_x(const int32_t *nbpart, const double *dimx, const double *dimy
 
 extern "C" int pic_set_f32_accumulation(pic_ctx *ctx, int mode) {
     PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
-    PIC_CHECK(mode == 0 || mode == 1, PIC_ERR_ARG, "accumulation mode %d", mode);
+    PIC_CHECK(mode >= 0 && mode <= 2, PIC_ERR_ARG, "accumulation mode %d", mode);
     ctx_of(ctx)->f32_acc = mode;
     return PIC_OK;
 }

@@ -123,9 +123,10 @@ struct Ctx {
     int graph_key = -1;
 
     // FP32 charge accumulation in the LDS windows: 0 compensated FP32
-    // (hi, lo) pairs, 1 FP64 (the default: gfx950 executes ds_add_f64 some
-    // twenty times faster than ds_add_f32)
-    int f32_acc = 1;
+    // (hi, lo) pairs, 1 FP64, 2 int32 fixed point (the default: the fastest
+    // on gfx950, whose ds_add_f32 is slow and whose 8-byte LDS atomics
+    // conflict on banks twice as often as 4-byte ones)
+    int f32_acc = 2;
 
     // pic_run kernel timing (HIP events on the context stream)
     bool timing = false;

@@ -22,9 +22,14 @@
 // the reference's operations; FP32: the cell-relative step), so results are
 // bit-identical to them up to the order of the charge sums.
 //
-// FP32 charge accumulation, two modes (Ctx::f32_acc):
-//  1 (default) FP64 LDS tiles: every FP32 contribution is added exactly
-//    converted to FP64 (ds_add_f64);
+// FP32 charge accumulation, three modes (Ctx::f32_acc):
+//  2 fixed-point int32 LDS tiles: every contribution is rounded to a multiple
+//    of a power-of-two quantum u (per block, 2^-30 of the largest sum a node
+//    of the block can reach) and summed exactly with ds_add_u32; the flush
+//    converts to FP64.  Four-byte words halve the LDS bank pressure of the
+//    deposit, whose lanes spread over neighbouring rows once particles drift.
+//  1 FP64 LDS tiles: every FP32 contribution is added exactly converted to
+//    FP64 (ds_add_f64);
 //  0 compensated (hi, lo) FP32 tile pairs.  Every contribution q is split as
 //    q_hi = (q + C) - C, q_lo = q - q_hi, with C = 1.5 * 2^23 * u for a
 //    power-of-two quantum u: q_hi is a multiple of u, so the hi tile sums
@@ -32,8 +37,7 @@
 //    block), and q_lo, below u/2, carries the rest.  Both adds are
 //    non-returning LDS atomics (ds_add_f32).  hi + lo is flushed in FP64.
 // Measured on MI355X, ds_add_f32 retires about 0.33 lanes per clock per CU
-// against 7.3 for ds_add_f64 and 13 for ds_add_u32, so mode 1 is both the
-// more accurate and the faster one there.
+// against 7.3 for ds_add_f64 and 13 for ds_add_u32.
 #include <climits>
 #include <cmath>
 
@@ -134,6 +138,23 @@ __device__ __forceinline__ float split_constant(float w, int64_t count) {
     const double m = frexp(bound, &ex);  // bound = m 2^ex, m in [0.5, 1)
     const int e = (m == 0.5 ? ex - 1 : ex) - 24;
     return (float)ldexp(1.5, e + 23);
+}
+
+// Fixed-point tiles: exponent e of the quantum u = 2^e with which a block of
+// `count` particles accumulates its charge as int32 multiples of u.  Every
+// contribution |q| <= qmax |w| (qmax = the largest product of two shape
+// weights: 1 for CIC, (66/120)^2 < 0.31 for M6) is rounded to the nearest
+// multiple of u (error <= u/2); u is large enough that |q| < 2^22 u (so the
+// rounding below is exact integer arithmetic) and that a node's sum stays
+// below 2^31 u (sum |q|/u <= 2^30, plus at most count/2 from the roundings).
+__device__ __forceinline__ int fixed_exponent(float w, double qmax, int64_t count) {
+    const double aw = qmax * fabs((double)w) + 1e-300;
+    int e1, e2;
+    const double m1 = frexp(aw * (double)(count + 5), &e1);
+    const double m2 = frexp(aw, &e2);
+    e1 = (m1 == 0.5 ? e1 - 1 : e1) - 30;  // ceil(log2(.)) - 30
+    e2 = (m2 == 0.5 ? e2 - 1 : e2) - 22;
+    return e1 > e2 ? e1 : e2;
 }
 
 __device__ __forceinline__ int drift1s(float o, float v, float dtd, float &onew) {
@@ -288,12 +309,18 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
     const Win W = seg_window<S, HL>(s, nx, ny, a.ncells, a.margin, a.cap);
     const bool fits = W.tw * W.th <= a.cap;
     const int cap = a.cap;
-    const float cq = ACC == 0 ? split_constant(a.w, p1 - p0) : 0.0f;
+    // ACC 0: the (hi, lo) split constant; ACC 2: 1.5 * 2^23 * u, so that
+    // bits(q + cq) - bits(cq) = q / u rounded to the nearest integer
+    const int qe = ACC == 2 ? fixed_exponent(a.w, K == PIC_CIC ? 1.0 : 0.31, p1 - p0) : 0;
+    const float cq = ACC == 0   ? split_constant(a.w, p1 - p0)
+                     : ACC == 2 ? ldexpf(1.5f, qe + 23)
+                                : 0.0f;
     float2 *te = reinterpret_cast<float2 *>(smem_s32);
     float *tb = reinterpret_cast<float *>(smem_s32 + 8 * (size_t)cap);
     char *dep = smem_s32 + (GATHER ? (BZ == PIC_BZ_GRID ? 12 : 8) * (size_t)cap : 0);
     float *hi = reinterpret_cast<float *>(dep), *lo = hi + cap;
     double *dacc = reinterpret_cast<double *>(dep);
+    int *iacc = reinterpret_cast<int *>(dep);
     if (fits) {
         const int nt = W.tw * W.th;
         for (int t = tid; t < nt; t += SEG_T) {
@@ -307,8 +334,10 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
                 if (ACC == 0) {
                     hi[t] = 0.0f;
                     lo[t] = 0.0f;
-                } else {
+                } else if (ACC == 1) {
                     dacc[t] = 0.0;
+                } else {
+                    iacc[t] = 0;
                 }
             }
         }
@@ -364,8 +393,12 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
                     __hip_atomic_fetch_add(&lo[t], ql, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
-                } else {
+                } else if (ACC == 1) {
                     __hip_atomic_fetch_add(&dacc[t], (double)q, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {
+                    const int qi = __float_as_int(q + cq) - __float_as_int(cq);
+                    __hip_atomic_fetch_add(&iacc[t], qi, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             });
@@ -375,7 +408,9 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
     __syncthreads();
     const int nt = W.tw * W.th;
     for (int t = tid; t < nt; t += SEG_T) {
-        const double q = ACC == 0 ? (double)hi[t] + (double)lo[t] : dacc[t];
+        const double q = ACC == 0   ? (double)hi[t] + (double)lo[t]
+                         : ACC == 1 ? dacc[t]
+                                    : ldexp((double)iacc[t], qe);
         if (q != 0.0) {
             const int u = t % W.tw, v = t / W.tw;
             unsafeAtomicAdd(&a.acc[(int64_t)((W.x0 + u) & mx) + (int64_t)nx * ((W.y0 + v) & my)],
@@ -825,7 +860,7 @@ int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     const bool gather = mode == 1 || mode == 2;
     const int acc = c->f32_acc;
     const size_t per_node = (gather ? (c->bz_mode == PIC_BZ_GRID ? 12 : 8) : 0) +
-                            (mode == 2 ? 0 : 8);
+                            (mode == 2 ? 0 : acc == 2 ? 4 : 8);
     a.cap = (int)(SEG_LDS_BYTES / (per_node ? per_node : 8)) & ~15;
     a.slow = c->slow_chunks;
     PIC_TRY(ensure_defer(c));
@@ -833,10 +868,12 @@ int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     a.dcount = c->defer_count;
     const size_t sm = (size_t)a.cap * (per_node ? per_node : 8);
     if (kernel == PIC_CIC)
-        return acc == 0 ? seg32_mode<PIC_CIC, 0>(c, a, mode, c->bz_mode, sm)
-                        : seg32_mode<PIC_CIC, 1>(c, a, mode, c->bz_mode, sm);
-    return acc == 0 ? seg32_mode<PIC_M6, 0>(c, a, mode, c->bz_mode, sm)
-                    : seg32_mode<PIC_M6, 1>(c, a, mode, c->bz_mode, sm);
+        return acc == 0   ? seg32_mode<PIC_CIC, 0>(c, a, mode, c->bz_mode, sm)
+               : acc == 1 ? seg32_mode<PIC_CIC, 1>(c, a, mode, c->bz_mode, sm)
+                          : seg32_mode<PIC_CIC, 2>(c, a, mode, c->bz_mode, sm);
+    return acc == 0   ? seg32_mode<PIC_M6, 0>(c, a, mode, c->bz_mode, sm)
+           : acc == 1 ? seg32_mode<PIC_M6, 1>(c, a, mode, c->bz_mode, sm)
+                      : seg32_mode<PIC_M6, 2>(c, a, mode, c->bz_mode, sm);
 }
 
 template <class Kern, class Defer>

@@ -379,14 +379,34 @@ def test_f32_step_bitwise(gpu, O, kernel):
     ctx.close()
 
 
-@pytest.mark.parametrize("acc", [0, 1])
+def fixed_point_bound(cell, nx, ny, kernel, w, n):
+    """Largest error the int32 fixed-point tiles (accumulation mode 2) may
+    add to each node: (contributions to the node) * u / 2, with u the
+    quantum of a block holding all n particles (an upper bound on every
+    block's quantum: 2^(max(ceil(log2(qmax w (n + 5))) - 30,
+    ceil(log2(qmax w)) - 22)), qmax = 1 for CIC, 0.31 for M6)."""
+    aw = (1.0 if kernel == "cic" else 0.31) * abs(float(w))
+    e = max(math.ceil(math.log2(aw * (n + 5))) - 30, math.ceil(math.log2(aw)) - 22)
+    counts = np.bincount(cell, minlength=nx * ny).reshape(ny, nx).T  # (i, j)
+    s, hl = (2, 0) if kernel == "cic" else (6, 2)
+    contrib = np.zeros((nx, ny))
+    for du in range(s):
+        for dv in range(s):
+            # a particle in cell (i, j) reaches node (i - hl + du, j - hl + dv)
+            contrib += np.roll(np.roll(counts, du - hl, axis=0), dv - hl, axis=1)
+    return contrib * 2.0 ** e / 2
+
+
+@pytest.mark.parametrize("acc", [0, 1, 2])
 @pytest.mark.parametrize("kernel", ["cic", "m6"])
 @pytest.mark.parametrize("nx,ny", [(128, 128), (64, 32), (32, 64)])
 def test_f32_segment_pass_bitwise(gpu, O, kernel, acc, nx, ny):
     """The segment-persistent FP32 pass of pic_run (after pic_sort_lanes):
-    cells, offsets, velocities bit-identical to the oracle; the charge of the
-    compensated FP32 pairs (acc 0) or FP64 windows (acc 1) within 1e-12.
-    Particles are spread wide enough that some leave their segment window."""
+    cells, offsets, velocities bit-identical to the oracle.  The charge of
+    the compensated FP32 pairs (acc 0) or FP64 windows (acc 1) within 1e-12
+    of the oracle's FP64 sum of the same FP32 contributions; that of the
+    int32 fixed-point windows (acc 2) within the rounding bound of every
+    node.  Particles are spread wide enough that some leave their window."""
     pic = gpu
     rng = np.random.default_rng(41 + nx)
     n = 200000
@@ -419,7 +439,12 @@ def test_f32_segment_pass_bitwise(gpu, O, kernel, acc, nx, ny):
         got = ctx.get_state_f32()
         for a, b, name in zip(got, st, ("cell", "ox", "oy", "vx", "vy")):
             assert np.array_equal(a, b), name
-        assert nrel(raw, acc_o) < 1e-12
+        if acc == 2:
+            bound = fixed_point_bound(st[0], nx, ny, kernel, w, n)
+            assert np.all(np.abs(raw - acc_o) <= bound + 1e-12 * np.abs(acc_o))
+            assert nrel(raw, acc_o) < 1e-6
+        else:
+            assert nrel(raw, acc_o) < 1e-12
     if nx * ny > 64 * 64:
         # windows smaller than the grid: the global-memory fallback of
         # out-of-window particles ran
@@ -467,11 +492,14 @@ def test_f64_segment_pass(gpu, O, kernel):
     ctx.close()
 
 
+@pytest.mark.parametrize("order", ["any", "lanes"])
 @pytest.mark.parametrize("bz", ["none", "uniform", "grid"])
 @pytest.mark.parametrize("kernel", ["cic", "m6"])
-def test_f32_step_bz_bitwise(gpu, O, kernel, bz):
+def test_f32_step_bz_bitwise(gpu, O, kernel, bz, order):
     """The FP32 pass with a magnetic field (uniform, as the strong-B
-    configuration, or a grid): cells, offsets, velocities bit-identical."""
+    configuration, or a grid), through the chunk kernels (any particle order)
+    and the segment kernels (after pic_sort_lanes, FP64 windows): cells,
+    offsets, velocities bit-identical, the deposit within 1e-12."""
     pic = gpu
     rng = np.random.default_rng(39)
     nx, ny, n = 32, 64, 60000
@@ -482,6 +510,11 @@ def test_f32_step_bz_bitwise(gpu, O, kernel, bz):
     ex, ey, bzg = (np.asfortranarray(rng.normal(size=(nx, ny))) for _ in range(3))
     ctx = pic.Context(mesh, capacity=n)
     ctx.set_state_f32(st, float(w))
+    if order == "lanes":
+        perm = np.zeros(n, np.int32)
+        pic._lib.call("pic_sort_lanes", ctx.h, perm.ctypes.data)
+        st = tuple(np.ascontiguousarray(a[perm]) for a in st)
+        pic._lib.call("pic_set_f32_accumulation", ctx.h, 1)
     ctx.set_grid(pic._lib.PIC_GRID_EX, ex)
     ctx.set_grid(pic._lib.PIC_GRID_EY, ey)
     e32 = np.empty(2 * nx * ny, np.float32)
