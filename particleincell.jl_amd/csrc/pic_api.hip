@@ -50,7 +50,7 @@ static void dfree(T *&p) {
 static void free_particles(Ctx *c) {
     dfree(c->x); dfree(c->y); dfree(c->vx); dfree(c->vy); dfree(c->w);
     dfree(c->epx); dfree(c->epy);
-    dfree(c->cell); dfree(c->ox); dfree(c->oy); dfree(c->vx32); dfree(c->vy32);
+    dfree(c->cell); dfree(c->ov32);
     dfree(c->swap);
     dfree(c->defer);
     c->swap_bytes = 0;
@@ -81,10 +81,7 @@ int ensure_particles(Ctx *c, int64_t n) {
 
 int ensure_f32(Ctx *c) {
     PIC_TRY(dalloc(&c->cell, c->cap));
-    PIC_TRY(dalloc(&c->ox, c->cap));
-    PIC_TRY(dalloc(&c->oy, c->cap));
-    PIC_TRY(dalloc(&c->vx32, c->cap));
-    PIC_TRY(dalloc(&c->vy32, c->cap));
+    PIC_TRY(dalloc(&c->ov32, c->cap));
     return PIC_OK;
 }
 
@@ -176,7 +173,7 @@ __global__ void k_weights_differ(const double *__restrict__ p, int64_t n, int *f
 // cell; cells wrap periodically.
 __global__ void k_f64_to_f32(int64_t n, const double *x, const double *y, const double *vx,
                              const double *vy, int nx, int ny, double dx, double dy,
-                             int32_t *cell, float *ox, float *oy, float *ux, float *uy) {
+                             int32_t *cell, float4 *ov) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     double xp = x[k] / dx, yp = y[k] / dy;
@@ -188,23 +185,37 @@ __global__ void k_f64_to_f32(int64_t n, const double *x, const double *y, const 
     i = picd::wrapi(i, nx);
     j = picd::wrapi(j, ny);
     cell[k] = i + nx * j;
-    ox[k] = fx;
-    oy[k] = fy;
-    ux[k] = (float)vx[k];
-    uy[k] = (float)vy[k];
+    ov[k] = make_float4(fx, fy, (float)vx[k], (float)vy[k]);
 }
 
-__global__ void k_f32_to_f64(int64_t n, const int32_t *cell, const float *ox, const float *oy,
-                             const float *ux, const float *uy, int nx, double dx, double dy,
-                             double *x, double *y, double *vx, double *vy) {
+__global__ void k_f32_to_f64(int64_t n, const int32_t *cell, const float4 *ov, int nx, double dx,
+                             double dy, double *x, double *y, double *vx, double *vy) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int c = cell[k];
     int i = c % nx, j = c / nx;
-    x[k] = ((double)i + (double)ox[k]) * dx;
-    y[k] = ((double)j + (double)oy[k]) * dy;
-    vx[k] = (double)ux[k];
-    vy[k] = (double)uy[k];
+    const float4 o = ov[k];
+    x[k] = ((double)i + (double)o.x) * dx;
+    y[k] = ((double)j + (double)o.y) * dy;
+    vx[k] = (double)o.z;
+    vy[k] = (double)o.w;
+}
+
+// (ox, oy, vx, vy) records <-> four planes of n floats
+__global__ void k_ov_unpack(int64_t n, const float4 *ov, float *planes) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float4 o = ov[k];
+    planes[k] = o.x;
+    planes[n + k] = o.y;
+    planes[2 * n + k] = o.z;
+    planes[3 * n + k] = o.w;
+}
+
+__global__ void k_ov_pack(int64_t n, const float *planes, float4 *ov) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    ov[k] = make_float4(planes[k], planes[n + k], planes[2 * n + k], planes[3 * n + k]);
 }
 
 static inline dim3 grid1(int64_t n, int t = 256) {
@@ -226,12 +237,10 @@ int to_state(Ctx *c, int state) {
                   c->ny);
         PIC_TRY(ensure_f32(c));
         hipLaunchKernelGGL(k_f64_to_f32, grid1(c->n), dim3(256), 0, c->stream, c->n, c->x, c->y,
-                           c->vx, c->vy, c->nx, c->ny, c->dx, c->dy, c->cell, c->ox, c->oy,
-                           c->vx32, c->vy32);
+                           c->vx, c->vy, c->nx, c->ny, c->dx, c->dy, c->cell, c->ov32);
     } else {
         hipLaunchKernelGGL(k_f32_to_f64, grid1(c->n), dim3(256), 0, c->stream, c->n, c->cell,
-                           c->ox, c->oy, c->vx32, c->vy32, c->nx, c->dx, c->dy, c->x, c->y,
-                           c->vx, c->vy);
+                           c->ov32, c->nx, c->dx, c->dy, c->x, c->y, c->vx, c->vy);
     }
     PIC_HIP(hipGetLastError());
     c->state = state;
@@ -529,12 +538,20 @@ int pic_get_state_f32(pic_ctx *ctx, int32_t *cell, float *ox, float *oy, float *
     Ctx *c = ctx_of(ctx);
     PIC_CHECK(c->state == PIC_STATE_F32, PIC_ERR_STATE, "context is not in FP32 state");
     PIC_HIP(hipSetDevice(c->device));
-    size_t b = (size_t)c->n * 4;
+    const int64_t n = c->n;
+    size_t b = (size_t)n * 4;
     if (cell) PIC_HIP(hipMemcpyAsync(cell, c->cell, b, hipMemcpyDeviceToHost, c->stream));
-    if (ox) PIC_HIP(hipMemcpyAsync(ox, c->ox, b, hipMemcpyDeviceToHost, c->stream));
-    if (oy) PIC_HIP(hipMemcpyAsync(oy, c->oy, b, hipMemcpyDeviceToHost, c->stream));
-    if (vx) PIC_HIP(hipMemcpyAsync(vx, c->vx32, b, hipMemcpyDeviceToHost, c->stream));
-    if (vy) PIC_HIP(hipMemcpyAsync(vy, c->vy32, b, hipMemcpyDeviceToHost, c->stream));
+    if (n > 0 && (ox || oy || vx || vy)) {
+        PIC_TRY(ensure_swap(c, (int64_t)n * 16));
+        float *planes = static_cast<float *>(c->swap);
+        hipLaunchKernelGGL(k_ov_unpack, grid1(n), dim3(256), 0, c->stream, n, c->ov32, planes);
+        PIC_HIP(hipGetLastError());
+        float *dst[4] = {ox, oy, vx, vy};
+        for (int q = 0; q < 4; ++q)
+            if (dst[q])
+                PIC_HIP(hipMemcpyAsync(dst[q], planes + (size_t)q * n, b, hipMemcpyDeviceToHost,
+                                       c->stream));
+    }
     PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
 }
@@ -560,11 +577,17 @@ int pic_set_state_f32(pic_ctx *ctx, const int32_t *cell, const float *ox, const 
     c->uniform_w = true;
     c->w0 = w;
     size_t b = (size_t)n * 4;
-    PIC_HIP(hipMemcpyAsync(c->cell, cell, b, hipMemcpyHostToDevice, c->stream));
-    PIC_HIP(hipMemcpyAsync(c->ox, ox, b, hipMemcpyHostToDevice, c->stream));
-    PIC_HIP(hipMemcpyAsync(c->oy, oy, b, hipMemcpyHostToDevice, c->stream));
-    PIC_HIP(hipMemcpyAsync(c->vx32, vx, b, hipMemcpyHostToDevice, c->stream));
-    PIC_HIP(hipMemcpyAsync(c->vy32, vy, b, hipMemcpyHostToDevice, c->stream));
+    if (n > 0) {
+        PIC_HIP(hipMemcpyAsync(c->cell, cell, b, hipMemcpyHostToDevice, c->stream));
+        PIC_TRY(ensure_swap(c, (int64_t)n * 16));
+        float *planes = static_cast<float *>(c->swap);
+        const float *src[4] = {ox, oy, vx, vy};
+        for (int q = 0; q < 4; ++q)
+            PIC_HIP(hipMemcpyAsync(planes + (size_t)q * n, src[q], b, hipMemcpyHostToDevice,
+                                   c->stream));
+        hipLaunchKernelGGL(k_ov_pack, grid1(n), dim3(256), 0, c->stream, n, planes, c->ov32);
+        PIC_HIP(hipGetLastError());
+    }
     PIC_HIP(hipStreamSynchronize(c->stream));
     c->state = PIC_STATE_F32;
     c->seg_valid = false;

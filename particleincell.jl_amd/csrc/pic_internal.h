@@ -61,8 +61,9 @@ struct Ctx {
     double w0 = 0.0;                    // the common weight when uniform_w
     double *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr, *w = nullptr;
     double *epx = nullptr, *epy = nullptr;  // particle fields (gather results)
+    // FP32 cell-relative state: cell index and (ox, oy, vx, vy) per particle
     int32_t *cell = nullptr;
-    float *ox = nullptr, *oy = nullptr, *vx32 = nullptr, *vy32 = nullptr;
+    float4 *ov32 = nullptr;
     void *swap = nullptr;               // 40 B/particle reorder scratch
     int64_t swap_bytes = 0;
     // second set of state arrays: the sort gathers into these and swaps the

@@ -113,7 +113,7 @@ __device__ __forceinline__ void defer_finish(unsigned cnt, unsigned *dcount,
 
 struct SegF32 {
     int32_t *cell;
-    float *ox, *oy, *vx, *vy;
+    float4 *ov;  // (ox, oy, vx, vy)
     float w;
     const float2 *e32;
     const float *bz;
@@ -347,25 +347,18 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
     for (int64_t p = p0 + tid; p < p1; p += SEG_T) {
         const int c = a.cell[p];
         int ci = c & mx, cj = c >> a.lognx;
-        float fx = a.ox[p], fy = a.oy[p];
-        float ux = 0.0f, uy = 0.0f;
+        const float4 o = a.ov[p];
+        float fx = o.x, fy = o.y, ux = o.z, uy = o.w;
         if (GATHER) {
             const int lx = (ci - HL - W.x0) & mx, ly = (cj - HL - W.y0) & my;
             const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
             defer_push(!in, (uint32_t)p, a.dlist, a.dcount);
             if (!in) continue;
-            ux = a.vx[p];
-            uy = a.vy[p];
             float e1, e2, b3 = 0.0f;
             interp32<K, BZ>(
                 fx, fy, [&](int u, int v) { return te[(lx + u) + W.tw * (ly + v)]; },
                 [&](int u, int v) { return tb[(lx + u) + W.tw * (ly + v)]; }, e1, e2, b3);
             kick32<BZ>(a, e1, e2, b3, ux, uy);
-            a.vx[p] = ux;
-            a.vy[p] = uy;
-        } else if (DRIFT) {
-            ux = a.vx[p];
-            uy = a.vy[p];
         }
         if (DRIFT) {
             float nfx, nfy;
@@ -376,9 +369,8 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
             fx = nfx;
             fy = nfy;
             a.cell[p] = ci + nx * cj;
-            a.ox[p] = nfx;
-            a.oy[p] = nfy;
         }
+        if (GATHER || DRIFT) a.ov[p] = make_float4(fx, fy, ux, uy);
         if (DEPOSIT) {
             const int lx = (ci - HL - W.x0) & mx, ly = (cj - HL - W.y0) & my;
             const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
@@ -435,13 +427,10 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f32(SegF32 a) {
         const bool whole = (e & DEFER_DEPOSIT_ONLY) == 0;
         const int c = a.cell[p];
         int ci = c & mx, cj = c >> a.lognx;
-        float fx = a.ox[p], fy = a.oy[p];
+        const float4 o = a.ov[p];
+        float fx = o.x, fy = o.y;
         if (whole) {
-            float ux = 0.0f, uy = 0.0f;
-            if (GATHER || DRIFT) {
-                ux = a.vx[p];
-                uy = a.vy[p];
-            }
+            float ux = o.z, uy = o.w;
             if (GATHER) {
                 float e1, e2, b3 = 0.0f;
                 interp32<K, BZ>(
@@ -456,8 +445,6 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f32(SegF32 a) {
                     },
                     e1, e2, b3);
                 kick32<BZ>(a, e1, e2, b3, ux, uy);
-                a.vx[p] = ux;
-                a.vy[p] = uy;
             }
             if (DRIFT) {
                 float nfx, nfy;
@@ -468,9 +455,8 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f32(SegF32 a) {
                 fx = nfx;
                 fy = nfy;
                 a.cell[p] = ci + nx * cj;
-                a.ox[p] = nfx;
-                a.oy[p] = nfy;
             }
+            if (GATHER || DRIFT) a.ov[p] = make_float4(fx, fy, ux, uy);
         }
         if (DEPOSIT)
             charge32<K>(fx, fy, a.w, [&](int u, int v, float q) {
@@ -841,7 +827,7 @@ int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     if (c->n == 0) return PIC_OK;
     PIC_CHECK(c->seg_valid, PIC_ERR_STATE, "segment passes need a lane sort first");
     SegF32 a;
-    a.cell = c->cell; a.ox = c->ox; a.oy = c->oy; a.vx = c->vx32; a.vy = c->vy32;
+    a.cell = c->cell; a.ov = c->ov32;
     a.w = (float)c->w0;
     a.e32 = reinterpret_cast<const float2 *>(c->e32);
     a.bz = c->bz32;

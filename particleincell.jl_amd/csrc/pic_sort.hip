@@ -360,22 +360,29 @@ static void alt_swap(Ctx *c, T *&arr, int slot) {
 }
 
 // out_q[r] = in_q[perm[r]] for the state arrays of one particle
-// representation, in one pass (the permutation is read once).  N4 arrays of
-// 4-byte words, N8 of 8-byte words; a null input is skipped.
+// representation, in one pass (the permutation is read once).  The FP32
+// state is the cell index and one 16-byte (ox, oy, vx, vy) record; the FP64
+// state N8 arrays of 8-byte words (a null input is skipped).
 struct StateGather {
-    const uint32_t *in4[5];
-    uint32_t *out4[5];
     const uint64_t *in8[5];
     uint64_t *out8[5];
 };
 
-template <int N4, int N8>
+__global__ void k_gather_f32state(int64_t n, const uint32_t *__restrict__ perm,
+                                  const int32_t *__restrict__ cell_in, int32_t *__restrict__ cell_out,
+                                  const float4 *__restrict__ ov_in, float4 *__restrict__ ov_out) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t src = perm[r];
+    cell_out[r] = cell_in[src];
+    ov_out[r] = ov_in[src];
+}
+
+template <int N8>
 __global__ void k_gather_state(int64_t n, const uint32_t *__restrict__ perm, StateGather g) {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const uint32_t src = perm[r];
-#pragma unroll
-    for (int q = 0; q < N4; ++q) g.out4[q][r] = g.in4[q][src];
 #pragma unroll
     for (int q = 0; q < N8; ++q)
         if (g.in8[q]) g.out8[q][r] = g.in8[q][src];
@@ -387,24 +394,15 @@ static int permute_state(Ctx *c, int64_t n, const uint32_t *perm) {
     StateGather g = {};
     dim3 grid((unsigned)((n + 255) / 256));
     if (c->state == PIC_STATE_F32) {
-        int sl[5];
+        int sl[2];
         PIC_TRY(alt_slot(c, c->cell, &sl[0]));
-        PIC_TRY(alt_slot(c, c->ox, &sl[1]));
-        PIC_TRY(alt_slot(c, c->oy, &sl[2]));
-        PIC_TRY(alt_slot(c, c->vx32, &sl[3]));
-        PIC_TRY(alt_slot(c, c->vy32, &sl[4]));
-        void *src[5] = {c->cell, c->ox, c->oy, c->vx32, c->vy32};
-        for (int q = 0; q < 5; ++q) {
-            g.in4[q] = static_cast<const uint32_t *>(src[q]);
-            g.out4[q] = static_cast<uint32_t *>(c->alt[sl[q]]);
-        }
-        hipLaunchKernelGGL((k_gather_state<5, 0>), grid, dim3(256), 0, c->stream, n, perm, g);
+        PIC_TRY(alt_slot(c, c->ov32, &sl[1]));
+        hipLaunchKernelGGL(k_gather_f32state, grid, dim3(256), 0, c->stream, n, perm, c->cell,
+                           static_cast<int32_t *>(c->alt[sl[0]]), c->ov32,
+                           static_cast<float4 *>(c->alt[sl[1]]));
         PIC_HIP(hipGetLastError());
         alt_swap(c, c->cell, sl[0]);
-        alt_swap(c, c->ox, sl[1]);
-        alt_swap(c, c->oy, sl[2]);
-        alt_swap(c, c->vx32, sl[3]);
-        alt_swap(c, c->vy32, sl[4]);
+        alt_swap(c, c->ov32, sl[1]);
     } else {
         const bool w = !c->uniform_w;
         int sl[5] = {0, 0, 0, 0, 0};
@@ -418,7 +416,7 @@ static int permute_state(Ctx *c, int64_t n, const uint32_t *perm) {
             g.in8[q] = reinterpret_cast<const uint64_t *>(src[q]);
             g.out8[q] = src[q] ? static_cast<uint64_t *>(c->alt[sl[q]]) : nullptr;
         }
-        hipLaunchKernelGGL((k_gather_state<0, 5>), grid, dim3(256), 0, c->stream, n, perm, g);
+        hipLaunchKernelGGL((k_gather_state<5>), grid, dim3(256), 0, c->stream, n, perm, g);
         PIC_HIP(hipGetLastError());
         alt_swap(c, c->x, sl[0]);
         alt_swap(c, c->y, sl[1]);

@@ -60,7 +60,7 @@ struct ArgsF64 {
 struct ArgsF32 {
     int64_t n;
     int32_t *cell;
-    float *ox, *oy, *vx, *vy;
+    float4 *ov;       // (ox, oy, vx, vy)
     float w;
     const float2 *e32;
     const float *bz;
@@ -462,12 +462,11 @@ __global__ __launch_bounds__(ST) void k_step_f32(ArgsF32 a) {
         int c = 0;
         if (live[k]) {
             c = a.cell[p];
-            fx[k] = a.ox[p];
-            fy[k] = a.oy[p];
-            if (GATHER || DRIFT) {
-                ux[k] = a.vx[p];
-                uy[k] = a.vy[p];
-            }
+            const float4 o = a.ov[p];
+            fx[k] = o.x;
+            fy[k] = o.y;
+            ux[k] = o.z;
+            uy[k] = o.w;
         }
         ci[k] = c & (nx - 1);
         cj[k] = c >> a.lognx;
@@ -601,13 +600,8 @@ __global__ __launch_bounds__(ST) void k_step_f32(ArgsF32 a) {
             fx[k] = nfx;
             fy[k] = nfy;
             a.cell[p] = ci[k] + nx * cj[k];
-            a.ox[p] = nfx;
-            a.oy[p] = nfy;
         }
-        if (GATHER) {
-            a.vx[p] = ux[k];
-            a.vy[p] = uy[k];
-        }
+        if (GATHER || DRIFT) a.ov[p] = make_float4(fx[k], fy[k], ux[k], uy[k]);
     }
     if (!DEPOSIT) return;
 
@@ -774,7 +768,7 @@ int launch_step32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     if (c->n == 0) return PIC_OK;
     ArgsF32 a;
     a.n = c->n;
-    a.cell = c->cell; a.ox = c->ox; a.oy = c->oy; a.vx = c->vx32; a.vy = c->vy32;
+    a.cell = c->cell; a.ov = c->ov32;
     a.w = (float)c->w0;
     a.e32 = reinterpret_cast<const float2 *>(c->e32);
     a.bz = c->bz32;
