@@ -32,42 +32,33 @@
  *   120 w3 = P3(t) = 26+50t+20t^2-20t^3-20t^4+10t^5 = f(1-t)
  *   120 w4 = P4(t) = 1+5t+10t^2+10t^3+5t^4-5t^5     = f(2-t)
  *   120 w5 = t^5                         = f(3-t)
- * with each coefficient pre-divided by 120 and rounded to float. */
-#define C120(v) ((float)((v) / 120.0))
-
-static inline float p3(float u)
-{
-    float r = C120(10.0);
-    r = fmaf(r, u, C120(-20.0));
-    r = fmaf(r, u, C120(-20.0));
-    r = fmaf(r, u, C120(20.0));
-    r = fmaf(r, u, C120(50.0));
-    r = fmaf(r, u, C120(26.0));
-    return r;
-}
-
-static inline float p4(float u)
-{
-    float r = C120(-5.0);
-    r = fmaf(r, u, C120(5.0));
-    r = fmaf(r, u, C120(10.0));
-    r = fmaf(r, u, C120(10.0));
-    r = fmaf(r, u, C120(5.0));
-    r = fmaf(r, u, C120(1.0));
-    return r;
-}
+ * Written in z = t - 1/2, each mirrored pair is
+ *   w5, w0 = E0(z^2) +/- z Q0(z^2),  E0 = 1/3840 + z^2/96 + z^4/48,
+ *                                    Q0 = 1/384 + z^2/48 + z^4/120,
+ *   w4, w1 = E1 +/- z Q1,            E1 = 79/1280 + 7/32 z^2 - z^4/16,
+ *                                    Q1 = 25/128 + z^2/16 - z^4/24,
+ *   w3, w2 = E2 +/- z Q2,            E2 = 841/1920 - 11/48 z^2 + z^4/24,
+ *                                    Q2 = 77/192 - 7/24 z^2 + z^4/12,
+ * (the binomial expansion of the pieces about t = 1/2), coefficients rounded
+ * to float, evaluated with fused multiply-adds in this order. */
+#define QF(num, den) ((float)((double)(num) / (double)(den)))
 
 void orf_m6_weights(float t, float w[6])
 {
-    float s = 1.0f - t;
-    float t2 = t * t, s2 = s * s;
-    float t4 = t2 * t2, s4 = s2 * s2;
-    w[0] = (s4 * s) * C120(1.0);
-    w[1] = p4(s);
-    w[2] = p3(s);
-    w[3] = p3(t);
-    w[4] = p4(t);
-    w[5] = (t4 * t) * C120(1.0);
+    const float z = t - 0.5f, z2 = z * z, mz = -z;
+    float e, q;
+    e = fmaf(fmaf(QF(1, 48), z2, QF(1, 96)), z2, QF(1, 3840));
+    q = fmaf(fmaf(QF(1, 120), z2, QF(1, 48)), z2, QF(1, 384));
+    w[5] = fmaf(z, q, e);
+    w[0] = fmaf(mz, q, e);
+    e = fmaf(fmaf(QF(-1, 16), z2, QF(7, 32)), z2, QF(79, 1280));
+    q = fmaf(fmaf(QF(-1, 24), z2, QF(1, 16)), z2, QF(25, 128));
+    w[4] = fmaf(z, q, e);
+    w[1] = fmaf(mz, q, e);
+    e = fmaf(fmaf(QF(1, 24), z2, QF(-11, 48)), z2, QF(841, 1920));
+    q = fmaf(fmaf(QF(1, 12), z2, QF(-7, 24)), z2, QF(77, 192));
+    w[3] = fmaf(z, q, e);
+    w[2] = fmaf(mz, q, e);
 }
 
 static inline void cic_weights(float ox, float oy, float a[4])

@@ -190,37 +190,29 @@ __host__ __device__ inline long long mod1z(long long x, long long n) {
 }
 
 // Quintic B-spline (M6) weights of nodes i-2..i+3 at offset t in [0,1), FP32.
-// Expanded pieces of f_m6 (compute_rho_m6.jl:10-25), coefficients divided by
-// 120 and rounded to float; evaluated by Horner with fused multiply-adds.
-#define PIC_C120(v) ((float)((v) / 120.0))
-__device__ __forceinline__ float m6_p3f(float u) {
-    float r = PIC_C120(10.0);
-    r = __builtin_fmaf(r, u, PIC_C120(-20.0));
-    r = __builtin_fmaf(r, u, PIC_C120(-20.0));
-    r = __builtin_fmaf(r, u, PIC_C120(20.0));
-    r = __builtin_fmaf(r, u, PIC_C120(50.0));
-    r = __builtin_fmaf(r, u, PIC_C120(26.0));
-    return r;
-}
-__device__ __forceinline__ float m6_p4f(float u) {
-    float r = PIC_C120(-5.0);
-    r = __builtin_fmaf(r, u, PIC_C120(5.0));
-    r = __builtin_fmaf(r, u, PIC_C120(10.0));
-    r = __builtin_fmaf(r, u, PIC_C120(10.0));
-    r = __builtin_fmaf(r, u, PIC_C120(5.0));
-    r = __builtin_fmaf(r, u, PIC_C120(1.0));
-    return r;
-}
+// The pieces of f_m6 (compute_rho_m6.jl:10-25) are 120 w5 = t^5, 120 w4 =
+// P4(t) = 1+5t+10t^2+10t^3+5t^4-5t^5, 120 w3 = P3(t) =
+// 26+50t+20t^2-20t^3-20t^4+10t^5 and w_k(t) = w_(5-k)(1-t).  In z = t - 1/2
+// each mirrored pair is w_(5-k), w_k = E_k(z^2) +/- z Q_k(z^2) with quadratic
+// E_k, Q_k (exact rational coefficients rounded to float), so the six weights
+// cost 20 operations: z, z^2, then per pair two Horner steps for E and Q and
+// two fused multiply-adds.
+#define PIC_F(num, den) ((float)((double)(num) / (double)(den)))
 __device__ __forceinline__ void m6_weights_f(float t, float w[6]) {
-    float s = 1.0f - t;
-    float t2 = t * t, s2 = s * s;
-    float t4 = t2 * t2, s4 = s2 * s2;
-    w[0] = (s4 * s) * PIC_C120(1.0);
-    w[1] = m6_p4f(s);
-    w[2] = m6_p3f(s);
-    w[3] = m6_p3f(t);
-    w[4] = m6_p4f(t);
-    w[5] = (t4 * t) * PIC_C120(1.0);
+    const float z = t - 0.5f, z2 = z * z, mz = -z;
+    float e, q;
+    e = __builtin_fmaf(__builtin_fmaf(PIC_F(1, 48), z2, PIC_F(1, 96)), z2, PIC_F(1, 3840));
+    q = __builtin_fmaf(__builtin_fmaf(PIC_F(1, 120), z2, PIC_F(1, 48)), z2, PIC_F(1, 384));
+    w[5] = __builtin_fmaf(z, q, e);
+    w[0] = __builtin_fmaf(mz, q, e);
+    e = __builtin_fmaf(__builtin_fmaf(PIC_F(-1, 16), z2, PIC_F(7, 32)), z2, PIC_F(79, 1280));
+    q = __builtin_fmaf(__builtin_fmaf(PIC_F(-1, 24), z2, PIC_F(1, 16)), z2, PIC_F(25, 128));
+    w[4] = __builtin_fmaf(z, q, e);
+    w[1] = __builtin_fmaf(mz, q, e);
+    e = __builtin_fmaf(__builtin_fmaf(PIC_F(1, 24), z2, PIC_F(-11, 48)), z2, PIC_F(841, 1920));
+    q = __builtin_fmaf(__builtin_fmaf(PIC_F(1, 12), z2, PIC_F(-7, 24)), z2, PIC_F(77, 192));
+    w[3] = __builtin_fmaf(z, q, e);
+    w[2] = __builtin_fmaf(mz, q, e);
 }
 
 // The same six weights in FP64, exact integer coefficients, divided by 120 at

@@ -271,6 +271,36 @@ __device__ __forceinline__ void charge32(float fx, float fy, float w, F &&add) {
     }
 }
 
+// The int32 fixed-point charges of a particle's stencil nodes, in units of
+// the quantum u = 2^qe: add(u, v, n) with n = the exact product of the two
+// shape factors times w/u, rounded to the nearest integer by one fused
+// multiply-add onto 1.5 * 2^23 (the float grid there is the integers; the
+// products stay below 2^22 by the choice of u).
+template <int K, class F>
+__device__ __forceinline__ void charge32_fixed(float fx, float fy, float wu, F &&add) {
+    constexpr float C = 12582912.0f;  // 1.5 * 2^23
+    auto n = [&](float a, float b) {
+        return __float_as_int(__builtin_fmaf(a, b, C)) - __float_as_int(C);
+    };
+    if (K == PIC_CIC) {
+        const float sx = 1.0f - fx, sy = (1.0f - fy) * wu, ty = fy * wu;
+        add(0, 0, n(sx, sy));
+        add(1, 0, n(fx, sy));
+        add(1, 1, n(fx, ty));
+        add(0, 1, n(sx, ty));
+    } else {
+        float wx[6], wy[6];
+        picd::m6_weights_f(fx, wx);
+        picd::m6_weights_f(fy, wy);
+#pragma unroll
+        for (int v = 0; v < 6; ++v) wy[v] = wy[v] * wu;
+#pragma unroll
+        for (int u = 0; u < 6; ++u)
+#pragma unroll
+            for (int v = 0; v < 6; ++v) add(u, v, n(wx[u], wy[v]));
+    }
+}
+
 // Appends `entry` for every lane with `want` set (one atomic per wave).  The
 // leader's slot is broadcast with v_readlane, not a shuffle: a shuffle is an
 // LDS instruction, and waiting for its result would also wait for every LDS
@@ -309,12 +339,11 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
     const Win W = seg_window<S, HL>(s, nx, ny, a.ncells, a.margin, a.cap);
     const bool fits = W.tw * W.th <= a.cap;
     const int cap = a.cap;
-    // ACC 0: the (hi, lo) split constant; ACC 2: 1.5 * 2^23 * u, so that
-    // bits(q + cq) - bits(cq) = q / u rounded to the nearest integer
+    // ACC 0: the (hi, lo) split constant; ACC 2: the fixed-point quantum
+    // 2^qe and w in units of it (exact: a power-of-two scaling)
+    const float cq = ACC == 0 ? split_constant(a.w, p1 - p0) : 0.0f;
     const int qe = ACC == 2 ? fixed_exponent(a.w, K == PIC_CIC ? 1.0 : 0.31, p1 - p0) : 0;
-    const float cq = ACC == 0   ? split_constant(a.w, p1 - p0)
-                     : ACC == 2 ? ldexpf(1.5f, qe + 23)
-                                : 0.0f;
+    const float wu = ACC == 2 ? ldexpf(a.w, -qe) : 0.0f;
     float2 *te = reinterpret_cast<float2 *>(smem_s32);
     float *tb = reinterpret_cast<float *>(smem_s32 + 8 * (size_t)cap);
     char *dep = smem_s32 + (GATHER ? (BZ == PIC_BZ_GRID ? 12 : 8) * (size_t)cap : 0);
@@ -344,7 +373,7 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
     }
     __syncthreads();
 
-    for (int64_t p = p0 + tid; p < p1; p += SEG_T) {
+    for (uint32_t p = (uint32_t)p0 + tid; p < (uint32_t)p1; p += SEG_T) {
         const int c = a.cell[p];
         int ci = c & mx, cj = c >> a.lognx;
         const float4 o = a.ov[p];
@@ -376,6 +405,13 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
             const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
             defer_push(!in, (uint32_t)p | DEFER_DEPOSIT_ONLY, a.dlist, a.dcount);
             if (!in) continue;
+            if (ACC == 2) {
+                charge32_fixed<K>(fx, fy, wu, [&](int u, int v, int qi) {
+                    __hip_atomic_fetch_add(&iacc[(lx + u) + W.tw * (ly + v)], qi,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                });
+                continue;
+            }
             charge32<K>(fx, fy, a.w, [&](int u, int v, float q) {
                 const int t = (lx + u) + W.tw * (ly + v);
                 if (ACC == 0) {
@@ -385,12 +421,8 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
                     __hip_atomic_fetch_add(&lo[t], ql, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
-                } else if (ACC == 1) {
-                    __hip_atomic_fetch_add(&dacc[t], (double)q, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
                 } else {
-                    const int qi = __float_as_int(q + cq) - __float_as_int(cq);
-                    __hip_atomic_fetch_add(&iacc[t], qi, __ATOMIC_RELAXED,
+                    __hip_atomic_fetch_add(&dacc[t], (double)q, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             });
@@ -642,7 +674,7 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
     }
     __syncthreads();
 
-    for (int64_t p = p0 + tid; p < p1; p += SEG_T) {
+    for (uint32_t p = (uint32_t)p0 + tid; p < (uint32_t)p1; p += SEG_T) {
         double px = a.x[p], py = a.y[p];
         double pvx = 0.0, pvy = 0.0;
         if (GATHER) {

@@ -381,10 +381,13 @@ def test_f32_step_bitwise(gpu, O, kernel):
 
 def fixed_point_bound(cell, nx, ny, kernel, w, n):
     """Largest error the int32 fixed-point tiles (accumulation mode 2) may
-    add to each node: (contributions to the node) * u / 2, with u the
+    add to each node, against the oracle's FP64 sum of FP32 contributions:
+    (contributions to the node) * (u / 2 + 2^-22 qmax |w|), with u the
     quantum of a block holding all n particles (an upper bound on every
     block's quantum: 2^(max(ceil(log2(qmax w (n + 5))) - 30,
-    ceil(log2(qmax w)) - 22)), qmax = 1 for CIC, 0.31 for M6)."""
+    ceil(log2(qmax w)) - 22)), qmax = 1 for CIC, 0.31 for M6).  u / 2 is the
+    rounding to the quantum; 2^-22 qmax |w| covers the different FP32
+    rounding of the product of the two shape factors and w."""
     aw = (1.0 if kernel == "cic" else 0.31) * abs(float(w))
     e = max(math.ceil(math.log2(aw * (n + 5))) - 30, math.ceil(math.log2(aw)) - 22)
     counts = np.bincount(cell, minlength=nx * ny).reshape(ny, nx).T  # (i, j)
@@ -394,7 +397,7 @@ def fixed_point_bound(cell, nx, ny, kernel, w, n):
         for dv in range(s):
             # a particle in cell (i, j) reaches node (i - hl + du, j - hl + dv)
             contrib += np.roll(np.roll(counts, du - hl, axis=0), dv - hl, axis=1)
-    return contrib * 2.0 ** e / 2
+    return contrib * (2.0 ** e / 2 + 2.0 ** -22 * aw)
 
 
 @pytest.mark.parametrize("acc", [0, 1, 2])
