@@ -60,7 +60,11 @@ static void free_particles(Ctx *c) {
         c->alt_of[k] = nullptr;
     }
     for (int k = 0; k < 2; ++k) { dfree(c->sort_keys[k]); dfree(c->sort_vals[k]); }
+    // the histograms and their scan block sums are sized by the particle
+    // capacity too: both are reallocated together by the next ensure_sort
+    dfree(c->sort_hist);
     dfree(c->sort_blk);
+    c->sort_hist_cap = 0;
     dfree(c->perm2);
     c->sort_cap = 0;
 }
@@ -401,7 +405,8 @@ int pic_ctx_destroy(pic_ctx *ctx) {
     dfree(c->rho); dfree(c->ex); c->ey = nullptr; dfree(c->bz); dfree(c->phi);
     dfree(c->rho_acc); dfree(c->e32); dfree(c->bz32);
     dfree(c->red_part); dfree(c->red_count); dfree(c->energy_hist); dfree(c->step_counter);
-    dfree(c->sort_hist); dfree(c->cell_off); dfree(c->slow_chunks); dfree(c->defer_count);
+    dfree(c->sort_hist); dfree(c->cell_off);
+    dfree(c->slow_chunks); dfree(c->defer_count);
     dfree(c->seg_k); dfree(c->seg_tbl); dfree(c->seg_A); dfree(c->seg_M);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

@@ -274,8 +274,6 @@ int ensure_sort(Ctx *c) {
             if (c->sort_vals[k]) (void)hipFree(c->sort_vals[k]);
             c->sort_keys[k] = c->sort_vals[k] = nullptr;
         }
-        if (c->sort_blk) (void)hipFree(c->sort_blk);
-        c->sort_blk = nullptr;
         int64_t cap = c->cap > 0 ? c->cap : 1;
         for (int k = 0; k < 2; ++k) {
             PIC_HIP(hipMalloc(&c->sort_keys[k], cap * 4));
@@ -284,7 +282,10 @@ int ensure_sort(Ctx *c) {
         c->sort_cap = cap;
     }
     const int64_t ntiles = (c->cap + SORT_TILE - 1) / SORT_TILE + 1;
-    const int64_t hcap = 256 * ntiles;
+    // the histograms, and room for the scans of the digit histograms, of the
+    // per-cell counts (ncells) and of the segment tables (block sums in
+    // sort_blk, one per SCAN_B items)
+    const int64_t hcap = std::max<int64_t>(256 * ntiles, (int64_t)c->nx * c->ny);
     if (hcap > c->sort_hist_cap) {
         if (c->sort_hist) (void)hipFree(c->sort_hist);
         if (c->sort_blk) (void)hipFree(c->sort_blk);
@@ -427,7 +428,7 @@ static int permute_state(Ctx *c, int64_t n, const uint32_t *perm) {
     return PIC_OK;
 }
 
-// Lane-spread permutation of the cell-sorted slots into c->perm2.  No host
+// Lane-spread tables (per-segment A_s(k), M_s(k)) from the cell offsets.  No host
 // synchronisation: the tables hold sum_s max_l count_l <= n entries, so
 // capacity n + 64 always suffices.
 static int lane_spread(Ctx *c) {
@@ -457,10 +458,8 @@ static int lane_spread(Ctx *c) {
     PIC_TRY(scan(c, c->seg_tbl, nseg));
     hipLaunchKernelGGL(k_seg_tables, gs, bs, 0, c->stream, c->cell_off, ncells, nseg, c->seg_k,
                        c->seg_tbl, c->seg_A, c->seg_M);
-    hipLaunchKernelGGL(k_lane_dest, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                       c->sort_keys[0], c->sort_vals[0], n, c->cell_off, c->seg_tbl, c->seg_A,
-                       c->seg_M, c->perm2);
     PIC_HIP(hipGetLastError());
+    (void)n;
     return PIC_OK;
 }
 
@@ -488,6 +487,10 @@ int sort_by_cell(Ctx *c, int32_t *perm_host, bool lanes) {
     const uint32_t *perm = c->sort_vals[0];
     if (lanes) {
         PIC_TRY(lane_spread(c));
+        hipLaunchKernelGGL(k_lane_dest, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           c->stream, c->sort_keys[0], c->sort_vals[0], n, c->cell_off,
+                           c->seg_tbl, c->seg_A, c->seg_M, c->perm2);
+        PIC_HIP(hipGetLastError());
         perm = c->perm2;
         c->seg_valid = true;
     }

@@ -337,6 +337,45 @@ def test_sort_lanes_bitwise(gpu, O, nx, ny, n):
     sim.close()
 
 
+def test_sort_after_context_growth(gpu, O):
+    """A context whose particle capacity grows between sorts (same number of
+    sort tiles) reallocates its sort scratch: the second sort is still the
+    stable sort."""
+    pic = gpu
+    rng = np.random.default_rng(17)
+    nx, ny = 32, 32
+    mesh = pic.TwoDGrid(4 * np.pi, nx, 4 * np.pi, ny)
+    ctx = pic.Context(mesh, capacity=1000)
+    for n in (1000, 3000):
+        p = random_particles(rng, n, mesh.dimx, mesh.dimy)
+        ctx.set_particles(p)
+        perm = np.zeros(n, np.int32)
+        pic._lib.call("pic_sort", ctx.h, perm.ctypes.data)
+        cell = (np.floor(p[0] / mesh.dx).astype(int) % nx
+                + nx * (np.floor(p[1] / mesh.dy).astype(int) % ny)).astype(np.int32)
+        want, _ = O.counting_sort(cell, nx * ny)
+        assert np.array_equal(perm, want), n
+    ctx.close()
+
+
+@pytest.mark.parametrize("state", [0, 1])
+def test_run_large_grid_few_particles(gpu, O, state):
+    """pic_run on a 512 x 512 grid with 300 particles (the per-cell scans
+    are far longer than the particle arrays): energies of the FP64 state
+    match the oracle loop, the FP32 state tracks them."""
+    pic = gpu
+    rng = np.random.default_rng(23)
+    nx = ny = 512
+    mesh = pic.TwoDGrid(4 * np.pi, nx, 4 * np.pi, ny)
+    p = random_particles(rng, 300, mesh.dimx, mesh.dimy)
+    sim = pic.Simulation(mesh, p, pic.CloudInCell(), state=state)
+    e = sim.run(0.1, 6, sort_every=2)
+    sim.close()
+    eo = O.landau_loop(p.copy(order="F"), "cic", nx, ny, mesh.dimx, mesh.dimy, 0.1, 6)
+    tol = 1e-10 if state == 0 else 1e-3
+    assert np.max(np.abs(np.sqrt(e[:, 0]) - eo) / eo) < tol
+
+
 # ------------------------------------------------------------ FP32 state
 
 @pytest.mark.parametrize("kernel", ["cic", "m6"])
