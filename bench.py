@@ -1,11 +1,18 @@
 """Benchmark of the particle-mesh step on one MI355X.
 
-Workload: 2D2V Landau damping with M6 (quintic spline) deposition and gather,
-grid 128 x 128 on [0, 4pi]^2, 1e7 particles, alpha = 0.01, k = 0.5,
-dt = 0.05, 400 steps; the FP32 cell-relative state and the FP64 state run
-side by side from the same quiet start.  A step is one iteration of the
-reference's time loop (examples/landau_damping_poisson_cic_m6.jl:70-83):
-Poisson solve, kick, drift, deposit, on device-resident particles.
+Default workload (--workload landau_m6, the headline): 2D2V Landau damping
+with M6 (quintic spline) deposition and gather, grid 128 x 128 on
+[0, 4pi]^2, 1e7 particles, alpha = 0.01, k = 0.5, dt = 0.05, 400 steps; the
+FP32 cell-relative state and the FP64 state run side by side from the same
+quiet start.  A step is one iteration of the reference's time loop
+(examples/landau_damping_poisson_cic_m6.jl:70-83): Poisson solve, kick,
+drift, deposit, on device-resident particles.
+
+Other workloads, measured the same way: landau_cic (CIC, 64 x 64, 204,800
+particles, alpha 0.05, dt 0.1, 100 steps), strong_b (CIC, 256 x 256, 1e8
+particles, density 1 + 0.05 cos(x/2) cos(y/2), Maxwellian v, uniform bz =
+1/eps = 20, dt = eps/4, 1000 steps) and large_m6 (M6, 512 x 512, 5e8
+particles, alpha 0.05, 200 steps); the last two are sampled on the GPU.
 
 Prints one JSON line.  value = particle-steps/s of the FP32 path over the
 timed steps; roofline = the fused FP32 particle pass (the dominant kernel),
@@ -82,10 +89,9 @@ def copy_probe(mib, reps=5):
                       "GBs": 2 * n * 4 / dt / 1e9}), flush=True)
 
 
-def run_path(pic, mesh, p0, state, dt, steps, warmup, sort_every, f32_acc=2):
+def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_acc=2, bz=None):
     """Warmup, then the timed run from the initial state. Returns dict."""
-    L = pic._lib
-    sim = pic.Simulation(mesh, p0, pic.M6(), state=state)
+    sim = pic.Simulation(mesh, p0, kernel, state=state, bz=bz)
     sim.set_f32_accumulation(f32_acc)
     sim.sort(lanes=True)
     if warmup:
@@ -111,25 +117,45 @@ def run_path(pic, mesh, p0, state, dt, steps, warmup, sort_every, f32_acc=2):
         "slow_chunks": sim.slow_chunks() - slow0,
     }
     sim.close()
-    del L
     return out
+
+
+# The configurations the benchmark knows.  landau_m6 is the headline
+# (particle-steps/s of the FP32 state beside the FP64 state); the others are
+# the remaining hot-path configurations, measured the same way.
+WORKLOADS = {
+    "landau_m6": dict(name="landau2d2v-m6-128x128-1e7-dt0.05-400steps", grid=128, n=10_000_000,
+                      kernel="m6", alpha=0.01, k=0.5, dt=0.05, steps=400, warmup=20,
+                      sampler="quiet", sort=(8, 4)),
+    "landau_cic": dict(name="landau2d2v-cic-64x64-204800-dt0.1-100steps", grid=64, n=204_800,
+                       kernel="cic", alpha=0.05, k=0.5, dt=0.1, steps=100, warmup=10,
+                       sampler="quiet", sort=(8, 4)),
+    "strong_b": dict(name="strongB-cic-256x256-1e8-eps0.05-dt0.0125-1000steps", grid=256,
+                     n=100_000_000, kernel="cic", alpha=0.05, k=0.5, dt=0.0125, steps=1000,
+                     warmup=10, sampler="maxwellian_device", bz=20.0, sort=(32, 16)),
+    "large_m6": dict(name="landau2d2v-m6-512x512-5e8-dt0.05-200steps", grid=512,
+                     n=500_000_000, kernel="m6", alpha=0.05, k=0.5, dt=0.05, steps=200,
+                     warmup=5, sampler="landau_device", sort=(2, 1)),
+}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--particles", type=int, default=10_000_000)
-    ap.add_argument("--sort-every-f32", type=int, default=8,
-                    help="steps between lane sorts on the FP32 path")
-    ap.add_argument("--sort-every-f64", type=int, default=4,
-                    help="steps between lane sorts on the FP64 path")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="landau_m6")
+    ap.add_argument("--steps", type=int, default=None, help="default: the workload's")
+    ap.add_argument("--warmup", type=int, default=None, help="default: the workload's")
+    ap.add_argument("--particles", type=int, default=None, help="default: the workload's")
+    ap.add_argument("--sort-every-f32", type=int, default=None,
+                    help="steps between lane sorts on the FP32 path (default: the workload's)")
+    ap.add_argument("--sort-every-f64", type=int, default=None,
+                    help="steps between lane sorts on the FP64 path (default: the workload's)")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["f32", "f64", "both"], default="both")
     ap.add_argument("--f32-acc", type=int, choices=[0, 1, 2], default=2,
-                    help="FP32 charge tiles: 0 compensated FP32 pairs, 1 FP64 sums, 2 int32 fixed point")
+                    help="FP32 charge tiles: 0 compensated FP32 pairs, 1 FP64 sums, "
+                         "2 int32 fixed point")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run plain device copies of MIB MiB (byte-counter calibration)")
     ap.add_argument("--dump", default="", help="write the energy histories to this .npz")
@@ -147,103 +173,128 @@ def main():
     pic = entry.load_package()
     from picmi355x import diagnostics
 
-    nx = ny = 128
-    alpha, kx, dt = 0.01, 0.5, 0.05
+    W = WORKLOADS[args.workload]
+    # steps between lane sorts: each path at the interval that suits it (the
+    # FP32 windows keep a wider drift margin than the FP64 ones; particles
+    # cross cells faster on finer grids and barely drift in a strong field)
+    if args.sort_every_f32 is None:
+        args.sort_every_f32 = W["sort"][0]
+    if args.sort_every_f64 is None:
+        args.sort_every_f64 = W["sort"][1]
+    steps = W["steps"] if args.steps is None else args.steps
+    warmup = W["warmup"] if args.warmup is None else args.warmup
+    n = W["n"] if args.particles is None else args.particles
+    nx = ny = W["grid"]
+    alpha, kx, dt = W["alpha"], W["k"], W["dt"]
+    bz = W.get("bz")
+    kernel = pic.M6() if W["kernel"] == "m6" else pic.CloudInCell()
     mesh = pic.TwoDGrid(0.0, 2 * math.pi / kx, nx, 0.0, 2 * math.pi / kx, ny)
-    n = args.particles
-    log(f"bench: sampling {n} particles (Landau quiet start)")
-    p0 = pic.sampling.landau_damping(n, mesh, alpha, kx)
+    log(f"bench: {args.workload}: sampling {n} particles ({W['sampler']})")
+    if W["sampler"] == "quiet":
+        p0 = pic.sampling.landau_damping(n, mesh, alpha, kx)
+    elif W["sampler"] == "landau_device":
+        p0 = pic.sampling.landau_damping_device(n, mesh, alpha, kx, seed=1234)
+    else:
+        p0 = pic.sampling.perturbed_maxwellian_device(n, mesh, alpha, kx, kx, seed=1234)
 
     res = {}
     for name, state in (("f32", 1), ("f64", 0)):
         if args.only not in ("both", name):
             continue
-        log(f"bench: {name} path, warmup {args.warmup}, timed {args.steps} steps")
+        log(f"bench: {name} path, warmup {warmup}, timed {steps} steps")
         sort_every = args.sort_every_f32 if name == "f32" else args.sort_every_f64
-        res[name] = run_path(pic, mesh, p0, state, dt, args.steps, args.warmup, sort_every,
-                             args.f32_acc)
+        res[name] = run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every,
+                             args.f32_acc, bz)
         r = res[name]
         log(f"bench: {name} {r['seconds']*1e3:.2f} ms total, pass {r['pass_ms_avg']*1e3:.1f} us,"
-            f" solve {r['solve_ms_avg']*1e3:.1f} us, slow chunks {r['slow_chunks']}")
+            f" solve {r['solve_ms_avg']*1e3:.1f} us, deferred {r['slow_chunks']}")
 
-    t = np.arange(1, args.steps + 1) * dt
+    t = np.arange(1, steps + 1) * dt
     if args.dump:
         np.savez(args.dump, t=t, **{k: v["energy"] for k, v in res.items()})
     gam = {}
-    for name, r in res.items():
-        # ey carries no signal in this problem: its level is the noise floor
-        noise = float(np.median(np.sqrt(r["energy"][:, 1])))
-        g, w, nm = diagnostics.damping_rate(t, np.sqrt(r["energy"][:, 0]), tmin=0.5,
-                                            noise=noise)
-        gam[name] = (g, w, nm)
-        log(f"bench: {name} gamma {g:.5f} omega {w:.4f} from {nm} maxima")
+    if W["sampler"] in ("quiet", "landau_device"):
+        for name, r in res.items():
+            # ey carries no signal in this problem: its level is the noise floor
+            noise = float(np.median(np.sqrt(r["energy"][:, 1])))
+            g, w, nm = diagnostics.damping_rate(t, np.sqrt(r["energy"][:, 0]), tmin=0.5,
+                                                noise=noise)
+            gam[name] = (g, w, nm)
+            log(f"bench: {name} gamma {g:.5f} omega {w:.4f} from {nm} maxima")
 
     head = "f32" if "f32" in res else "f64"
     r = res[head]
-    value = n * args.steps / r["seconds"]
+    value = n * steps / r["seconds"]
     bytes_pp = BYTES_F32 if head == "f32" else BYTES_F64
     achieved = bytes_pp * n / (r["pass_ms_avg"] * 1e-3) / 1e9
+    kname = "M6" if W["kernel"] == "m6" else "CIC"
+    tiles = ({0: " (FP32 pair tiles)", 1: " (FP64 tiles)", 2: " (int32 fixed-point tiles)"}
+             [args.f32_acc] if head == "f32" else " (FP64 tiles)")
     out = {
         "metric": "particle-steps/s FP32 vs FP64 at <=1% damping-rate error; % HBM roofline",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": r["seconds"] * 1e3 / args.steps,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": r["seconds"] * 1e3 / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if head == "f32" else "f64",
-        "data": "synthetic (Landau quiet start, Sobol)",
-        "config": {"workload": "landau2d2v-m6-128x128-1e7-dt0.05-400steps", "grid": [nx, ny],
-                   "particles": n, "kernel": "M6", "alpha": alpha, "k": kx, "dt": dt,
+        "data": "synthetic (Landau quiet start, Sobol)" if W["sampler"] == "quiet" else
+                "synthetic (random start sampled on the GPU, seed 1234)",
+        "config": {"workload": W["name"], "grid": [nx, ny], "particles": n, "kernel": kname,
+                   "alpha": alpha, "k": kx, "dt": dt, "bz": bz,
                    "state": "int32 cell + fp32 offset + fp32 velocity" if head == "f32" else
                    "fp64 soa", "sort_every": args.sort_every_f32 if head == "f32" else
                    args.sort_every_f64, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_seg_{head}<M6> gather+kick+drift+deposit pass"
-                               + ({0: " (FP32 pair tiles)", 1: " (FP64 tiles)",
-                                   2: " (int32 fixed-point tiles)"}[args.f32_acc]
-                                  if head == "f32" else " (FP64 tiles)"),
+                     "kernel": f"k_seg_{head}<{kname}> gather+kick+drift+deposit pass" + tiles,
                      "bytes_per_particle": bytes_pp, "avg_us": r["pass_ms_avg"] * 1e3},
     }
     if "f32" in res and "f64" in res:
-        v64 = n * args.steps / res["f64"]["seconds"]
-        g32, g64 = gam["f32"][0], gam["f64"][0]
-        out["fp64"] = {"value": v64, "ms_per_step": res["f64"]["seconds"] * 1e3 / args.steps,
+        v64 = n * steps / res["f64"]["seconds"]
+        out["fp64"] = {"value": v64, "ms_per_step": res["f64"]["seconds"] * 1e3 / steps,
                        "pass_us": res["f64"]["pass_ms_avg"] * 1e3,
                        "achieved_GBs": BYTES_F64 * n / (res["f64"]["pass_ms_avg"] * 1e-3) / 1e9,
                        "sort_every": args.sort_every_f64,
                        "deferred_particles": res["f64"]["slow_chunks"]}
         out["speedup_fp32_vs_fp64"] = value / v64
-        out["gamma"] = {"fp32": g32, "fp64": g64, "rel_err": abs(g32 - g64) / abs(g64),
-                        "analytic": -0.153359}
+        if gam:
+            g32, g64 = gam["f32"][0], gam["f64"][0]
+            out["gamma"] = {"fp32": g32, "fp64": g64, "rel_err": abs(g32 - g64) / abs(g64),
+                            "analytic": -0.153359}
+        e32, e64 = res["f32"]["energy"][:, 0], res["f64"]["energy"][:, 0]
+        out["energy_rel_diff_fp32_fp64"] = float(np.max(np.abs(e32 - e64)) / np.max(e64))
     out["solve_us"] = r["solve_ms_avg"] * 1e3
     out["deferred_particles"] = r["slow_chunks"]
-    traffic = pmc_traffic(f"k_seg_{head}")
-    if traffic is not None:
-        out["roofline"]["traffic"] = traffic[0]
-        out["roofline"]["traffic_source"] = traffic[1]
+    if args.workload == "landau_m6":
+        traffic = pmc_traffic(f"k_seg_{head}")
+        if traffic is not None:
+            out["roofline"]["traffic"] = traffic[0]
+            out["roofline"]["traffic_source"] = traffic[1]
 
-    if not args.no_cpu:
-        import __graft_entry__ as entry2
-        O = entry2.load_oracle()
+    if not args.no_cpu and args.cpu_steps > 0:
+        O = entry.load_oracle()
         steps_cpu = args.cpu_steps
-        log(f"bench: cpu baseline, oracle loop (M6), {n} particles, {steps_cpu} steps, "
+        m = min(n, 10_000_000)
+        q = (np.asfortranarray(p0[:, :m]) if isinstance(p0, np.ndarray)
+             else np.asfortranarray(p0[:m].cpu().numpy().T))
+        q[4] = mesh.dimx * mesh.dimy / m
+        log(f"bench: cpu baseline, oracle loop ({kname}), {m} particles, {steps_cpu} steps, "
             f"{O.num_threads()} threads")
-        q = np.asfortranarray(p0.copy())
         c0 = time.perf_counter()
-        O.landau_loop(q, "m6", nx, ny, mesh.dimx, mesh.dimy, dt, steps_cpu)
+        O.landau_loop(q, W["kernel"], nx, ny, mesh.dimx, mesh.dimy, dt, steps_cpu)
         c1 = time.perf_counter()
         cores, model = cpu_info()
         out["cpu_baseline"] = {
-            "value": n * steps_cpu / (c1 - c0), "unit": "particle-steps/s",
+            "value": m * steps_cpu / (c1 - c0), "unit": "particle-steps/s",
             "cores": O.num_threads(), "kind": "port",
-            "sample": f"same 1e7 particles, first {steps_cpu} steps of the reference loop "
-                      f"(oracle C restatement: serial deposit, OpenMP push), {model}, "
-                      f"nproc={cores}"}
+            "sample": f"first {m} particles of the same workload, first {steps_cpu} steps of "
+                      f"the reference loop without magnetic field (oracle C restatement: "
+                      f"serial deposit, OpenMP push), {model}, nproc={cores}"}
     print(json.dumps(out), flush=True)
 
 

@@ -87,6 +87,21 @@ def _kcode(kernel):
 
 # --------------------------------------------------------- device context
 
+def _is_device_array(arr):
+    """A torch float64 tensor of shape (n, 5) in GPU memory, contiguous (the
+    memory of a 5 x n column-major particle array)."""
+    if not (hasattr(arr, "is_cuda") and hasattr(arr, "data_ptr")):
+        return False
+    if not arr.is_cuda:
+        raise ValueError("a torch particle array must be on the GPU")
+    import torch
+    if arr.dtype != torch.float64 or arr.dim() != 2 or arr.shape[1] != 5 or \
+            not arr.is_contiguous():
+        raise ValueError("a device particle array must be a contiguous float64 (n, 5) tensor")
+    return True
+
+
+
 class Context:
     """One libpicmi355x context: mesh, particles and grids on one GPU."""
 
@@ -110,6 +125,15 @@ class Context:
 
     # particles
     def set_particles(self, arr):
+        if _is_device_array(arr):
+            # (n, 5) float64 on the GPU: the 5 x n column-major p.array.  The
+            # tensor was written on torch's stream and is read on the
+            # context's: wait for both sides.
+            import torch
+            torch.cuda.synchronize()
+            L.call("pic_set_particles_device", self.h, C.c_void_p(arr.data_ptr()), arr.shape[0])
+            self.sync()
+            return arr
         a = np.asfortranarray(arr, dtype=np.float64)
         assert a.ndim == 2 and a.shape[0] == 5
         L.call("pic_set_particles", self.h, a.ctypes.data, a.shape[1])
@@ -398,7 +422,8 @@ class Simulation:
         self.mesh = mesh
         self.kernel = _kcode(kernel)
         arr = p.array if isinstance(p, ParticleGroup) else p
-        self.ctx = Context(mesh, capacity=arr.shape[1], device=device)
+        n = arr.shape[0] if _is_device_array(arr) else arr.shape[1]
+        self.ctx = Context(mesh, capacity=n, device=device)
         self.ctx.set_particles(arr)
         if bz is None:
             self.ctx.set_bz(L.PIC_BZ_NONE)
@@ -408,7 +433,7 @@ class Simulation:
             self.ctx.set_grid(L.PIC_GRID_BZ, bz)
             self.ctx.set_bz(L.PIC_BZ_GRID)
         self.ctx.set_state(state)
-        self.n = arr.shape[1]
+        self.n = n
 
     def sort(self, lanes=False):
         """Stable sort by cell (pic_sort), or that followed by the lane-spread
@@ -471,7 +496,7 @@ class Simulation:
         arr = p.array if isinstance(p, ParticleGroup) else p
         self.ctx.set_particles(arr)
         self.ctx.set_state(state)
-        self.n = arr.shape[1]
+        self.n = arr.shape[0] if _is_device_array(arr) else arr.shape[1]
 
     def slow_chunks(self):
         v = C.c_uint64()
