@@ -111,6 +111,8 @@ struct Ctx {
     // after a lane sort, segment s occupies slots [cell_off[64 s],
     // cell_off[min(64 s + 64, ncells)]) until the next sort
     bool seg_valid = false;             // cell_off describes the current slots
+    // a lane sort's permutation not yet applied to the state (see sort_by_cell)
+    const uint32_t *pending_perm = nullptr;
 
     // fused-step tile statistics (diagnostic): chunks that took the slow path
     unsigned long long *slow_chunks = nullptr;
@@ -156,7 +158,16 @@ int finalize_rho(Ctx *c, int kernel);
 int field_energy(Ctx *c, double *out2);
 int field_energy_mf(Ctx *c, double *out);
 
-int sort_by_cell(Ctx *c, int32_t *perm_host, bool lanes);
+// Stable sort by cell (lanes: then the lane-spread order of the segment
+// passes).  apply = false with lanes leaves the permutation pending in
+// c->pending_perm for the next segment pass to apply while it reads the
+// particles (no host permutation requested).
+int sort_by_cell(Ctx *c, int32_t *perm_host, bool lanes, bool apply = true);
+int settle_state(Ctx *c);
+int alt_f32(Ctx *c, int32_t **cell, float4 **ov);
+void swap_alt_f32(Ctx *c);
+int alt_f64(Ctx *c, double **x, double **y, double **vx, double **vy, double **w);
+void swap_alt_f64(Ctx *c);
 
 int launch_fill(Ctx *c, double *p, double v, int64_t n);
 

@@ -64,7 +64,7 @@ extern "C" int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort
     const double scale = 1.0 / ((double)c->nx * (double)c->ny * c->dx * c->dy);
     const int f32 = c->state == PIC_STATE_F32;
     // the passes work segment by segment on lane-sorted slots
-    PIC_TRY(sort_by_cell(c, nullptr, true));
+    PIC_TRY(sort_by_cell(c, nullptr, true, false));
     PIC_TRY(launch_fill(c, c->rho_acc, 0.0, m));
     PIC_TRY(seg_pass(c, kernel, 0, dt));
     // timing: events around every solve and every particle pass
@@ -89,7 +89,12 @@ extern "C" int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort
         if (rc == PIC_OK) rc = seg_pass(c, kernel, last ? 2 : 1, dt);
         if (rc == PIC_OK && c->timing) rc = mark();
         if (rc == PIC_OK && !last && sort_every > 0 && (s + 1) % sort_every == 0)
-            rc = sort_by_cell(c, nullptr, true);
+            rc = sort_by_cell(c, nullptr, true, false);
+    }
+    // a sort whose permutation no pass consumed (an error stopped the loop)
+    if (c->pending_perm) {
+        const int rs = settle_state(c);
+        if (rc == PIC_OK) rc = rs;
     }
     if (rc == PIC_OK && energy)
         rc = hipMemcpyAsync(energy, c->energy_hist, (size_t)nsteps * 2 * sizeof(double),

@@ -112,8 +112,14 @@ __device__ __forceinline__ void defer_finish(unsigned cnt, unsigned *dcount,
 // ============================================================ FP32 state
 
 struct SegF32 {
-    int32_t *cell;
-    float4 *ov;  // (ox, oy, vx, vy)
+    int32_t *cell;  // state written by the pass (slot p)
+    float4 *ov;     // (ox, oy, vx, vy)
+    // state read by the pass: the same arrays, or, right after a lane sort
+    // whose permutation has not been applied, the unsorted arrays read at
+    // perm[p] (the pass then moves every particle into its sorted slot)
+    const int32_t *cell_in;
+    const float4 *ov_in;
+    const uint32_t *perm;
     float w;
     const float2 *e32;
     const float *bz;
@@ -323,7 +329,7 @@ constexpr uint32_t DEFER_DEPOSIT_ONLY = 0x80000000u;
 // the block's LDS window is not touched here: it goes to the deferred list
 // whole (gather outside) or for its deposit only (drifted outside), and
 // k_defer_f32 completes it from global memory after this kernel.
-template <int K, bool GATHER, bool DRIFT, bool DEPOSIT, int BZ, int ACC>
+template <int K, bool GATHER, bool DRIFT, bool DEPOSIT, int BZ, int ACC, bool PERM>
 __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
     using St = SegStencil<K>;
     constexpr int S = St::S, HL = St::HL;
@@ -374,9 +380,10 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
     __syncthreads();
 
     for (uint32_t p = (uint32_t)p0 + tid; p < (uint32_t)p1; p += SEG_T) {
-        const int c = a.cell[p];
+        const uint32_t src = PERM ? a.perm[p] : p;
+        const int c = a.cell_in[src];
         int ci = c & mx, cj = c >> a.lognx;
-        const float4 o = a.ov[p];
+        const float4 o = a.ov_in[src];
         float fx = o.x, fy = o.y, ux = o.z, uy = o.w;
         if (GATHER) {
             const int lx = (ci - HL - W.x0) & mx, ly = (cj - HL - W.y0) & my;
@@ -397,9 +404,9 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
             cj = (cj + sj) & my;
             fx = nfx;
             fy = nfy;
-            a.cell[p] = ci + nx * cj;
         }
-        if (GATHER || DRIFT) a.ov[p] = make_float4(fx, fy, ux, uy);
+        if (DRIFT || PERM) a.cell[p] = ci + nx * cj;
+        if (GATHER || DRIFT || PERM) a.ov[p] = make_float4(fx, fy, ux, uy);
         if (DEPOSIT) {
             const int lx = (ci - HL - W.x0) & mx, ly = (cj - HL - W.y0) & my;
             const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
@@ -448,7 +455,7 @@ constexpr int DEFER_T = 256, DEFER_BLOCKS = 256;
 // Completes the particles k_seg_f32 deferred, from global memory (same
 // arithmetic), adds their number to the slow-path counter, and the last
 // block to finish resets the list for the next pass.
-template <int K, bool GATHER, bool DRIFT, bool DEPOSIT, int BZ>
+template <int K, bool GATHER, bool DRIFT, bool DEPOSIT, int BZ, bool PERM>
 __global__ __launch_bounds__(DEFER_T) void k_defer_f32(SegF32 a) {
     constexpr int HL = SegStencil<K>::HL;
     const unsigned cnt = __hip_atomic_load(a.dcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -457,9 +464,12 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f32(SegF32 a) {
         const uint32_t e = a.dlist[i];
         const int64_t p = e & ~DEFER_DEPOSIT_ONLY;
         const bool whole = (e & DEFER_DEPOSIT_ONLY) == 0;
-        const int c = a.cell[p];
+        // a whole particle has not been touched: read it where the pass
+        // would have; a deposit-only one is already in its slot
+        const uint32_t src = (PERM && whole) ? a.perm[p] : (uint32_t)p;
+        const int c = whole ? a.cell_in[src] : a.cell[p];
         int ci = c & mx, cj = c >> a.lognx;
-        const float4 o = a.ov[p];
+        const float4 o = whole ? a.ov_in[src] : a.ov[p];
         float fx = o.x, fy = o.y;
         if (whole) {
             float ux = o.z, uy = o.w;
@@ -486,9 +496,9 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f32(SegF32 a) {
                 cj = (cj + sj) & my;
                 fx = nfx;
                 fy = nfy;
-                a.cell[p] = ci + nx * cj;
             }
-            if (GATHER || DRIFT) a.ov[p] = make_float4(fx, fy, ux, uy);
+            if (DRIFT || PERM) a.cell[p] = ci + nx * cj;
+            if (GATHER || DRIFT || PERM) a.ov[p] = make_float4(fx, fy, ux, uy);
         }
         if (DEPOSIT)
             charge32<K>(fx, fy, a.w, [&](int u, int v, float q) {
@@ -503,8 +513,14 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f32(SegF32 a) {
 // ============================================================ FP64 state
 
 struct SegF64 {
-    double *x, *y, *vx, *vy;
+    double *x, *y, *vx, *vy;  // state written by the pass (slot p)
+    // state read by the pass (see SegF32): the same arrays or the unsorted
+    // ones read at perm[p]; weights w (null when uniform) are read at the
+    // same place and, with a permutation, copied to w_out[p]
+    const double *x_in, *y_in, *vx_in, *vy_in;
+    const uint32_t *perm;
     const double *w;
+    double *w_out;
     double w0;
     const double *ex, *ey, *bz;
     double bz0;
@@ -640,7 +656,7 @@ __device__ __forceinline__ void charge64(double fx, double fy, double wgt, F &&a
 }
 
 // FP64 counterpart of k_seg_f32; out-of-window particles go to k_defer_f64.
-template <int K, bool GATHER, int NDRIFT, bool DEPOSIT, int BZ>
+template <int K, bool GATHER, int NDRIFT, bool DEPOSIT, int BZ, bool PERM>
 __global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
     using St = SegStencil<K>;
     constexpr int S = St::S, HL = St::HL;
@@ -675,7 +691,8 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
     __syncthreads();
 
     for (uint32_t p = (uint32_t)p0 + tid; p < (uint32_t)p1; p += SEG_T) {
-        double px = a.x[p], py = a.y[p];
+        const uint32_t src = PERM ? a.perm[p] : p;
+        double px = a.x_in[src], py = a.y_in[src];
         double pvx = 0.0, pvy = 0.0;
         if (GATHER) {
             double fx, fy;
@@ -685,24 +702,27 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
             const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
             defer_push(!in, (uint32_t)p, a.dlist, a.dcount);
             if (!in) continue;
-            pvx = a.vx[p];
-            pvy = a.vy[p];
+            pvx = a.vx_in[src];
+            pvy = a.vy_in[src];
             double e1, e2, b3 = 0.0;
             interp64<K, BZ>(
                 fx, fy, [&](int u, int v) { return te[(lx + u) + W.tw * (ly + v)]; },
                 [&](int u, int v) { return tb[(lx + u) + W.tw * (ly + v)]; }, e1, e2, b3);
             kick64<BZ>(a, e1, e2, b3, pvx, pvy);
+        } else if (NDRIFT > 0 || PERM) {
+            pvx = a.vx_in[src];
+            pvy = a.vy_in[src];
+        }
+        if (NDRIFT > 0) drift64<NDRIFT>(a, pvx, pvy, px, py);
+        if (GATHER || PERM) {
             a.vx[p] = pvx;
             a.vy[p] = pvy;
-        } else if (NDRIFT > 0) {
-            pvx = a.vx[p];
-            pvy = a.vy[p];
         }
-        if (NDRIFT > 0) {
-            drift64<NDRIFT>(a, pvx, pvy, px, py);
+        if (NDRIFT > 0 || PERM) {
             a.x[p] = px;
             a.y[p] = py;
         }
+        if (PERM && a.w) a.w_out[p] = a.w[src];
         if (DEPOSIT) {
             double fx, fy;
             const int ci = ref_cell_s(px, a.dx, nx, K == PIC_CIC, fx);
@@ -711,7 +731,7 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
             const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
             defer_push(!in, (uint32_t)p | DEFER_DEPOSIT_ONLY, a.dlist, a.dcount);
             if (!in) continue;
-            charge64<K>(fx, fy, a.w ? a.w[p] : a.w0, [&](int u, int v, double q) {
+            charge64<K>(fx, fy, a.w ? a.w[src] : a.w0, [&](int u, int v, double q) {
                 __hip_atomic_fetch_add(&dacc[(lx + u) + W.tw * (ly + v)], q, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
             });
@@ -732,7 +752,7 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
 }
 
 // Completes the particles k_seg_f64 deferred, from global memory.
-template <int K, bool GATHER, int NDRIFT, bool DEPOSIT, int BZ>
+template <int K, bool GATHER, int NDRIFT, bool DEPOSIT, int BZ, bool PERM>
 __global__ __launch_bounds__(DEFER_T) void k_defer_f64(SegF64 a) {
     constexpr int HL = SegStencil<K>::HL;
     const unsigned cnt = __hip_atomic_load(a.dcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -740,12 +760,17 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f64(SegF64 a) {
     for (unsigned i = blockIdx.x * DEFER_T + threadIdx.x; i < cnt; i += gridDim.x * DEFER_T) {
         const uint32_t e = a.dlist[i];
         const int64_t p = e & ~DEFER_DEPOSIT_ONLY;
-        double px = a.x[p], py = a.y[p];
-        if ((e & DEFER_DEPOSIT_ONLY) == 0) {
+        const bool whole = (e & DEFER_DEPOSIT_ONLY) == 0;
+        // a whole particle is read where the pass would have read it; a
+        // deposit-only one is already in its slot
+        const uint32_t src = (PERM && whole) ? a.perm[p] : (uint32_t)p;
+        double px = whole ? a.x_in[src] : a.x[p], py = whole ? a.y_in[src] : a.y[p];
+        const double wgt = !a.w ? a.w0 : whole ? a.w[src] : PERM ? a.w_out[p] : a.w[p];
+        if (whole) {
             double pvx = 0.0, pvy = 0.0;
-            if (GATHER || NDRIFT > 0) {
-                pvx = a.vx[p];
-                pvy = a.vy[p];
+            if (GATHER || NDRIFT > 0 || PERM) {
+                pvx = a.vx_in[src];
+                pvy = a.vy_in[src];
             }
             if (GATHER) {
                 double fx, fy;
@@ -765,20 +790,23 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f64(SegF64 a) {
                     },
                     e1, e2, b3);
                 kick64<BZ>(a, e1, e2, b3, pvx, pvy);
+            }
+            if (NDRIFT > 0) drift64<NDRIFT>(a, pvx, pvy, px, py);
+            if (GATHER || PERM) {
                 a.vx[p] = pvx;
                 a.vy[p] = pvy;
             }
-            if (NDRIFT > 0) {
-                drift64<NDRIFT>(a, pvx, pvy, px, py);
+            if (NDRIFT > 0 || PERM) {
                 a.x[p] = px;
                 a.y[p] = py;
             }
+            if (PERM && a.w) a.w_out[p] = a.w[src];
         }
         if (DEPOSIT) {
             double fx, fy;
             const int ci = ref_cell_s(px, a.dx, nx, K == PIC_CIC, fx);
             const int cj = ref_cell_s(py, a.dy, ny, K == PIC_CIC, fy);
-            charge64<K>(fx, fy, a.w ? a.w[p] : a.w0, [&](int u, int v, double q) {
+            charge64<K>(fx, fy, wgt, [&](int u, int v, double q) {
                 unsafeAtomicAdd(&a.acc[(int64_t)picd::wrap2(ci - HL + u, nx) +
                                        (int64_t)nx * picd::wrap2(cj - HL + v, ny)],
                                 q);
@@ -813,8 +841,8 @@ static int seg_bps(const Ctx *c, const void *kernel, size_t sm) {
     return bps;
 }
 
-// Sets the blocks per segment and the FP32 split quantum for kernel `k`,
-// then launches it over every segment.
+// Launches the pass kernel `k` over every segment, then its deferred-list
+// kernel `kd`.
 template <class Kern, class Defer>
 static void seg32_launch(const Ctx *c, SegF32 a, Kern k, Defer kd, size_t sm) {
     a.bps = seg_bps(c, reinterpret_cast<const void *>(k), sm);
@@ -823,43 +851,55 @@ static void seg32_launch(const Ctx *c, SegF32 a, Kern k, Defer kd, size_t sm) {
     hipLaunchKernelGGL(kd, dim3(DEFER_BLOCKS), dim3(DEFER_T), 0, c->stream, a);
 }
 
-template <int K, bool G, bool DR, bool D, int ACC>
+template <int K, bool G, bool DR, bool D, int ACC, bool P>
 static void seg32_bz(const Ctx *c, const SegF32 &a, int bz, size_t sm) {
     switch (bz) {
         case PIC_BZ_NONE:
-            seg32_launch(c, a, k_seg_f32<K, G, DR, D, PIC_BZ_NONE, ACC>,
-                         k_defer_f32<K, G, DR, D, PIC_BZ_NONE>, sm);
+            seg32_launch(c, a, k_seg_f32<K, G, DR, D, PIC_BZ_NONE, ACC, P>,
+                         k_defer_f32<K, G, DR, D, PIC_BZ_NONE, P>, sm);
             break;
         case PIC_BZ_UNIFORM:
-            seg32_launch(c, a, k_seg_f32<K, G, DR, D, PIC_BZ_UNIFORM, ACC>,
-                         k_defer_f32<K, G, DR, D, PIC_BZ_UNIFORM>, sm);
+            seg32_launch(c, a, k_seg_f32<K, G, DR, D, PIC_BZ_UNIFORM, ACC, P>,
+                         k_defer_f32<K, G, DR, D, PIC_BZ_UNIFORM, P>, sm);
             break;
         default:
-            seg32_launch(c, a, k_seg_f32<K, G, DR, D, PIC_BZ_GRID, ACC>,
-                         k_defer_f32<K, G, DR, D, PIC_BZ_GRID>, sm);
+            seg32_launch(c, a, k_seg_f32<K, G, DR, D, PIC_BZ_GRID, ACC, P>,
+                         k_defer_f32<K, G, DR, D, PIC_BZ_GRID, P>, sm);
     }
 }
 
-template <int K, int ACC>
+template <int K, int ACC, bool P>
 static int seg32_mode(const Ctx *c, const SegF32 &a, int mode, int bz, size_t sm) {
     switch (mode) {
-        case 0: seg32_bz<K, false, true, true, ACC>(c, a, PIC_BZ_NONE, sm); break;
-        case 1: seg32_bz<K, true, true, true, ACC>(c, a, bz, sm); break;
-        case 2: seg32_bz<K, true, true, false, ACC>(c, a, bz, sm); break;
-        case 3: seg32_bz<K, false, false, true, ACC>(c, a, PIC_BZ_NONE, sm); break;
+        case 0: seg32_bz<K, false, true, true, ACC, P>(c, a, PIC_BZ_NONE, sm); break;
+        case 1: seg32_bz<K, true, true, true, ACC, P>(c, a, bz, sm); break;
+        case 2: seg32_bz<K, true, true, false, ACC, P>(c, a, bz, sm); break;
+        case 3: seg32_bz<K, false, false, true, ACC, P>(c, a, PIC_BZ_NONE, sm); break;
         default: return fail(PIC_ERR_ARG, "pass mode %d", mode);
     }
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
 
-// One segment-persistent pass in FP32 state (the particles must be in the
-// lane order of the last sort_by_cell(..., lanes = true)).
+template <int K, int ACC>
+static int seg32_perm(const Ctx *c, const SegF32 &a, int mode, int bz, size_t sm) {
+    return a.perm ? seg32_mode<K, ACC, true>(c, a, mode, bz, sm)
+                  : seg32_mode<K, ACC, false>(c, a, mode, bz, sm);
+}
+
+// One segment-persistent pass in FP32 state.  The particles are in the lane
+// order of the last sort_by_cell(..., lanes = true), or that sort left its
+// permutation pending (c->pending_perm): then this pass reads every particle
+// through it, writes it into its sorted slot of the alternate arrays, and
+// swaps those in.
 int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     if (c->n == 0) return PIC_OK;
     PIC_CHECK(c->seg_valid, PIC_ERR_STATE, "segment passes need a lane sort first");
     SegF32 a;
     a.cell = c->cell; a.ov = c->ov32;
+    a.cell_in = c->cell; a.ov_in = c->ov32;
+    a.perm = c->pending_perm;
+    if (a.perm) PIC_TRY(alt_f32(c, &a.cell, &a.ov));
     a.w = (float)c->w0;
     a.e32 = reinterpret_cast<const float2 *>(c->e32);
     a.bz = c->bz32;
@@ -885,13 +925,20 @@ int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     a.dlist = c->defer;
     a.dcount = c->defer_count;
     const size_t sm = (size_t)a.cap * (per_node ? per_node : 8);
+    int rc;
     if (kernel == PIC_CIC)
-        return acc == 0   ? seg32_mode<PIC_CIC, 0>(c, a, mode, c->bz_mode, sm)
-               : acc == 1 ? seg32_mode<PIC_CIC, 1>(c, a, mode, c->bz_mode, sm)
-                          : seg32_mode<PIC_CIC, 2>(c, a, mode, c->bz_mode, sm);
-    return acc == 0   ? seg32_mode<PIC_M6, 0>(c, a, mode, c->bz_mode, sm)
-           : acc == 1 ? seg32_mode<PIC_M6, 1>(c, a, mode, c->bz_mode, sm)
-                      : seg32_mode<PIC_M6, 2>(c, a, mode, c->bz_mode, sm);
+        rc = acc == 0   ? seg32_perm<PIC_CIC, 0>(c, a, mode, c->bz_mode, sm)
+             : acc == 1 ? seg32_perm<PIC_CIC, 1>(c, a, mode, c->bz_mode, sm)
+                        : seg32_perm<PIC_CIC, 2>(c, a, mode, c->bz_mode, sm);
+    else
+        rc = acc == 0   ? seg32_perm<PIC_M6, 0>(c, a, mode, c->bz_mode, sm)
+             : acc == 1 ? seg32_perm<PIC_M6, 1>(c, a, mode, c->bz_mode, sm)
+                        : seg32_perm<PIC_M6, 2>(c, a, mode, c->bz_mode, sm);
+    if (rc == PIC_OK && a.perm) {
+        swap_alt_f32(c);
+        c->pending_perm = nullptr;
+    }
+    return rc;
 }
 
 template <class Kern, class Defer>
@@ -902,42 +949,47 @@ static void seg64_launch(const Ctx *c, SegF64 a, Kern k, Defer kd, size_t sm) {
     hipLaunchKernelGGL(kd, dim3(DEFER_BLOCKS), dim3(DEFER_T), 0, c->stream, a);
 }
 
-template <int K, bool G, int ND, bool D>
+template <int K, bool G, int ND, bool D, bool P>
 static void seg64_bz(const Ctx *c, const SegF64 &a, int bz, size_t sm) {
     switch (bz) {
         case PIC_BZ_NONE:
-            seg64_launch(c, a, k_seg_f64<K, G, ND, D, PIC_BZ_NONE>,
-                         k_defer_f64<K, G, ND, D, PIC_BZ_NONE>, sm);
+            seg64_launch(c, a, k_seg_f64<K, G, ND, D, PIC_BZ_NONE, P>,
+                         k_defer_f64<K, G, ND, D, PIC_BZ_NONE, P>, sm);
             break;
         case PIC_BZ_UNIFORM:
-            seg64_launch(c, a, k_seg_f64<K, G, ND, D, PIC_BZ_UNIFORM>,
-                         k_defer_f64<K, G, ND, D, PIC_BZ_UNIFORM>, sm);
+            seg64_launch(c, a, k_seg_f64<K, G, ND, D, PIC_BZ_UNIFORM, P>,
+                         k_defer_f64<K, G, ND, D, PIC_BZ_UNIFORM, P>, sm);
             break;
         default:
-            seg64_launch(c, a, k_seg_f64<K, G, ND, D, PIC_BZ_GRID>,
-                         k_defer_f64<K, G, ND, D, PIC_BZ_GRID>, sm);
+            seg64_launch(c, a, k_seg_f64<K, G, ND, D, PIC_BZ_GRID, P>,
+                         k_defer_f64<K, G, ND, D, PIC_BZ_GRID, P>, sm);
     }
 }
 
-template <int K>
+template <int K, bool P>
 static int seg64_mode(const Ctx *c, const SegF64 &a, int mode, int bz, size_t sm) {
     switch (mode) {
-        case 0: seg64_bz<K, false, 1, true>(c, a, PIC_BZ_NONE, sm); break;
-        case 1: seg64_bz<K, true, 2, true>(c, a, bz, sm); break;
-        case 2: seg64_bz<K, true, 1, false>(c, a, bz, sm); break;
-        case 3: seg64_bz<K, false, 0, true>(c, a, PIC_BZ_NONE, sm); break;
+        case 0: seg64_bz<K, false, 1, true, P>(c, a, PIC_BZ_NONE, sm); break;
+        case 1: seg64_bz<K, true, 2, true, P>(c, a, bz, sm); break;
+        case 2: seg64_bz<K, true, 1, false, P>(c, a, bz, sm); break;
+        case 3: seg64_bz<K, false, 0, true, P>(c, a, PIC_BZ_NONE, sm); break;
         default: return fail(PIC_ERR_ARG, "pass mode %d", mode);
     }
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
 
+// The FP64 counterpart of launch_seg32 (same pending-permutation rule).
 int launch_seg64(Ctx *c, int kernel, int mode, double dt, double dtd) {
     if (c->n == 0) return PIC_OK;
     PIC_CHECK(c->seg_valid, PIC_ERR_STATE, "segment passes need a lane sort first");
     SegF64 a;
     a.x = c->x; a.y = c->y; a.vx = c->vx; a.vy = c->vy;
+    a.x_in = c->x; a.y_in = c->y; a.vx_in = c->vx; a.vy_in = c->vy;
     a.w = c->uniform_w ? nullptr : c->w;
+    a.w_out = nullptr;
+    a.perm = c->pending_perm;
+    if (a.perm) PIC_TRY(alt_f64(c, &a.x, &a.y, &a.vx, &a.vy, a.w ? &a.w_out : nullptr));
     a.w0 = c->w0;
     a.ex = c->ex; a.ey = c->ey; a.bz = c->bz; a.bz0 = c->bz0;
     a.nx = c->nx; a.ny = c->ny; a.ncells = c->nx * c->ny;
@@ -956,8 +1008,18 @@ int launch_seg64(Ctx *c, int kernel, int mode, double dt, double dtd) {
     a.dlist = c->defer;
     a.dcount = c->defer_count;
     const size_t sm = (size_t)a.cap * (per_node ? per_node : 8);
-    return kernel == PIC_CIC ? seg64_mode<PIC_CIC>(c, a, mode, c->bz_mode, sm)
-                             : seg64_mode<PIC_M6>(c, a, mode, c->bz_mode, sm);
+    int rc;
+    if (kernel == PIC_CIC)
+        rc = a.perm ? seg64_mode<PIC_CIC, true>(c, a, mode, c->bz_mode, sm)
+                    : seg64_mode<PIC_CIC, false>(c, a, mode, c->bz_mode, sm);
+    else
+        rc = a.perm ? seg64_mode<PIC_M6, true>(c, a, mode, c->bz_mode, sm)
+                    : seg64_mode<PIC_M6, false>(c, a, mode, c->bz_mode, sm);
+    if (rc == PIC_OK && a.perm) {
+        swap_alt_f64(c);
+        c->pending_perm = nullptr;
+    }
+    return rc;
 }
 
 }  // namespace pic

@@ -389,6 +389,58 @@ __global__ void k_gather_state(int64_t n, const uint32_t *__restrict__ perm, Sta
         if (g.in8[q]) g.out8[q][r] = g.in8[q][src];
 }
 
+// The alternate state arrays (allocated on first use) and their swap with
+// the current ones, for a pass that moves the particles into sorted slots.
+int alt_f32(Ctx *c, int32_t **cell, float4 **ov) {
+    int s0, s1;
+    PIC_TRY(alt_slot(c, c->cell, &s0));
+    PIC_TRY(alt_slot(c, c->ov32, &s1));
+    *cell = static_cast<int32_t *>(c->alt[s0]);
+    *ov = static_cast<float4 *>(c->alt[s1]);
+    return PIC_OK;
+}
+
+void swap_alt_f32(Ctx *c) {
+    int s0 = 0, s1 = 0;
+    (void)alt_slot(c, c->cell, &s0);
+    (void)alt_slot(c, c->ov32, &s1);
+    alt_swap(c, c->cell, s0);
+    alt_swap(c, c->ov32, s1);
+}
+
+int alt_f64(Ctx *c, double **x, double **y, double **vx, double **vy, double **w) {
+    int s[5];
+    PIC_TRY(alt_slot(c, c->x, &s[0]));
+    PIC_TRY(alt_slot(c, c->y, &s[1]));
+    PIC_TRY(alt_slot(c, c->vx, &s[2]));
+    PIC_TRY(alt_slot(c, c->vy, &s[3]));
+    *x = static_cast<double *>(c->alt[s[0]]);
+    *y = static_cast<double *>(c->alt[s[1]]);
+    *vx = static_cast<double *>(c->alt[s[2]]);
+    *vy = static_cast<double *>(c->alt[s[3]]);
+    if (w) {
+        PIC_TRY(alt_slot(c, c->w, &s[4]));
+        *w = static_cast<double *>(c->alt[s[4]]);
+    }
+    return PIC_OK;
+}
+
+void swap_alt_f64(Ctx *c) {
+    int s[5] = {0, 0, 0, 0, 0};
+    (void)alt_slot(c, c->x, &s[0]);
+    (void)alt_slot(c, c->y, &s[1]);
+    (void)alt_slot(c, c->vx, &s[2]);
+    (void)alt_slot(c, c->vy, &s[3]);
+    alt_swap(c, c->x, s[0]);
+    alt_swap(c, c->y, s[1]);
+    alt_swap(c, c->vx, s[2]);
+    alt_swap(c, c->vy, s[3]);
+    if (!c->uniform_w) {
+        (void)alt_slot(c, c->w, &s[4]);
+        alt_swap(c, c->w, s[4]);
+    }
+}
+
 // Applies `perm` to the current state (FP32: cell, ox, oy, vx, vy; FP64: x,
 // y, vx, vy and w when weights differ).
 static int permute_state(Ctx *c, int64_t n, const uint32_t *perm) {
@@ -463,8 +515,17 @@ static int lane_spread(Ctx *c) {
     return PIC_OK;
 }
 
-int sort_by_cell(Ctx *c, int32_t *perm_host, bool lanes) {
+// Applies a permutation a lane sort left pending (no pass consumed it).
+int settle_state(Ctx *c) {
+    if (!c->pending_perm) return PIC_OK;
+    const uint32_t *perm = c->pending_perm;
+    c->pending_perm = nullptr;
+    return permute_state(c, c->n, perm);
+}
+
+int sort_by_cell(Ctx *c, int32_t *perm_host, bool lanes, bool apply) {
     const int64_t n = c->n;
+    PIC_TRY(settle_state(c));
     c->lanes_order = lanes;
     c->seg_valid = false;
     if (n == 0) return PIC_OK;
@@ -494,7 +555,10 @@ int sort_by_cell(Ctx *c, int32_t *perm_host, bool lanes) {
         perm = c->perm2;
         c->seg_valid = true;
     }
-    PIC_TRY(permute_state(c, n, perm));
+    if (apply || !lanes || perm_host)
+        PIC_TRY(permute_state(c, n, perm));
+    else
+        c->pending_perm = perm;  // the next segment pass moves the particles
     if (perm_host) {
         PIC_HIP(hipMemcpyAsync(perm_host, perm, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
         PIC_HIP(hipStreamSynchronize(c->stream));

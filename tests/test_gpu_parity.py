@@ -643,6 +643,31 @@ def test_run_matches_reference_loop(gpu, O, kernel):
     assert np.unique(idx).size == q.shape[1]
 
 
+def test_run_nonuniform_weights(gpu, O):
+    """pic_run with per-particle weights (FP64 state, re-sorted every 2 steps,
+    so the weights travel with the particles through the sorts): energies
+    match the oracle loop and the particles, weights included, are the
+    oracle's as a set."""
+    pic = gpu
+    rng = np.random.default_rng(29)
+    nx = ny = 32
+    mesh = pic.TwoDGrid(4 * np.pi, nx, 4 * np.pi, ny)
+    n = 40000
+    p = random_particles(rng, n, mesh.dimx, mesh.dimy,
+                         w=rng.uniform(0.5, 1.5, n) * mesh.dimx * mesh.dimy / n)
+    sim = pic.Simulation(mesh, p, pic.M6())
+    e = sim.run(0.1, 7, sort_every=2)
+    got = sim.particles()
+    sim.close()
+    q = p.copy(order="F")
+    eo = O.landau_loop(q, "m6", nx, ny, mesh.dimx, mesh.dimy, 0.1, 7)
+    assert np.max(np.abs(np.sqrt(e[:, 0]) - eo) / eo) < 1e-10
+    from scipy.spatial import cKDTree
+    dist, idx = cKDTree(q.T).query(got.T)
+    assert np.max(dist) < 1e-9
+    assert np.unique(idx).size == n
+
+
 def test_run_sorted_matches_unsorted(gpu):
     """Sorting every step only permutes the particles: the energy history is
     the same to rounding."""
