@@ -254,7 +254,10 @@ int pic_get_timing(pic_ctx *ctx, double *pass_ms, int64_t *passes, double *solve
  * J = sum(ex^2 + ey^2) dx dy of the last step's field (the energy the loop
  * records at t = nsteps*dt), with respect to the initial positions and
  * velocities of every particle: the exact discrete adjoint of the loop,
- * bz = 0, FP64 state.  The forward pass keeps the solved field of every step
+ * bz = 0, uniform weights.  In the FP64 state every operation is FP64; in
+ * the FP32 state the loop and the particle adjoints are FP32 (cell-relative
+ * positions, as pic_run) while the charge sums, the Poisson solve and its
+ * adjoint stay FP64.  The forward pass keeps the solved field of every step
  * and the particle state every `ckpt` steps; the reverse sweep recomputes
  * each segment from its checkpoint.  gx, gv (host, 2 x n, may be NULL)
  * receive dJ/d(x0, y0) and dJ/d(vx0, vy0) in the context's particle order;

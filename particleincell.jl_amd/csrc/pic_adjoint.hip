@@ -246,18 +246,6 @@ __global__ void k_scatter_back(int64_t n, const uint32_t *__restrict__ perm,
 
 static inline dim3 ga(int64_t n) { return dim3((unsigned)((n + AT - 1) / AT)); }
 
-template <typename T>
-struct DevBuf {
-    T *p = nullptr;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    int alloc(size_t count) {
-        PIC_HIP(hipMalloc(&p, (count ? count : 1) * sizeof(T)));
-        return PIC_OK;
-    }
-};
-
 template <int K>
 static int adjoint_impl(Ctx *c, double dt, int nsteps, int ckpt, double *J, double *gx,
                         double *gv) {
@@ -409,10 +397,10 @@ extern "C" int pic_adjoint(pic_ctx *ctx, int kernel, double dt, int nsteps, int 
     PIC_CHECK(nsteps >= 1 && ckpt >= 1, PIC_ERR_ARG, "nsteps %d, ckpt %d", nsteps, ckpt);
     Ctx *c = ctx_of(ctx);
     PIC_HIP(hipSetDevice(c->device));
-    PIC_CHECK(c->state == PIC_STATE_F64, PIC_ERR_STATE, "the adjoint runs in the FP64 state");
     PIC_CHECK(c->uniform_w, PIC_ERR_STATE, "the adjoint needs uniform particle weights");
     PIC_CHECK(c->bz_mode == PIC_BZ_NONE, PIC_ERR_STATE, "the adjoint supports bz = 0 only");
     PIC_CHECK(c->n > 0, PIC_ERR_STATE, "no particles");
+    if (c->state == PIC_STATE_F32) return adjoint_f32(c, kernel, dt, nsteps, ckpt, J, gx, gv);
     return kernel == PIC_CIC ? adjoint_impl<PIC_CIC>(c, dt, nsteps, ckpt, J, gx, gv)
                              : adjoint_impl<PIC_M6>(c, dt, nsteps, ckpt, J, gx, gv);
 }

@@ -171,6 +171,26 @@ void swap_alt_f64(Ctx *c);
 
 int launch_fill(Ctx *c, double *p, double v, int64_t n);
 
+// The adjoint sweep of the FP32 cell-relative state (pic_adjoint32.hip).
+int adjoint_f32(Ctx *c, int kernel, double dt, int nsteps, int ckpt, double *J, double *gx,
+                double *gv);
+
+// Device scratch owned by one call, freed when it goes out of scope.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t count) {
+        PIC_HIP(hipMalloc(&p, (count ? count : 1) * sizeof(T)));
+        return PIC_OK;
+    }
+};
+
 // ------------------------------------------------------------ constants
 constexpr int RED_BLOCKS = 256;
 constexpr int RED_THREADS = 256;

@@ -42,6 +42,7 @@
 #include <cmath>
 
 #include "pic_internal.h"
+#include "pic_push32.h"
 
 namespace pic {
 
@@ -161,97 +162,6 @@ __device__ __forceinline__ int fixed_exponent(float w, double qmax, int64_t coun
     e1 = (m1 == 0.5 ? e1 - 1 : e1) - 30;  // ceil(log2(.)) - 30
     e2 = (m2 == 0.5 ? e2 - 1 : e2) - 22;
     return e1 > e2 ? e1 : e2;
-}
-
-__device__ __forceinline__ int drift1s(float o, float v, float dtd, float &onew) {
-    float t = __builtin_fmaf(v, dtd, o);
-    float s = floorf(t);
-    float r = t - s;
-    if (r >= 1.0f) {
-        r = 0.0f;
-        s = s + 1.0f;
-    }
-    onew = r;
-    return (int)s;
-}
-
-// Field at a particle from the nodes (u, v) of its stencil, u, v in [0, S):
-// CIC bilinear or M6 tensor-product weights, FP32, in a fixed operation order.
-template <int K, int BZ, class FE, class FB>
-__device__ __forceinline__ void interp32(float fx, float fy, FE node_e, FB node_b, float &e1,
-                                         float &e2, float &b3) {
-    if (K == PIC_CIC) {
-        float sx = 1.0f - fx, sy = 1.0f - fy;
-        float a0 = sx * sy, a1 = fx * sy, a2 = fx * fy, a3 = sx * fy;
-        float2 E00 = node_e(0, 0), E10 = node_e(1, 0), E11 = node_e(1, 1),
-               E01 = node_e(0, 1);
-        e1 = a0 * E00.x;
-        e1 = __builtin_fmaf(a1, E10.x, e1);
-        e1 = __builtin_fmaf(a2, E11.x, e1);
-        e1 = __builtin_fmaf(a3, E01.x, e1);
-        e2 = a0 * E00.y;
-        e2 = __builtin_fmaf(a1, E10.y, e2);
-        e2 = __builtin_fmaf(a2, E11.y, e2);
-        e2 = __builtin_fmaf(a3, E01.y, e2);
-        if (BZ == PIC_BZ_GRID) {
-            b3 = a0 * node_b(0, 0);
-            b3 = __builtin_fmaf(a1, node_b(1, 0), b3);
-            b3 = __builtin_fmaf(a2, node_b(1, 1), b3);
-            b3 = __builtin_fmaf(a3, node_b(0, 1), b3);
-        }
-    } else {
-        float wx[6], wy[6];
-        picd::m6_weights_f(fx, wx);
-        picd::m6_weights_f(fy, wy);
-        e1 = 0.0f;
-        e2 = 0.0f;
-#pragma unroll
-        for (int u = 0; u < 6; ++u) {
-            float2 E0 = node_e(u, 0);
-            float t1 = wy[0] * E0.x, t2 = wy[0] * E0.y;
-#pragma unroll
-            for (int v = 1; v < 6; ++v) {
-                float2 E = node_e(u, v);
-                t1 = __builtin_fmaf(wy[v], E.x, t1);
-                t2 = __builtin_fmaf(wy[v], E.y, t2);
-            }
-            if (u == 0) {
-                e1 = wx[0] * t1;
-                e2 = wx[0] * t2;
-            } else {
-                e1 = __builtin_fmaf(wx[u], t1, e1);
-                e2 = __builtin_fmaf(wx[u], t2, e2);
-            }
-        }
-        if (BZ == PIC_BZ_GRID) {
-#pragma unroll
-            for (int u = 0; u < 6; ++u) {
-                float t3 = wy[0] * node_b(u, 0);
-#pragma unroll
-                for (int v = 1; v < 6; ++v) t3 = __builtin_fmaf(wy[v], node_b(u, v), t3);
-                b3 = u == 0 ? wx[0] * t3 : __builtin_fmaf(wx[u], t3, b3);
-            }
-        }
-    }
-}
-
-// Velocity update of the FP32 step from the gathered field: half kick,
-// rotation by bz, half kick (pushers_cic.jl:38-49 in FP32 with FMAs).
-template <int BZ>
-__device__ __forceinline__ void kick32(const SegF32 &a, float e1, float e2, float b3, float &ux,
-                                       float &uy) {
-    if (BZ == PIC_BZ_UNIFORM) b3 = a.bz0;
-    float v1 = __builtin_fmaf(a.hdt, e1, ux);
-    float v2 = __builtin_fmaf(a.hdt, e2, uy);
-    if (BZ != PIC_BZ_NONE) {
-        float tn = a.hdt * b3;
-        float sn = (2.0f * tn) / __builtin_fmaf(tn, tn, 1.0f);
-        v1 = __builtin_fmaf(v2, tn, v1);
-        v2 = __builtin_fmaf(-v1, sn, v2);
-        v1 = __builtin_fmaf(v2, tn, v1);
-    }
-    ux = __builtin_fmaf(a.hdt, e1, v1);
-    uy = __builtin_fmaf(a.hdt, e2, v2);
 }
 
 // Calls add(u, v, q) with the FP32 charge q of stencil node (u, v) of a
@@ -394,7 +304,7 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
             interp32<K, BZ>(
                 fx, fy, [&](int u, int v) { return te[(lx + u) + W.tw * (ly + v)]; },
                 [&](int u, int v) { return tb[(lx + u) + W.tw * (ly + v)]; }, e1, e2, b3);
-            kick32<BZ>(a, e1, e2, b3, ux, uy);
+            kick32<BZ>(a.hdt, a.bz0, e1, e2, b3, ux, uy);
         }
         if (DRIFT) {
             float nfx, nfy;
@@ -486,7 +396,7 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f32(SegF32 a) {
                                     (int64_t)nx * ((cj - HL + v) & my)];
                     },
                     e1, e2, b3);
-                kick32<BZ>(a, e1, e2, b3, ux, uy);
+                kick32<BZ>(a.hdt, a.bz0, e1, e2, b3, ux, uy);
             }
             if (DRIFT) {
                 float nfx, nfy;

@@ -81,3 +81,76 @@ def test_adjoint_finite_differences(gpu, kernel):
     qm[2] -= h * u
     fdv = (fwd(qp) - fwd(qm)) / (2 * h)
     assert float(np.sum(gv[0] * u)) == pytest.approx(fdv, rel=1e-4)
+
+
+# ------------------------------------------------------------ FP32 state
+# The FP32 adjoint differentiates the FP32 loop (cell-relative FP32 particle
+# arithmetic, FP64 charge sums and Poisson solve); its reference values are
+# the FP64 adjoint and central differences of FP64 runs.  Over 20 steps the
+# two trajectories stay within a few FP32 roundings of each other, so the
+# gradients agree to about 1e-6.
+
+
+@pytest.mark.parametrize("kernel", ["cic", "m6"])
+def test_adjoint_f32_against_f64_and_fd(gpu, kernel):
+    pic = gpu
+    F32 = pic._lib.PIC_STATE_F32
+    n, nx, alpha, dt, steps, ck = 65536, 32, 0.05, 0.1, 20, 5
+    k = pic.CloudInCell() if kernel == "cic" else pic.M6()
+    mesh, sob, p = _landau(pic, n, nx, alpha)
+    sim = pic.Simulation(mesh, p, k)
+    J64, gx64, gv64 = sim.adjoint(dt, steps, ck)
+    sim.close()
+    sim = pic.Simulation(mesh, p, k, state=F32)
+    before = sim.state_f32()
+    J32, gx32, gv32 = sim.adjoint(dt, steps, ck)
+    after = sim.state_f32()
+    sim.close()
+    # the FP32 state is left bit for bit as it was
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b)
+    # J is the FP32 loop's own field energy (same sorts, same charge sums)
+    sim = pic.Simulation(mesh, p, k, state=F32)
+    e = sim.run(dt, steps, ck)
+    sim.close()
+    assert J32 == pytest.approx(e[-1, 0] + e[-1, 1], rel=1e-12)
+    assert J32 == pytest.approx(J64, rel=1e-5)
+    for a, b in ((gx32, gx64), (gv32, gv64)):
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 2e-5
+    dxa = pic.sampling.dnewton_dalpha(p[0], alpha, 0.5)
+
+    def fwd(q):
+        s = pic.Simulation(mesh, q, k)
+        en = s.run(dt, steps)
+        s.close()
+        return en[-1, 0] + en[-1, 1]
+
+    h = 1e-6
+    fd = (fwd(_landau(pic, n, nx, alpha + h, sob=sob)[2])
+          - fwd(_landau(pic, n, nx, alpha - h, sob=sob)[2])) / (2 * h)
+    assert float(np.sum(gx32[0] * dxa)) == pytest.approx(fd, rel=1e-5)
+    u = np.random.default_rng(7).choice([-1.0, 1.0], size=n)
+    qp, qm = p.copy(order="F"), p.copy(order="F")
+    qp[2] += h * u
+    qm[2] -= h * u
+    fdv = (fwd(qp) - fwd(qm)) / (2 * h)
+    assert float(np.sum(gv32[0] * u)) == pytest.approx(fdv, rel=1e-4)
+
+
+def test_adjoint_f32_checkpoint_independent(gpu):
+    """FP32: checkpoints every 1, 3 or all steps give the same gradient up to
+    the order of the FP64 charge sums (the recomputed positions are
+    bit-identical to the forward's)."""
+    pic = gpu
+    F32 = pic._lib.PIC_STATE_F32
+    n, nx, dt, steps = 16384, 32, 0.1, 8
+    mesh, sob, p = _landau(pic, n, nx, 0.05)
+    res = []
+    for ck in (1, 3, 8):
+        sim = pic.Simulation(mesh, p, pic.M6(), state=F32)
+        res.append(sim.adjoint(dt, steps, ck))
+        sim.close()
+    for J, gx, gv in res[1:]:
+        assert J == pytest.approx(res[0][0], rel=1e-6)
+        assert np.linalg.norm(gx - res[0][1]) / np.linalg.norm(res[0][1]) < 1e-5
+        assert np.linalg.norm(gv - res[0][2]) / np.linalg.norm(res[0][2]) < 1e-5
