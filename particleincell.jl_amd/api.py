@@ -465,15 +465,16 @@ class Simulation:
     def grid(self, which):
         return self.ctx.get_grid(which)
 
-    def adjoint(self, dt, nsteps, ckpt):
+    def adjoint(self, dt, nsteps, ckpt, sort_every=0):
         """(J, gx, gv): J = sum(ex^2+ey^2) dx dy of the last step's field after
         nsteps loop iterations, gx = dJ/d(x0, y0), gv = dJ/d(vx0, vy0), each
-        2 x n in the particle order of the context (pic_adjoint)."""
+        2 x n in the particle order of the context (pic_adjoint).  In the FP32
+        state the sweeps also lane-sort every sort_every steps."""
         J = C.c_double()
         gx = np.zeros((2, self.n), order="F")
         gv = np.zeros((2, self.n), order="F")
         L.call("pic_adjoint", self.ctx.h, self.kernel, float(dt), int(nsteps), int(ckpt),
-               C.byref(J), gx.ctypes.data, gv.ctypes.data)
+               int(sort_every), C.byref(J), gx.ctypes.data, gv.ctypes.data)
         return J.value, gx, gv
 
     def set_timing(self, on=True):

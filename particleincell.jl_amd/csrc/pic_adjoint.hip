@@ -390,17 +390,19 @@ static int adjoint_impl(Ctx *c, double dt, int nsteps, int ckpt, double *J, doub
 
 using namespace pic;
 
-extern "C" int pic_adjoint(pic_ctx *ctx, int kernel, double dt, int nsteps, int ckpt, double *J,
-                           double *gx, double *gv) {
+extern "C" int pic_adjoint(pic_ctx *ctx, int kernel, double dt, int nsteps, int ckpt,
+                           int sort_every, double *J, double *gx, double *gv) {
     PIC_CHECK(ctx && J, PIC_ERR_ARG, "null pointer");
     PIC_CHECK(kernel == PIC_CIC || kernel == PIC_M6, PIC_ERR_ARG, "kernel %d", kernel);
     PIC_CHECK(nsteps >= 1 && ckpt >= 1, PIC_ERR_ARG, "nsteps %d, ckpt %d", nsteps, ckpt);
+    PIC_CHECK(sort_every >= 0, PIC_ERR_ARG, "sort_every %d", sort_every);
     Ctx *c = ctx_of(ctx);
     PIC_HIP(hipSetDevice(c->device));
     PIC_CHECK(c->uniform_w, PIC_ERR_STATE, "the adjoint needs uniform particle weights");
     PIC_CHECK(c->bz_mode == PIC_BZ_NONE, PIC_ERR_STATE, "the adjoint supports bz = 0 only");
     PIC_CHECK(c->n > 0, PIC_ERR_STATE, "no particles");
-    if (c->state == PIC_STATE_F32) return adjoint_f32(c, kernel, dt, nsteps, ckpt, J, gx, gv);
+    if (c->state == PIC_STATE_F32)
+        return adjoint_f32(c, kernel, dt, nsteps, ckpt, sort_every, J, gx, gv);
     return kernel == PIC_CIC ? adjoint_impl<PIC_CIC>(c, dt, nsteps, ckpt, J, gx, gv)
                              : adjoint_impl<PIC_M6>(c, dt, nsteps, ckpt, J, gx, gv);
 }

@@ -29,6 +29,7 @@
 
 #include "pic_internal.h"
 #include "pic_push32.h"
+#include "pic_segcore.h"
 
 namespace pic {
 
@@ -112,42 +113,31 @@ __global__ __launch_bounds__(AT) void k_adj32_pos(int64_t n, const int32_t *__re
     po[k] = make_float2(o.x, o.y);
 }
 
-// Adjoint of the gather and kick of step s < last at x'_s.  lam = (A, B).
-template <int K>
-__global__ __launch_bounds__(AT) void k_adj32_gather(int64_t n, const int32_t *__restrict__ pc,
-                                                     const float2 *__restrict__ po, float4 *lam,
-                                                     const float2 *__restrict__ e32, double *mux,
-                                                     double *muy, Grid32 g, float dt, float dtdx,
-                                                     float dtdy) {
-    constexpr int S = St32<K>::S, HL = St32<K>::HL;
-    const int64_t k = (int64_t)blockIdx.x * AT + threadIdx.x;
-    if (k >= n) return;
-    const int mx = g.nx - 1, my = g.ny - 1;
-    const int c = pc[k];
-    const int ci = c & mx, cj = c >> g.lognx;
-    const float2 f = po[k];
+// Adjoint of the gather and kick of step s < last at x'_s for one particle:
+// lam = (A, B) -> (A + dt (Bt . grad G E_s), Bt) with Bt = B + dt A, and
+// add_mu(u, v, qx, qy) receives dt Bt times the shape weight of node (u, v).
+template <int K, class FE, class FA>
+__device__ __forceinline__ float4 adj_gather_one(float fx, float fy, float4 L, float dt,
+                                                 float dtdx, float dtdy, FE node_e, FA add_mu) {
+    constexpr int S = St32<K>::S;
     float wx[6], wy[6], dwx[6], dwy[6];
-    shape32<K>(f.x, wx, dwx);
-    shape32<K>(f.y, wy, dwy);
-    const float4 L = lam[k];
+    shape32<K>(fx, wx, dwx);
+    shape32<K>(fy, wy, dwy);
     const float tx = __builtin_fmaf(dt, L.x, L.z), ty = __builtin_fmaf(dt, L.y, L.w);
     const float qx = dt * tx, qy = dt * ty;
     float gxx = 0.0f, gxy = 0.0f, gyx = 0.0f, gyy = 0.0f;  // d ex/dfx, d ex/dfy, d ey/dfx, ...
 #pragma unroll
     for (int u = 0; u < S; ++u) {
-        const int64_t iu = (ci - HL + u) & mx;
         float sx1 = 0.0f, sx2 = 0.0f, sy1 = 0.0f, sy2 = 0.0f;
 #pragma unroll
         for (int v = 0; v < S; ++v) {
-            const int64_t gi = iu + (int64_t)g.nx * ((cj - HL + v) & my);
-            const float2 E = e32[gi];
+            const float2 E = node_e(u, v);
             sx1 = __builtin_fmaf(wy[v], E.x, sx1);
             sx2 = __builtin_fmaf(wy[v], E.y, sx2);
             sy1 = __builtin_fmaf(dwy[v], E.x, sy1);
             sy2 = __builtin_fmaf(dwy[v], E.y, sy2);
             const float wab = wx[u] * wy[v];
-            unsafeAtomicAdd(&mux[gi], (double)(qx * wab));
-            unsafeAtomicAdd(&muy[gi], (double)(qy * wab));
+            add_mu(u, v, qx * wab, qy * wab);
         }
         gxx = __builtin_fmaf(dwx[u], sx1, gxx);
         gyx = __builtin_fmaf(dwx[u], sx2, gyx);
@@ -156,44 +146,227 @@ __global__ __launch_bounds__(AT) void k_adj32_gather(int64_t n, const int32_t *_
     }
     const float ax = __builtin_fmaf(dtdx, __builtin_fmaf(tx, gxx, ty * gyx), L.x);
     const float ay = __builtin_fmaf(dtdy, __builtin_fmaf(tx, gxy, ty * gyy), L.y);
-    lam[k] = make_float4(ax, ay, tx, ty);
+    return make_float4(ax, ay, tx, ty);
 }
 
-// Adjoint of the deposit at x'_s: A += (w / dx dy) grad W . lrho, with
-// cx = w / (dx dy dx), cy = w / (dx dy dy).
-template <int K>
-__global__ __launch_bounds__(AT) void k_adj32_deposit(int64_t n, const int32_t *__restrict__ pc,
-                                                      const float2 *__restrict__ po, float4 *lam,
-                                                      const double *__restrict__ lrho, Grid32 g,
-                                                      float cx, float cy) {
-    constexpr int S = St32<K>::S, HL = St32<K>::HL;
-    const int64_t k = (int64_t)blockIdx.x * AT + threadIdx.x;
-    if (k >= n) return;
-    const int mx = g.nx - 1, my = g.ny - 1;
-    const int c = pc[k];
-    const int ci = c & mx, cj = c >> g.lognx;
-    const float2 f = po[k];
+// Adjoint of the deposit at x'_s for one particle: A += (w / dx dy) grad W .
+// lrho, with cx = w / (dx dy dx), cy = w / (dx dy dy); node_r(u, v) is lrho
+// at node (u, v) rounded to FP32.
+template <int K, class FR>
+__device__ __forceinline__ float4 adj_deposit_one(float fx, float fy, float4 L, float cx,
+                                                  float cy, FR node_r) {
+    constexpr int S = St32<K>::S;
     float wx[6], wy[6], dwx[6], dwy[6];
-    shape32<K>(f.x, wx, dwx);
-    shape32<K>(f.y, wy, dwy);
+    shape32<K>(fx, wx, dwx);
+    shape32<K>(fy, wy, dwy);
     float gx = 0.0f, gy = 0.0f;
 #pragma unroll
     for (int u = 0; u < S; ++u) {
-        const int64_t iu = (ci - HL + u) & mx;
         float s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
         for (int v = 0; v < S; ++v) {
-            const float r = (float)lrho[iu + (int64_t)g.nx * ((cj - HL + v) & my)];
+            const float r = node_r(u, v);
             s1 = __builtin_fmaf(wy[v], r, s1);
             s2 = __builtin_fmaf(dwy[v], r, s2);
         }
         gx = __builtin_fmaf(dwx[u], s1, gx);
         gy = __builtin_fmaf(wx[u], s2, gy);
     }
-    float4 L = lam[k];
     L.x = __builtin_fmaf(cx, gx, L.x);
     L.y = __builtin_fmaf(cy, gy, L.y);
-    lam[k] = L;
+    return L;
+}
+
+// Arguments of the reverse passes over the positions of one step, in the
+// slot order of the lane sort whose offsets `off` are.
+struct AdjSeg32 {
+    const int32_t *pc;     // cell of x'_s per slot
+    const float2 *po;      // in-cell offset of x'_s per slot
+    float4 *lam;           // (A, B) per slot
+    const float2 *e32;     // gather adjoint: E_s (FP32 copy)
+    const double *lrho;    // deposit adjoint: P^T mu
+    double *mux, *muy;     // gather adjoint: mu accumulators
+    int nx, ny, lognx, ncells;
+    float dt, dtdx, dtdy;  // gather adjoint: dt, dt/dx, dt/dy
+    float cx, cy;          // deposit adjoint
+    const int64_t *off;
+    int bps, margin, cap;
+    unsigned long long *slow;
+    uint32_t *dlist;
+    unsigned *dcount;
+};
+
+// Gather adjoint over one segment slice: E_s staged in an LDS window, mu
+// summed in FP64 LDS tiles (ds_add_f64) and flushed once with FP64 global
+// atomics; particles outside the window go to the deferred list.
+template <int K>
+__global__ __launch_bounds__(SEG_T) void k_adjseg_gather(AdjSeg32 a) {
+    using St = SegStencil<K>;
+    constexpr int S = St::S, HL = St::HL;
+    extern __shared__ __align__(16) char smem_adj[];
+    const int tid = threadIdx.x;
+    const int nx = a.nx, ny = a.ny, mx = nx - 1, my = ny - 1;
+    const int s = blockIdx.x / a.bps, bi = blockIdx.x - s * a.bps;
+    const int ce = min(64 * s + 64, a.ncells);
+    const int64_t s0 = a.off[64 * s], s1 = a.off[ce];
+    const int64_t len = s1 - s0;
+    const int64_t p0 = s0 + len * bi / a.bps, p1 = s0 + len * (bi + 1) / a.bps;
+    const Win W = seg_window<S, HL>(s, nx, ny, a.ncells, a.margin, a.cap);
+    const bool fits = W.tw * W.th <= a.cap;
+    float2 *te = reinterpret_cast<float2 *>(smem_adj);
+    double *tmx = reinterpret_cast<double *>(smem_adj + 8 * (size_t)a.cap), *tmy = tmx + a.cap;
+    if (fits) {
+        const int nt = W.tw * W.th;
+        for (int t = tid; t < nt; t += SEG_T) {
+            const int u = t % W.tw, v = t / W.tw;
+            te[t] = a.e32[(int64_t)((W.x0 + u) & mx) + (int64_t)nx * ((W.y0 + v) & my)];
+            tmx[t] = 0.0;
+            tmy[t] = 0.0;
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = (uint32_t)p0 + tid; p < (uint32_t)p1; p += SEG_T) {
+        const int c = a.pc[p];
+        const int ci = c & mx, cj = c >> a.lognx;
+        const int lx = (ci - HL - W.x0) & mx, ly = (cj - HL - W.y0) & my;
+        const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
+        defer_push(!in, p, a.dlist, a.dcount);
+        if (!in) continue;
+        const float2 f = a.po[p];
+        a.lam[p] = adj_gather_one<K>(
+            f.x, f.y, a.lam[p], a.dt, a.dtdx, a.dtdy,
+            [&](int u, int v) { return te[(lx + u) + W.tw * (ly + v)]; },
+            [&](int u, int v, float qa, float qb) {
+                const int t = (lx + u) + W.tw * (ly + v);
+                __hip_atomic_fetch_add(&tmx[t], (double)qa, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(&tmy[t], (double)qb, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            });
+    }
+    if (!fits) return;
+    __syncthreads();
+    const int nt = W.tw * W.th;
+    for (int t = tid; t < nt; t += SEG_T) {
+        const double qa = tmx[t], qb = tmy[t];
+        if (qa != 0.0 || qb != 0.0) {
+            const int u = t % W.tw, v = t / W.tw;
+            const int64_t g = (int64_t)((W.x0 + u) & mx) + (int64_t)nx * ((W.y0 + v) & my);
+            unsafeAtomicAdd(&a.mux[g], qa);
+            unsafeAtomicAdd(&a.muy[g], qb);
+        }
+    }
+}
+
+// The deferred particles of k_adjseg_gather, from global memory.
+template <int K>
+__global__ __launch_bounds__(DEFER_T) void k_adjdefer_gather(AdjSeg32 a) {
+    constexpr int HL = SegStencil<K>::HL;
+    const unsigned cnt = __hip_atomic_load(a.dcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int nx = a.nx, mx = a.nx - 1, my = a.ny - 1;
+    for (unsigned i = blockIdx.x * DEFER_T + threadIdx.x; i < cnt; i += gridDim.x * DEFER_T) {
+        const uint32_t p = a.dlist[i];
+        const int c = a.pc[p];
+        const int ci = c & mx, cj = c >> a.lognx;
+        auto node = [&](int u, int v) {
+            return (int64_t)((ci - HL + u) & mx) + (int64_t)nx * ((cj - HL + v) & my);
+        };
+        const float2 f = a.po[p];
+        a.lam[p] = adj_gather_one<K>(
+            f.x, f.y, a.lam[p], a.dt, a.dtdx, a.dtdy,
+            [&](int u, int v) { return a.e32[node(u, v)]; },
+            [&](int u, int v, float qa, float qb) {
+                const int64_t g = node(u, v);
+                unsafeAtomicAdd(&a.mux[g], (double)qa);
+                unsafeAtomicAdd(&a.muy[g], (double)qb);
+            });
+    }
+    defer_finish(cnt, a.dcount, a.slow);
+}
+
+// Deposit adjoint over one segment slice: lrho (rounded to FP32) staged in
+// an LDS window.
+template <int K>
+__global__ __launch_bounds__(SEG_T) void k_adjseg_deposit(AdjSeg32 a) {
+    using St = SegStencil<K>;
+    constexpr int S = St::S, HL = St::HL;
+    extern __shared__ __align__(16) char smem_adj[];
+    const int tid = threadIdx.x;
+    const int nx = a.nx, ny = a.ny, mx = nx - 1, my = ny - 1;
+    const int s = blockIdx.x / a.bps, bi = blockIdx.x - s * a.bps;
+    const int ce = min(64 * s + 64, a.ncells);
+    const int64_t s0 = a.off[64 * s], s1 = a.off[ce];
+    const int64_t len = s1 - s0;
+    const int64_t p0 = s0 + len * bi / a.bps, p1 = s0 + len * (bi + 1) / a.bps;
+    const Win W = seg_window<S, HL>(s, nx, ny, a.ncells, a.margin, a.cap);
+    const bool fits = W.tw * W.th <= a.cap;
+    float *tr = reinterpret_cast<float *>(smem_adj);
+    if (fits) {
+        const int nt = W.tw * W.th;
+        for (int t = tid; t < nt; t += SEG_T) {
+            const int u = t % W.tw, v = t / W.tw;
+            tr[t] = (float)a.lrho[(int64_t)((W.x0 + u) & mx) + (int64_t)nx * ((W.y0 + v) & my)];
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = (uint32_t)p0 + tid; p < (uint32_t)p1; p += SEG_T) {
+        const int c = a.pc[p];
+        const int ci = c & mx, cj = c >> a.lognx;
+        const int lx = (ci - HL - W.x0) & mx, ly = (cj - HL - W.y0) & my;
+        const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
+        defer_push(!in, p, a.dlist, a.dcount);
+        if (!in) continue;
+        const float2 f = a.po[p];
+        a.lam[p] = adj_deposit_one<K>(f.x, f.y, a.lam[p], a.cx, a.cy, [&](int u, int v) {
+            return tr[(lx + u) + W.tw * (ly + v)];
+        });
+    }
+}
+
+// The deferred particles of k_adjseg_deposit, from global memory.
+template <int K>
+__global__ __launch_bounds__(DEFER_T) void k_adjdefer_deposit(AdjSeg32 a) {
+    constexpr int HL = SegStencil<K>::HL;
+    const unsigned cnt = __hip_atomic_load(a.dcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int nx = a.nx, mx = a.nx - 1, my = a.ny - 1;
+    for (unsigned i = blockIdx.x * DEFER_T + threadIdx.x; i < cnt; i += gridDim.x * DEFER_T) {
+        const uint32_t p = a.dlist[i];
+        const int c = a.pc[p];
+        const int ci = c & mx, cj = c >> a.lognx;
+        const float2 f = a.po[p];
+        a.lam[p] = adj_deposit_one<K>(f.x, f.y, a.lam[p], a.cx, a.cy, [&](int u, int v) {
+            return (float)a.lrho[(int64_t)((ci - HL + u) & mx) +
+                                 (int64_t)nx * ((cj - HL + v) & my)];
+        });
+    }
+    defer_finish(cnt, a.dcount, a.slow);
+}
+
+// LDS per window node: FP32 field + two FP64 sums (gather adjoint), FP32
+// lrho (deposit adjoint).
+constexpr size_t ADJ_GATHER_NODE = 24, ADJ_DEPOSIT_NODE = 4;
+
+template <int K, bool GATHER>
+int launch_adjseg(const Ctx *c, AdjSeg32 a) {
+    const size_t per_node = GATHER ? ADJ_GATHER_NODE : ADJ_DEPOSIT_NODE;
+    a.cap = (int)(SEG_LDS_BYTES / per_node) & ~15;
+    a.margin = 32;
+    const size_t sm = (size_t)a.cap * per_node;
+    const void *k = GATHER ? reinterpret_cast<const void *>(k_adjseg_gather<K>)
+                           : reinterpret_cast<const void *>(k_adjseg_deposit<K>);
+    a.bps = seg_bps(c, k, sm);
+    const dim3 grid((unsigned)(((c->nx * c->ny + 63) / 64) * a.bps));
+    if (GATHER) {
+        hipLaunchKernelGGL(k_adjseg_gather<K>, grid, dim3(SEG_T), sm, c->stream, a);
+        hipLaunchKernelGGL(k_adjdefer_gather<K>, dim3(DEFER_BLOCKS), dim3(DEFER_T), 0, c->stream,
+                           a);
+    } else {
+        hipLaunchKernelGGL(k_adjseg_deposit<K>, grid, dim3(SEG_T), sm, c->stream, a);
+        hipLaunchKernelGGL(k_adjdefer_deposit<K>, dim3(DEFER_BLOCKS), dim3(DEFER_T), 0,
+                           c->stream, a);
+    }
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
 }
 
 __global__ void k_adj32_seed(int64_t m, const double *ex, const double *ey, double dxdy,
@@ -213,8 +386,20 @@ __global__ void k_adj32_back(int64_t n, const uint32_t *__restrict__ perm,
     if (r < n) out[perm[r]] = in[r];
 }
 
+// Steps at which the loop lane-sorts the particles: every checkpoint step and,
+// with sort_every > 0, every multiple of it before the last step.  The
+// forward pass and the reverse sweep's recomputation follow the same list,
+// and the stable sort of a bit-identical state is the same permutation.
+struct SortPlan {
+    int ckpt, every, nsteps;
+    bool at(int s) const {
+        return s % ckpt == 0 || (every > 0 && s % every == 0 && s < nsteps - 1);
+    }
+};
+
 template <int K>
-int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, double *J, double *gx, double *gv) {
+int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, double *J,
+                   double *gx, double *gv) {
     const int64_t n = c->n;
     const int nx = c->nx, ny = c->ny;
     const int64_t m = (int64_t)nx * ny;
@@ -226,18 +411,26 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, double *J, double *g
     const double h = 0.5 * dt, dxdy = c->dx * c->dy;
     const int nseg = (nsteps + ckpt - 1) / ckpt;
     const int len = std::min(ckpt, nsteps);
+    const SortPlan plan{ckpt, sort_every, nsteps};
+    int nsub = 0;  // most sorts inside one segment (after its checkpoint)
+    for (int g = 0; g < nseg; ++g) {
+        int k = 0;
+        for (int s = g * ckpt + 1; s < std::min(nsteps, g * ckpt + ckpt); ++s) k += plan.at(s);
+        nsub = std::max(nsub, k);
+    }
     const double scale_rho = 1.0 / ((double)nx * (double)ny * dxdy);
     const double scale = 1.0 / ((double)nx * (double)ny);
     // the forward passes' FP32 constants (launch_seg32)
     const float hdt = (float)(0.5 * dt);
     const float dtdx_h = (float)(h / c->dx), dtdy_h = (float)(h / c->dy);
-    const float dtdx = (float)(dt / c->dx), dtdy = (float)(dt / c->dy);
+    const size_t noff = (size_t)m + 1;
 
     DevBuf<float2> Es, po;
     DevBuf<double> Elast, mu, lrho;
     DevBuf<int32_t> ckc, pc, tc;
     DevBuf<float4> cko, lam, t4;
-    DevBuf<uint32_t> perms;
+    DevBuf<uint32_t> perms, subperm;
+    DevBuf<int64_t> offs, suboff;
     PIC_TRY(Es.alloc((size_t)m * nsteps));
     PIC_TRY(Elast.alloc((size_t)2 * m));
     PIC_TRY(mu.alloc((size_t)2 * m));
@@ -245,53 +438,62 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, double *J, double *g
     PIC_TRY(ckc.alloc((size_t)n * nseg));
     PIC_TRY(cko.alloc((size_t)n * nseg));
     PIC_TRY(perms.alloc((size_t)n * nseg));
+    PIC_TRY(offs.alloc(noff * nseg));
+    PIC_TRY(subperm.alloc((size_t)n * nsub));
+    PIC_TRY(suboff.alloc(noff * nsub));
     PIC_TRY(pc.alloc((size_t)n * len));
     PIC_TRY(po.alloc((size_t)n * len));
     PIC_TRY(lam.alloc((size_t)n));
     PIC_TRY(t4.alloc((size_t)n));
     PIC_TRY(tc.alloc((size_t)n));
     PIC_TRY(ensure_energy(c, nsteps));
+    PIC_TRY(ensure_defer(c));
     if (c->pending_perm) PIC_TRY(settle_state(c));
     PIC_HIP(hipMemsetAsync(c->step_counter, 0, sizeof(int), c->stream));
 
-    auto save = [&](int g) -> int {
-        PIC_HIP(hipMemcpyAsync(ckc.p + (size_t)g * n, c->cell, (size_t)n * 4,
-                               hipMemcpyDeviceToDevice, c->stream));
-        PIC_HIP(hipMemcpyAsync(cko.p + (size_t)g * n, c->ov32, (size_t)n * 16,
-                               hipMemcpyDeviceToDevice, c->stream));
-        PIC_HIP(hipMemcpyAsync(perms.p + (size_t)g * n, c->perm2, (size_t)n * 4,
-                               hipMemcpyDeviceToDevice, c->stream));
+    auto dcopy = [&](void *dst, const void *src, size_t bytes) -> int {
+        PIC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
         return PIC_OK;
     };
+    auto save = [&](int g) -> int {
+        PIC_TRY(dcopy(ckc.p + (size_t)g * n, c->cell, (size_t)n * 4));
+        PIC_TRY(dcopy(cko.p + (size_t)g * n, c->ov32, (size_t)n * 16));
+        PIC_TRY(dcopy(perms.p + (size_t)g * n, c->perm2, (size_t)n * 4));
+        PIC_TRY(dcopy(offs.p + noff * g, c->cell_off, noff * 8));
+        return PIC_OK;
+    };
+    // the state of checkpoint g, in its slot order, ready for segment passes
     auto restore = [&](int g) -> int {
-        PIC_HIP(hipMemcpyAsync(c->cell, ckc.p + (size_t)g * n, (size_t)n * 4,
-                               hipMemcpyDeviceToDevice, c->stream));
-        PIC_HIP(hipMemcpyAsync(c->ov32, cko.p + (size_t)g * n, (size_t)n * 16,
-                               hipMemcpyDeviceToDevice, c->stream));
+        PIC_TRY(dcopy(c->cell, ckc.p + (size_t)g * n, (size_t)n * 4));
+        PIC_TRY(dcopy(c->ov32, cko.p + (size_t)g * n, (size_t)n * 16));
+        PIC_TRY(dcopy(c->cell_off, offs.p + noff * g, noff * 8));
+        c->seg_valid = true;
+        c->pending_perm = nullptr;
         return PIC_OK;
     };
 
     // ---------------------------------------------------------- forward
     // checkpoint 0 holds x_0 (before the first half drift); checkpoint g > 0
-    // holds (x'_s, v_s) at s = g ckpt
+    // holds (x'_s, v_s) at s = g ckpt.  Sorts between checkpoints leave their
+    // permutation for the next pass to apply.
     PIC_TRY(sort_by_cell(c, nullptr, true));
     PIC_TRY(save(0));
     PIC_TRY(launch_fill(c, c->rho_acc, 0.0, m));
     PIC_TRY(launch_seg32(c, K, 0, dt, h));
     for (int s = 0; s < nsteps; ++s) {
         const bool last = s == nsteps - 1;
-        if (s > 0 && s % ckpt == 0) {
-            PIC_TRY(sort_by_cell(c, nullptr, true));
-            PIC_TRY(save(s / ckpt));
+        if (s > 0 && plan.at(s)) {
+            const bool ck = s % ckpt == 0;
+            PIC_TRY(sort_by_cell(c, nullptr, true, ck));
+            if (ck) PIC_TRY(save(s / ckpt));
         }
         PIC_TRY(poisson_solve_e(c, c->rho_acc, scale_rho, last ? 1 : 3, -2));
-        PIC_HIP(hipMemcpyAsync(Es.p + (size_t)m * s, c->e32, (size_t)m * 8,
-                               hipMemcpyDeviceToDevice, c->stream));
+        PIC_TRY(dcopy(Es.p + (size_t)m * s, c->e32, (size_t)m * 8));
         if (last) break;
         PIC_TRY(launch_seg32(c, K, 1, dt, dt));
     }
-    PIC_HIP(hipMemcpyAsync(Elast.p, c->ex, (size_t)m * 8, hipMemcpyDeviceToDevice, c->stream));
-    PIC_HIP(hipMemcpyAsync(Elast.p + m, c->ey, (size_t)m * 8, hipMemcpyDeviceToDevice, c->stream));
+    PIC_TRY(dcopy(Elast.p, c->ex, (size_t)m * 8));
+    PIC_TRY(dcopy(Elast.p + m, c->ey, (size_t)m * 8));
     {
         double e2[2];
         PIC_HIP(hipMemcpyAsync(e2, c->energy_hist + 2 * (nsteps - 1), 16, hipMemcpyDeviceToHost,
@@ -302,11 +504,28 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, double *J, double *g
 
     // ---------------------------------------------------------- reverse
     PIC_HIP(hipMemsetAsync(lam.p, 0, (size_t)n * 16, c->stream));
-    const float cx = (float)(c->w0 / (dxdy * c->dx)), cy = (float)(c->w0 / (dxdy * c->dy));
-    const float dtf = (float)dt, dtdx_g = (float)(dt / c->dx), dtdy_g = (float)(dt / c->dy);
+    AdjSeg32 a;
+    a.lam = lam.p;
+    a.mux = mu.p;
+    a.muy = mu.p + m;
+    a.lrho = lrho.p;
+    a.nx = nx;
+    a.ny = ny;
+    a.lognx = gr.lognx;
+    a.ncells = (int)m;
+    a.dt = (float)dt;
+    a.dtdx = (float)(dt / c->dx);
+    a.dtdy = (float)(dt / c->dy);
+    a.cx = (float)(c->w0 / (dxdy * c->dx));
+    a.cy = (float)(c->w0 / (dxdy * c->dy));
+    a.slow = c->slow_chunks;
+    a.dlist = c->defer;
+    a.dcount = c->defer_count;
+    std::vector<int> sub_of((size_t)len);
     for (int g = nseg - 1; g >= 0; --g) {
         const int s0 = g * ckpt, s1 = std::min(nsteps, s0 + ckpt);
         PIC_TRY(restore(g));
+        // recompute x'_s, s0 <= s < s1, in the slot order of the sort in force
         if (g == 0)
             hipLaunchKernelGGL((k_adj32_advance<K, false>), grid_for(n), dim3(AT), 0, c->stream,
                                n, c->cell, c->ov32, (const float2 *)nullptr, gr, hdt, dtdx_h,
@@ -314,35 +533,50 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, double *J, double *g
         else
             hipLaunchKernelGGL(k_adj32_pos, grid_for(n), dim3(AT), 0, c->stream, n,
                                (const int32_t *)c->cell, (const float4 *)c->ov32, pc.p, po.p);
-        for (int s = s0; s + 1 < s1; ++s) {
-            const size_t q = (size_t)(s + 1 - s0) * n;
-            hipLaunchKernelGGL((k_adj32_advance<K, true>), grid_for(n), dim3(AT), 0, c->stream,
-                               n, c->cell, c->ov32, (const float2 *)(Es.p + (size_t)m * s), gr,
-                               hdt, dtdx, dtdy, pc.p + q, po.p + q);
+        int ns = 0;
+        sub_of[0] = 0;
+        for (int s = s0 + 1; s < s1; ++s) {
+            PIC_TRY(dcopy(c->e32, Es.p + (size_t)m * (s - 1), (size_t)m * 8));
+            PIC_TRY(launch_seg32(c, K, 2, dt, dt));  // kick + drift, no deposit
+            if (plan.at(s)) {
+                PIC_TRY(sort_by_cell(c, nullptr, true));
+                PIC_TRY(dcopy(subperm.p + (size_t)ns * n, c->perm2, (size_t)n * 4));
+                PIC_TRY(dcopy(suboff.p + noff * ns, c->cell_off, noff * 8));
+                ++ns;
+            }
+            const size_t q = (size_t)(s - s0) * n;
+            hipLaunchKernelGGL(k_adj32_pos, grid_for(n), dim3(AT), 0, c->stream, n,
+                               (const int32_t *)c->cell, (const float4 *)c->ov32, pc.p + q,
+                               po.p + q);
+            sub_of[s - s0] = ns;
         }
         for (int s = s1 - 1; s >= s0; --s) {
             const size_t q = (size_t)(s - s0) * n;
+            const int j = sub_of[s - s0];
+            a.pc = pc.p + q;
+            a.po = po.p + q;
+            a.off = j == 0 ? offs.p + noff * g : suboff.p + noff * (j - 1);
             if (s == nsteps - 1) {
                 hipLaunchKernelGGL(k_adj32_seed, grid_for(m), dim3(AT), 0, c->stream, m,
                                    (const double *)Elast.p, (const double *)(Elast.p + m), dxdy,
                                    mu.p, mu.p + m);
             } else {
                 PIC_TRY(launch_fill(c, mu.p, 0.0, 2 * m));
-                hipLaunchKernelGGL((k_adj32_gather<K>), grid_for(n), dim3(AT), 0, c->stream, n,
-                                   (const int32_t *)(pc.p + q), (const float2 *)(po.p + q), lam.p,
-                                   (const float2 *)(Es.p + (size_t)m * s), mu.p, mu.p + m, gr,
-                                   dtf, dtdx_g, dtdy_g);
+                a.e32 = Es.p + (size_t)m * s;
+                PIC_TRY((launch_adjseg<K, true>(c, a)));
             }
             PIC_TRY(poisson_adjoint(c, mu.p, mu.p + m, lrho.p, scale));
-            hipLaunchKernelGGL((k_adj32_deposit<K>), grid_for(n), dim3(AT), 0, c->stream, n,
-                               (const int32_t *)(pc.p + q), (const float2 *)(po.p + q), lam.p,
-                               (const double *)lrho.p, gr, cx, cy);
+            PIC_TRY((launch_adjseg<K, false>(c, a)));
+            // the first step of a sort's slots: into the order before it
+            if (s == s0 || plan.at(s)) {
+                const uint32_t *pg = j == 0 ? perms.p + (size_t)g * n
+                                            : subperm.p + (size_t)(j - 1) * n;
+                hipLaunchKernelGGL((k_adj32_back<float4>), grid_for(n), dim3(AT), 0, c->stream,
+                                   n, pg, (const float4 *)lam.p, t4.p);
+                std::swap(lam.p, t4.p);
+                a.lam = lam.p;
+            }
         }
-        // into the slot order of the previous segment (or the caller's order)
-        hipLaunchKernelGGL((k_adj32_back<float4>), grid_for(n), dim3(AT), 0, c->stream, n,
-                           (const uint32_t *)(perms.p + (size_t)g * n), (const float4 *)lam.p,
-                           t4.p);
-        std::swap(lam.p, t4.p);
     }
     PIC_HIP(hipGetLastError());
     std::vector<float4> hl((size_t)n);
@@ -366,8 +600,8 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, double *J, double *g
                        (const uint32_t *)perms.p, (const int32_t *)c->cell, tc.p);
     hipLaunchKernelGGL((k_adj32_back<float4>), grid_for(n), dim3(AT), 0, c->stream, n,
                        (const uint32_t *)perms.p, (const float4 *)c->ov32, t4.p);
-    PIC_HIP(hipMemcpyAsync(c->cell, tc.p, (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
-    PIC_HIP(hipMemcpyAsync(c->ov32, t4.p, (size_t)n * 16, hipMemcpyDeviceToDevice, c->stream));
+    PIC_TRY(dcopy(c->cell, tc.p, (size_t)n * 4));
+    PIC_TRY(dcopy(c->ov32, t4.p, (size_t)n * 16));
     PIC_HIP(hipStreamSynchronize(c->stream));
     c->seg_valid = false;
     return PIC_OK;
@@ -375,10 +609,11 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, double *J, double *g
 
 }  // namespace
 
-int adjoint_f32(Ctx *c, int kernel, double dt, int nsteps, int ckpt, double *J, double *gx,
-                double *gv) {
-    return kernel == PIC_CIC ? adjoint32_impl<PIC_CIC>(c, dt, nsteps, ckpt, J, gx, gv)
-                             : adjoint32_impl<PIC_M6>(c, dt, nsteps, ckpt, J, gx, gv);
+int adjoint_f32(Ctx *c, int kernel, double dt, int nsteps, int ckpt, int sort_every, double *J,
+                double *gx, double *gv) {
+    return kernel == PIC_CIC
+               ? adjoint32_impl<PIC_CIC>(c, dt, nsteps, ckpt, sort_every, J, gx, gv)
+               : adjoint32_impl<PIC_M6>(c, dt, nsteps, ckpt, sort_every, J, gx, gv);
 }
 
 }  // namespace pic

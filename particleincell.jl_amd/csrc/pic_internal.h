@@ -172,8 +172,8 @@ void swap_alt_f64(Ctx *c);
 int launch_fill(Ctx *c, double *p, double v, int64_t n);
 
 // The adjoint sweep of the FP32 cell-relative state (pic_adjoint32.hip).
-int adjoint_f32(Ctx *c, int kernel, double dt, int nsteps, int ckpt, double *J, double *gx,
-                double *gv);
+int adjoint_f32(Ctx *c, int kernel, double dt, int nsteps, int ckpt, int sort_every, double *J,
+                double *gx, double *gv);
 
 // Device scratch owned by one call, freed when it goes out of scope.
 template <typename T>

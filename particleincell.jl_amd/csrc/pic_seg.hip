@@ -43,72 +43,9 @@
 
 #include "pic_internal.h"
 #include "pic_push32.h"
+#include "pic_segcore.h"
 
 namespace pic {
-
-constexpr int SEG_T = 512;  // threads per block
-
-template <int K>
-struct SegStencil;
-template <>
-struct SegStencil<PIC_CIC> {
-    static constexpr int S = 2, HL = 0;
-};
-template <>
-struct SegStencil<PIC_M6> {
-    static constexpr int S = 6, HL = 2;
-};
-
-struct Win {
-    int x0, y0, tw, th;  // global index of window node (0,0) (may be negative), size
-};
-
-// Window of segment s: its home cells widened by the stencil and a drift
-// margin, the margin reduced until the window fits `cap` nodes.  A window as
-// wide as the periodic grid plus the stencil covers every cell of that
-// direction (x0 = -HL then), duplicating S-1 columns.
-template <int S, int HL>
-__device__ Win seg_window(int s, int nx, int ny, int ncells, int margin, int cap) {
-    const int cf = 64 * s, cl = min(64 * s + 63, ncells - 1);
-    const int jh0 = cf / nx, jh1 = cl / nx;
-    int ih0, ih1;
-    if (jh0 == jh1) {
-        ih0 = cf - jh0 * nx;
-        ih1 = cl - jh0 * nx;
-    } else {
-        ih0 = 0;
-        ih1 = nx - 1;
-    }
-    const int hw = ih1 - ih0 + 1, hh = jh1 - jh0 + 1;
-    int M = margin;
-    int tw = 0, th = 0;
-    for (;; --M) {
-        tw = min(hw + S - 1 + 2 * M, nx + S - 1);
-        th = min(hh + S - 1 + 2 * M, ny + S - 1);
-        if (tw * th <= cap || M == 0) break;
-    }
-    Win w;
-    w.tw = tw;
-    w.th = th;
-    w.x0 = tw == nx + S - 1 ? -HL : ih0 - HL - M;
-    w.y0 = th == ny + S - 1 ? -HL : jh0 - HL - M;
-    return w;
-}
-
-// End of a deferred-list kernel: block 0 adds the list length to the
-// slow-path counter; the last block to arrive (every block has read the
-// count by then) clears count and arrivals for the next pass.
-__device__ __forceinline__ void defer_finish(unsigned cnt, unsigned *dcount,
-                                             unsigned long long *slow) {
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    if (blockIdx.x == 0 && cnt) atomicAdd(slow, (unsigned long long)cnt);
-    const unsigned arrived = atomicAdd(&dcount[1], 1u);
-    if (arrived == gridDim.x - 1) {
-        __hip_atomic_store(&dcount[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&dcount[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-}
 
 // ============================================================ FP32 state
 
@@ -216,24 +153,6 @@ __device__ __forceinline__ void charge32_fixed(float fx, float fy, float wu, F &
             for (int v = 0; v < 6; ++v) add(u, v, n(wx[u], wy[v]));
     }
 }
-
-// Appends `entry` for every lane with `want` set (one atomic per wave).  The
-// leader's slot is broadcast with v_readlane, not a shuffle: a shuffle is an
-// LDS instruction, and waiting for its result would also wait for every LDS
-// atomic the wave still has in flight.
-__device__ __forceinline__ void defer_push(bool want, uint32_t entry, uint32_t *list,
-                                           unsigned *count) {
-    const unsigned long long b = __ballot(want);
-    if (!b) return;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)b) - 1;
-    unsigned base = 0;
-    if (lane == leader) base = atomicAdd(count, (unsigned)__popcll(b));
-    base = (unsigned)__builtin_amdgcn_readlane((int)base, leader);
-    if (want) list[base + (unsigned)__popcll(b & ((1ull << lane) - 1ull))] = entry;
-}
-
-constexpr uint32_t DEFER_DEPOSIT_ONLY = 0x80000000u;
 
 // The particles of one segment slice.  A particle whose stencil lies outside
 // the block's LDS window is not touched here: it goes to the deferred list
@@ -359,8 +278,6 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
         }
     }
 }
-
-constexpr int DEFER_T = 256, DEFER_BLOCKS = 256;
 
 // Completes the particles k_seg_f32 deferred, from global memory (same
 // arithmetic), adds their number to the slow-path counter, and the last
@@ -727,29 +644,6 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f64(SegF64 a) {
 }
 
 // ============================================================== dispatch
-
-// LDS per block for the window: FP32 8 B field (+4 B bz) + 8 B charge
-// (pair or FP64), FP64 16 B field (+8 B bz) + 8 B charge.  40 KiB lets four
-// 512-thread blocks share a CU's 160 KiB; an M6 window of a 64-cell row
-// segment then keeps a drift margin of 11 cells (FP32) or 7 cells (FP64).
-constexpr size_t SEG_LDS_BYTES = 40 * 1024;
-
-// Blocks per segment: as many as keep every CU busy in one round of
-// resident blocks (the occupancy of this kernel instantiation at this LDS
-// size), at least one.  A larger grid than one round leaves a partial last
-// round; a grid with more segments than resident blocks takes several full
-// rounds.
-static int seg_bps(const Ctx *c, const void *kernel, size_t sm) {
-    const int nseg = (c->nx * c->ny + 63) / 64;
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, SEG_T, sm) != hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
-    int bps = c->ncu * per_cu / nseg;
-    if (bps < 1) bps = 1;
-    if (bps > 64) bps = 64;
-    return bps;
-}
 
 // Launches the pass kernel `k` over every segment, then its deferred-list
 // kernel `kd`.
