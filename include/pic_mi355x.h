@@ -269,6 +269,18 @@ int pic_get_timing(pic_ctx *ctx, double *pass_ms, int64_t *passes, double *solve
 int pic_adjoint(pic_ctx *ctx, int kernel, double dt, int nsteps, int ckpt, int sort_every,
                 double *J, double *gx, double *gv);
 
+/* Device memory of pic_adjoint's reverse sweep in the FP32 state.  Inside a
+ * checkpoint segment the sweep keeps the positions of `block` consecutive
+ * steps at once: it recomputes the segment once to store restart points
+ * (state, sort offsets and permutation) every `block` steps, then each block
+ * from its restart point.  block = 0 (the default) takes the largest block,
+ * up to `ckpt`, whose buffers fit the free device memory (PIC_ERR_NOMEM when
+ * even one step does not); the gradients do not depend on it beyond the
+ * order of floating-point sums.  pic_get_adjoint_plan returns the block and
+ * the scratch bytes the last pic_adjoint used.  No reference counterpart. */
+int pic_set_adjoint_block(pic_ctx *ctx, int block);
+int pic_get_adjoint_plan(pic_ctx *ctx, int *block, int64_t *bytes);
+
 /* Number of particle chunks that took the global-memory slow path in the
  * fused kernels since the context was created (diagnostic). */
 int pic_slow_chunks(pic_ctx *ctx, uint64_t *count);

@@ -477,6 +477,17 @@ class Simulation:
                int(sort_every), C.byref(J), gx.ctypes.data, gv.ctypes.data)
         return J.value, gx, gv
 
+    def set_adjoint_block(self, block):
+        """Steps whose positions the FP32 adjoint sweep keeps at once (0: the
+        most that fits the free device memory) (pic_set_adjoint_block)."""
+        L.call("pic_set_adjoint_block", self.ctx.h, int(block))
+
+    def adjoint_plan(self):
+        """(block, scratch bytes) of the last adjoint sweep (pic_get_adjoint_plan)."""
+        b, nb = C.c_int(), C.c_int64()
+        L.call("pic_get_adjoint_plan", self.ctx.h, C.byref(b), C.byref(nb))
+        return b.value, nb.value
+
     def set_timing(self, on=True):
         L.call("pic_set_timing", self.ctx.h, 1 if on else 0)
 

@@ -406,3 +406,18 @@ extern "C" int pic_adjoint(pic_ctx *ctx, int kernel, double dt, int nsteps, int 
     return kernel == PIC_CIC ? adjoint_impl<PIC_CIC>(c, dt, nsteps, ckpt, J, gx, gv)
                              : adjoint_impl<PIC_M6>(c, dt, nsteps, ckpt, J, gx, gv);
 }
+
+extern "C" int pic_set_adjoint_block(pic_ctx *ctx, int block) {
+    PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
+    PIC_CHECK(block >= 0, PIC_ERR_ARG, "block %d", block);
+    ctx_of(ctx)->adj_block = block;
+    return PIC_OK;
+}
+
+extern "C" int pic_get_adjoint_plan(pic_ctx *ctx, int *block, int64_t *bytes) {
+    PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
+    const Ctx *c = ctx_of(ctx);
+    if (block) *block = c->adj_block_used;
+    if (bytes) *bytes = c->adj_bytes_used;
+    return PIC_OK;
+}

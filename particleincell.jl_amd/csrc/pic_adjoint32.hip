@@ -397,6 +397,37 @@ struct SortPlan {
     }
 };
 
+// Memory plan of the reverse sweep.  Within a segment of `len` steps the
+// positions of `q` consecutive steps are kept at once: the segment is
+// recomputed once to store restart points (state, sort offsets and the
+// permutation of a sort at that step) every q steps, then block by block
+// from its restart point while the block's positions are stored and
+// reversed.  q = len is the one-level sweep (one recomputation).
+struct AdjPlan {
+    int q = 0, nrestart = 0, nsub = 0;
+    int64_t bytes = 0;
+};
+
+static AdjPlan plan_for(const SortPlan &sp, int nsteps, int ckpt, int q, int64_t n, int64_t m,
+                        int64_t fixed) {
+    AdjPlan p;
+    p.q = q;
+    const int len = std::min(ckpt, nsteps);
+    p.nrestart = (len + q - 1) / q - 1;
+    for (int s0 = 0; s0 < nsteps; s0 += ckpt) {
+        const int s1 = std::min(nsteps, s0 + ckpt);
+        for (int t0 = s0; t0 < s1; t0 += q) {
+            int k = 0;
+            for (int s = t0 + 1; s < std::min(s1, t0 + q); ++s) k += sp.at(s);
+            p.nsub = std::max(p.nsub, k);
+        }
+    }
+    const int64_t noff8 = (m + 1) * 8;
+    p.bytes = fixed + (int64_t)p.nrestart * (24 * n + noff8) + (int64_t)q * 12 * n +
+              (int64_t)p.nsub * (4 * n + noff8);
+    return p;
+}
+
 template <int K>
 int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, double *J,
                    double *gx, double *gv) {
@@ -411,13 +442,7 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, doub
     const double h = 0.5 * dt, dxdy = c->dx * c->dy;
     const int nseg = (nsteps + ckpt - 1) / ckpt;
     const int len = std::min(ckpt, nsteps);
-    const SortPlan plan{ckpt, sort_every, nsteps};
-    int nsub = 0;  // most sorts inside one segment (after its checkpoint)
-    for (int g = 0; g < nseg; ++g) {
-        int k = 0;
-        for (int s = g * ckpt + 1; s < std::min(nsteps, g * ckpt + ckpt); ++s) k += plan.at(s);
-        nsub = std::max(nsub, k);
-    }
+    const SortPlan sp{ckpt, sort_every, nsteps};
     const double scale_rho = 1.0 / ((double)nx * (double)ny * dxdy);
     const double scale = 1.0 / ((double)nx * (double)ny);
     // the forward passes' FP32 constants (launch_seg32)
@@ -425,12 +450,40 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, doub
     const float dtdx_h = (float)(h / c->dx), dtdy_h = (float)(h / c->dy);
     const size_t noff = (size_t)m + 1;
 
+    PIC_TRY(ensure_energy(c, nsteps));
+    PIC_TRY(ensure_defer(c));
+    if (c->pending_perm) PIC_TRY(settle_state(c));
+    // the first sort also brings every sort and pass buffer into existence
+    // before the free memory is measured
+    PIC_TRY(sort_by_cell(c, nullptr, true));
+
+    const int64_t fixed = (int64_t)m * nsteps * 8 + 40 * m + (int64_t)nseg * (24 * n + 8 * noff) +
+                          36 * n;
+    AdjPlan pl;
+    if (c->adj_block > 0) {
+        pl = plan_for(sp, nsteps, ckpt, std::min(c->adj_block, len), n, m, fixed);
+    } else {
+        size_t free_b = 0, total_b = 0;
+        PIC_HIP(hipMemGetInfo(&free_b, &total_b));
+        const int64_t budget = (int64_t)free_b - (int64_t)(512u << 20) - (int64_t)(total_b / 100);
+        for (int q = len; q >= 1; --q) {
+            pl = plan_for(sp, nsteps, ckpt, q, n, m, fixed);
+            if (pl.bytes <= budget) break;
+        }
+        PIC_CHECK(pl.bytes <= budget, PIC_ERR_NOMEM,
+                  "adjoint needs %lld bytes of device memory at one stored step, %lld free",
+                  (long long)pl.bytes, (long long)budget);
+    }
+    c->adj_block_used = pl.q;
+    c->adj_bytes_used = pl.bytes;
+    const int q = pl.q;
+
     DevBuf<float2> Es, po;
     DevBuf<double> Elast, mu, lrho;
-    DevBuf<int32_t> ckc, pc, tc;
-    DevBuf<float4> cko, lam, t4;
-    DevBuf<uint32_t> perms, subperm;
-    DevBuf<int64_t> offs, suboff;
+    DevBuf<int32_t> ckc, pc, tc, rsc;
+    DevBuf<float4> cko, lam, t4, rso;
+    DevBuf<uint32_t> perms, subperm, rsp;
+    DevBuf<int64_t> offs, suboff, rsoff;
     PIC_TRY(Es.alloc((size_t)m * nsteps));
     PIC_TRY(Elast.alloc((size_t)2 * m));
     PIC_TRY(mu.alloc((size_t)2 * m));
@@ -439,53 +492,75 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, doub
     PIC_TRY(cko.alloc((size_t)n * nseg));
     PIC_TRY(perms.alloc((size_t)n * nseg));
     PIC_TRY(offs.alloc(noff * nseg));
-    PIC_TRY(subperm.alloc((size_t)n * nsub));
-    PIC_TRY(suboff.alloc(noff * nsub));
-    PIC_TRY(pc.alloc((size_t)n * len));
-    PIC_TRY(po.alloc((size_t)n * len));
+    PIC_TRY(rsc.alloc((size_t)n * pl.nrestart));
+    PIC_TRY(rso.alloc((size_t)n * pl.nrestart));
+    PIC_TRY(rsp.alloc((size_t)n * pl.nrestart));
+    PIC_TRY(rsoff.alloc(noff * pl.nrestart));
+    PIC_TRY(subperm.alloc((size_t)n * pl.nsub));
+    PIC_TRY(suboff.alloc(noff * pl.nsub));
+    PIC_TRY(pc.alloc((size_t)n * q));
+    PIC_TRY(po.alloc((size_t)n * q));
     PIC_TRY(lam.alloc((size_t)n));
     PIC_TRY(t4.alloc((size_t)n));
     PIC_TRY(tc.alloc((size_t)n));
-    PIC_TRY(ensure_energy(c, nsteps));
-    PIC_TRY(ensure_defer(c));
-    if (c->pending_perm) PIC_TRY(settle_state(c));
     PIC_HIP(hipMemsetAsync(c->step_counter, 0, sizeof(int), c->stream));
 
     auto dcopy = [&](void *dst, const void *src, size_t bytes) -> int {
         PIC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
         return PIC_OK;
     };
-    auto save = [&](int g) -> int {
-        PIC_TRY(dcopy(ckc.p + (size_t)g * n, c->cell, (size_t)n * 4));
-        PIC_TRY(dcopy(cko.p + (size_t)g * n, c->ov32, (size_t)n * 16));
-        PIC_TRY(dcopy(perms.p + (size_t)g * n, c->perm2, (size_t)n * 4));
-        PIC_TRY(dcopy(offs.p + noff * g, c->cell_off, noff * 8));
+    // a restart point: state, sort offsets and permutation, in slot `g` of
+    // the given buffers
+    auto save_to = [&](int32_t *cc, float4 *co, uint32_t *cp, int64_t *cf, int g) -> int {
+        PIC_TRY(dcopy(cc + (size_t)g * n, c->cell, (size_t)n * 4));
+        PIC_TRY(dcopy(co + (size_t)g * n, c->ov32, (size_t)n * 16));
+        PIC_TRY(dcopy(cp + (size_t)g * n, c->perm2, (size_t)n * 4));
+        PIC_TRY(dcopy(cf + noff * g, c->cell_off, noff * 8));
         return PIC_OK;
     };
-    // the state of checkpoint g, in its slot order, ready for segment passes
-    auto restore = [&](int g) -> int {
-        PIC_TRY(dcopy(c->cell, ckc.p + (size_t)g * n, (size_t)n * 4));
-        PIC_TRY(dcopy(c->ov32, cko.p + (size_t)g * n, (size_t)n * 16));
-        PIC_TRY(dcopy(c->cell_off, offs.p + noff * g, noff * 8));
+    // the state of a restart point, in its slot order, ready for segment passes
+    auto restore_from = [&](const int32_t *cc, const float4 *co, const int64_t *cf,
+                            int g) -> int {
+        PIC_TRY(dcopy(c->cell, cc + (size_t)g * n, (size_t)n * 4));
+        PIC_TRY(dcopy(c->ov32, co + (size_t)g * n, (size_t)n * 16));
+        PIC_TRY(dcopy(c->cell_off, cf + noff * g, noff * 8));
         c->seg_valid = true;
         c->pending_perm = nullptr;
         return PIC_OK;
+    };
+    // x'_s from x'_{s-1}: the loop's kick with E_{s-1} and drift (its segment
+    // pass without the deposit), then the sort of step s if there is one
+    auto recompute = [&](int s, bool *sorted) -> int {
+        PIC_TRY(dcopy(c->e32, Es.p + (size_t)m * (s - 1), (size_t)m * 8));
+        PIC_TRY(launch_seg32(c, K, 2, dt, dt));
+        *sorted = sp.at(s);
+        if (*sorted) PIC_TRY(sort_by_cell(c, nullptr, true));
+        return PIC_OK;
+    };
+    auto drift_half = [&](int32_t *pcq, float2 *poq) {
+        hipLaunchKernelGGL((k_adj32_advance<K, false>), grid_for(n), dim3(AT), 0, c->stream, n,
+                           c->cell, c->ov32, (const float2 *)nullptr, gr, hdt, dtdx_h, dtdy_h,
+                           pcq, poq);
+    };
+    auto positions = [&](size_t slot) {
+        hipLaunchKernelGGL(k_adj32_pos, grid_for(n), dim3(AT), 0, c->stream, n,
+                           (const int32_t *)c->cell, (const float4 *)c->ov32, pc.p + slot * n,
+                           po.p + slot * n);
     };
 
     // ---------------------------------------------------------- forward
     // checkpoint 0 holds x_0 (before the first half drift); checkpoint g > 0
     // holds (x'_s, v_s) at s = g ckpt.  Sorts between checkpoints leave their
     // permutation for the next pass to apply.
-    PIC_TRY(sort_by_cell(c, nullptr, true));
-    PIC_TRY(save(0));
+    PIC_TRY(save_to(ckc.p, cko.p, perms.p, offs.p, 0));
     PIC_TRY(launch_fill(c, c->rho_acc, 0.0, m));
     PIC_TRY(launch_seg32(c, K, 0, dt, h));
     for (int s = 0; s < nsteps; ++s) {
         const bool last = s == nsteps - 1;
-        if (s > 0 && plan.at(s)) {
+        if (s > 0 && sp.at(s)) {
             const bool ck = s % ckpt == 0;
             PIC_TRY(sort_by_cell(c, nullptr, true, ck));
-            if (ck) PIC_TRY(save(s / ckpt));
+            if (ck) PIC_TRY(save_to(ckc.p, cko.p, perms.p, offs.p, s / ckpt));
         }
         PIC_TRY(poisson_solve_e(c, c->rho_acc, scale_rho, last ? 1 : 3, -2));
         PIC_TRY(dcopy(Es.p + (size_t)m * s, c->e32, (size_t)m * 8));
@@ -521,60 +596,74 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, doub
     a.slow = c->slow_chunks;
     a.dlist = c->defer;
     a.dcount = c->defer_count;
-    std::vector<int> sub_of((size_t)len);
+    std::vector<int> sub_of((size_t)q);
     for (int g = nseg - 1; g >= 0; --g) {
         const int s0 = g * ckpt, s1 = std::min(nsteps, s0 + ckpt);
-        PIC_TRY(restore(g));
-        // recompute x'_s, s0 <= s < s1, in the slot order of the sort in force
-        if (g == 0)
-            hipLaunchKernelGGL((k_adj32_advance<K, false>), grid_for(n), dim3(AT), 0, c->stream,
-                               n, c->cell, c->ov32, (const float2 *)nullptr, gr, hdt, dtdx_h,
-                               dtdy_h, pc.p, po.p);
-        else
-            hipLaunchKernelGGL(k_adj32_pos, grid_for(n), dim3(AT), 0, c->stream, n,
-                               (const int32_t *)c->cell, (const float4 *)c->ov32, pc.p, po.p);
-        int ns = 0;
-        sub_of[0] = 0;
-        for (int s = s0 + 1; s < s1; ++s) {
-            PIC_TRY(dcopy(c->e32, Es.p + (size_t)m * (s - 1), (size_t)m * 8));
-            PIC_TRY(launch_seg32(c, K, 2, dt, dt));  // kick + drift, no deposit
-            if (plan.at(s)) {
-                PIC_TRY(sort_by_cell(c, nullptr, true));
-                PIC_TRY(dcopy(subperm.p + (size_t)ns * n, c->perm2, (size_t)n * 4));
-                PIC_TRY(dcopy(suboff.p + noff * ns, c->cell_off, noff * 8));
-                ++ns;
+        const int nblk = (s1 - s0 + q - 1) / q;
+        if (nblk > 1) {  // restart points every q steps through the segment
+            PIC_TRY(restore_from(ckc.p, cko.p, offs.p, g));
+            if (g == 0) drift_half(pc.p, po.p);
+            for (int s = s0 + 1; s < s0 + (nblk - 1) * q + 1; ++s) {
+                bool sorted;
+                PIC_TRY(recompute(s, &sorted));
+                if ((s - s0) % q == 0)
+                    PIC_TRY(save_to(rsc.p, rso.p, rsp.p, rsoff.p, (s - s0) / q - 1));
             }
-            const size_t q = (size_t)(s - s0) * n;
-            hipLaunchKernelGGL(k_adj32_pos, grid_for(n), dim3(AT), 0, c->stream, n,
-                               (const int32_t *)c->cell, (const float4 *)c->ov32, pc.p + q,
-                               po.p + q);
-            sub_of[s - s0] = ns;
         }
-        for (int s = s1 - 1; s >= s0; --s) {
-            const size_t q = (size_t)(s - s0) * n;
-            const int j = sub_of[s - s0];
-            a.pc = pc.p + q;
-            a.po = po.p + q;
-            a.off = j == 0 ? offs.p + noff * g : suboff.p + noff * (j - 1);
-            if (s == nsteps - 1) {
-                hipLaunchKernelGGL(k_adj32_seed, grid_for(m), dim3(AT), 0, c->stream, m,
-                                   (const double *)Elast.p, (const double *)(Elast.p + m), dxdy,
-                                   mu.p, mu.p + m);
+        for (int j = nblk - 1; j >= 0; --j) {
+            const int t0 = s0 + j * q, t1 = std::min(s1, t0 + q);
+            // the restart point of the block: its offsets and permutation
+            const int64_t *off0 = j == 0 ? offs.p + noff * g : rsoff.p + noff * (j - 1);
+            const uint32_t *perm0 = j == 0 ? perms.p + (size_t)g * n : rsp.p + (size_t)(j - 1) * n;
+            if (j == 0) {
+                PIC_TRY(restore_from(ckc.p, cko.p, offs.p, g));
+                if (g == 0)
+                    drift_half(pc.p, po.p);
+                else
+                    positions(0);
             } else {
-                PIC_TRY(launch_fill(c, mu.p, 0.0, 2 * m));
-                a.e32 = Es.p + (size_t)m * s;
-                PIC_TRY((launch_adjseg<K, true>(c, a)));
+                PIC_TRY(restore_from(rsc.p, rso.p, rsoff.p, j - 1));
+                positions(0);
             }
-            PIC_TRY(poisson_adjoint(c, mu.p, mu.p + m, lrho.p, scale));
-            PIC_TRY((launch_adjseg<K, false>(c, a)));
-            // the first step of a sort's slots: into the order before it
-            if (s == s0 || plan.at(s)) {
-                const uint32_t *pg = j == 0 ? perms.p + (size_t)g * n
-                                            : subperm.p + (size_t)(j - 1) * n;
-                hipLaunchKernelGGL((k_adj32_back<float4>), grid_for(n), dim3(AT), 0, c->stream,
-                                   n, pg, (const float4 *)lam.p, t4.p);
-                std::swap(lam.p, t4.p);
-                a.lam = lam.p;
+            // recompute x'_s, t0 <= s < t1, in the slot order of the sort in force
+            int ns = 0;
+            sub_of[0] = 0;
+            for (int s = t0 + 1; s < t1; ++s) {
+                bool sorted;
+                PIC_TRY(recompute(s, &sorted));
+                if (sorted) {
+                    PIC_TRY(dcopy(subperm.p + (size_t)ns * n, c->perm2, (size_t)n * 4));
+                    PIC_TRY(dcopy(suboff.p + noff * ns, c->cell_off, noff * 8));
+                    ++ns;
+                }
+                positions((size_t)(s - t0));
+                sub_of[s - t0] = ns;
+            }
+            for (int s = t1 - 1; s >= t0; --s) {
+                const size_t qs = (size_t)(s - t0) * n;
+                const int k = sub_of[s - t0];
+                a.pc = pc.p + qs;
+                a.po = po.p + qs;
+                a.off = k == 0 ? off0 : suboff.p + noff * (k - 1);
+                if (s == nsteps - 1) {
+                    hipLaunchKernelGGL(k_adj32_seed, grid_for(m), dim3(AT), 0, c->stream, m,
+                                       (const double *)Elast.p, (const double *)(Elast.p + m),
+                                       dxdy, mu.p, mu.p + m);
+                } else {
+                    PIC_TRY(launch_fill(c, mu.p, 0.0, 2 * m));
+                    a.e32 = Es.p + (size_t)m * s;
+                    PIC_TRY((launch_adjseg<K, true>(c, a)));
+                }
+                PIC_TRY(poisson_adjoint(c, mu.p, mu.p + m, lrho.p, scale));
+                PIC_TRY((launch_adjseg<K, false>(c, a)));
+                // a sort at step s: the slots into their order before it
+                if (sp.at(s)) {
+                    const uint32_t *pg = k == 0 ? perm0 : subperm.p + (size_t)(k - 1) * n;
+                    hipLaunchKernelGGL((k_adj32_back<float4>), grid_for(n), dim3(AT), 0,
+                                       c->stream, n, pg, (const float4 *)lam.p, t4.p);
+                    std::swap(lam.p, t4.p);
+                    a.lam = lam.p;
+                }
             }
         }
     }
@@ -595,7 +684,7 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, doub
     }
     // leave the particles as they were at entry: checkpoint 0 carried back
     // through its permutation into the caller's order
-    PIC_TRY(restore(0));
+    PIC_TRY(restore_from(ckc.p, cko.p, offs.p, 0));
     hipLaunchKernelGGL((k_adj32_back<int32_t>), grid_for(n), dim3(AT), 0, c->stream, n,
                        (const uint32_t *)perms.p, (const int32_t *)c->cell, tc.p);
     hipLaunchKernelGGL((k_adj32_back<float4>), grid_for(n), dim3(AT), 0, c->stream, n,

@@ -131,6 +131,12 @@ struct Ctx {
     // conflict on banks twice as often as 4-byte ones)
     int f32_acc = 2;
 
+    // adjoint reverse sweep: steps whose positions are kept at once (0 =
+    // the most the free device memory allows), and the last sweep's plan
+    int adj_block = 0;
+    int adj_block_used = 0;
+    int64_t adj_bytes_used = 0;
+
     // pic_run kernel timing (HIP events on the context stream)
     bool timing = false;
     double t_pass_ms = 0.0, t_solve_ms = 0.0;

@@ -199,3 +199,26 @@ def test_adjoint_f32_windows_and_sorts(gpu, kernel, sort_every):
         e = sim.run(dt, steps, sort_every)
         sim.close()
         assert J32 == pytest.approx(e[-1, 0] + e[-1, 1], rel=1e-6)
+
+
+def test_adjoint_f32_two_level_matches_one_level(gpu):
+    """Restart points inside the checkpoint segments (positions of 1 or 3
+    steps kept at once, restart points at sorted and unsorted steps) give the
+    one-level sweep's gradient up to the order of its FP64 sums."""
+    pic = gpu
+    F32 = pic._lib.PIC_STATE_F32
+    n, dt, steps, ck, se = 32768, 0.1, 20, 10, 2
+    mesh = pic.TwoDGrid(0.0, 4 * math.pi, 256, 0.0, 4 * math.pi, 32)
+    p = pic.sampling.landau_damping(n, mesh, 0.05, 0.5, sobol=pic.sampling.sobol3(n))
+    res = []
+    for block in (0, 3, 1):
+        sim = pic.Simulation(mesh, p, pic.M6(), state=F32)
+        sim.set_adjoint_block(block)
+        res.append(sim.adjoint(dt, steps, ck, se))
+        used, nbytes = sim.adjoint_plan()
+        assert used == (block if block else ck) and nbytes > 0
+        sim.close()
+    for J, gx, gv in res[1:]:
+        assert J == pytest.approx(res[0][0], rel=1e-12)
+        assert np.linalg.norm(gx - res[0][1]) / np.linalg.norm(res[0][1]) < 1e-6
+        assert np.linalg.norm(gv - res[0][2]) / np.linalg.norm(res[0][2]) < 1e-6
