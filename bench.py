@@ -136,7 +136,172 @@ WORKLOADS = {
     "large_m6": dict(name="landau2d2v-m6-512x512-5e8-dt0.05-200steps", grid=512,
                      n=500_000_000, kernel="m6", alpha=0.05, k=0.5, dt=0.05, steps=200,
                      warmup=5, sampler="landau_device", sort=(2, 1)),
+    # adjoint sweeps: forward loop with checkpoints + reverse sweep, timed as
+    # one call; value = particle-steps/s of the whole sweep
+    "adjoint_cic": dict(name="adjoint-landau2d2v-cic-128x128-2e6-dt0.1-200steps-ckpt20",
+                        grid=128, n=2_000_000, kernel="cic", alpha=0.05, k=0.5, dt=0.1,
+                        steps=200, warmup=20, sampler="quiet", sort=(4, 0),
+                        adjoint=dict(ckpt=20, fd=True, f64=True)),
+    "adjoint_large_m6": dict(name="adjoint-landau2d2v-m6-512x512-5e8-dt0.05-200steps-ckpt25",
+                             grid=512, n=500_000_000, kernel="m6", alpha=0.05, k=0.5, dt=0.05,
+                             steps=200, warmup=0, sampler="landau_device", sort=(2, 0),
+                             adjoint=dict(ckpt=25, fd=False, f64=False)),
 }
+
+# Algorithmic bytes per particle of the adjoint sweep's passes (FP32 state):
+# forward pass and recompute pass read and write cell + 4 floats; the
+# adjoint passes read the stored position (cell + 2 floats) and the
+# particle adjoint (4 floats) and write the adjoint back.
+ADJ_PASS_BYTES = {"forward": 40.0, "gather_adjoint": 44.0, "deposit_adjoint": 44.0,
+                  "recompute": 40.0}
+
+
+def bench_adjoint(args, W, pic, entry, mesh, held, n, steps, warmup):
+    """The adjoint workloads: FP32 sweep (and the FP64 one), gradient checks
+    against central differences of FP64 runs, per-pass roofline.  `held`
+    holds the only reference to the sampled particles: a GPU-sampled set is
+    dropped once the context holds its copy."""
+    A = W["adjoint"]
+    p0 = held.pop()
+    ckpt = A["ckpt"]
+    dt, alpha, kx = W["dt"], W["alpha"], W["k"]
+    nx, ny = mesh.nx, mesh.ny
+    kernel = pic.M6() if W["kernel"] == "m6" else pic.CloudInCell()
+    kname = "M6" if W["kernel"] == "m6" else "CIC"
+    se = args.sort_every_f32
+    # dx0/dalpha of the sampler at fixed uniform numbers
+    x0 = p0[0] if isinstance(p0, np.ndarray) else p0[:, 0].cpu().numpy()
+    dxa = pic.sampling.dnewton_dalpha(x0, alpha, kx)
+    del x0
+    on_device = not isinstance(p0, np.ndarray)
+    cpu_sample = None
+    if on_device:  # what the CPU baseline needs, then free the device copy
+        import torch
+        cpu_sample = np.asfortranarray(p0[:200_000].cpu().numpy().T)
+    res = {}
+    for name, state in (("f32", 1), ("f64", 0)):
+        if name == "f64" and not A["f64"]:
+            continue
+        if args.only not in ("both", name):
+            continue
+        sim = pic.Simulation(mesh, p0, kernel, state=state)
+        if on_device:
+            del p0
+            torch.cuda.empty_cache()
+            free, total = torch.cuda.mem_get_info()
+            log(f"bench: device memory free {free / 2**30:.1f} GiB of {total / 2**30:.1f} GiB")
+        if warmup:
+            log(f"bench: {name} adjoint warmup, {warmup} steps")
+            sim.adjoint(dt, warmup, min(ckpt, warmup), se)
+        sim.set_timing(True)
+        log(f"bench: {name} adjoint, {steps} steps, checkpoints every {ckpt}, sort every {se}")
+        sim.ctx.sync()
+        t0 = time.perf_counter()
+        J, gx, gv = sim.adjoint(dt, steps, ckpt, se)
+        t1 = time.perf_counter()
+        r = {"seconds": t1 - t0, "J": J, "dJ_dalpha": float(np.sum(gx[0] * dxa))}
+        u = np.random.default_rng(7).choice([-1.0, 1.0], size=n)
+        r["dJ_dvx_dir"] = float(np.sum(gv[0] * u))
+        del gx, gv, u
+        if name == "f32":
+            r["plan"] = sim.adjoint_plan()
+            r["passes"] = sim.adjoint_timing()
+        sim.close()
+        res[name] = r
+        log(f"bench: {name} adjoint {r['seconds']:.3f} s, J {J:.10e}, dJ/dalpha "
+            f"{r['dJ_dalpha']:.10e}")
+    fd = {}
+    if A["fd"]:
+        h = 1e-6
+
+        def fwd(q):
+            s_ = pic.Simulation(mesh, q, kernel)
+            e = s_.run(dt, steps, sort_every=max(se, 1))
+            s_.close()
+            return e[-1, 0] + e[-1, 1]
+
+        log("bench: central differences of FP64 runs, h = 1e-6")
+        sob = pic.sampling.sobol3(n)
+        fd["dJ_dalpha"] = (fwd(pic.sampling.landau_damping(n, mesh, alpha + h, kx, sobol=sob))
+                           - fwd(pic.sampling.landau_damping(n, mesh, alpha - h, kx,
+                                                             sobol=sob))) / (2 * h)
+        u = np.random.default_rng(7).choice([-1.0, 1.0], size=n)
+        qp, qm = p0.copy(order="F"), p0.copy(order="F")
+        qp[2] += h * u
+        qm[2] -= h * u
+        fd["dJ_dvx_dir"] = (fwd(qp) - fwd(qm)) / (2 * h)
+
+    head = "f32" if "f32" in res else "f64"
+    r = res[head]
+    value = n * steps / r["seconds"]
+    out = {
+        "metric": "particle-steps/s FP32 vs FP64 at <=1% damping-rate error; % HBM roofline",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": r["seconds"] * 1e3 / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": head,
+        "data": "synthetic (Landau quiet start, Sobol)" if W["sampler"] == "quiet" else
+                "synthetic (Landau start sampled on the GPU, seed 1234)",
+        "config": {"workload": W["name"], "grid": [nx, ny], "particles": n, "kernel": kname,
+                   "alpha": alpha, "k": kx, "dt": dt, "checkpoint_every": ckpt,
+                   "sort_every": se, "sweep": "forward with checkpoints + reverse adjoint",
+                   "state": "int32 cell + fp32 offset + fp32 velocity" if head == "f32" else
+                   "fp64 soa", "parallelism": "single-gpu"},
+    }
+    if "passes" in r:
+        passes = {}
+        for k, (ms, cnt) in r["passes"].items():
+            if cnt:
+                avg = ms / cnt
+                gbs = ADJ_PASS_BYTES[k] * n / (avg * 1e-3) / 1e9
+                passes[k] = {"avg_us": avg * 1e3, "count": cnt, "bytes_per_particle":
+                             ADJ_PASS_BYTES[k], "GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+        dom = max(passes, key=lambda k: passes[k]["avg_us"] * passes[k]["count"])
+        d = passes[dom]
+        out["roofline"] = {"bound": "hbm", "achieved": d["GBs"], "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": d["frac"], "traffic": None,
+                           "kernel": f"{dom} pass <{kname}> (FP32 state)",
+                           "bytes_per_particle": d["bytes_per_particle"], "avg_us": d["avg_us"]}
+        out["adjoint_passes"] = passes
+        out["adjoint_plan"] = {"positions_kept_steps": r["plan"][0],
+                               "scratch_bytes": r["plan"][1]}
+    grad = {}
+    for name, rr in res.items():
+        grad[name] = {"J": rr["J"], "dJ_dalpha": rr["dJ_dalpha"], "dJ_dvx_dir": rr["dJ_dvx_dir"]}
+        if fd:
+            grad[name]["rel_err_vs_fd_dalpha"] = abs(rr["dJ_dalpha"] - fd["dJ_dalpha"]) / abs(
+                fd["dJ_dalpha"])
+            grad[name]["rel_err_vs_fd_dvx_dir"] = abs(rr["dJ_dvx_dir"] - fd["dJ_dvx_dir"]) / abs(
+                fd["dJ_dvx_dir"])
+    if fd:
+        grad["fd_fp64_h1e-6"] = fd
+    if "f32" in res and "f64" in res:
+        grad["f32_vs_f64_dalpha_rel"] = abs(res["f32"]["dJ_dalpha"] - res["f64"]["dJ_dalpha"]) / \
+            abs(res["f64"]["dJ_dalpha"])
+        v64 = n * steps / res["f64"]["seconds"]
+        out["fp64"] = {"value": v64, "ms_per_step": res["f64"]["seconds"] * 1e3 / steps,
+                       "sort_every": ckpt}
+        out["speedup_fp32_vs_fp64"] = value / v64
+    out["gradient"] = grad
+
+    if not args.no_cpu and args.cpu_steps > 0:
+        O = entry.load_oracle()
+        m = min(n, 200_000)
+        steps_cpu = max(args.cpu_steps, 4)
+        q = np.asfortranarray(p0[:, :m]) if not on_device else np.asfortranarray(cpu_sample)
+        q[4] = mesh.dimx * mesh.dimy / m
+        log(f"bench: cpu baseline, oracle adjoint ({kname}), {m} particles, {steps_cpu} steps")
+        c0 = time.perf_counter()
+        O.adjoint(q, W["kernel"], nx, ny, mesh.dimx, mesh.dimy, dt, steps_cpu,
+                  min(ckpt, steps_cpu))
+        c1 = time.perf_counter()
+        cores, model = cpu_info()
+        out["cpu_baseline"] = {
+            "value": m * steps_cpu / (c1 - c0), "unit": "particle-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"first {m} particles of the same workload, {steps_cpu} steps of the "
+                      f"FP64 forward loop + adjoint sweep (oracle C restatement, serial), "
+                      f"{model}, nproc={cores}"}
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -196,6 +361,12 @@ def main():
         p0 = pic.sampling.landau_damping_device(n, mesh, alpha, kx, seed=1234)
     else:
         p0 = pic.sampling.perturbed_maxwellian_device(n, mesh, alpha, kx, kx, seed=1234)
+
+    if "adjoint" in W:
+        held = [p0]
+        del p0
+        bench_adjoint(args, W, pic, entry, mesh, held, n, steps, warmup)
+        return
 
     res = {}
     for name, state in (("f32", 1), ("f64", 0)):

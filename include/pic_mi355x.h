@@ -281,6 +281,14 @@ int pic_adjoint(pic_ctx *ctx, int kernel, double dt, int nsteps, int ckpt, int s
 int pic_set_adjoint_block(pic_ctx *ctx, int block);
 int pic_get_adjoint_plan(pic_ctx *ctx, int *block, int64_t *bytes);
 
+/* With timing enabled (pic_set_timing, which also clears these), the FP32
+ * adjoint records HIP events around its particle passes: ms[k] (host, 4)
+ * receives the summed durations and passes[k] (host, 4) the counts of
+ * k = 0 forward passes, 1 gather-adjoint passes, 2 deposit-adjoint passes,
+ * 3 recomputation passes (each with its deferred-particle kernel).  No
+ * reference counterpart. */
+int pic_get_adjoint_timing(pic_ctx *ctx, double *ms, int64_t *passes);
+
 /* Number of particle chunks that took the global-memory slow path in the
  * fused kernels since the context was created (diagnostic). */
 int pic_slow_chunks(pic_ctx *ctx, uint64_t *count);

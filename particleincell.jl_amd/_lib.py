@@ -62,6 +62,7 @@ SIGNATURES = {
     "pic_adjoint": [_vp, _i, _d, _i, _i, _i, _pd, _vp, _vp],
     "pic_set_adjoint_block": [_vp, _i],
     "pic_get_adjoint_plan": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int64)],
+    "pic_get_adjoint_timing": [_vp, _pd, C.POINTER(C.c_int64)],
     "pic_set_timing": [_vp, _i],
     "pic_set_f32_accumulation": [_vp, _i],
     "pic_get_timing": [_vp, _pd, C.POINTER(C.c_int64), _pd],

@@ -488,6 +488,16 @@ class Simulation:
         L.call("pic_get_adjoint_plan", self.ctx.h, C.byref(b), C.byref(nb))
         return b.value, nb.value
 
+    def adjoint_timing(self):
+        """{pass: (total_ms, count)} of the last adjoint sweeps since timing
+        was enabled, for pass in forward, gather_adjoint, deposit_adjoint,
+        recompute (pic_get_adjoint_timing)."""
+        ms = (C.c_double * 4)()
+        cnt = (C.c_int64 * 4)()
+        L.call("pic_get_adjoint_timing", self.ctx.h, ms, cnt)
+        names = ("forward", "gather_adjoint", "deposit_adjoint", "recompute")
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(names)}
+
     def set_timing(self, on=True):
         L.call("pic_set_timing", self.ctx.h, 1 if on else 0)
 

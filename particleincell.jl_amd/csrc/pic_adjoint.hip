@@ -421,3 +421,13 @@ extern "C" int pic_get_adjoint_plan(pic_ctx *ctx, int *block, int64_t *bytes) {
     if (bytes) *bytes = c->adj_bytes_used;
     return PIC_OK;
 }
+
+extern "C" int pic_get_adjoint_timing(pic_ctx *ctx, double *ms, int64_t *passes) {
+    PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
+    const Ctx *c = ctx_of(ctx);
+    for (int k = 0; k < 4; ++k) {
+        if (ms) ms[k] = c->t_adj_ms[k];
+        if (passes) passes[k] = c->t_adj_n[k];
+    }
+    return PIC_OK;
+}

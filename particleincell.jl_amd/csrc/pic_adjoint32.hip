@@ -386,6 +386,36 @@ __global__ void k_adj32_back(int64_t n, const uint32_t *__restrict__ perm,
     if (r < n) out[perm[r]] = in[r];
 }
 
+// HIP events around the adjoint's particle passes when the context's timing
+// is on: 0 forward passes, 1 gather adjoint, 2 deposit adjoint, 3 recompute.
+struct PassTimer {
+    Ctx *c;
+    std::vector<hipEvent_t> ev[4];
+    explicit PassTimer(Ctx *cc) : c(cc) {}
+    ~PassTimer() {
+        for (auto &v : ev)
+            for (hipEvent_t e : v) (void)hipEventDestroy(e);
+    }
+    int mark(int k) {
+        if (!c->timing) return PIC_OK;
+        hipEvent_t e;
+        PIC_HIP(hipEventCreate(&e));
+        ev[k].push_back(e);
+        PIC_HIP(hipEventRecord(e, c->stream));
+        return PIC_OK;
+    }
+    // after the stream has drained: add the (begin, end) pairs to the context
+    void collect() {
+        for (int k = 0; k < 4; ++k)
+            for (size_t i = 0; i + 1 < ev[k].size(); i += 2) {
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, ev[k][i], ev[k][i + 1]);
+                c->t_adj_ms[k] += ms;
+                c->t_adj_n[k] += 1;
+            }
+    }
+};
+
 // Steps at which the loop lane-sorts the particles: every checkpoint step and,
 // with sort_every > 0, every multiple of it before the last step.  The
 // forward pass and the reverse sweep's recomputation follow the same list,
@@ -505,6 +535,7 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, doub
     PIC_TRY(tc.alloc((size_t)n));
     PIC_HIP(hipMemsetAsync(c->step_counter, 0, sizeof(int), c->stream));
 
+    PassTimer tm(c);
     auto dcopy = [&](void *dst, const void *src, size_t bytes) -> int {
         PIC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
         return PIC_OK;
@@ -532,7 +563,9 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, doub
     // pass without the deposit), then the sort of step s if there is one
     auto recompute = [&](int s, bool *sorted) -> int {
         PIC_TRY(dcopy(c->e32, Es.p + (size_t)m * (s - 1), (size_t)m * 8));
+        PIC_TRY(tm.mark(3));
         PIC_TRY(launch_seg32(c, K, 2, dt, dt));
+        PIC_TRY(tm.mark(3));
         *sorted = sp.at(s);
         if (*sorted) PIC_TRY(sort_by_cell(c, nullptr, true));
         return PIC_OK;
@@ -554,7 +587,9 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, doub
     // permutation for the next pass to apply.
     PIC_TRY(save_to(ckc.p, cko.p, perms.p, offs.p, 0));
     PIC_TRY(launch_fill(c, c->rho_acc, 0.0, m));
+    PIC_TRY(tm.mark(0));
     PIC_TRY(launch_seg32(c, K, 0, dt, h));
+    PIC_TRY(tm.mark(0));
     for (int s = 0; s < nsteps; ++s) {
         const bool last = s == nsteps - 1;
         if (s > 0 && sp.at(s)) {
@@ -565,7 +600,9 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, doub
         PIC_TRY(poisson_solve_e(c, c->rho_acc, scale_rho, last ? 1 : 3, -2));
         PIC_TRY(dcopy(Es.p + (size_t)m * s, c->e32, (size_t)m * 8));
         if (last) break;
+        PIC_TRY(tm.mark(0));
         PIC_TRY(launch_seg32(c, K, 1, dt, dt));
+        PIC_TRY(tm.mark(0));
     }
     PIC_TRY(dcopy(Elast.p, c->ex, (size_t)m * 8));
     PIC_TRY(dcopy(Elast.p + m, c->ey, (size_t)m * 8));
@@ -652,10 +689,14 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, doub
                 } else {
                     PIC_TRY(launch_fill(c, mu.p, 0.0, 2 * m));
                     a.e32 = Es.p + (size_t)m * s;
+                    PIC_TRY(tm.mark(1));
                     PIC_TRY((launch_adjseg<K, true>(c, a)));
+                    PIC_TRY(tm.mark(1));
                 }
                 PIC_TRY(poisson_adjoint(c, mu.p, mu.p + m, lrho.p, scale));
+                PIC_TRY(tm.mark(2));
                 PIC_TRY((launch_adjseg<K, false>(c, a)));
+                PIC_TRY(tm.mark(2));
                 // a sort at step s: the slots into their order before it
                 if (sp.at(s)) {
                     const uint32_t *pg = k == 0 ? perm0 : subperm.p + (size_t)(k - 1) * n;
@@ -692,6 +733,7 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, doub
     PIC_TRY(dcopy(c->cell, tc.p, (size_t)n * 4));
     PIC_TRY(dcopy(c->ov32, t4.p, (size_t)n * 16));
     PIC_HIP(hipStreamSynchronize(c->stream));
+    tm.collect();
     c->seg_valid = false;
     return PIC_OK;
 }

@@ -136,6 +136,10 @@ struct Ctx {
     int adj_block = 0;
     int adj_block_used = 0;
     int64_t adj_bytes_used = 0;
+    // with timing enabled: summed durations (ms) and counts of the adjoint's
+    // forward passes, gather-adjoint, deposit-adjoint and recompute passes
+    double t_adj_ms[4] = {0.0, 0.0, 0.0, 0.0};
+    int64_t t_adj_n[4] = {0, 0, 0, 0};
 
     // pic_run kernel timing (HIP events on the context stream)
     bool timing = false;

@@ -124,6 +124,10 @@ extern "C" int pic_set_timing(pic_ctx *ctx, int enable) {
     c->timing = enable != 0;
     c->t_pass_ms = c->t_solve_ms = 0.0;
     c->t_passes = 0;
+    for (int k = 0; k < 4; ++k) {
+        c->t_adj_ms[k] = 0.0;
+        c->t_adj_n[k] = 0;
+    }
     return PIC_OK;
 }
 
