@@ -115,7 +115,21 @@ def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_ac
         "solve_ms_avg": solve_ms / max(passes, 1),
         "passes": passes,
         "slow_chunks": sim.slow_chunks() - slow0,
+        # the last step's grids, for the FP32-vs-FP64 comparison
+        "grids": {g: sim.grid(i) for g, i in (("rho", 0), ("ex", 1), ("ey", 2))},
     }
+    # the fused pass split into its halves, each on freshly lane-sorted
+    # particles: deposit only, and gather + kick + drift
+    split = {}
+    for mode, name in ((3, "deposit"), (2, "gather_push"), (1, "fused")):
+        sim.sort(lanes=True)
+        sim.set_timing(True)
+        for _ in range(3):
+            sim.step_pass(mode, dt)
+        ms, cnt, _ = sim.timing()
+        split[name] = ms / max(cnt, 1)
+    sim.set_timing(False)
+    out["split_ms"] = split
     sim.close()
     return out
 
@@ -439,8 +453,26 @@ def main():
                             "analytic": -0.153359}
         e32, e64 = res["f32"]["energy"][:, 0], res["f64"]["energy"][:, 0]
         out["energy_rel_diff_fp32_fp64"] = float(np.max(np.abs(e32 - e64)) / np.max(e64))
+        # normwise relative difference of the last step's rho and E
+        g32, g64 = res["f32"]["grids"], res["f64"]["grids"]
+        out["grid_rel_diff_fp32_fp64"] = {
+            "rho": float(np.linalg.norm(g32["rho"] - g64["rho"]) / np.linalg.norm(g64["rho"])),
+            "e": float(np.sqrt(np.sum((g32["ex"] - g64["ex"]) ** 2 + (g32["ey"] - g64["ey"]) ** 2)
+                               / np.sum(g64["ex"] ** 2 + g64["ey"] ** 2)))}
     out["solve_us"] = r["solve_ms_avg"] * 1e3
     out["deferred_particles"] = r["slow_chunks"]
+    # the halves of the fused pass timed on their own (HIP events), against
+    # their own algorithmic bytes: the deposit reads the state (FP32 20 B,
+    # FP64 x, y 16 B); gather + kick + drift reads and writes it
+    split_bytes = {"f32": {"deposit": 20.0, "gather_push": 40.0, "fused": 40.0},
+                   "f64": {"deposit": 16.0, "gather_push": 64.0, "fused": 64.0}}
+    out["passes_split"] = {}
+    for name, rr in res.items():
+        out["passes_split"][name] = {
+            k: {"avg_us": ms * 1e3, "bytes_per_particle": split_bytes[name][k],
+                "GBs": split_bytes[name][k] * n / (ms * 1e-3) / 1e9,
+                "frac": split_bytes[name][k] * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            for k, ms in rr["split_ms"].items()}
     if args.workload == "landau_m6":
         traffic = pmc_traffic(f"k_seg_{head}")
         if traffic is not None:

@@ -244,9 +244,10 @@ int pic_step_pass(pic_ctx *ctx, int kernel, int mode, double dt, double *rho_raw
 
 /* Kernel timing of pic_run: with timing enabled, pic_run records HIP events
  * on the context's stream around every particle pass (kick, drift,
- * deposit) and around every Poisson solve.  pic_get_timing returns the
- * summed durations in milliseconds and the number of particle passes since
- * timing was enabled (any pointer may be NULL). */
+ * deposit) and around every Poisson solve, and pic_step_pass around its
+ * pass.  pic_get_timing returns the summed durations in milliseconds and the
+ * number of particle passes since timing was enabled (any pointer may be
+ * NULL). */
 int pic_set_timing(pic_ctx *ctx, int enable);
 int pic_get_timing(pic_ctx *ctx, double *pass_ms, int64_t *passes, double *solve_ms);
 

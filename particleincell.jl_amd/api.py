@@ -477,6 +477,12 @@ class Simulation:
                int(sort_every), C.byref(J), gx.ctypes.data, gv.ctypes.data)
         return J.value, gx, gv
 
+    def step_pass(self, mode, dt):
+        """One particle pass of the loop (pic_step_pass): 0 drift dt/2 +
+        deposit, 1 kick + drift + drift + deposit, 2 kick + drift dt/2,
+        3 deposit only.  The raw deposit stays on the device."""
+        L.call("pic_step_pass", self.ctx.h, self.kernel, int(mode), float(dt), None)
+
     def set_adjoint_block(self, block):
         """Steps whose positions the FP32 adjoint sweep keeps at once (0: the
         most that fits the free device memory) (pic_set_adjoint_block)."""

@@ -150,11 +150,27 @@ extern "C" int pic_step_pass(pic_ctx *ctx, int kernel, int mode, double dt, doub
     const int64_t m = (int64_t)c->nx * c->ny;
     PIC_TRY(launch_fill(c, c->rho_acc, 0.0, m));
     // particles in the lane order of a pic_sort_lanes: the segment kernels of
-    // pic_run; any other order: the chunk kernels
-    if (c->seg_valid)
-        PIC_TRY(seg_pass(c, kernel, mode, dt));
-    else
-        PIC_TRY(step_pass(c, kernel, mode, dt));
+    // pic_run; any other order: the chunk kernels.  With timing on, the pass
+    // is timed like pic_run's (HIP events on the context stream).
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (c->timing) {
+        PIC_HIP(hipEventCreate(&ev[0]));
+        PIC_HIP(hipEventCreate(&ev[1]));
+        PIC_HIP(hipEventRecord(ev[0], c->stream));
+    }
+    const int rc = c->seg_valid ? seg_pass(c, kernel, mode, dt) : step_pass(c, kernel, mode, dt);
+    if (c->timing) {
+        if (rc == PIC_OK && hipEventRecord(ev[1], c->stream) == hipSuccess &&
+            hipEventSynchronize(ev[1]) == hipSuccess) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+            c->t_pass_ms += ms;
+            c->t_passes += 1;
+        }
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+    }
+    PIC_TRY(rc);
     if (rho_raw)
         PIC_HIP(hipMemcpyAsync(rho_raw, c->rho_acc, (size_t)m * sizeof(double),
                                hipMemcpyDeviceToHost, c->stream));

@@ -55,11 +55,19 @@ __global__ __launch_bounds__(SORT_T) void k_digit_hist(const uint32_t *__restric
     for (int d = tid; d < R; d += SORT_T) cnt[d] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
-    uint32_t cur = 0xffffffffu, run = 0;
+    // every key of this thread first, so the loads are in flight together
+    uint32_t kr[SORT_R];
+#pragma unroll
     for (int r = 0; r < SORT_R; ++r) {
-        int64_t p = base + (int64_t)r * SORT_T + tid;
+        const int64_t p = base + (int64_t)r * SORT_T + tid;
+        kr[r] = p < n ? key[p] : 0u;
+    }
+    uint32_t cur = 0xffffffffu, run = 0;
+#pragma unroll
+    for (int r = 0; r < SORT_R; ++r) {
+        const int64_t p = base + (int64_t)r * SORT_T + tid;
         if (p >= n) break;
-        uint32_t d = (key[p] >> shift) & (uint32_t)(R - 1);
+        uint32_t d = (kr[r] >> shift) & (uint32_t)(R - 1);
         if (d == cur) {
             ++run;
         } else {
@@ -155,10 +163,20 @@ __global__ __launch_bounds__(SORT_T) void k_digit_scatter(
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    // every (key, value) of this thread first, so the loads are in flight
+    // together rather than one round at a time between the barriers
+    uint32_t kr[SORT_R], vr[SORT_R];
+#pragma unroll
+    for (int r = 0; r < SORT_R; ++r) {
+        const int64_t p = base + (int64_t)r * SORT_T + tid;
+        kr[r] = p < n ? kin[p] : 0u;
+        vr[r] = p < n ? vin[p] : 0u;
+    }
+#pragma unroll
     for (int r = 0; r < SORT_R; ++r) {
         const int64_t p = base + (int64_t)r * SORT_T + tid;
         const bool valid = p < n;
-        uint32_t kk = valid ? kin[p] : 0u, vv = valid ? vin[p] : 0u;
+        const uint32_t kk = kr[r], vv = vr[r];
         uint32_t d = (kk >> shift) & (uint32_t)(R - 1);
         uint64_t peers = __ballot(valid);
         for (int b = 0; b < bits; ++b) {
