@@ -1,16 +1,19 @@
 // pic_sort.hip -- stable sort of the particles by cell.
 //
-// LSD radix sort of (cell, index) pairs: ceil(B/8) passes over digits of at
-// most 8 bits, B = bits of the largest cell index.  Every pass is a stable
-// counting sort of one digit:
+// LSD radix sort of (cell, index) pairs: ceil(B/9) passes over digits of at
+// most 9 bits, B = bits of the largest cell index (2 passes up to 512 x 512
+// cells).  Every pass is a stable counting sort of one digit:
 //   k_digit_hist   per tile of 4096 keys, a histogram of the digit
 //                  (LDS counters, one atomic per run of equal digits)
 //   k_scan_*       exclusive scan of the histograms in digit-major order,
 //                  giving each (digit, tile) its first output slot
 //   k_digit_scatter  per tile, in item order: the rank of each key among the
 //                  equal digits before it (wave ballots for the lanes of one
-//                  wave, LDS counts for earlier waves and rounds), then the
-//                  scatter
+//                  wave, LDS counts for earlier waves and rounds), which
+//                  places it in the tile's digit-ordered copy in LDS; the copy
+//                  is then written out, each digit's run to consecutive slots
+// The first pass reads the keys where they are (the FP32 state's cell array)
+// and takes the item index as the value.
 // Equal keys keep their input order in every pass, so the result is the
 // permutation std::stable_sort by cell produces.  The permutation is then
 // applied to the whole particle state by one gather kernel into a second set
@@ -26,30 +29,24 @@ namespace pic {
 constexpr int SORT_T = 256;          // threads per tile
 constexpr int SORT_R = 16;           // rounds (items per thread)
 constexpr int SORT_TILE = SORT_T * SORT_R;
+constexpr int SORT_MAXB = 9;         // widest digit
+constexpr int SORT_MAXR = 1 << SORT_MAXB;
 
 // keys from the current state: cell index i + nx*j (FP64 state: the cell of
 // floor(x/dx), floor(y/dy), wrapped).
 __global__ void k_keys_f64(int64_t n, const double *x, const double *y, double dx, double dy,
-                           int nx, int ny, uint32_t *key, uint32_t *val) {
+                           int nx, int ny, uint32_t *key) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int i = picd::wrapi((int)floor(x[k] / dx), nx);
     int j = picd::wrapi((int)floor(y[k] / dy), ny);
     key[k] = (uint32_t)(i + nx * j);
-    val[k] = (uint32_t)k;
-}
-
-__global__ void k_keys_f32(int64_t n, const int32_t *cell, uint32_t *key, uint32_t *val) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    key[k] = (uint32_t)cell[k];
-    val[k] = (uint32_t)k;
 }
 
 __global__ __launch_bounds__(SORT_T) void k_digit_hist(const uint32_t *__restrict__ key, int64_t n,
                                                        int shift, int bits, int ntiles,
                                                        uint32_t *__restrict__ hist) {
-    __shared__ uint32_t cnt[256];
+    __shared__ uint32_t cnt[SORT_MAXR];
     const int R = 1 << bits;
     const int tid = threadIdx.x;
     for (int d = tid; d < R; d += SORT_T) cnt[d] = 0;
@@ -148,21 +145,44 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_down(uint32_t *inout, int64_t n
     }
 }
 
+template <bool IDX>
 __global__ __launch_bounds__(SORT_T) void k_digit_scatter(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, int64_t n, int shift,
     int bits, int ntiles, const uint32_t *__restrict__ offs, uint32_t *__restrict__ kout,
     uint32_t *__restrict__ vout) {
-    __shared__ uint32_t base_d[256];         // next slot of each digit in this tile
-    __shared__ uint32_t wcnt[SORT_T / 64][256];
+    __shared__ uint32_t gbase[SORT_MAXR];   // first output slot of each digit of this tile
+    __shared__ uint32_t dstart[SORT_MAXR];  // its first position in the tile's digit order
+    __shared__ uint32_t run[SORT_MAXR];     // items of each digit placed so far
+    __shared__ uint32_t wcnt[SORT_T / 64][SORT_MAXR];
+    __shared__ uint32_t lk[SORT_TILE], lv[SORT_TILE];
+    __shared__ uint32_t sh[SCAN_T];
     const int R = 1 << bits;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int d = tid; d < R; d += SORT_T) {
-        base_d[d] = offs[(int64_t)d * ntiles + blockIdx.x];
-        for (int w = 0; w < SORT_T / 64; ++w) wcnt[w][d] = 0;
-    }
-    __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
-    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const int cnt_tile = (int)min((int64_t)SORT_TILE, n - base);
+    // this tile's count of digits 2 tid and 2 tid + 1, from the scanned
+    // histogram (the next entry in digit-major order ends the run; the last
+    // entry ends at n)
+    const int64_t total = (int64_t)R * ntiles;
+    uint32_t cd[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int d = 2 * tid + q;
+        cd[q] = 0;
+        if (d < R) {
+            const int64_t e = (int64_t)d * ntiles + blockIdx.x;
+            const uint32_t g0 = offs[e];
+            const uint32_t g1 = e + 1 < total ? offs[e + 1] : (uint32_t)n;
+            gbase[d] = g0;
+            cd[q] = g1 - g0;
+            run[d] = 0;
+            for (int w = 0; w < SORT_T / 64; ++w) wcnt[w][d] = 0;
+        }
+    }
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan(cd[0] + cd[1], sh, tot);
+    if (2 * tid < R) dstart[2 * tid] = ex;
+    if (2 * tid + 1 < R) dstart[2 * tid + 1] = ex + cd[0];
     // every (key, value) of this thread first, so the loads are in flight
     // together rather than one round at a time between the barriers
     uint32_t kr[SORT_R], vr[SORT_R];
@@ -170,8 +190,10 @@ __global__ __launch_bounds__(SORT_T) void k_digit_scatter(
     for (int r = 0; r < SORT_R; ++r) {
         const int64_t p = base + (int64_t)r * SORT_T + tid;
         kr[r] = p < n ? kin[p] : 0u;
-        vr[r] = p < n ? vin[p] : 0u;
+        vr[r] = IDX ? (uint32_t)p : (p < n ? vin[p] : 0u);
     }
+    __syncthreads();
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
 #pragma unroll
     for (int r = 0; r < SORT_R; ++r) {
         const int64_t p = base + (int64_t)r * SORT_T + tid;
@@ -190,17 +212,24 @@ __global__ __launch_bounds__(SORT_T) void k_digit_scatter(
         if (valid) {
             uint32_t before = 0;
             for (int w = 0; w < wv; ++w) before += wcnt[w][d];
-            uint32_t dst = base_d[d] + before + (uint32_t)rank_w;
-            kout[dst] = kk;
-            vout[dst] = vv;
+            const uint32_t lp = dstart[d] + run[d] + before + (uint32_t)rank_w;
+            lk[lp] = kk;
+            lv[lp] = vv;
         }
         __syncthreads();
-        if (leader) {
-            atomicAdd(&base_d[d], wcnt[wv][d]);
-        }
+        if (leader) atomicAdd(&run[d], wcnt[wv][d]);
         __syncthreads();
         if (leader) wcnt[wv][d] = 0;
         // the next round reads wcnt only after its own __syncthreads
+    }
+    __syncthreads();
+    // the tile in digit order: each digit's items go to consecutive slots
+    for (int i = tid; i < cnt_tile; i += SORT_T) {
+        const uint32_t kk = lk[i];
+        const uint32_t d = (kk >> shift) & (uint32_t)(R - 1);
+        const uint32_t g = gbase[d] + ((uint32_t)i - dstart[d]);
+        kout[g] = kk;
+        vout[g] = lv[i];
     }
 }
 
@@ -268,21 +297,54 @@ __global__ void k_seg_tables(const int64_t *__restrict__ off, int ncells, int ns
     }
 }
 
-// Destination of every sorted slot r; writes the lane-spread permutation.
-__global__ void k_lane_dest(const uint32_t *__restrict__ key, const uint32_t *__restrict__ val,
-                            int64_t n, const int64_t *__restrict__ off,
-                            const uint32_t *__restrict__ tbl, const uint32_t *__restrict__ A,
-                            const unsigned long long *__restrict__ M,
-                            uint32_t *__restrict__ perm2) {
-    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const uint32_t c = key[r];
-    const uint32_t s = c >> 6, l = c & 63u;
-    const uint32_t k = (uint32_t)(r - off[c]);
-    const uint32_t t = tbl[s] + k;
-    const unsigned long long below = l ? (M[t] & (~0ull >> (64 - l))) : 0ull;
-    const int64_t dest = off[64 * (int64_t)s] + A[t] + __popcll(below);
-    perm2[dest] = val[r];
+// The lane-spread permutation as a transpose, segment by segment: in the
+// sorted order a segment's values are cell-major (cell l, rank k); in the
+// lane-spread order they are rank-major.  A block moves LANE_KC ranks of all
+// 64 cells at a time through LDS: it reads each cell's run of ranks
+// (consecutive sorted slots) and writes each rank's cells (consecutive
+// destination slots).  Blocks are (segment, b) with b = 0 .. gridDim.y-1
+// taking the rank chunks b, b + gridDim.y, ... of the segment.
+constexpr int LANE_KC = 32, LANE_T = 256;
+
+__global__ __launch_bounds__(LANE_T) void k_lane_transpose(
+    const uint32_t *__restrict__ val, const int64_t *__restrict__ off, int ncells,
+    const uint32_t *__restrict__ kmax, const uint32_t *__restrict__ tbl,
+    const uint32_t *__restrict__ A, const unsigned long long *__restrict__ M,
+    uint32_t *__restrict__ perm2) {
+    __shared__ uint32_t tile[LANE_KC][65];
+    __shared__ int64_t coff[65];
+    const int s = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int c0 = 64 * s;
+    const int nc = min(64, ncells - c0);
+    if (tid <= 64) coff[tid] = off[min(c0 + tid, ncells)];
+    __syncthreads();
+    const uint32_t K = kmax[s], tb = tbl[s];
+    const int64_t sbase = coff[0];
+    for (uint32_t k0 = (uint32_t)blockIdx.y * LANE_KC; k0 < K; k0 += gridDim.y * LANE_KC) {
+        // read: cell l's ranks k0 .. k0 + LANE_KC - 1 (those it has)
+        for (int i = tid; i < 64 * LANE_KC; i += LANE_T) {
+            const int l = i / LANE_KC, kk = i % LANE_KC;
+            if (l < nc) {
+                const int64_t r = coff[l] + k0 + kk;
+                if (r < coff[l + 1]) tile[kk][l] = val[r];
+            }
+        }
+        __syncthreads();
+        // write: rank k's cells, in cell order, to consecutive slots
+        for (int i = tid; i < 64 * LANE_KC; i += LANE_T) {
+            const int kk = i / 64, l = i % 64;
+            const uint32_t k = k0 + kk;
+            if (k < K) {
+                const unsigned long long m = M[tb + k];
+                if ((m >> l) & 1ull) {
+                    const unsigned long long below = l ? (m & (~0ull >> (64 - l))) : 0ull;
+                    perm2[sbase + A[tb + k] + __popcll(below)] = tile[kk][l];
+                }
+            }
+        }
+        __syncthreads();
+    }
 }
 
 int ensure_sort(Ctx *c) {
@@ -303,7 +365,7 @@ int ensure_sort(Ctx *c) {
     // the histograms, and room for the scans of the digit histograms, of the
     // per-cell counts (ncells) and of the segment tables (block sums in
     // sort_blk, one per SCAN_B items)
-    const int64_t hcap = std::max<int64_t>(256 * ntiles, (int64_t)c->nx * c->ny);
+    const int64_t hcap = std::max<int64_t>((int64_t)SORT_MAXR * ntiles, (int64_t)c->nx * c->ny);
     if (hcap > c->sort_hist_cap) {
         if (c->sort_hist) (void)hipFree(c->sort_hist);
         if (c->sort_blk) (void)hipFree(c->sort_blk);
@@ -324,25 +386,34 @@ static int scan(Ctx *c, uint32_t *a, int64_t n) {
     return PIC_OK;
 }
 
-// Sorts (sort_keys[0], sort_vals[0]) in place (ping-pong), stable.
-static int radix_sort_pairs(Ctx *c, int64_t n, int keybits) {
-    const int passes = (keybits + 7) / 8;
+// Stable sort by key of the first n items of `keys0` (read, not written; the
+// values are the item indices): the sorted keys and values end in
+// (sort_keys[0], sort_vals[0]).  `keys0` may be sort_keys[0] itself.
+static int radix_sort_pairs(Ctx *c, const uint32_t *keys0, int64_t n, int keybits) {
+    const int passes = (keybits + SORT_MAXB - 1) / SORT_MAXB;
     const int ntiles = (int)((n + SORT_TILE - 1) / SORT_TILE);
-    int cur = 0;
-    int shift = 0;
+    const uint32_t *kin = keys0, *vin = nullptr;
+    int out = 1, shift = 0;
     for (int ps = 0; ps < passes; ++ps) {
         const int bits = (keybits - shift + (passes - ps) - 1) / (passes - ps);
-        hipLaunchKernelGGL(k_digit_hist, dim3(ntiles), dim3(SORT_T), 0, c->stream,
-                           c->sort_keys[cur], n, shift, bits, ntiles, c->sort_hist);
+        hipLaunchKernelGGL(k_digit_hist, dim3(ntiles), dim3(SORT_T), 0, c->stream, kin, n, shift,
+                           bits, ntiles, c->sort_hist);
         PIC_TRY(scan(c, c->sort_hist, (int64_t)(1 << bits) * ntiles));
-        hipLaunchKernelGGL(k_digit_scatter, dim3(ntiles), dim3(SORT_T), 0, c->stream,
-                           c->sort_keys[cur], c->sort_vals[cur], n, shift, bits, ntiles,
-                           c->sort_hist, c->sort_keys[1 - cur], c->sort_vals[1 - cur]);
+        if (ps == 0)
+            hipLaunchKernelGGL(k_digit_scatter<true>, dim3(ntiles), dim3(SORT_T), 0, c->stream,
+                               kin, vin, n, shift, bits, ntiles, c->sort_hist,
+                               c->sort_keys[out], c->sort_vals[out]);
+        else
+            hipLaunchKernelGGL(k_digit_scatter<false>, dim3(ntiles), dim3(SORT_T), 0, c->stream,
+                               kin, vin, n, shift, bits, ntiles, c->sort_hist,
+                               c->sort_keys[out], c->sort_vals[out]);
         PIC_HIP(hipGetLastError());
-        cur = 1 - cur;
+        kin = c->sort_keys[out];
+        vin = c->sort_vals[out];
+        out = 1 - out;
         shift += bits;
     }
-    if (cur != 0) {
+    if (kin != c->sort_keys[0]) {  // the result is in slot 1
         std::swap(c->sort_keys[0], c->sort_keys[1]);
         std::swap(c->sort_vals[0], c->sort_vals[1]);
     }
@@ -551,24 +622,34 @@ int sort_by_cell(Ctx *c, int32_t *perm_host, bool lanes, bool apply) {
     const int ncells = c->nx * c->ny;
     int keybits = 1;
     while ((1 << keybits) < ncells) ++keybits;
-    dim3 g((unsigned)((n + 255) / 256));
-    if (c->state == PIC_STATE_F32)
-        hipLaunchKernelGGL(k_keys_f32, g, dim3(256), 0, c->stream, n, c->cell, c->sort_keys[0],
-                           c->sort_vals[0]);
-    else
-        hipLaunchKernelGGL(k_keys_f64, g, dim3(256), 0, c->stream, n, c->x, c->y, c->dx, c->dy,
-                           c->nx, c->ny, c->sort_keys[0], c->sort_vals[0]);
-    PIC_HIP(hipGetLastError());
-    PIC_TRY(radix_sort_pairs(c, n, keybits));
+    // keys: the FP32 state's cell array as it is, or the FP64 state's cells
+    const uint32_t *keys0 = reinterpret_cast<const uint32_t *>(c->cell);
+    if (c->state != PIC_STATE_F32) {
+        hipLaunchKernelGGL(k_keys_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                           n, c->x, c->y, c->dx, c->dy, c->nx, c->ny, c->sort_keys[0]);
+        PIC_HIP(hipGetLastError());
+        keys0 = c->sort_keys[0];
+    }
+    PIC_TRY(radix_sort_pairs(c, keys0, n, keybits));
     hipLaunchKernelGGL(k_cell_offsets, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0,
                        c->stream, c->sort_keys[0], n, ncells, c->cell_off);
     PIC_HIP(hipGetLastError());
     const uint32_t *perm = c->sort_vals[0];
     if (lanes) {
         PIC_TRY(lane_spread(c));
-        hipLaunchKernelGGL(k_lane_dest, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                           c->stream, c->sort_keys[0], c->sort_vals[0], n, c->cell_off,
-                           c->seg_tbl, c->seg_A, c->seg_M, c->perm2);
+        // rank chunks per segment: enough blocks to fill the device several
+        // times over at the mean cell count (blocks past a segment's largest
+        // count exit at once)
+        const int nseg = (ncells + 63) / 64;
+        const int64_t per_cell = n / ncells + 1;
+        const int64_t chunks = (per_cell + LANE_KC - 1) / LANE_KC + 1;
+        const int64_t want = std::max<int64_t>(1, (int64_t)c->ncu * 16 / nseg);
+        const unsigned gy = (unsigned)std::max<int64_t>(1, std::min<int64_t>(chunks, want));
+        hipLaunchKernelGGL(k_lane_transpose, dim3((unsigned)nseg, gy), dim3(LANE_T), 0,
+                           c->stream, (const uint32_t *)c->sort_vals[0],
+                           (const int64_t *)c->cell_off, ncells, (const uint32_t *)c->seg_k,
+                           (const uint32_t *)c->seg_tbl, (const uint32_t *)c->seg_A,
+                           (const unsigned long long *)c->seg_M, c->perm2);
         PIC_HIP(hipGetLastError());
         perm = c->perm2;
         c->seg_valid = true;
