@@ -249,6 +249,14 @@ int pic_step_pass(pic_ctx *ctx, int kernel, int mode, double dt, double *rho_raw
  * number of particle passes since timing was enabled (any pointer may be
  * NULL). */
 int pic_set_timing(pic_ctx *ctx, int enable);
+
+/* The E solve of pic_solve_e and pic_run on power-of-two grids up to
+ * 512 x 512: fused = 1 (the default) runs it as three kernels with the FFTs
+ * in LDS, fused = 0 as rocFFT transforms around the spectral kernel.  Both
+ * compute the same transforms and spectral values (poisson_2d.jl:97-113);
+ * results agree to rounding.  Other grids always use rocFFT.  No reference
+ * counterpart. */
+int pic_set_fused_solve(pic_ctx *ctx, int fused);
 int pic_get_timing(pic_ctx *ctx, double *pass_ms, int64_t *passes, double *solve_ms);
 
 /* Gradient of the field energy after `nsteps` iterations of pic_run's loop,

@@ -507,6 +507,11 @@ class Simulation:
     def set_timing(self, on=True):
         L.call("pic_set_timing", self.ctx.h, 1 if on else 0)
 
+    def set_fused_solve(self, on=True):
+        """E solve as three LDS-FFT kernels (power-of-two grids up to 512) or
+        as rocFFT transforms (pic_set_fused_solve)."""
+        L.call("pic_set_fused_solve", self.ctx.h, 1 if on else 0)
+
     def set_f32_accumulation(self, mode):
         """Charge accumulation of the FP32 state: 0 = compensated FP32
         (hi, lo) tile pairs, 1 = FP64 tile sums, 2 = int32 fixed-point tile

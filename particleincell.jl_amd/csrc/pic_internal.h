@@ -131,6 +131,10 @@ struct Ctx {
     // conflict on banks twice as often as 4-byte ones)
     int f32_acc = 2;
 
+    // E solve on power-of-two grids up to 512 x 512: three LDS-transform
+    // kernels (true) or the rocFFT sequence (false)
+    bool fused_solve = true;
+
     // adjoint reverse sweep: steps whose positions are kept at once (0 =
     // the most the free device memory allows), and the last sweep's plan
     int adj_block = 0;

@@ -16,6 +16,7 @@
 // inverse along y for those columns, which is this projection.  The kernel
 // also folds in the 1/(nx*ny) normalisation and, in the time loop, the
 // 1/(dx*dy) charge normalisation of the raw deposit.
+#include <algorithm>
 #include <cstdio>
 #include <cmath>
 #include <vector>
@@ -287,6 +288,216 @@ __global__ __launch_bounds__(RED_THREADS) void k_post(const double2 *__restrict_
 
 static inline dim3 grid_of(int64_t m) { return dim3((unsigned)((m + 255) / 256)); }
 
+// ------------------------------------------- fused solve for small grids
+//
+// For power-of-two nx, ny <= 512 the E solve runs as three kernels with the
+// transforms in LDS instead of the rocFFT sequence (seven launches of a few
+// microseconds each at these sizes):
+//   k_fft_rows_fwd   per row: forward FFT along x of rho (imaginary part 0);
+//                    keeps the half spectrum ik <= nx/2, column-major
+//   k_fft_cols_e     per half-spectrum column ih: forward FFT along y, the
+//                    spectral values of k_spectral<SPEC_E> for column ih and
+//                    its mirror nx - ih (which depends on column ih only), and
+//                    the inverse FFTs along y of both
+//   k_fft_rows_inv   per row: inverse FFT along x; ex = Re, ey = Im, with
+//                    k_post's FP32 copy, accumulator clearing and energy sums
+// Same transforms (unnormalised, exp(-+2 pi i jk/N)) and the same spectral
+// arithmetic as the rocFFT path; results agree to rounding.
+constexpr int FFT_T = 64;
+constexpr int FFT_MAX_LOG = 9;
+
+__device__ void fft_twiddles(double2 *w, int logN, int sign) {
+    const int N = 1 << logN;
+    for (int k = threadIdx.x; k < N / 2; k += blockDim.x) {
+        double sn, cs;
+        sincospi((double)(sign * 2 * k) / (double)N, &sn, &cs);
+        w[k] = make_double2(cs, sn);
+    }
+}
+
+// In-place radix-2 decimation-in-time FFT of x[0, N) in LDS by the block's
+// threads, with twiddles w[k] = exp(sign 2 pi i k / N).  Ends synchronised.
+__device__ void fft_lds(double2 *x, const double2 *w, int logN) {
+    const int N = 1 << logN, tid = threadIdx.x, T = blockDim.x;
+    for (int i = tid; i < N; i += T) {
+        const int r = (int)(__brev((unsigned)i) >> (32 - logN));
+        if (i < r) {
+            const double2 t = x[i];
+            x[i] = x[r];
+            x[r] = t;
+        }
+    }
+    __syncthreads();
+    for (int s = 1; s <= logN; ++s) {
+        const int h = 1 << (s - 1), stride = N >> s;
+        for (int b = tid; b < N / 2; b += T) {
+            const int j = b & (h - 1);
+            const int k = ((b >> (s - 1)) << s) + j;
+            const double2 tw = w[j * stride];
+            const double2 u = x[k], v = x[k + h];
+            const double2 vt = make_double2(v.x * tw.x - v.y * tw.y, v.x * tw.y + v.y * tw.x);
+            x[k] = make_double2(u.x + vt.x, u.y + vt.y);
+            x[k + h] = make_double2(u.x - vt.x, u.y - vt.y);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(FFT_T) void k_fft_rows_fwd(const double *__restrict__ rho, int nx,
+                                                        int ny, int lognx,
+                                                        double2 *__restrict__ At) {
+    extern __shared__ double2 fsm[];
+    double2 *x = fsm, *w = fsm + nx;
+    const int j = blockIdx.x;
+    fft_twiddles(w, lognx, -1);
+    for (int i = threadIdx.x; i < nx; i += FFT_T)
+        x[i] = make_double2(rho[(int64_t)j * nx + i], 0.0);
+    __syncthreads();
+    fft_lds(x, w, lognx);
+    for (int ih = threadIdx.x; ih <= nx / 2; ih += FFT_T) At[(int64_t)ih * ny + j] = x[ih];
+}
+
+__global__ __launch_bounds__(FFT_T) void k_fft_cols_e(const double2 *__restrict__ At, int nx,
+                                                      int ny, int logny,
+                                                      const double *__restrict__ kxk2,
+                                                      const double *__restrict__ kyk2,
+                                                      double scale, double2 *__restrict__ B) {
+    extern __shared__ double2 fsm[];
+    double2 *S = fsm, *Z0 = fsm + ny, *Z1 = fsm + 2 * ny, *wf = fsm + 3 * ny,
+            *wi = fsm + 3 * ny + ny / 2;
+    const int ih = blockIdx.x, h = nx / 2 + 1, tid = threadIdx.x;
+    fft_twiddles(wf, logny, -1);
+    fft_twiddles(wi, logny, 1);
+    for (int j = tid; j < ny; j += FFT_T) S[j] = At[(int64_t)ih * ny + j];
+    __syncthreads();
+    fft_lds(S, wf, logny);
+    const bool self = ih == 0 || 2 * ih == nx;  // self-conjugate column
+    // half_value<SPEC_E> of (ih, jh) from the column in LDS
+    auto hv = [&](int jh, double2 &f, double2 &g) {
+        const int64_t q = (int64_t)ih + (int64_t)h * jh;
+        if (q == 0) {
+            f = g = make_double2(0.0, 0.0);
+        } else {
+            f = mul_mi(kxk2[q], S[jh], scale);
+            g = mul_mi(kyk2[q], S[jh], scale);
+        }
+    };
+    for (int jk = tid; jk < ny; jk += FFT_T) {
+        double2 f, g;
+        hv(jk, f, g);
+        if (self) {
+            double2 fm, gm;
+            hv((ny - jk) % ny, fm, gm);
+            f = make_double2(0.5 * (f.x + fm.x), 0.5 * (f.y - fm.y));
+            g = make_double2(0.5 * (g.x + gm.x), 0.5 * (g.y - gm.y));
+        }
+        Z0[jk] = make_double2(f.x - g.y, f.y + g.x);
+        if (!self) {  // column nx - ih: the conjugate half values at -jk
+            double2 f1, g1;
+            hv((ny - jk) % ny, f1, g1);
+            f1 = cconj(f1);
+            g1 = cconj(g1);
+            Z1[jk] = make_double2(f1.x - g1.y, f1.y + g1.x);
+        }
+    }
+    __syncthreads();
+    fft_lds(Z0, wi, logny);
+    if (!self) fft_lds(Z1, wi, logny);
+    for (int j = tid; j < ny; j += FFT_T) {
+        B[(int64_t)j * nx + ih] = Z0[j];
+        if (!self) B[(int64_t)j * nx + (nx - ih)] = Z1[j];
+    }
+}
+
+// Inverse transform along x and k_post's work, row by row; partial sums per
+// block over its rows in a fixed order, combined in block order by the last
+// block to arrive.
+__global__ __launch_bounds__(FFT_T) void k_fft_rows_inv(
+    const double2 *__restrict__ B, int nx, int ny, int lognx, double *__restrict__ ex,
+    double *__restrict__ ey, float2 *__restrict__ e32, double *__restrict__ zero_acc,
+    double *part, unsigned *count, double *hist, int slot, int *counter, double dxdy) {
+    extern __shared__ double2 fsm[];
+    __shared__ double sh[2 * FFT_T];
+    __shared__ bool last;
+    double2 *x = fsm, *w = fsm + nx;
+    fft_twiddles(w, lognx, 1);
+    double sx = 0.0, sy = 0.0;
+    for (int j = blockIdx.x; j < ny; j += gridDim.x) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < nx; i += FFT_T) x[i] = B[(int64_t)j * nx + i];
+        __syncthreads();
+        fft_lds(x, w, lognx);
+        for (int i = threadIdx.x; i < nx; i += FFT_T) {
+            const int64_t k = (int64_t)j * nx + i;
+            const double a = x[i].x, b = x[i].y;
+            ex[k] = a;
+            ey[k] = b;
+            sx += a * a;
+            sy += b * b;
+            if (e32) e32[k] = make_float2((float)a, (float)b);
+            if (zero_acc) zero_acc[k] = 0.0;
+        }
+    }
+    block_sum2<FFT_T>(sx, sy, sh);
+    if (slot == -1) return;
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sx;
+        part[2 * blockIdx.x + 1] = sy;
+        __threadfence();
+        const unsigned ticket = atomicAdd(count, 1u);
+        last = (ticket == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < (int)gridDim.x; k += FFT_T) {
+        a += __hip_atomic_load(&part[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        b += __hip_atomic_load(&part[2 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_sum2<FFT_T>(a, b, sh);
+    if (threadIdx.x == 0) {
+        int s = slot;
+        if (slot == -2) {
+            s = *counter;
+            *counter = s + 1;
+        }
+        hist[2 * s] = a * dxdy;
+        hist[2 * s + 1] = b * dxdy;
+        *count = 0u;
+    }
+}
+
+static int log2_exact(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return (1 << l) == n ? l : -1;
+}
+
+// The fused path covers power-of-two grids from 4 to 512 nodes per side.
+static bool fused_solve_ok(const Ctx *c) {
+    const int lx = log2_exact(c->nx), ly = log2_exact(c->ny);
+    return lx >= 2 && ly >= 2 && lx <= FFT_MAX_LOG && ly <= FFT_MAX_LOG;
+}
+
+static int solve_e_fused(Ctx *c, const double *rho_in, double scale, int post, int energy_slot) {
+    const int nx = c->nx, ny = c->ny, lx = log2_exact(nx), ly = log2_exact(ny);
+    double2 *At = c->zf, *B = c->zg;  // (nx/2+1) x ny column-major; nx x ny
+    hipLaunchKernelGGL(k_fft_rows_fwd, dim3((unsigned)ny), dim3(FFT_T),
+                       (size_t)(nx + nx / 2) * sizeof(double2), c->stream, rho_in, nx, ny, lx, At);
+    hipLaunchKernelGGL(k_fft_cols_e, dim3((unsigned)(nx / 2 + 1)), dim3(FFT_T),
+                       (size_t)(4 * ny) * sizeof(double2), c->stream, (const double2 *)At, nx, ny,
+                       ly, (const double *)c->kx_k2, (const double *)c->ky_k2, scale, B);
+    const int blocks = std::min(ny, RED_BLOCKS);
+    hipLaunchKernelGGL(k_fft_rows_inv, dim3((unsigned)blocks), dim3(FFT_T),
+                       (size_t)(nx + nx / 2) * sizeof(double2), c->stream, (const double2 *)B,
+                       nx, ny, lx, c->ex, c->ey, (post & 1) ? (float2 *)c->e32 : nullptr,
+                       (post & 2) ? c->rho_acc : nullptr, c->red_part, c->red_count,
+                       c->energy_hist, energy_slot, c->step_counter, c->dx * c->dy);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
 template <int MODE>
 static int spectral(Ctx *c, double scale) {
     const int64_t mm = (int64_t)c->nx * c->ny;
@@ -300,6 +511,8 @@ static int spectral(Ctx *c, double scale) {
 // field copy, bit 1 clears the raw deposit accumulator (after the forward
 // transform has read it).  energy_slot as in k_post.
 int poisson_solve_e(Ctx *c, const double *rho_in, double scale, int post, int energy_slot) {
+    if (c->fused_solve && fused_solve_ok(c))
+        return solve_e_fused(c, rho_in, scale, post, energy_slot);
     const int64_t mm = (int64_t)c->nx * c->ny;
     hipLaunchKernelGGL(k_pack, grid_of(mm), dim3(256), 0, c->stream, rho_in, (const double *)nullptr,
                        mm, c->zf);
@@ -481,3 +694,9 @@ int pic_solve_fields(pic_ctx *ctx, double *energy) {
 }
 
 }  // extern "C"
+
+extern "C" int pic_set_fused_solve(pic_ctx *ctx, int fused) {
+    PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
+    ::pic::ctx_of(ctx)->fused_solve = fused != 0;
+    return PIC_OK;
+}

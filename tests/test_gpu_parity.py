@@ -251,6 +251,39 @@ def test_solve_e_phi(gpu, O):
         assert nrel(phi, O.solve_phi(rho, dimx, dimy)) < RTOL
 
 
+@pytest.mark.parametrize("nx,ny", [(8, 8), (32, 16), (128, 128), (512, 64), (64, 512)])
+def test_solve_e_fused_matches_rocfft(gpu, nx, ny):
+    """The E solve's LDS-FFT kernels (power-of-two grids up to 512) against
+    the rocFFT transforms around the same spectral kernel: fields to 1e-13
+    normwise, and the fused path's field energy as the reduction of those
+    fields (one pic_run step from a random deposit)."""
+    pic = gpu
+    rng = np.random.default_rng(nx + 7 * ny)
+    grid = pic.TwoDGrid(0.0, 2.0, nx, 0.0, 3.0, ny)
+    rho = np.asfortranarray(rng.normal(size=(nx, ny)))
+    out = []
+    for fused in (True, False):
+        ctx = pic.Context(grid)
+        pic._lib.call("pic_set_fused_solve", ctx.h, 1 if fused else 0)
+        ctx.set_grid(pic._lib.PIC_GRID_RHO, rho)
+        pic._lib.call("pic_solve_e", ctx.h)
+        out.append((ctx.get_grid(pic._lib.PIC_GRID_EX), ctx.get_grid(pic._lib.PIC_GRID_EY)))
+        ctx.close()
+    (ef, ff), (er, fr) = out
+    assert nrel(ef, er) < 1e-13 and nrel(ff, fr) < 1e-13
+    # the loop's energy history with either solve
+    n = 4096
+    p = random_particles(rng, n, grid.dimx, grid.dimy)
+    en = []
+    for fused in (True, False):
+        sim = pic.Simulation(grid, p, pic.CloudInCell())
+        sim.ctx.sync()
+        pic._lib.call("pic_set_fused_solve", sim.ctx.h, 1 if fused else 0)
+        en.append(sim.run(0.05, 3))
+        sim.close()
+    assert np.allclose(en[0], en[1], rtol=1e-12, atol=0)
+
+
 def test_solve_fields_fixture(gpu, O):
     """solve!(fields, Poisson2D) on the input of test/test2.dat: FFTW's output
     and the energy of meshfields.jl:105-106."""
