@@ -122,12 +122,14 @@ def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_ac
     # particles: deposit only, and gather + kick + drift
     split = {}
     for mode, name in ((3, "deposit"), (2, "gather_push"), (1, "fused")):
-        sim.sort(lanes=True)
-        sim.set_timing(True)
+        ms_sum = 0.0
         for _ in range(3):
+            sim.sort(lanes=True)
+            sim.set_timing(True)
             sim.step_pass(mode, dt)
-        ms, cnt, _ = sim.timing()
-        split[name] = ms / max(cnt, 1)
+            ms, cnt, _ = sim.timing()
+            ms_sum += ms / max(cnt, 1)
+        split[name] = ms_sum / 3
     sim.set_timing(False)
     out["split_ms"] = split
     sim.close()
