@@ -40,9 +40,16 @@ typedef struct pic_ctx pic_ctx;
 #define PIC_ERR_NOMEM -5      /* device allocation failed */
 
 /* particle shapes: the reference's dispatch tags CloudInCell
- * (src/cloud_in_cell.jl:6) and M6 (src/pushers_m6.jl:3) */
+ * (src/cloud_in_cell.jl:6), M6 (src/pushers_m6.jl:3), NearestGridPoint
+ * (src/pushers_ngp.jl:3), TriangularShapeCloud (src/pushers_tsc.jl:14) and
+ * M4 (src/pushers_m4.jl:6).  As in the reference, TriangularShapeCloud
+ * deposits with the TSC weights f_tsc but its push_v! gathers with the M4
+ * weights f_m4 (pushers_tsc.jl:46-53).  pic_adjoint takes CIC and M6. */
 #define PIC_CIC 0
 #define PIC_M6 1
+#define PIC_NGP 2
+#define PIC_TSC 3
+#define PIC_M4 4
 
 /* particle state held on the device */
 #define PIC_STATE_F64 0       /* x, y, vx, vy (+w) FP64 structure of arrays */

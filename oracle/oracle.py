@@ -49,6 +49,12 @@ def lib():
             "or_interpolate_cic": (None, [_i64, _dp, _i32, _i32, _d, _d, _dp, _dp, _dpc, _dpc]),
             "or_push_v_m6": (None, [_i64, _dp, _d, _i32, _i32, _d, _d, _dp, _dp, C.c_void_p, _i32]),
             "or_compute_rho_m6": (None, [_i64, _dp, _i32, _i32, _d, _d, _d, _d, _dp]),
+            "or_f_tsc": (_d, [_d]),
+            "or_f_m4": (_d, [_d]),
+            "or_push_v_ngp": (None, [_i64, _dp, _d, _i32, _i32, _d, _d, _dp, _dp, C.c_void_p, _i32]),
+            "or_push_v_m4": (None, [_i64, _dp, _d, _i32, _i32, _d, _d, _dp, _dp, C.c_void_p, _i32]),
+            "or_compute_rho_ngp": (None, [_i64, _dp, _i32, _i32, _d, _d, _d, _d, _dp]),
+            "or_compute_rho_shape4": (None, [_i64, _dp, _i32, _i32, _d, _d, _d, _d, _i32, _dp]),
             "or_compute_rho_m6_fields": (None, [_i64, _dp, _d, _i32, _i32, _d, _d, _d, _d, _i32, _dp]),
             "or_interpol_eb_m6": (None, [_i64, _dp, _i32, _i32, _d, _d, _d, _d, _dp, _dp]),
             "or_rfft2": (_i32, [_i32, _i32, _dp, _i32, _cp]),
@@ -95,11 +101,25 @@ def f_m6(q):
     return lib().or_f_m6(float(q))
 
 
+# kernel names and the loop's kernel codes
+KERNELS = {"cic": 0, "m6": 1, "ngp": 2, "tsc": 3, "m4": 4}
+
+
+def f_tsc(x):
+    return lib().or_f_tsc(float(x))
+
+
+def f_m4(x):
+    return lib().or_f_m4(float(x))
+
+
 def push_v(p, kernel, nx, ny, dx, dy, ex, ey, bz, dt):
-    """push_v!(p, kernel, mesh, ex, ey, bz, dt) on a (5, n) array, in place."""
+    """push_v!(p, kernel, mesh, ex, ey, bz, dt) on a (5, n) array, in place.
+    TSC gathers with the M4 weights, as the reference's method does."""
     ld = ex.shape[0]
     assert p.flags.f_contiguous and p.dtype == np.float64
-    f = lib().or_push_v_cic if kernel == "cic" else lib().or_push_v_m6
+    f = {"cic": lib().or_push_v_cic, "m6": lib().or_push_v_m6, "ngp": lib().or_push_v_ngp,
+         "tsc": lib().or_push_v_m4, "m4": lib().or_push_v_m4}[kernel]
     bzf = None if bz is None else _F(bz)
     f(p.shape[1], p, dt, nx, ny, dx, dy, _F(ex), _F(ey), _ptr_or_none(bzf), ld)
 
@@ -112,8 +132,13 @@ def compute_rho(p, kernel, nx, ny, dx, dy, dimx=None, dimy=None):
     rho = np.zeros((nx, ny), order="F")
     if kernel == "cic":
         lib().or_compute_rho_cic(p.shape[1], p, nx, ny, dx, dy, rho)
-    else:
+    elif kernel == "m6":
         lib().or_compute_rho_m6(p.shape[1], p, nx, ny, dx, dy, dimx, dimy, rho)
+    elif kernel == "ngp":
+        lib().or_compute_rho_ngp(p.shape[1], p, nx, ny, dx, dy, dimx, dimy, rho)
+    else:
+        lib().or_compute_rho_shape4(p.shape[1], p, nx, ny, dx, dy, dimx, dimy,
+                                    1 if kernel == "m4" else 0, rho)
     return rho
 
 
@@ -221,7 +246,7 @@ def lane_spread(cell, ncells):
 def landau_loop(p, kernel, nx, ny, dimx, dimy, dt, nsteps, bz0=0.0):
     e = np.zeros(nsteps)
     rc = lib().or_landau_loop(p.shape[1], p, nx, ny, dimx, dimy, dt, nsteps,
-                              0 if kernel == "cic" else 1, bz0, e)
+                              KERNELS[kernel], bz0, e)
     assert rc == 0
     return e
 

@@ -680,11 +680,191 @@ void or_counting_sort(int64_t n, const int32_t *key, int32_t nkeys, int32_t *per
     free(cnt);
 }
 
+/* ------------------------------------------ NGP, TSC and M4 (shapes) */
+
+/* f_tsc(x): pushers_tsc.jl:3-12 (x^2 is x*x in Julia). */
+double or_f_tsc(double x)
+{
+    if (x <= 0.5)
+        return 0.75 - x * x;
+    else if (0.5 < x && x <= 1.5)
+        return 0.5 * ((1.5 - x) * (1.5 - x));
+    return 0.0;
+}
+
+/* f_m4(x): pushers_m4.jl:8-19 (x^3 is (x*x)*x), times 2/3 as (f*2)/3. */
+double or_f_m4(double x)
+{
+    double f = 0.0;
+    if (x > 2)
+        f = 0.;
+    else if (x >= 1 && x <= 2) {
+        double t = 2 - x;
+        f = 0.25 * ((t * t) * t);
+    } else if (x <= 1)
+        f = (1 - 1.5 * (x * x)) + 0.75 * ((x * x) * x);
+    return f * 2 / 3;
+}
+
+/* Weights of the offsets -1, 0, +1, +2 (and +3 for M4's deposit, whose
+ * weight f_m4(3-d) is always 0), in the order of the reference's
+ * cm1, c, cp1, cp2(, cp3). */
+static inline void shape4_weights(double (*f)(double), double d, double c[5])
+{
+    c[0] = f(1 + d);
+    c[1] = f(d);
+    c[2] = f(1 - d);
+    c[3] = f(2 - d);
+    c[4] = f(3 - d);
+}
+
+/* push_v! of NearestGridPoint (pushers_ngp.jl:5-44): the node of
+ * round(Int, xp) (ties to even) + 1, mod1. */
+void or_push_v_ngp(int64_t n, double *p, double dt, int nx, int ny, double dx, double dy,
+                   const double *ex, const double *ey, const double *bz, int ld)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n; ++k) {
+        double *q = p + 5 * k;
+        double v1 = q[2];
+        double v2 = q[3];
+        double xp = q[0] / dx;
+        double yp = q[1] / dy;
+        int64_t i = mod1((int64_t)nearbyint(xp) + 1, nx);
+        int64_t j = mod1((int64_t)nearbyint(yp) + 1, ny);
+        double e1 = ex[IX(i - 1, j - 1, ld)];
+        double e2 = ey[IX(i - 1, j - 1, ld)];
+        double b3 = bz ? bz[IX(i - 1, j - 1, ld)] : 0.0;
+        v1 += 0.5 * dt * e1;
+        v2 += 0.5 * dt * e2;
+        double tantheta = 0.5 * dt * b3;
+        double sintheta = 2 * tantheta / (1 + tantheta * tantheta);
+        v1 += v2 * tantheta;
+        v2 += -v1 * sintheta;
+        v1 += v2 * tantheta;
+        q[2] = v1 + 0.5 * dt * e1;
+        q[3] = v2 + 0.5 * dt * e2;
+    }
+}
+
+/* push_v! of TriangularShapeCloud (pushers_tsc.jl:16-127) and of M4
+ * (pushers_m4.jl:22-131): both gather with the M4 weights f_m4 on the 4 x 4
+ * nodes i-1 .. i+2 (the TSC method calls f_m4, pushers_tsc.jl:46-53), x
+ * offset outer, y offset inner, each term (cx*cy)*field. */
+void or_push_v_m4(int64_t n, double *p, double dt, int nx, int ny, double dx, double dy,
+                  const double *ex, const double *ey, const double *bz, int ld)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n; ++k) {
+        double *q = p + 5 * k;
+        double v1 = q[2];
+        double v2 = q[3];
+        double xp = q[0] / dx;
+        double yp = q[1] / dy;
+        int64_t i = (int64_t)floor(xp) + 1;
+        int64_t j = (int64_t)floor(yp) + 1;
+        double dpx = (xp - (double)i) + 1;
+        double dpy = (yp - (double)j) + 1;
+        int64_t ii[4], jj[4];
+        for (int a = 0; a < 4; ++a) {
+            ii[a] = mod1(i - 1 + a, nx);
+            jj[a] = mod1(j - 1 + a, ny);
+        }
+        double cx[5], cy[5];
+        shape4_weights(or_f_m4, dpx, cx);
+        shape4_weights(or_f_m4, dpy, cy);
+        double e1 = 0.0, e2 = 0.0, b3 = 0.0;
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b)
+                e1 += cx[a] * cy[b] * ex[IX(ii[a] - 1, jj[b] - 1, ld)];
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b)
+                e2 += cx[a] * cy[b] * ey[IX(ii[a] - 1, jj[b] - 1, ld)];
+        if (bz)
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b)
+                    b3 += cx[a] * cy[b] * bz[IX(ii[a] - 1, jj[b] - 1, ld)];
+        v1 += 0.5 * dt * e1;
+        v2 += 0.5 * dt * e2;
+        double tantheta = 0.5 * dt * b3;
+        double sintheta = 2 * tantheta / (1 + tantheta * tantheta);
+        v1 += v2 * tantheta;
+        v2 += -v1 * sintheta;
+        v1 += v2 * tantheta;
+        q[2] = v1 + 0.5 * dt * e1;
+        q[3] = v2 + 0.5 * dt * e2;
+    }
+}
+
+/* The tail shared by the NGP, TSC and M4 compute_rho: rho ./= dx*dy, then
+ * rho .- (sum(rho) dx dy)/dimx/dimy (pushers_ngp.jl:74-80). */
+static void shape_rho_finish(double *rho, int nx, int ny, double dx, double dy, double dimx,
+                             double dimy)
+{
+    size_t m = (size_t)nx * ny;
+    for (size_t k = 0; k < m; ++k) rho[k] /= (dx * dy);
+    double rho_total = pairwise_sum(rho, m) * dx * dy;
+    double shift = rho_total / dimx / dimy;
+    for (size_t k = 0; k < m; ++k) rho[k] -= shift;
+}
+
+/* compute_rho(p, ::NearestGridPoint, mesh): pushers_ngp.jl:46-82. */
+void or_compute_rho_ngp(int64_t n, const double *p, int nx, int ny, double dx, double dy,
+                        double dimx, double dimy, double *rho)
+{
+    memset(rho, 0, sizeof(double) * (size_t)nx * ny);
+    for (int64_t k = 0; k < n; ++k) {
+        const double *q = p + 5 * k;
+        double xp = q[0] / dx;
+        double yp = q[1] / dy;
+        int64_t i = mod1((int64_t)nearbyint(xp) + 1, nx);
+        int64_t j = mod1((int64_t)nearbyint(yp) + 1, ny);
+        rho[IX(i - 1, j - 1, nx)] += q[4];
+    }
+    shape_rho_finish(rho, nx, ny, dx, dy, dimx, dimy);
+}
+
+/* compute_rho(p, ::TriangularShapeCloud, mesh) (pushers_tsc.jl:129-204, the
+ * 4 x 4 nodes i-1 .. i+2 with f_tsc) and compute_rho(p, ::M4, mesh)
+ * (pushers_m4.jl:135-228, the 5 x 5 nodes i-1 .. i+3 with f_m4, the last row
+ * and column of weight 0).  Scatter order x offset outer, y offset inner,
+ * each charge (cx*cy)*weight.  m4 selects the M4 method. */
+void or_compute_rho_shape4(int64_t n, const double *p, int nx, int ny, double dx, double dy,
+                           double dimx, double dimy, int m4, double *rho)
+{
+    memset(rho, 0, sizeof(double) * (size_t)nx * ny);
+    const int S = m4 ? 5 : 4;
+    for (int64_t k = 0; k < n; ++k) {
+        const double *q = p + 5 * k;
+        double xp = q[0] / dx;
+        double yp = q[1] / dy;
+        int64_t i = (int64_t)floor(xp) + 1;
+        int64_t j = (int64_t)floor(yp) + 1;
+        double dpx = (xp - (double)i) + 1;
+        double dpy = (yp - (double)j) + 1;
+        double weight = q[4];
+        int64_t ii[5], jj[5];
+        for (int a = 0; a < 5; ++a) {
+            ii[a] = mod1(i - 1 + a, nx);
+            jj[a] = mod1(j - 1 + a, ny);
+        }
+        double cx[5], cy[5];
+        shape4_weights(m4 ? or_f_m4 : or_f_tsc, dpx, cx);
+        shape4_weights(m4 ? or_f_m4 : or_f_tsc, dpy, cy);
+        for (int a = 0; a < S; ++a)
+            for (int b = 0; b < S; ++b)
+                rho[IX(ii[a] - 1, jj[b] - 1, nx)] += cx[a] * cy[b] * weight;
+    }
+    shape_rho_finish(rho, nx, ny, dx, dy, dimx, dimy);
+}
+
 /* ------------------------------------------------------------- time loop */
 
 /* The hot-path driver's time loop (examples/landau_damping_poisson_cic_m6.jl
  * :70-83): drift 1/2, deposit, Poisson solve, energy, kick, drift 1/2.
- * kernel 0 = CloudInCell, 1 = M6.  push_v!/push_x! run with OpenMP threads
+ * kernel 0 = CloudInCell, 1 = M6, 2 = NearestGridPoint, 3 =
+ * TriangularShapeCloud, 4 = M4 (examples/landau_damping_poisson_cic_m6.jl
+ * :88-103 runs the loop with each).  push_v!/push_x! run with OpenMP threads
  * (as @threads), the deposit is serial (as compute_rho).  energy[s] receives
  * sqrt(sum(ex^2) dx dy) after the solve of step s.  The domain is [0,dimx) x
  * [0,dimy) and bz is identically `bz0`. */
@@ -703,14 +883,22 @@ int or_landau_loop(int64_t n, double *p, int nx, int ny, double dimx, double dim
         or_push_x(n, p, 0.5 * dt, dimx, dimy);
         if (kernel == 0)
             or_compute_rho_cic(n, p, nx, ny, dx, dy, rho);
-        else
+        else if (kernel == 1)
             or_compute_rho_m6(n, p, nx, ny, dx, dy, dimx, dimy, rho);
+        else if (kernel == 2)
+            or_compute_rho_ngp(n, p, nx, ny, dx, dy, dimx, dimy, rho);
+        else
+            or_compute_rho_shape4(n, p, nx, ny, dx, dy, dimx, dimy, kernel == 4, rho);
         rc = or_solve_e(nx, ny, dimx, dimy, rho, ex, ey);
         if (energy) energy[s] = or_energy(nx, ny, dx, dy, ex, NULL, NULL);
         if (kernel == 0)
             or_push_v_cic(n, p, dt, nx, ny, dx, dy, ex, ey, bz, nx);
-        else
+        else if (kernel == 1)
             or_push_v_m6(n, p, dt, nx, ny, dx, dy, ex, ey, bz, nx);
+        else if (kernel == 2)
+            or_push_v_ngp(n, p, dt, nx, ny, dx, dy, ex, ey, bz, nx);
+        else
+            or_push_v_m4(n, p, dt, nx, ny, dx, dy, ex, ey, bz, nx);
         or_push_x(n, p, 0.5 * dt, dimx, dimy);
     }
     free(rho); free(ex); free(ey); free(bz);

@@ -11,7 +11,7 @@ LIB_NAME = "libpicmi355x.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 
 PIC_OK = 0
-PIC_CIC, PIC_M6 = 0, 1
+PIC_CIC, PIC_M6, PIC_NGP, PIC_TSC, PIC_M4 = 0, 1, 2, 3, 4
 PIC_STATE_F64, PIC_STATE_F32 = 0, 1
 PIC_GRID_RHO, PIC_GRID_EX, PIC_GRID_EY, PIC_GRID_BZ, PIC_GRID_PHI = range(5)
 PIC_BZ_NONE, PIC_BZ_UNIFORM, PIC_BZ_GRID = range(3)

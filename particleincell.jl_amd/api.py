@@ -76,13 +76,31 @@ class M6:
     code = L.PIC_M6
 
 
+class NearestGridPoint:
+    """pushers_ngp.jl: the nearest node (round half to even)."""
+    code = L.PIC_NGP
+
+
+class TriangularShapeCloud:
+    """pushers_tsc.jl: TSC deposit; its push_v! gathers with the M4 weights,
+    as the reference's does."""
+    code = L.PIC_TSC
+
+
+class M4:
+    """pushers_m4.jl: the cubic B-spline."""
+    code = L.PIC_M4
+
+
+_SHAPES = (CloudInCell, M6, NearestGridPoint, TriangularShapeCloud, M4)
+
+
 def _kcode(kernel):
     k = kernel if isinstance(kernel, type) else type(kernel)
-    if k is CloudInCell:
-        return L.PIC_CIC
-    if k is M6:
-        return L.PIC_M6
-    raise TypeError(f"unsupported kernel {kernel!r}: CloudInCell() or M6()")
+    if k in _SHAPES:
+        return k.code
+    raise TypeError(f"unsupported kernel {kernel!r}: CloudInCell(), M6(), NearestGridPoint(), "
+                    "TriangularShapeCloud() or M4()")
 
 
 # --------------------------------------------------------- device context

@@ -84,7 +84,7 @@ __global__ __launch_bounds__(AT) void k_adj32_advance(int64_t n, int32_t *cell, 
     if (KICK) {
         float e1, e2, b3 = 0.0f;
         interp32<K, PIC_BZ_NONE>(
-            fx, fy,
+            fx, fy, ci, cj,
             [&](int u, int v) {
                 return e32[(int64_t)((ci - HL + u) & mx) + (int64_t)g.nx * ((cj - HL + v) & my)];
             },

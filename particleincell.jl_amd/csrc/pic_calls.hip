@@ -187,7 +187,7 @@ extern "C" {
 
 int pic_compute_rho(pic_ctx *ctx, int kernel) {
     PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
-    PIC_CHECK(kernel == PIC_CIC || kernel == PIC_M6, PIC_ERR_ARG, "kernel %d", kernel);
+    PIC_CHECK(valid_shape(kernel), PIC_ERR_ARG, "kernel %d", kernel);
     Ctx *c = ctx_of(ctx);
     PIC_HIP(hipSetDevice(c->device));
     PIC_TRY(launch_fill(c, c->rho_acc, 0.0, (int64_t)c->nx * c->ny));
@@ -202,7 +202,7 @@ int pic_compute_rho(pic_ctx *ctx, int kernel) {
 
 int pic_push_v(pic_ctx *ctx, int kernel, double dt) {
     PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
-    PIC_CHECK(kernel == PIC_CIC || kernel == PIC_M6, PIC_ERR_ARG, "kernel %d", kernel);
+    PIC_CHECK(valid_shape(kernel), PIC_ERR_ARG, "kernel %d", kernel);
     Ctx *c = ctx_of(ctx);
     PIC_HIP(hipSetDevice(c->device));
     PIC_TRY(need_f64(c));

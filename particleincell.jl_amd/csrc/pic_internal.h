@@ -292,4 +292,151 @@ __device__ __forceinline__ void m6_weights_d(double t, double w[6]) {
     w[5] = (t2 * t2 * t) * c;
 }
 
+// ------------------------------------------------ NGP, TSC, M4 weights
+//
+// Stencils of four nodes i-1 .. i+2 around the cell i of the particle, at the
+// in-cell offset d, in the reference's order cm1, c, cp1, cp2: the weights
+// f(1+d), f(d), f(1-d), f(2-d) with f = f_tsc (pushers_tsc.jl:3-12) or f_m4
+// (pushers_m4.jl:8-19), evaluated with the reference's operations.
+__device__ __forceinline__ double f_tsc_d(double x) {
+    if (x <= 0.5) return 0.75 - x * x;
+    if (x <= 1.5) return 0.5 * ((1.5 - x) * (1.5 - x));
+    return 0.0;
+}
+__device__ __forceinline__ double f_m4_d(double x) {
+    double f = 0.0;
+    if (x > 2) {
+        f = 0.0;
+    } else if (x >= 1) {
+        const double t = 2 - x;
+        f = 0.25 * ((t * t) * t);
+    } else {
+        f = (1 - 1.5 * (x * x)) + 0.75 * ((x * x) * x);
+    }
+    return f * 2 / 3;
+}
+__device__ __forceinline__ float f_tsc_f(float x) {
+    if (x <= 0.5f) return 0.75f - x * x;
+    if (x <= 1.5f) return 0.5f * ((1.5f - x) * (1.5f - x));
+    return 0.0f;
+}
+__device__ __forceinline__ float f_m4_f(float x) {
+    if (x > 2.0f) return 0.0f;
+    if (x >= 1.0f) {
+        const float t = 2.0f - x;
+        return (t * t) * t * PIC_F(1, 6);
+    }
+    return __builtin_fmaf(x * x, __builtin_fmaf(0.5f, x, -1.0f), PIC_F(2, 3));
+}
+
+// NGP as a two-node stencil i, i+1 with weights (1, 0) or (0, 1): the node
+// of round(Int, xp) (ties to even, pushers_ngp.jl:22-23) with xp = i + d.
+// The parity of the 0-based cell stands for that of floor(xp) (equal on
+// grids with an even number of cells).
+template <typename T>
+__device__ __forceinline__ void ngp_weights(T d, int cell, T w[2]) {
+    const bool up = d > T(0.5) || (d == T(0.5) && (cell & 1));
+    w[0] = up ? T(0) : T(1);
+    w[1] = up ? T(1) : T(0);
+}
+
 }  // namespace picd
+
+namespace pic {
+
+// Stencil of a particle shape: S nodes per direction starting HL nodes left
+// of the particle's cell; QMAX bounds the product of two deposit weights.
+template <int K>
+struct Shape;
+template <>
+struct Shape<PIC_CIC> {
+    static constexpr int S = 2, HL = 0;
+    static constexpr double QMAX = 1.0;
+};
+template <>
+struct Shape<PIC_M6> {
+    static constexpr int S = 6, HL = 2;
+    static constexpr double QMAX = 0.31;  // (66/120)^2
+};
+template <>
+struct Shape<PIC_NGP> {
+    static constexpr int S = 2, HL = 0;
+    static constexpr double QMAX = 1.0;
+};
+template <>
+struct Shape<PIC_TSC> {
+    static constexpr int S = 4, HL = 1;
+    static constexpr double QMAX = 0.5625;  // (3/4)^2
+};
+template <>
+struct Shape<PIC_M4> {
+    static constexpr int S = 4, HL = 1;
+    static constexpr double QMAX = 0.445;  // (2/3)^2
+};
+
+// Gather (push_v!) and deposit (compute_rho) weights of the separable
+// shapes, FP64 and FP32, for the S nodes of a direction.  CIC keeps its own
+// bilinear code in the kernels.
+template <int K>
+__device__ __forceinline__ void gather_w(double d, int cell, double *w) {
+    if (K == PIC_M6) {
+        picd::m6_weights_d(d, w);
+    } else if (K == PIC_NGP) {
+        picd::ngp_weights(d, cell, w);
+    } else {  // TSC and M4 both gather with f_m4
+        w[0] = picd::f_m4_d(1 + d);
+        w[1] = picd::f_m4_d(d);
+        w[2] = picd::f_m4_d(1 - d);
+        w[3] = picd::f_m4_d(2 - d);
+    }
+}
+template <int K>
+__device__ __forceinline__ void deposit_w(double d, int cell, double *w) {
+    if (K == PIC_TSC) {
+        w[0] = picd::f_tsc_d(1 + d);
+        w[1] = picd::f_tsc_d(d);
+        w[2] = picd::f_tsc_d(1 - d);
+        w[3] = picd::f_tsc_d(2 - d);
+    } else {
+        gather_w<K>(d, cell, w);
+    }
+}
+template <int K>
+__device__ __forceinline__ void gather_w(float d, int cell, float *w) {
+    if (K == PIC_M6) {
+        picd::m6_weights_f(d, w);
+    } else if (K == PIC_NGP) {
+        picd::ngp_weights(d, cell, w);
+    } else {
+        w[0] = picd::f_m4_f(1.0f + d);
+        w[1] = picd::f_m4_f(d);
+        w[2] = picd::f_m4_f(1.0f - d);
+        w[3] = picd::f_m4_f(2.0f - d);
+    }
+}
+template <int K>
+__device__ __forceinline__ void deposit_w(float d, int cell, float *w) {
+    if (K == PIC_TSC) {
+        w[0] = picd::f_tsc_f(1.0f + d);
+        w[1] = picd::f_tsc_f(d);
+        w[2] = picd::f_tsc_f(1.0f - d);
+        w[3] = picd::f_tsc_f(2.0f - d);
+    } else {
+        gather_w<K>(d, cell, w);
+    }
+}
+
+// Runs the statement(s) given after `kernel` with the compile-time constant
+// KK set to the run-time shape code `kernel`.
+#define PIC_SHAPE_SWITCH(kernel, ...)                                    \
+    switch (kernel) {                                                    \
+        case PIC_CIC: { constexpr int KK = PIC_CIC; __VA_ARGS__; } break; \
+        case PIC_M6: { constexpr int KK = PIC_M6; __VA_ARGS__; } break;   \
+        case PIC_NGP: { constexpr int KK = PIC_NGP; __VA_ARGS__; } break; \
+        case PIC_TSC: { constexpr int KK = PIC_TSC; __VA_ARGS__; } break; \
+        default: { constexpr int KK = PIC_M4; __VA_ARGS__; } break;       \
+    }
+
+inline bool valid_shape(int kernel) { return kernel >= PIC_CIC && kernel <= PIC_M4; }
+
+}  // namespace pic

@@ -24,8 +24,9 @@ __device__ __forceinline__ int drift1s(float o, float v, float dtd, float &onew)
 // Field at a particle from the nodes (u, v) of its stencil, u, v in [0, S):
 // CIC bilinear or M6 tensor-product weights, FP32, in a fixed operation order.
 template <int K, int BZ, class FE, class FB>
-__device__ __forceinline__ void interp32(float fx, float fy, FE node_e, FB node_b, float &e1,
-                                         float &e2, float &b3) {
+__device__ __forceinline__ void interp32(float fx, float fy, int ci, int cj, FE node_e, FB node_b,
+                                         float &e1, float &e2, float &b3) {
+    constexpr int S = Shape<K>::S;
     if (K == PIC_CIC) {
         float sx = 1.0f - fx, sy = 1.0f - fy;
         float a0 = sx * sy, a1 = fx * sy, a2 = fx * fy, a3 = sx * fy;
@@ -47,16 +48,16 @@ __device__ __forceinline__ void interp32(float fx, float fy, FE node_e, FB node_
         }
     } else {
         float wx[6], wy[6];
-        picd::m6_weights_f(fx, wx);
-        picd::m6_weights_f(fy, wy);
+        gather_w<K>(fx, ci, wx);
+        gather_w<K>(fy, cj, wy);
         e1 = 0.0f;
         e2 = 0.0f;
 #pragma unroll
-        for (int u = 0; u < 6; ++u) {
+        for (int u = 0; u < S; ++u) {
             float2 E0 = node_e(u, 0);
             float t1 = wy[0] * E0.x, t2 = wy[0] * E0.y;
 #pragma unroll
-            for (int v = 1; v < 6; ++v) {
+            for (int v = 1; v < S; ++v) {
                 float2 E = node_e(u, v);
                 t1 = __builtin_fmaf(wy[v], E.x, t1);
                 t2 = __builtin_fmaf(wy[v], E.y, t2);
@@ -71,10 +72,10 @@ __device__ __forceinline__ void interp32(float fx, float fy, FE node_e, FB node_
         }
         if (BZ == PIC_BZ_GRID) {
 #pragma unroll
-            for (int u = 0; u < 6; ++u) {
+            for (int u = 0; u < S; ++u) {
                 float t3 = wy[0] * node_b(u, 0);
 #pragma unroll
-                for (int v = 1; v < 6; ++v) t3 = __builtin_fmaf(wy[v], node_b(u, v), t3);
+                for (int v = 1; v < S; ++v) t3 = __builtin_fmaf(wy[v], node_b(u, v), t3);
                 b3 = u == 0 ? wx[0] * t3 : __builtin_fmaf(wx[u], t3, b3);
             }
         }

@@ -51,7 +51,7 @@ using namespace pic;
 extern "C" int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort_every,
                        double *energy) {
     PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
-    PIC_CHECK(kernel == PIC_CIC || kernel == PIC_M6, PIC_ERR_ARG, "kernel %d", kernel);
+    PIC_CHECK(valid_shape(kernel), PIC_ERR_ARG, "kernel %d", kernel);
     PIC_CHECK(nsteps >= 0, PIC_ERR_ARG, "nsteps %d", nsteps);
     PIC_CHECK(sort_every >= 0, PIC_ERR_ARG, "sort_every %d", sort_every);
     Ctx *c = ctx_of(ctx);
@@ -142,7 +142,7 @@ extern "C" int pic_get_timing(pic_ctx *ctx, double *pass_ms, int64_t *passes, do
 
 extern "C" int pic_step_pass(pic_ctx *ctx, int kernel, int mode, double dt, double *rho_raw) {
     PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
-    PIC_CHECK(kernel == PIC_CIC || kernel == PIC_M6, PIC_ERR_ARG, "kernel %d", kernel);
+    PIC_CHECK(valid_shape(kernel), PIC_ERR_ARG, "kernel %d", kernel);
     PIC_CHECK(mode >= 0 && mode <= 3, PIC_ERR_ARG, "pass mode %d", mode);
     Ctx *c = ctx_of(ctx);
     PIC_HIP(hipSetDevice(c->device));

@@ -104,7 +104,8 @@ __device__ __forceinline__ int fixed_exponent(float w, double qmax, int64_t coun
 // Calls add(u, v, q) with the FP32 charge q of stencil node (u, v) of a
 // particle at offset (fx, fy), in a fixed order.
 template <int K, class F>
-__device__ __forceinline__ void charge32(float fx, float fy, float w, F &&add) {
+__device__ __forceinline__ void charge32(float fx, float fy, int ci, int cj, float w, F &&add) {
+    constexpr int S = Shape<K>::S;
     if (K == PIC_CIC) {
         float sx = 1.0f - fx, sy = 1.0f - fy;
         add(0, 0, (sx * sy) * w);
@@ -113,14 +114,14 @@ __device__ __forceinline__ void charge32(float fx, float fy, float w, F &&add) {
         add(0, 1, (sx * fy) * w);
     } else {
         float wx[6], wy[6];
-        picd::m6_weights_f(fx, wx);
-        picd::m6_weights_f(fy, wy);
+        deposit_w<K>(fx, ci, wx);
+        deposit_w<K>(fy, cj, wy);
 #pragma unroll
-        for (int v = 0; v < 6; ++v) wy[v] = wy[v] * w;
+        for (int v = 0; v < S; ++v) wy[v] = wy[v] * w;
 #pragma unroll
-        for (int u = 0; u < 6; ++u)
+        for (int u = 0; u < S; ++u)
 #pragma unroll
-            for (int v = 0; v < 6; ++v) add(u, v, wx[u] * wy[v]);
+            for (int v = 0; v < S; ++v) add(u, v, wx[u] * wy[v]);
     }
 }
 
@@ -130,7 +131,9 @@ __device__ __forceinline__ void charge32(float fx, float fy, float w, F &&add) {
 // multiply-add onto 1.5 * 2^23 (the float grid there is the integers; the
 // products stay below 2^22 by the choice of u).
 template <int K, class F>
-__device__ __forceinline__ void charge32_fixed(float fx, float fy, float wu, F &&add) {
+__device__ __forceinline__ void charge32_fixed(float fx, float fy, int ci, int cj, float wu,
+                                               F &&add) {
+    constexpr int S = Shape<K>::S;
     constexpr float C = 12582912.0f;  // 1.5 * 2^23
     auto n = [&](float a, float b) {
         return __float_as_int(__builtin_fmaf(a, b, C)) - __float_as_int(C);
@@ -143,14 +146,14 @@ __device__ __forceinline__ void charge32_fixed(float fx, float fy, float wu, F &
         add(0, 1, n(sx, ty));
     } else {
         float wx[6], wy[6];
-        picd::m6_weights_f(fx, wx);
-        picd::m6_weights_f(fy, wy);
+        deposit_w<K>(fx, ci, wx);
+        deposit_w<K>(fy, cj, wy);
 #pragma unroll
-        for (int v = 0; v < 6; ++v) wy[v] = wy[v] * wu;
+        for (int v = 0; v < S; ++v) wy[v] = wy[v] * wu;
 #pragma unroll
-        for (int u = 0; u < 6; ++u)
+        for (int u = 0; u < S; ++u)
 #pragma unroll
-            for (int v = 0; v < 6; ++v) add(u, v, n(wx[u], wy[v]));
+            for (int v = 0; v < S; ++v) add(u, v, n(wx[u], wy[v]));
     }
 }
 
@@ -177,7 +180,7 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
     // ACC 0: the (hi, lo) split constant; ACC 2: the fixed-point quantum
     // 2^qe and w in units of it (exact: a power-of-two scaling)
     const float cq = ACC == 0 ? split_constant(a.w, p1 - p0) : 0.0f;
-    const int qe = ACC == 2 ? fixed_exponent(a.w, K == PIC_CIC ? 1.0 : 0.31, p1 - p0) : 0;
+    const int qe = ACC == 2 ? fixed_exponent(a.w, Shape<K>::QMAX, p1 - p0) : 0;
     const float wu = ACC == 2 ? ldexpf(a.w, -qe) : 0.0f;
     float2 *te = reinterpret_cast<float2 *>(smem_s32);
     float *tb = reinterpret_cast<float *>(smem_s32 + 8 * (size_t)cap);
@@ -221,7 +224,7 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
             if (!in) continue;
             float e1, e2, b3 = 0.0f;
             interp32<K, BZ>(
-                fx, fy, [&](int u, int v) { return te[(lx + u) + W.tw * (ly + v)]; },
+                fx, fy, ci, cj, [&](int u, int v) { return te[(lx + u) + W.tw * (ly + v)]; },
                 [&](int u, int v) { return tb[(lx + u) + W.tw * (ly + v)]; }, e1, e2, b3);
             kick32<BZ>(a.hdt, a.bz0, e1, e2, b3, ux, uy);
         }
@@ -242,13 +245,13 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
             defer_push(!in, (uint32_t)p | DEFER_DEPOSIT_ONLY, a.dlist, a.dcount);
             if (!in) continue;
             if (ACC == 2) {
-                charge32_fixed<K>(fx, fy, wu, [&](int u, int v, int qi) {
+                charge32_fixed<K>(fx, fy, ci, cj, wu, [&](int u, int v, int qi) {
                     __hip_atomic_fetch_add(&iacc[(lx + u) + W.tw * (ly + v)], qi,
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 });
                 continue;
             }
-            charge32<K>(fx, fy, a.w, [&](int u, int v, float q) {
+            charge32<K>(fx, fy, ci, cj, a.w, [&](int u, int v, float q) {
                 const int t = (lx + u) + W.tw * (ly + v);
                 if (ACC == 0) {
                     const float qh = (q + cq) - cq;
@@ -303,7 +306,7 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f32(SegF32 a) {
             if (GATHER) {
                 float e1, e2, b3 = 0.0f;
                 interp32<K, BZ>(
-                    fx, fy,
+                    fx, fy, ci, cj,
                     [&](int u, int v) {
                         return a.e32[(int64_t)((ci - HL + u) & mx) +
                                      (int64_t)nx * ((cj - HL + v) & my)];
@@ -328,7 +331,7 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f32(SegF32 a) {
             if (GATHER || DRIFT || PERM) a.ov[p] = make_float4(fx, fy, ux, uy);
         }
         if (DEPOSIT)
-            charge32<K>(fx, fy, a.w, [&](int u, int v, float q) {
+            charge32<K>(fx, fy, ci, cj, a.w, [&](int u, int v, float q) {
                 unsafeAtomicAdd(
                     &a.acc[(int64_t)((ci - HL + u) & mx) + (int64_t)nx * ((cj - HL + v) & my)],
                     (double)q);
@@ -374,8 +377,9 @@ __device__ __forceinline__ int ref_cell_s(double x, double dx, int n, bool trunc
 // Field at a particle from the nodes (u, v) of its stencil, FP64, with the
 // reference's operation order (pushers_cic.jl:31-44, pushers_m6.jl:63-83).
 template <int K, int BZ, class FE, class FB>
-__device__ __forceinline__ void interp64(double fx, double fy, FE node_e, FB node_b, double &e1,
-                                         double &e2, double &b3) {
+__device__ __forceinline__ void interp64(double fx, double fy, int ci, int cj, FE node_e,
+                                         FB node_b, double &e1, double &e2, double &b3) {
+    constexpr int S = Shape<K>::S;
     if (K == PIC_CIC) {
         double a1 = (1 - fx) * (1 - fy);
         double a2 = fx * (1 - fy);
@@ -390,16 +394,16 @@ __device__ __forceinline__ void interp64(double fx, double fy, FE node_e, FB nod
                  a4 * node_b(0, 1);
     } else {
         double wx[6], wy[6];
-        picd::m6_weights_d(fx, wx);
-        picd::m6_weights_d(fy, wy);
+        gather_w<K>(fx, ci, wx);
+        gather_w<K>(fy, cj, wy);
         e1 = 0.0;
         e2 = 0.0;
 #pragma unroll
-        for (int u = 0; u < 6; ++u) {
+        for (int u = 0; u < S; ++u) {
             double2 E0 = node_e(u, 0);
             double t1 = wy[0] * E0.x, t2 = wy[0] * E0.y;
 #pragma unroll
-            for (int v = 1; v < 6; ++v) {
+            for (int v = 1; v < S; ++v) {
                 double2 E = node_e(u, v);
                 t1 = __builtin_fma(wy[v], E.x, t1);
                 t2 = __builtin_fma(wy[v], E.y, t2);
@@ -409,10 +413,10 @@ __device__ __forceinline__ void interp64(double fx, double fy, FE node_e, FB nod
         }
         if (BZ == PIC_BZ_GRID) {
 #pragma unroll
-            for (int u = 0; u < 6; ++u) {
+            for (int u = 0; u < S; ++u) {
                 double t3 = wy[0] * node_b(u, 0);
 #pragma unroll
-                for (int v = 1; v < 6; ++v) t3 = __builtin_fma(wy[v], node_b(u, v), t3);
+                for (int v = 1; v < S; ++v) t3 = __builtin_fma(wy[v], node_b(u, v), t3);
                 b3 = __builtin_fma(wx[u], t3, b3);
             }
         }
@@ -461,7 +465,9 @@ __device__ __forceinline__ void drift64(const SegF64 &a, double pvx, double pvy,
 // Calls add(u, v, q) with the charge of stencil node (u, v)
 // (pushers_cic.jl:91-94, pushers_m6.jl:20-27).
 template <int K, class F>
-__device__ __forceinline__ void charge64(double fx, double fy, double wgt, F &&add) {
+__device__ __forceinline__ void charge64(double fx, double fy, int ci, int cj, double wgt,
+                                         F &&add) {
+    constexpr int S = Shape<K>::S;
     if (K == PIC_CIC) {
         double a1 = (1 - fx) * (1 - fy);
         double a2 = fx * (1 - fy);
@@ -473,12 +479,12 @@ __device__ __forceinline__ void charge64(double fx, double fy, double wgt, F &&a
         add(0, 1, a4 * wgt);
     } else {
         double wx[6], wy[6];
-        picd::m6_weights_d(fx, wx);
-        picd::m6_weights_d(fy, wy);
+        deposit_w<K>(fx, ci, wx);
+        deposit_w<K>(fy, cj, wy);
 #pragma unroll
-        for (int u = 0; u < 6; ++u)
+        for (int u = 0; u < S; ++u)
 #pragma unroll
-            for (int v = 0; v < 6; ++v) add(u, v, wx[u] * wy[v] * wgt);
+            for (int v = 0; v < S; ++v) add(u, v, wx[u] * wy[v] * wgt);
     }
 }
 
@@ -533,7 +539,7 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
             pvy = a.vy_in[src];
             double e1, e2, b3 = 0.0;
             interp64<K, BZ>(
-                fx, fy, [&](int u, int v) { return te[(lx + u) + W.tw * (ly + v)]; },
+                fx, fy, ci, cj, [&](int u, int v) { return te[(lx + u) + W.tw * (ly + v)]; },
                 [&](int u, int v) { return tb[(lx + u) + W.tw * (ly + v)]; }, e1, e2, b3);
             kick64<BZ>(a, e1, e2, b3, pvx, pvy);
         } else if (NDRIFT > 0 || PERM) {
@@ -558,7 +564,7 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
             const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
             defer_push(!in, (uint32_t)p | DEFER_DEPOSIT_ONLY, a.dlist, a.dcount);
             if (!in) continue;
-            charge64<K>(fx, fy, a.w ? a.w[src] : a.w0, [&](int u, int v, double q) {
+            charge64<K>(fx, fy, ci, cj, a.w ? a.w[src] : a.w0, [&](int u, int v, double q) {
                 __hip_atomic_fetch_add(&dacc[(lx + u) + W.tw * (ly + v)], q, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
             });
@@ -605,7 +611,7 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f64(SegF64 a) {
                 const int cj = ref_cell_s(py, a.dy, ny, false, fy);
                 double e1, e2, b3 = 0.0;
                 interp64<K, BZ>(
-                    fx, fy,
+                    fx, fy, ci, cj,
                     [&](int u, int v) {
                         const int64_t g = (int64_t)picd::wrap2(ci - HL + u, nx) +
                                           (int64_t)nx * picd::wrap2(cj - HL + v, ny);
@@ -633,7 +639,7 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f64(SegF64 a) {
             double fx, fy;
             const int ci = ref_cell_s(px, a.dx, nx, K == PIC_CIC, fx);
             const int cj = ref_cell_s(py, a.dy, ny, K == PIC_CIC, fy);
-            charge64<K>(fx, fy, wgt, [&](int u, int v, double q) {
+            charge64<K>(fx, fy, ci, cj, wgt, [&](int u, int v, double q) {
                 unsafeAtomicAdd(&a.acc[(int64_t)picd::wrap2(ci - HL + u, nx) +
                                        (int64_t)nx * picd::wrap2(cj - HL + v, ny)],
                                 q);
@@ -730,14 +736,9 @@ int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     a.dcount = c->defer_count;
     const size_t sm = (size_t)a.cap * (per_node ? per_node : 8);
     int rc;
-    if (kernel == PIC_CIC)
-        rc = acc == 0   ? seg32_perm<PIC_CIC, 0>(c, a, mode, c->bz_mode, sm)
-             : acc == 1 ? seg32_perm<PIC_CIC, 1>(c, a, mode, c->bz_mode, sm)
-                        : seg32_perm<PIC_CIC, 2>(c, a, mode, c->bz_mode, sm);
-    else
-        rc = acc == 0   ? seg32_perm<PIC_M6, 0>(c, a, mode, c->bz_mode, sm)
-             : acc == 1 ? seg32_perm<PIC_M6, 1>(c, a, mode, c->bz_mode, sm)
-                        : seg32_perm<PIC_M6, 2>(c, a, mode, c->bz_mode, sm);
+    PIC_SHAPE_SWITCH(kernel, rc = acc == 0   ? seg32_perm<KK, 0>(c, a, mode, c->bz_mode, sm)
+                                  : acc == 1 ? seg32_perm<KK, 1>(c, a, mode, c->bz_mode, sm)
+                                             : seg32_perm<KK, 2>(c, a, mode, c->bz_mode, sm));
     if (rc == PIC_OK && a.perm) {
         swap_alt_f32(c);
         c->pending_perm = nullptr;
@@ -813,12 +814,8 @@ int launch_seg64(Ctx *c, int kernel, int mode, double dt, double dtd) {
     a.dcount = c->defer_count;
     const size_t sm = (size_t)a.cap * (per_node ? per_node : 8);
     int rc;
-    if (kernel == PIC_CIC)
-        rc = a.perm ? seg64_mode<PIC_CIC, true>(c, a, mode, c->bz_mode, sm)
-                    : seg64_mode<PIC_CIC, false>(c, a, mode, c->bz_mode, sm);
-    else
-        rc = a.perm ? seg64_mode<PIC_M6, true>(c, a, mode, c->bz_mode, sm)
-                    : seg64_mode<PIC_M6, false>(c, a, mode, c->bz_mode, sm);
+    PIC_SHAPE_SWITCH(kernel, rc = a.perm ? seg64_mode<KK, true>(c, a, mode, c->bz_mode, sm)
+                                         : seg64_mode<KK, false>(c, a, mode, c->bz_mode, sm));
     if (rc == PIC_OK && a.perm) {
         swap_alt_f64(c);
         c->pending_perm = nullptr;

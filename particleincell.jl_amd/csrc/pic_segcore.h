@@ -15,16 +15,9 @@ namespace pic {
 
 constexpr int SEG_T = 512;  // threads per block
 
+// Stencil of shape K (S nodes per direction, HL left of the cell).
 template <int K>
-struct SegStencil;
-template <>
-struct SegStencil<PIC_CIC> {
-    static constexpr int S = 2, HL = 0;
-};
-template <>
-struct SegStencil<PIC_M6> {
-    static constexpr int S = 6, HL = 2;
-};
+using SegStencil = Shape<K>;
 
 struct Win {
     int x0, y0, tw, th;  // global index of window node (0,0) (may be negative), size

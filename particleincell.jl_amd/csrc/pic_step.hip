@@ -72,15 +72,11 @@ struct ArgsF32 {
     unsigned long long *slow;
 };
 
+// Stencil of shape K: S nodes per direction, HL left and HR right of the
+// particle's cell.
 template <int K>
-struct Stencil;
-template <>
-struct Stencil<PIC_CIC> {
-    static constexpr int S = 2, HL = 0, HR = 1;
-};
-template <>
-struct Stencil<PIC_M6> {
-    static constexpr int S = 6, HL = 2, HR = 3;
+struct Stencil {
+    static constexpr int S = Shape<K>::S, HL = Shape<K>::HL, HR = S - 1 - HL;
 };
 
 // Wrapped difference of cell indices i, iref in [0, n) into [-n/2, n/2).
@@ -270,16 +266,16 @@ __global__ __launch_bounds__(ST) void k_step_f64(ArgsF64 a) {
                          a4 * node_b(0, 1);
             } else {
                 double wx[6], wy[6];
-                picd::m6_weights_d(fx[k], wx);
-                picd::m6_weights_d(fy[k], wy);
+                gather_w<K>(fx[k], ci[k], wx);
+                gather_w<K>(fy[k], cj[k], wy);
                 e1 = 0.0;
                 e2 = 0.0;
 #pragma unroll
-                for (int u = 0; u < 6; ++u) {
+                for (int u = 0; u < St::S; ++u) {
                     double2 E0 = node_e(u, 0);
                     double t1 = wy[0] * E0.x, t2 = wy[0] * E0.y;
 #pragma unroll
-                    for (int v = 1; v < 6; ++v) {
+                    for (int v = 1; v < St::S; ++v) {
                         double2 E = node_e(u, v);
                         t1 = __builtin_fma(wy[v], E.x, t1);
                         t2 = __builtin_fma(wy[v], E.y, t2);
@@ -289,10 +285,10 @@ __global__ __launch_bounds__(ST) void k_step_f64(ArgsF64 a) {
                 }
                 if (BZ == PIC_BZ_GRID) {
 #pragma unroll
-                    for (int u = 0; u < 6; ++u) {
+                    for (int u = 0; u < St::S; ++u) {
                         double t3 = wy[0] * node_b(u, 0);
 #pragma unroll
-                        for (int v = 1; v < 6; ++v) t3 = __builtin_fma(wy[v], node_b(u, v), t3);
+                        for (int v = 1; v < St::S; ++v) t3 = __builtin_fma(wy[v], node_b(u, v), t3);
                         b3 = __builtin_fma(wx[u], t3, b3);
                     }
                 }
@@ -403,12 +399,12 @@ __global__ __launch_bounds__(ST) void k_step_f64(ArgsF64 a) {
             add(0, 1, a4 * wgt);
         } else {
             double wx[6], wy[6];
-            picd::m6_weights_d(fx[k], wx);
-            picd::m6_weights_d(fy[k], wy);
+            deposit_w<K>(fx[k], ci[k], wx);
+            deposit_w<K>(fy[k], cj[k], wy);
 #pragma unroll
-            for (int u = 0; u < 6; ++u)
+            for (int u = 0; u < St::S; ++u)
 #pragma unroll
-                for (int v = 0; v < 6; ++v) add(u, v, wx[u] * wy[v] * wgt);
+                for (int v = 0; v < St::S; ++v) add(u, v, wx[u] * wy[v] * wgt);
         }
     }
     if (!b.fits) return;
@@ -541,16 +537,16 @@ __global__ __launch_bounds__(ST) void k_step_f32(ArgsF32 a) {
                 }
             } else {
                 float wx[6], wy[6];
-                picd::m6_weights_f(fx[k], wx);
-                picd::m6_weights_f(fy[k], wy);
+                gather_w<K>(fx[k], ci[k], wx);
+                gather_w<K>(fy[k], cj[k], wy);
                 e1 = 0.0f;
                 e2 = 0.0f;
 #pragma unroll
-                for (int u = 0; u < 6; ++u) {
+                for (int u = 0; u < St::S; ++u) {
                     float2 E0 = node_e(u, 0);
                     float t1 = wy[0] * E0.x, t2 = wy[0] * E0.y;
 #pragma unroll
-                    for (int v = 1; v < 6; ++v) {
+                    for (int v = 1; v < St::S; ++v) {
                         float2 E = node_e(u, v);
                         t1 = __builtin_fmaf(wy[v], E.x, t1);
                         t2 = __builtin_fmaf(wy[v], E.y, t2);
@@ -565,10 +561,10 @@ __global__ __launch_bounds__(ST) void k_step_f32(ArgsF32 a) {
                 }
                 if (BZ == PIC_BZ_GRID) {
 #pragma unroll
-                    for (int u = 0; u < 6; ++u) {
+                    for (int u = 0; u < St::S; ++u) {
                         float t3 = wy[0] * node_b(u, 0);
 #pragma unroll
-                        for (int v = 1; v < 6; ++v) t3 = __builtin_fmaf(wy[v], node_b(u, v), t3);
+                        for (int v = 1; v < St::S; ++v) t3 = __builtin_fmaf(wy[v], node_b(u, v), t3);
                         b3 = u == 0 ? wx[0] * t3 : __builtin_fmaf(wx[u], t3, b3);
                     }
                 }
@@ -651,14 +647,14 @@ __global__ __launch_bounds__(ST) void k_step_f32(ArgsF32 a) {
             add(0, 1, (sx * fy[k]) * a.w);
         } else {
             float wx[6], wy[6];
-            picd::m6_weights_f(fx[k], wx);
-            picd::m6_weights_f(fy[k], wy);
+            deposit_w<K>(fx[k], ci[k], wx);
+            deposit_w<K>(fy[k], cj[k], wy);
 #pragma unroll
-            for (int v = 0; v < 6; ++v) wy[v] = wy[v] * a.w;
+            for (int v = 0; v < St::S; ++v) wy[v] = wy[v] * a.w;
 #pragma unroll
-            for (int u = 0; u < 6; ++u)
+            for (int u = 0; u < St::S; ++u)
 #pragma unroll
-                for (int v = 0; v < 6; ++v) add(u, v, wx[u] * wy[v]);
+                for (int v = 0; v < St::S; ++v) add(u, v, wx[u] * wy[v]);
         }
     }
     if (!b.fits) return;
@@ -726,8 +722,9 @@ int launch_step64(Ctx *c, int kernel, int mode, double dt, double dtd) {
     a.dt = dt; a.dtd = dtd;
     a.acc = c->rho_acc;
     a.slow = c->slow_chunks;
-    return kernel == PIC_CIC ? launch64_k<PIC_CIC>(c, a, mode, c->bz_mode)
-                             : launch64_k<PIC_M6>(c, a, mode, c->bz_mode);
+    int rc = PIC_OK;
+    PIC_SHAPE_SWITCH(kernel, rc = launch64_k<KK>(c, a, mode, c->bz_mode));
+    return rc;
 }
 
 template <int K, bool G, bool DR, bool D>
@@ -781,8 +778,9 @@ int launch_step32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     a.dtdy = (float)(dtd / c->dy);
     a.acc = c->rho_acc;
     a.slow = c->slow_chunks;
-    return kernel == PIC_CIC ? launch32_k<PIC_CIC>(c, a, mode, c->bz_mode)
-                             : launch32_k<PIC_M6>(c, a, mode, c->bz_mode);
+    int rc = PIC_OK;
+    PIC_SHAPE_SWITCH(kernel, rc = launch32_k<KK>(c, a, mode, c->bz_mode));
+    return rc;
 }
 
 }  // namespace pic
