@@ -152,6 +152,17 @@ WORKLOADS = {
     "large_m6": dict(name="landau2d2v-m6-512x512-5e8-dt0.05-200steps", grid=512,
                      n=500_000_000, kernel="m6", alpha=0.05, k=0.5, dt=0.05, steps=200,
                      warmup=5, sampler="landau_device", sort=(2, 1)),
+    # the other shapes of the reference's Landau example
+    # (examples/landau_damping_poisson_cic_m6.jl:88-103) on the M6 workload
+    "landau_ngp": dict(name="landau2d2v-ngp-128x128-1e7-dt0.05-400steps", grid=128,
+                       n=10_000_000, kernel="ngp", alpha=0.01, k=0.5, dt=0.05, steps=400,
+                       warmup=20, sampler="quiet", sort=(8, 4)),
+    "landau_tsc": dict(name="landau2d2v-tsc-128x128-1e7-dt0.05-400steps", grid=128,
+                       n=10_000_000, kernel="tsc", alpha=0.01, k=0.5, dt=0.05, steps=400,
+                       warmup=20, sampler="quiet", sort=(8, 4)),
+    "landau_m4": dict(name="landau2d2v-m4-128x128-1e7-dt0.05-400steps", grid=128,
+                      n=10_000_000, kernel="m4", alpha=0.01, k=0.5, dt=0.05, steps=400,
+                      warmup=20, sampler="quiet", sort=(8, 4)),
     # adjoint sweeps: forward loop with checkpoints + reverse sweep, timed as
     # one call; value = particle-steps/s of the whole sweep
     "adjoint_cic": dict(name="adjoint-landau2d2v-cic-128x128-2e6-dt0.1-200steps-ckpt20",
@@ -163,6 +174,11 @@ WORKLOADS = {
                              steps=200, warmup=0, sampler="landau_device", sort=(2, 0),
                              adjoint=dict(ckpt=25, fd=False, f64=False)),
 }
+
+SHAPES = {"cic": lambda pic: pic.CloudInCell(), "m6": lambda pic: pic.M6(),
+          "ngp": lambda pic: pic.NearestGridPoint(),
+          "tsc": lambda pic: pic.TriangularShapeCloud(), "m4": lambda pic: pic.M4()}
+SHAPE_NAMES = {"cic": "CIC", "m6": "M6", "ngp": "NGP", "tsc": "TSC", "m4": "M4"}
 
 # Algorithmic bytes per particle of the adjoint sweep's passes (FP32 state):
 # forward pass and recompute pass read and write cell + 4 floats; the
@@ -368,7 +384,7 @@ def main():
     nx = ny = W["grid"]
     alpha, kx, dt = W["alpha"], W["k"], W["dt"]
     bz = W.get("bz")
-    kernel = pic.M6() if W["kernel"] == "m6" else pic.CloudInCell()
+    kernel = SHAPES[W["kernel"]](pic)
     mesh = pic.TwoDGrid(0.0, 2 * math.pi / kx, nx, 0.0, 2 * math.pi / kx, ny)
     log(f"bench: {args.workload}: sampling {n} particles ({W['sampler']})")
     if W["sampler"] == "quiet":
@@ -414,7 +430,7 @@ def main():
     value = n * steps / r["seconds"]
     bytes_pp = BYTES_F32 if head == "f32" else BYTES_F64
     achieved = bytes_pp * n / (r["pass_ms_avg"] * 1e-3) / 1e9
-    kname = "M6" if W["kernel"] == "m6" else "CIC"
+    kname = SHAPE_NAMES[W["kernel"]]
     tiles = ({0: " (FP32 pair tiles)", 1: " (FP64 tiles)", 2: " (int32 fixed-point tiles)"}
              [args.f32_acc] if head == "f32" else " (FP64 tiles)")
     out = {

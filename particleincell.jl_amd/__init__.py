@@ -1,9 +1,9 @@
 """MI355X-native particle-mesh step of ParticleInCell.jl.
 
 The hot path (charge deposition, field gather, the pushers of the CIC, M6,
-NGP, TSC and M4 shapes, and the
-periodic FFT Poisson solve) runs in libpicmi355x.so: hand-written HIP
-kernels for gfx950 plus rocFFT behind the C ABI of include/pic_mi355x.h.
+NGP, TSC and M4 shapes, and the periodic FFT Poisson solve) runs in
+libpicmi355x.so: hand-written HIP kernels for gfx950 plus rocFFT behind the
+C ABI of include/pic_mi355x.h.
 This package is the host side: ctypes bindings (_lib), the reference's API
 (api), the Landau quiet start (sampling) and the damping-rate fit
 (diagnostics).
