@@ -309,6 +309,57 @@ int pic_get_adjoint_timing(pic_ctx *ctx, double *ms, int64_t *passes);
  * fused kernels since the context was created (diagnostic). */
 int pic_slow_chunks(pic_ctx *ctx, uint64_t *count);
 
+/* ------------------------------------------ Vlasov-Maxwell (FDTD) */
+
+/* The FDTD solver of src/fdtd.jl on the Yee grid and the CIC current
+ * deposit of src/cloud_in_cell.jl, for the electromagnetic loop of
+ * examples/landau_damping_maxwell_fdtd.jl:58-83.  The context holds the
+ * FDTD fields (fdtd.jl:3-20: ex nx x (ny+1), ey (nx+1) x ny, bz nx x ny,
+ * column-major, zero until set) and a periodic nx x ny current.  The node
+ * fields that update_fields! produces live in the context's ex, ey, bz
+ * grids (nx x ny; the reference's ghost row and column are their periodic
+ * images, which they equal whenever fdtd.ex's row ny+1 equals its row 1 and
+ * fdtd.ey's column nx+1 its column 1, as ampere_maxwell! keeps them).
+ * Particle work needs the FP64 state (pic_set_state). */
+
+/* Copy the FDTD fields from / to the host (fdtd.ex, fdtd.ey, fdtd.bz; any
+ * pointer may be NULL to skip that field). */
+int pic_fdtd_set(pic_ctx *ctx, const double *ex, const double *ey, const double *bz);
+int pic_fdtd_get(pic_ctx *ctx, double *ex, double *ey, double *bz);
+
+/* faraday!(fdtd, m, dt): fdtd.jl:24-36. */
+int pic_faraday(pic_ctx *ctx, double dt);
+
+/* ampere_maxwell!(fdtd, m, jx, jy, dt): fdtd.jl:39-54, with the context's
+ * current (pic_compute_current or pic_set_current). */
+int pic_ampere_maxwell(pic_ctx *ctx, double dt);
+
+/* update_fields!(ex, ey, bz, m, fdtd): fdtd.jl:58-77, into the context's
+ * node grids (read them with pic_get_grid; bz mode becomes PIC_BZ_GRID). */
+int pic_update_fields(pic_ctx *ctx);
+
+/* compute_current!(jx, jy, m, ::CloudInCell, p): cloud_in_cell.jl:10-81,
+ * into the context's current; when jx, jy are not NULL also out to them as
+ * the reference's (nx+1) x (ny+1) arrays with folded ghosts.  kernel must be
+ * PIC_CIC. */
+int pic_compute_current(pic_ctx *ctx, int kernel, double *jx, double *jy);
+
+/* Load the context's current from (nx+1) x (ny+1) host arrays (their
+ * first nx x ny nodes; the rest is the periodic image). */
+int pic_set_current(pic_ctx *ctx, const double *jx, const double *jy);
+
+/* compute_energy(fdtd, m) = 0.5 log(sum(fdtd.ex .^ 2) dx dy): fdtd.jl:79-80. */
+int pic_fdtd_energy(pic_ctx *ctx, double *energy);
+
+/* nsteps iterations of the loop of examples/landau_damping_maxwell_fdtd.jl
+ * :58-83 on the resident particles and FDTD fields: faraday!(dt/2) from the
+ * second iteration, update_fields!, push_v!(dt), push_x!(dt/2),
+ * compute_current!, push_x!(dt/2), faraday!(dt/2), ampere_maxwell!(dt);
+ * energy[s] = compute_energy after iteration s+1 (examples/vm2d.jl:55; may
+ * be NULL).  The particle part is one fused pass per iteration.  kernel must
+ * be PIC_CIC; FP64 state. */
+int pic_run_maxwell(pic_ctx *ctx, int kernel, double dt, int nsteps, double *energy);
+
 /* ------------------------------------------ Fortran-twin drop-ins */
 
 /* The symbols of src/pic.f90, same arguments (scalars by reference, arrays

@@ -49,6 +49,13 @@ def lib():
             "or_interpolate_cic": (None, [_i64, _dp, _i32, _i32, _d, _d, _dp, _dp, _dpc, _dpc]),
             "or_push_v_m6": (None, [_i64, _dp, _d, _i32, _i32, _d, _d, _dp, _dp, C.c_void_p, _i32]),
             "or_compute_rho_m6": (None, [_i64, _dp, _i32, _i32, _d, _d, _d, _d, _dp]),
+            "or_faraday": (None, [_i32, _i32, _d, _d, _dp, _dp, _dp, _d]),
+            "or_ampere_maxwell": (None, [_i32, _i32, _d, _d, _dp, _dp, _dp, _dp, _dp, _d]),
+            "or_update_fields": (None, [_i32, _i32, _dp, _dp, _dp, _dp, _dp, _dp]),
+            "or_compute_current": (None, [_i64, _dp, _i32, _i32, _d, _d, _dp, _dp]),
+            "or_fdtd_energy": (_d, [_i32, _i32, _d, _d, _dp]),
+            "or_maxwell_loop": (_i32, [_i64, _dp, _i32, _i32, _d, _d, _d, _i32, _dp, _dp, _dp,
+                                       _dpc]),
             "or_f_tsc": (_d, [_d]),
             "or_f_m4": (_d, [_d]),
             "or_push_v_ngp": (None, [_i64, _dp, _d, _i32, _i32, _d, _d, _dp, _dp, C.c_void_p, _i32]),
@@ -379,3 +386,48 @@ def ref_deposition(p, nx, ny, dx, dy):
     jy = np.zeros((nx + 1, ny + 1), order="F")
     ref().deposition(_i(p.shape[1]), _i(nx), _i(ny), _r(dx), _r(dy), p, jx, jy)
     return jx, jy
+
+
+# ------------------------------------------------------- Vlasov-Maxwell
+
+class FDTD:
+    """fdtd.jl's FDTD struct: ex nx x (ny+1), ey (nx+1) x ny, bz nx x ny."""
+
+    def __init__(self, nx, ny):
+        self.ex = np.zeros((nx, ny + 1), order="F")
+        self.ey = np.zeros((nx + 1, ny), order="F")
+        self.bz = np.zeros((nx, ny), order="F")
+
+
+def faraday(f, nx, ny, dx, dy, dt):
+    lib().or_faraday(nx, ny, dx, dy, f.ex, f.ey, f.bz, dt)
+
+
+def ampere_maxwell(f, nx, ny, dx, dy, jx, jy, dt):
+    lib().or_ampere_maxwell(nx, ny, dx, dy, f.ex, f.ey, f.bz, _F(jx), _F(jy), dt)
+
+
+def update_fields(f, nx, ny):
+    ex, ey, bz = (np.zeros((nx + 1, ny + 1), order="F") for _ in range(3))
+    lib().or_update_fields(nx, ny, f.ex, f.ey, f.bz, ex, ey, bz)
+    return ex, ey, bz
+
+
+def compute_current(p, nx, ny, dx, dy):
+    jx, jy = (np.zeros((nx + 1, ny + 1), order="F") for _ in range(2))
+    lib().or_compute_current(p.shape[1], _F(p), nx, ny, dx, dy, jx, jy)
+    return jx, jy
+
+
+def fdtd_energy(f, nx, ny, dx, dy):
+    return lib().or_fdtd_energy(nx, ny, dx, dy, f.ex)
+
+
+def maxwell_loop(p, f, nx, ny, dimx, dimy, dt, nsteps):
+    """The loop of examples/landau_damping_maxwell_fdtd.jl; p and f are
+    updated in place; returns compute_energy after every step."""
+    e = np.zeros(nsteps)
+    rc = lib().or_maxwell_loop(p.shape[1], p, nx, ny, dimx, dimy, dt, nsteps, f.ex, f.ey, f.bz,
+                               e)
+    assert rc == 0
+    return e

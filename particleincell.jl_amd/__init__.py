@@ -1,7 +1,8 @@
 """MI355X-native particle-mesh step of ParticleInCell.jl.
 
 The hot path (charge deposition, field gather, the pushers of the CIC, M6,
-NGP, TSC and M4 shapes, and the periodic FFT Poisson solve) runs in
+NGP, TSC and M4 shapes, the periodic FFT Poisson solve, and the FDTD
+Vlasov-Maxwell loop with its CIC current deposit) runs in
 libpicmi355x.so: hand-written HIP kernels for gfx950 plus rocFFT behind the
 C ABI of include/pic_mi355x.h.
 This package is the host side: ctypes bindings (_lib), the reference's API
@@ -9,14 +10,17 @@ This package is the host side: ctypes bindings (_lib), the reference's API
 (diagnostics).
 """
 from . import _lib
-from .api import (CloudInCell, Context, F90, M4, M6, MeshFields2D, NearestGridPoint,
+from .api import (FDTD, CloudInCell, Context, F90, M4, M6, MeshFields2D, NearestGridPoint,
                   ParticleGroup, Particles, TriangularShapeCloud,
-                  Poisson2D, Simulation, TwoDGrid, TwoDPoissonPeriodic, compute_rho,
-                  compute_rho_m6_, energy, interpol_eb_m6_, interpolate_cic_, push_v_, push_x_,
-                  solve_)
+                  Poisson2D, Simulation, TwoDGrid, TwoDPoissonPeriodic, ampere_maxwell_,
+                  compute_current_, compute_energy, compute_rho, compute_rho_m6_, energy,
+                  faraday_, interpol_eb_m6_, interpolate_cic_, push_v_, push_x_, solve_,
+                  update_fields_)
 from . import sampling, diagnostics
 
 __all__ = [
+    "FDTD", "ampere_maxwell_", "compute_current_", "compute_energy", "faraday_",
+    "update_fields_",
     "CloudInCell", "Context", "F90", "M4", "M6", "MeshFields2D", "NearestGridPoint",
     "ParticleGroup", "Particles", "TriangularShapeCloud",
     "Poisson2D", "Simulation", "TwoDGrid", "TwoDPoissonPeriodic", "compute_rho",

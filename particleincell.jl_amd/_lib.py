@@ -68,6 +68,15 @@ SIGNATURES = {
     "pic_set_fused_solve": [_vp, _i],
     "pic_get_timing": [_vp, _pd, C.POINTER(C.c_int64), _pd],
     "pic_slow_chunks": [_vp, C.POINTER(C.c_uint64)],
+    "pic_fdtd_set": [_vp, _vp, _vp, _vp],
+    "pic_fdtd_get": [_vp, _vp, _vp, _vp],
+    "pic_faraday": [_vp, _d],
+    "pic_ampere_maxwell": [_vp, _d],
+    "pic_update_fields": [_vp],
+    "pic_compute_current": [_vp, _i, _vp, _vp],
+    "pic_set_current": [_vp, _vp, _vp],
+    "pic_fdtd_energy": [_vp, _pd],
+    "pic_run_maxwell": [_vp, _i, _d, _i, _vp],
     "pic_f90_push_v": [_pi32, _vp, _pd, _pi32, _pi32, _pd, _pd, _vp, _vp, _vp],
     "pic_f90_push_x": [_pi32, _pd, _pd, _vp, _pd],
     "pic_f90_deposition": [_pi32, _pi32, _pi32, _pd, _pd, _vp, _vp, _vp],

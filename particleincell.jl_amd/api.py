@@ -18,6 +18,9 @@ argument order and array layouts (a Julia `f!` is written `f_` here):
     compute_rho_m6_(fields, particles)       compute_rho_m6.jl:181
     interpol_eb_m6_(particles, fields)       interpolation_m6.jl:125
     F90.push_v_/push_x_/compute_current_     pic.jl:18-114 (Fortran-twin ABI)
+    FDTD(mesh), faraday_, ampere_maxwell_, update_fields_, compute_energy
+                                             fdtd.jl:3-80
+    compute_current_(jx, jy, mesh, CloudInCell(), p)   cloud_in_cell.jl:10
 
 Arrays are numpy float64 in Fortran order, so the memory layout is the Julia
 one.  Each call copies its inputs to the GPU and its outputs back (the
@@ -430,6 +433,94 @@ class F90:
             jx[nx, j] = jx[0, j]
 
 
+# ------------------------------------------------------- Vlasov-Maxwell
+
+class FDTD:
+    """FDTD(mesh) (fdtd.jl:3-20): the Yee-grid fields ex nx x (ny+1),
+    ey (nx+1) x ny, bz nx x ny, float64 in Fortran order, zero."""
+
+    def __init__(self, mesh):
+        nx, ny = mesh.nx, mesh.ny
+        self.ex = np.zeros((nx, ny + 1), order="F")
+        self.ey = np.zeros((nx + 1, ny), order="F")
+        self.bz = np.zeros((nx, ny), order="F")
+
+    def _check(self, mesh):
+        nx, ny = mesh.nx, mesh.ny
+        for a, shp, nm in ((self.ex, (nx, ny + 1), "ex"), (self.ey, (nx + 1, ny), "ey"),
+                           (self.bz, (nx, ny), "bz")):
+            if a.shape != shp or a.dtype != np.float64 or not a.flags.f_contiguous:
+                raise ValueError(f"fdtd.{nm} must be a float64 Fortran array of shape {shp}")
+
+
+def _fdtd_in(ctx, fdtd, mesh):
+    fdtd._check(mesh)
+    L.call("pic_fdtd_set", ctx.h, fdtd.ex.ctypes.data, fdtd.ey.ctypes.data, fdtd.bz.ctypes.data)
+
+
+def _fdtd_out(ctx, fdtd):
+    L.call("pic_fdtd_get", ctx.h, fdtd.ex.ctypes.data, fdtd.ey.ctypes.data, fdtd.bz.ctypes.data)
+
+
+def _node_array(a, mesh, name):
+    if (a.shape != (mesh.nx + 1, mesh.ny + 1) or a.dtype != np.float64
+            or not a.flags.f_contiguous):
+        raise ValueError(f"{name} must be a float64 Fortran array of shape "
+                         f"({mesh.nx + 1}, {mesh.ny + 1})")
+
+
+def faraday_(fdtd, mesh, dt):
+    """faraday!(fdtd, m, dt) (fdtd.jl:24-36), in place on fdtd.bz."""
+    ctx = _context(mesh)
+    _fdtd_in(ctx, fdtd, mesh)
+    L.call("pic_faraday", ctx.h, float(dt))
+    _fdtd_out(ctx, fdtd)
+
+
+def ampere_maxwell_(fdtd, mesh, jx, jy, dt):
+    """ampere_maxwell!(fdtd, m, jx, jy, dt) (fdtd.jl:39-54), in place on
+    fdtd.ex, fdtd.ey; jx, jy are (nx+1) x (ny+1) node currents."""
+    _node_array(jx, mesh, "jx")
+    _node_array(jy, mesh, "jy")
+    ctx = _context(mesh)
+    _fdtd_in(ctx, fdtd, mesh)
+    L.call("pic_set_current", ctx.h, jx.ctypes.data, jy.ctypes.data)
+    L.call("pic_ampere_maxwell", ctx.h, float(dt))
+    _fdtd_out(ctx, fdtd)
+
+
+def update_fields_(ex, ey, bz, mesh, fdtd):
+    """update_fields!(ex, ey, bz, m, fdtd) (fdtd.jl:58-77): fills the
+    (nx+1) x (ny+1) node arrays; the ghost row and column are the periodic
+    images of the first."""
+    for a, nm in ((ex, "ex"), (ey, "ey"), (bz, "bz")):
+        _node_array(a, mesh, nm)
+    ctx = _context(mesh)
+    _fdtd_in(ctx, fdtd, mesh)
+    L.call("pic_update_fields", ctx.h)
+    for a, which in ((ex, L.PIC_GRID_EX), (ey, L.PIC_GRID_EY), (bz, L.PIC_GRID_BZ)):
+        a[...] = ctx.get_grid(which, ghost=True)
+
+
+def compute_current_(jx, jy, mesh, kernel, p):
+    """compute_current!(jx, jy, m, ::CloudInCell, p) (cloud_in_cell.jl:10-81):
+    fills the (nx+1) x (ny+1) arrays, ghosts folded."""
+    _node_array(jx, mesh, "jx")
+    _node_array(jy, mesh, "jy")
+    ctx = _context(mesh)
+    ctx.set_particles(p.array if isinstance(p, ParticleGroup) else p)
+    L.call("pic_compute_current", ctx.h, _kcode(kernel), jx.ctypes.data, jy.ctypes.data)
+
+
+def compute_energy(fdtd, mesh):
+    """compute_energy(fdtd, m) = 0.5 log(sum(fdtd.ex .^ 2) dx dy) (fdtd.jl:79-80)."""
+    ctx = _context(mesh)
+    _fdtd_in(ctx, fdtd, mesh)
+    e = C.c_double()
+    L.call("pic_fdtd_energy", ctx.h, C.byref(e))
+    return e.value
+
+
 # ------------------------------------------------- device-resident run
 
 class Simulation:
@@ -521,6 +612,24 @@ class Simulation:
         L.call("pic_get_adjoint_timing", self.ctx.h, ms, cnt)
         names = ("forward", "gather_adjoint", "deposit_adjoint", "recompute")
         return {k: (ms[i], cnt[i]) for i, k in enumerate(names)}
+
+    def set_fdtd(self, fdtd):
+        """Loads the FDTD fields (fdtd.jl:3-20) of the Vlasov-Maxwell loop."""
+        _fdtd_in(self.ctx, fdtd, self.mesh)
+
+    def fdtd(self):
+        """The resident FDTD fields, as an FDTD."""
+        f = FDTD(self.mesh)
+        _fdtd_out(self.ctx, f)
+        return f
+
+    def run_maxwell(self, dt, nsteps):
+        """nsteps iterations of the Vlasov-Maxwell loop
+        (examples/landau_damping_maxwell_fdtd.jl:58-83, pic_run_maxwell);
+        returns compute_energy after each iteration."""
+        e = np.zeros(nsteps)
+        L.call("pic_run_maxwell", self.ctx.h, self.kernel, float(dt), int(nsteps), e.ctypes.data)
+        return e
 
     def set_timing(self, on=True):
         L.call("pic_set_timing", self.ctx.h, 1 if on else 0)

@@ -408,6 +408,7 @@ int pic_ctx_destroy(pic_ctx *ctx) {
     dfree(c->sort_hist); dfree(c->cell_off);
     dfree(c->slow_chunks); dfree(c->defer_count);
     dfree(c->seg_k); dfree(c->seg_tbl); dfree(c->seg_A); dfree(c->seg_M);
+    dfree(c->fd_ex); dfree(c->fd_ey); dfree(c->fd_bz); dfree(c->jx); dfree(c->jy);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PIC_OK;

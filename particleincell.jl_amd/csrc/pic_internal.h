@@ -135,6 +135,13 @@ struct Ctx {
     // kernels (true) or the rocFFT sequence (false)
     bool fused_solve = true;
 
+    // Vlasov-Maxwell: FDTD fields on the Yee grid (fdtd.jl:3-20; ex nx x
+    // (ny+1), ey (nx+1) x ny, bz nx x ny, column-major) and the periodic CIC
+    // current jx, jy (nx x ny: the ghost row and column of compute_current!
+    // folded in)
+    double *fd_ex = nullptr, *fd_ey = nullptr, *fd_bz = nullptr;
+    double *jx = nullptr, *jy = nullptr;
+
     // adjoint reverse sweep: steps whose positions are kept at once (0 =
     // the most the free device memory allows), and the last sweep's plan
     int adj_block = 0;
