@@ -8,6 +8,10 @@ quiet start.  A step is one iteration of the reference's time loop
 (examples/landau_damping_poisson_cic_m6.jl:70-83): Poisson solve, kick,
 drift, deposit, on device-resident particles.
 
+The Vlasov-Maxwell workloads vm2d and vm2d_npm500 time the FDTD + CIC
+current loop of examples/vm2d.jl (FP64, 128 x 16 cells, 204,800 or
+1,024,000 particles, dt = 0.01, 1000 steps) the same way.
+
 Other workloads, measured the same way: landau_cic (CIC, 64 x 64, 204,800
 particles, alpha 0.05, dt 0.1, 100 steps), strong_b (CIC, 256 x 256, 1e8
 particles, density 1 + 0.05 cos(x/2) cos(y/2), Maxwellian v, uniform bz =
@@ -56,11 +60,11 @@ def cpu_info():
     return os.cpu_count(), model or platform.processor()
 
 
-def pmc_traffic(kernel):
+def pmc_traffic(kernel, profile="pmc_traffic_landau_m6_128x128_1e7.txt"):
     """HBM bytes per launch of `kernel` from the committed byte-counter profile
     of this workload (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, each scaled
     by the same counter on a plain copy of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_landau_m6_128x128_1e7.txt")
+    path = os.path.join(ROOT, "profiles", profile)
     try:
         for line in open(path):
             f = line.split()
@@ -163,6 +167,15 @@ WORKLOADS = {
     "landau_m4": dict(name="landau2d2v-m4-128x128-1e7-dt0.05-400steps", grid=128,
                       n=10_000_000, kernel="m4", alpha=0.01, k=0.5, dt=0.05, steps=400,
                       warmup=20, sampler="quiet", sort=(8, 4)),
+    # the Vlasov-Maxwell loop of examples/vm2d.jl (FDTD fields + CIC current
+    # deposit, FP64): 100 particles per cell (the README's 204,800
+    # particles) and 500 per cell (vm2d.jl's timed run), 1000 steps
+    "vm2d": dict(name="vm2d-fdtd-cic-128x16-204800-dt0.01-1000steps", grid=(128, 16),
+                 dimy=1.0, n=204_800, kernel="cic", alpha=0.1, k=0.5, dt=0.01, steps=1000,
+                 warmup=20, sampler="quiet", sort=(0, 0), maxwell=True),
+    "vm2d_npm500": dict(name="vm2d-fdtd-cic-128x16-1024000-dt0.01-1000steps", grid=(128, 16),
+                        dimy=1.0, n=1_024_000, kernel="cic", alpha=0.1, k=0.5, dt=0.01,
+                        steps=1000, warmup=20, sampler="quiet", sort=(0, 0), maxwell=True),
     # adjoint sweeps: forward loop with checkpoints + reverse sweep, timed as
     # one call; value = particle-steps/s of the whole sweep
     "adjoint_cic": dict(name="adjoint-landau2d2v-cic-128x128-2e6-dt0.1-200steps-ckpt20",
@@ -336,6 +349,95 @@ def bench_adjoint(args, W, pic, entry, mesh, held, n, steps, warmup):
     print(json.dumps(out), flush=True)
 
 
+VM_BYTES = 64.0  # per particle per pass: x, y, vx, vy read and written (uniform weights)
+
+
+def bench_maxwell(args, W, pic, entry, steps, warmup):
+    """The Vlasov-Maxwell workloads: pic_run_maxwell over `steps` iterations
+    from the initial state of examples/vm2d.jl (Landau alpha 0.1, kx 0.5,
+    fdtd.ex = alpha/kx sin(kx x) at the cell midpoints), timed as one call;
+    the particle pass timed with HIP events in a second run; the energy
+    history of the first 100 iterations against the oracle's loop."""
+    nx, ny = W["grid"]
+    n = W["n"] if args.particles is None else args.particles
+    alpha, kx, dt = W["alpha"], W["k"], W["dt"]
+    mesh = pic.TwoDGrid(2 * math.pi / kx, nx, W["dimy"], ny)
+    log(f"bench: {W['name']}: sampling {n} particles (quiet)")
+    p0 = np.asfortranarray(pic.sampling.landau_damping(n, mesh, alpha, kx))
+    f0 = pic.FDTD(mesh)
+    xn = np.linspace(0.0, mesh.dimx, nx + 1)
+    f0.ex[...] = (alpha / kx * np.sin(kx * (xn[:-1] + xn[1:]) / 2))[:, None]
+    sim = pic.Simulation(mesh, p0, pic.CloudInCell())
+
+    def reset():
+        sim.set_particles(p0)
+        sim.set_fdtd(f0)
+        sim.ctx.sync()
+
+    if warmup:
+        reset()
+        sim.run_maxwell(dt, warmup)
+    reset()
+    log(f"bench: maxwell loop, {steps} steps")
+    t0 = time.perf_counter()
+    energy = sim.run_maxwell(dt, steps)
+    t1 = time.perf_counter()
+    reset()
+    sim.set_timing(True)
+    nt = min(steps, 200)
+    sim.run_maxwell(dt, nt)
+    pass_ms, npass, _ = sim.timing()
+    sim.set_timing(False)
+    pass_ms /= max(npass, 1)
+    sim.close()
+    seconds = t1 - t0
+    value = n * steps / seconds
+    achieved = VM_BYTES * n / (pass_ms * 1e-3) / 1e9
+    log(f"bench: maxwell {seconds * 1e3:.2f} ms for {steps} steps, pass {pass_ms * 1e3:.2f} us")
+    out = {
+        "metric": "particle-steps/s of the Vlasov-Maxwell FDTD + CIC loop",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": seconds * 1e3 / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (Landau quiet start, Sobol)",
+        "config": {"workload": W["name"], "grid": [nx, ny], "particles": n, "kernel": "CIC",
+                   "alpha": alpha, "k": kx, "dt": dt, "dimy": W["dimy"],
+                   "state": "fp64 soa", "parallelism": "single-gpu"},
+        "seconds_total": seconds,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_vm_pass_lds<CIC> update_fields+push_v+push_x+current pass",
+                     "bytes_per_particle": VM_BYTES, "avg_us": pass_ms * 1e3},
+        "energy_first_last": [float(energy[0]), float(energy[-1])],
+    }
+    if args.workload == "vm2d" and n == W["n"]:
+        traffic = pmc_traffic("k_vm_pass_lds", "pmc_traffic_vm2d_128x16_204800.txt")
+        if traffic is not None:
+            out["roofline"]["traffic"] = traffic[0]
+            out["roofline"]["traffic_source"] = traffic[1]
+    if not args.no_cpu and args.cpu_steps > 0:
+        O = entry.load_oracle()
+        steps_cpu = min(steps, 100)
+        q = p0.copy(order="F")
+        fo = O.FDTD(nx, ny)
+        fo.ex[...] = f0.ex
+        log(f"bench: cpu baseline, oracle Maxwell loop, {n} particles, {steps_cpu} steps, "
+            f"{O.num_threads()} threads")
+        c0 = time.perf_counter()
+        eo = O.maxwell_loop(q, fo, nx, ny, mesh.dimx, mesh.dimy, dt, steps_cpu)
+        c1 = time.perf_counter()
+        out["energy_rel_diff_vs_oracle"] = float(np.max(np.abs(energy[:steps_cpu] - eo))
+                                                 / np.max(np.abs(eo)))
+        cores, model = cpu_info()
+        out["cpu_baseline"] = {
+            "value": n * steps_cpu / (c1 - c0), "unit": "particle-steps/s",
+            "cores": O.num_threads(), "kind": "port",
+            "sample": f"all {n} particles of the same workload, first {steps_cpu} steps of the "
+                      f"loop (oracle C restatement: OpenMP push_v, serial current deposit and "
+                      f"FDTD), {model}, nproc={cores}"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -371,6 +473,10 @@ def main():
     from picmi355x import diagnostics
 
     W = WORKLOADS[args.workload]
+    if W.get("maxwell"):
+        bench_maxwell(args, W, pic, entry, W["steps"] if args.steps is None else args.steps,
+                      W["warmup"] if args.warmup is None else args.warmup)
+        return
     # steps between lane sorts: each path at the interval that suits it (the
     # FP32 windows keep a wider drift margin than the FP64 ones; particles
     # cross cells faster on finer grids and barely drift in a strong field)

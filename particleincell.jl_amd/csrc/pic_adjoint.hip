@@ -63,10 +63,8 @@ __device__ __forceinline__ void shape_d(double f, double *w, double *dw) {
 }
 
 __device__ __forceinline__ int ref_cell_a(double x, double dx, int n, double &d) {
-    double xp = x / dx;
-    long long i1 = (long long)floor(xp) + 1;
-    d = (xp - (double)i1) + 1;
-    return (int)picd::mod1z(i1, n);
+    const double xp = x / dx;
+    return picd::cell_of(xp, floor(xp), n, d);
 }
 
 // x' = x + h v (push_x!, pushers_cic.jl:55-72); xs receives a copy.

@@ -241,6 +241,22 @@ __host__ __device__ inline long long mod1z(long long x, long long n) {
     return r < 0 ? r + n : r;
 }
 
+// The reference's cell of a coordinate, i = fl + 1 with fl = floor(xp) (or
+// trunc(xp)), as the 0-based periodic cell mod1(i, n) - 1 = fl mod n, and
+// the in-cell offset d = (xp - i) + 1.  32-bit arithmetic: fl is clamped to
+// +-2^30 first (the reference's index is out of bounds long before; NaN
+// becomes -2^30), so the result is a valid cell for every input and equals
+// the 64-bit mod1 for |fl| < 2^30.
+__device__ __forceinline__ int cell_of(double xp, double fl, int n, double &d) {
+    d = (xp - (fl + 1.0)) + 1;
+    int i = (int)fmin(fmax(fl, -1073741824.0), 1073741824.0);
+    if ((unsigned)i >= (unsigned)n) {
+        i %= n;
+        if (i < 0) i += n;
+    }
+    return i;
+}
+
 // Quintic B-spline (M6) weights of nodes i-2..i+3 at offset t in [0,1), FP32.
 // The pieces of f_m6 (compute_rho_m6.jl:10-25) are 120 w5 = t^5, 120 w4 =
 // P4(t) = 1+5t+10t^2+10t^3+5t^4-5t^5, 120 w3 = P3(t) =

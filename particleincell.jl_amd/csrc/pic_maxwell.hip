@@ -80,13 +80,11 @@ __global__ void k_update_fields(int nx, int ny, const double *__restrict__ fex,
                     fbz[m1z(i, nx) + nx * m1z(j, ny)]);
 }
 
-// The reference's CIC cell (floor) and offset of a coordinate.
+// The reference's CIC cell (floor) and offset of a coordinate
+// (pushers_cic.jl:18-24, cloud_in_cell.jl:36-42).
 __device__ __forceinline__ int cic_cell(double x, double d, int n, double &off) {
     const double xp = x / d;
-    const long long i1 = (long long)floor(xp) + 1;
-    off = (xp - (double)i1) + 1;
-    long long r = (i1 - 1) % n;
-    return (int)(r < 0 ? r + n : r);
+    return picd::cell_of(xp, floor(xp), n, off);
 }
 
 struct VmArgs {
@@ -103,15 +101,16 @@ struct VmArgs {
 
 // push_v!(dt) (pushers_cic.jl:3-53), push_x!(dt/2), the particle's current
 // (cloud_in_cell.jl:31-60), push_x!(dt/2), in that order, per particle.
-__global__ __launch_bounds__(MT) void k_vm_pass(VmArgs a) {
+template <int BS>
+__global__ __launch_bounds__(BS) void k_vm_pass(VmArgs a) {
     extern __shared__ double jsh[];  // jx then jy, nx*ny each, when tiled
     const int nx = a.nx, ny = a.ny, m = nx * ny;
     if (a.tiled) {
-        for (int t = threadIdx.x; t < 2 * m; t += MT) jsh[t] = 0.0;
+        for (int t = threadIdx.x; t < 2 * m; t += BS) jsh[t] = 0.0;
         __syncthreads();
     }
-    for (int64_t k = (int64_t)blockIdx.x * MT + threadIdx.x; k < a.n;
-         k += (int64_t)gridDim.x * MT) {
+    for (int64_t k = (int64_t)blockIdx.x * BS + threadIdx.x; k < a.n;
+         k += (int64_t)gridDim.x * BS) {
         double px = a.x[k], py = a.y[k], v1 = a.vx[k], v2 = a.vy[k];
         const double wgt = a.w ? a.w[k] : a.w0;
         {  // push_v!
@@ -176,7 +175,7 @@ __global__ __launch_bounds__(MT) void k_vm_pass(VmArgs a) {
     }
     if (!a.tiled) return;
     __syncthreads();
-    for (int t = threadIdx.x; t < m; t += MT) {
+    for (int t = threadIdx.x; t < m; t += BS) {
         if (jsh[t] != 0.0) unsafeAtomicAdd(&a.jx[t], jsh[t]);
         if (jsh[m + t] != 0.0) unsafeAtomicAdd(&a.jy[t], jsh[m + t]);
     }
@@ -229,6 +228,241 @@ __global__ __launch_bounds__(MT) void k_fdtd_energy(const double *__restrict__ f
     }
 }
 
+// (i, j) of t = i + ld j, walked by a fixed stride without divisions.
+struct Walk {
+    int i, j, di, dj, ld;
+    __device__ Walk(int t, int ld_, int stride)
+        : i(t % ld_), j(t / ld_), di(stride % ld_), dj(stride / ld_), ld(ld_) {}
+    __device__ void next() {
+        i += di;
+        j += dj;
+        if (i >= ld) {
+            i -= ld;
+            ++j;
+        }
+    }
+};
+
+// The loop's particle work for grids whose node fields and current fit in
+// one workgroup's LDS.  Each block forms the node fields of update_fields!
+// from the FDTD fields (k_update_fields' arithmetic) in LDS and writes its
+// share of them to the context's node grids, then runs P particles per
+// thread through push_v!, push_x!(dt/2), compute_current!, push_x!(dt/2)
+// (loads of all P first), sums the current in LDS and adds it to jx, jy
+// with global atomics.
+template <int BS, int P>
+__global__ __launch_bounds__(BS) void k_vm_pass_lds(VmArgs a, const double *__restrict__ fex,
+                                                    const double *__restrict__ fey,
+                                                    const double *__restrict__ fbz,
+                                                    double *__restrict__ nex,
+                                                    double *__restrict__ ney,
+                                                    double *__restrict__ nbz) {
+    extern __shared__ double lds[];
+    const int nx = a.nx, ny = a.ny, m = nx * ny, tid = threadIdx.x;
+    double *sex = lds, *sey = lds + m, *sbz = lds + 2 * m, *sjx = lds + 3 * m, *sjy = lds + 4 * m;
+    const int64_t n = a.n, stride = (int64_t)gridDim.x * (BS * P);
+    int64_t base = (int64_t)blockIdx.x * (BS * P);
+    double px[P], py[P], v1[P], v2[P], wg[P];
+    auto load = [&]() {
+#pragma unroll
+        for (int r = 0; r < P; ++r) {
+            const int64_t k = base + r * BS + tid;
+            if (k < n) {
+                px[r] = a.x[k];
+                py[r] = a.y[k];
+                v1[r] = a.vx[k];
+                v2[r] = a.vy[k];
+                wg[r] = a.w ? a.w[k] : a.w0;
+            }
+        }
+    };
+    load();
+    {
+        const int lo = (int)((int64_t)m * blockIdx.x / gridDim.x);
+        const int hi = (int)((int64_t)m * (blockIdx.x + 1) / gridDim.x);
+        Walk w(tid, nx, BS);
+        for (int t = tid; t < m; t += BS, w.next()) {
+            const int i = w.i, j = w.j;
+            const int im = i == 0 ? nx - 1 : i - 1, jm = j == 0 ? ny - 1 : j - 1;
+            const double e1 = 0.5 * (fex[im + nx * j] + fex[i + nx * j]);
+            const double e2 = 0.5 * (fey[i + (nx + 1) * jm] + fey[i + (nx + 1) * j]);
+            const double b3 =
+                0.25 * (((fbz[im + nx * jm] + fbz[i + nx * jm]) + fbz[im + nx * j]) + fbz[i + nx * j]);
+            sex[t] = e1;
+            sey[t] = e2;
+            sbz[t] = b3;
+            sjx[t] = 0.0;
+            sjy[t] = 0.0;
+            if (t >= lo && t < hi) {
+                nex[t] = e1;
+                ney[t] = e2;
+                nbz[t] = b3;
+            }
+        }
+    }
+    __syncthreads();
+    const double dt = a.dt, h = 0.5 * dt;
+    while (base < n) {
+#pragma unroll
+        for (int r = 0; r < P; ++r) {
+            const int64_t k = base + r * BS + tid;
+            if (k >= n) break;
+            double x = px[r], y = py[r], u1 = v1[r], u2 = v2[r];
+            {  // push_v!: pushers_cic.jl:13-51
+                double dxp, dyp;
+                const int i = cic_cell(x, a.dx, nx, dxp), j = cic_cell(y, a.dy, ny, dyp);
+                const int ip = i + 1 == nx ? 0 : i + 1, jp = j + 1 == ny ? 0 : j + 1;
+                const double a1 = (1 - dxp) * (1 - dyp), a2 = dxp * (1 - dyp), a3 = dxp * dyp,
+                             a4 = (1 - dxp) * dyp;
+                const int c00 = i + nx * j, c10 = ip + nx * j, c11 = ip + nx * jp, c01 = i + nx * jp;
+                const double e1 = a1 * sex[c00] + a2 * sex[c10] + a3 * sex[c11] + a4 * sex[c01];
+                const double e2 = a1 * sey[c00] + a2 * sey[c10] + a3 * sey[c11] + a4 * sey[c01];
+                const double b3 = a1 * sbz[c00] + a2 * sbz[c10] + a3 * sbz[c11] + a4 * sbz[c01];
+                u1 += 0.5 * dt * e1;
+                u2 += 0.5 * dt * e2;
+                const double tantheta = 0.5 * dt * b3;
+                const double sintheta = 2 * tantheta / (1 + tantheta * tantheta);
+                u1 += u2 * tantheta;
+                u2 += -u1 * sintheta;
+                u1 += u2 * tantheta;
+                u1 = u1 + 0.5 * dt * e1;
+                u2 = u2 + 0.5 * dt * e2;
+            }
+            auto drift = [&]() {  // push_x!(dt/2): pushers_cic.jl:55-72
+                double p1 = x + h * u1, p2 = y + h * u2;
+                if (p1 > a.dimx) p1 -= a.dimx;
+                if (p2 > a.dimy) p2 -= a.dimy;
+                if (p1 < 0.0) p1 += a.dimx;
+                if (p2 < 0.0) p2 += a.dimy;
+                x = p1;
+                y = p2;
+            };
+            drift();
+            {  // compute_current!: cloud_in_cell.jl:31-60
+                double dxp, dyp;
+                const int i = cic_cell(x, a.dx, nx, dxp), j = cic_cell(y, a.dy, ny, dyp);
+                const int ip = i + 1 == nx ? 0 : i + 1, jp = j + 1 == ny ? 0 : j + 1;
+                const double a1 = (1 - dxp) * (1 - dyp), a2 = dxp * (1 - dyp), a3 = dxp * dyp,
+                             a4 = (1 - dxp) * dyp;
+                const double w1 = u1 * a.scaling * wg[r], w2 = u2 * a.scaling * wg[r];
+                const int c[4] = {i + nx * j, ip + nx * j, ip + nx * jp, i + nx * jp};
+                const double q[4] = {a1, a2, a3, a4};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    __hip_atomic_fetch_add(&sjx[c[e]], q[e] * w1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(&sjy[c[e]], q[e] * w2, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+            drift();
+            a.x[k] = x;
+            a.y[k] = y;
+            a.vx[k] = u1;
+            a.vy[k] = u2;
+        }
+        base += stride;
+        if (base < n) load();
+    }
+    __syncthreads();
+    for (int t = tid; t < m; t += BS) {
+        if (sjx[t] != 0.0) unsafeAtomicAdd(&a.jx[t], sjx[t]);
+        if (sjy[t] != 0.0) unsafeAtomicAdd(&a.jy[t], sjy[t]);
+    }
+}
+
+// The field work between two such passes, in one workgroup with the FDTD
+// fields and the current in LDS: one launch instead of five.  The same
+// arithmetic as k_faraday, k_ampere and k_fdtd_energy, in the loop's order:
+//   faraday!(dt/2), ampere_maxwell!(dt) with jx, jy, the energy into
+//   energy[*counter] (then incremented), and with `next` (every iteration
+//   but the last) jx = jy = 0 and the next iteration's faraday!(dt/2).
+constexpr int FT = 1024;
+constexpr int VT = 1024;  // threads of a pass block that sums the current in LDS
+constexpr size_t VM_LDS_BYTES_MAX = 120 * 1024;
+
+__global__ __launch_bounds__(FT) void k_vm_fields(int nx, int ny, double dx, double dy, double dt,
+                                                  double *__restrict__ fex,
+                                                  double *__restrict__ fey,
+                                                  double *__restrict__ fbz, double *__restrict__ jx,
+                                                  double *__restrict__ jy, double *energy,
+                                                  int *counter, int next) {
+    extern __shared__ double lds[];
+    __shared__ double red[FT / 64];
+    const int m = nx * ny, mex = nx * (ny + 1), mey = (nx + 1) * ny, tid = threadIdx.x;
+    double *sex = lds, *sey = lds + mex, *sbz = sey + mey, *sjx = sbz + m, *sjy = sjx + m;
+    for (int t = tid; t < mex; t += FT) sex[t] = fex[t];
+    for (int t = tid; t < mey; t += FT) sey[t] = fey[t];
+    for (int t = tid; t < m; t += FT) {
+        sbz[t] = fbz[t];
+        sjx[t] = jx[t];
+        sjy[t] = jy[t];
+    }
+    __syncthreads();
+    const double h = 0.5 * dt;
+    auto faraday_lds = [&]() {  // fdtd.jl:24-36
+        Walk w(tid, nx, FT);
+        for (int t = tid; t < m; t += FT, w.next()) {
+            const int i = w.i, j = w.j;
+            const int ip = i + 1 == nx ? 0 : i + 1, jp = j + 1 == ny ? 0 : j + 1;
+            const double dex_dy = (sex[i + nx * jp] - sex[i + nx * j]) / dy;
+            const double dey_dx = (sey[ip + (nx + 1) * j] - sey[i + (nx + 1) * j]) / dx;
+            sbz[t] += h * (dex_dy - dey_dx);
+        }
+        __syncthreads();
+    };
+    faraday_lds();
+    {  // ampere_maxwell!: fdtd.jl:40-54
+        Walk w(tid, nx, FT);
+        for (int t = tid; t < mex; t += FT, w.next()) {
+            const int i = w.i, j = w.j;  // j in [0, ny]
+            const int jc = j == ny ? 0 : j, jm = j == 0 ? ny - 1 : j - 1;
+            const int ip = i + 1 == nx ? 0 : i + 1;
+            const double dbz_dy = (sbz[i + nx * jc] - sbz[i + nx * jm]) / dy;
+            sex[t] += dt * dbz_dy - dt * 0.5 * (sjx[i + nx * jc] + sjx[ip + nx * jc]);
+        }
+        Walk v(tid, nx + 1, FT);
+        for (int u = tid; u < mey; u += FT, v.next()) {
+            const int i = v.i, j = v.j;  // i in [0, nx]
+            const int ic = i == nx ? 0 : i, im = i == 0 ? nx - 1 : i - 1;
+            const int jp = j + 1 == ny ? 0 : j + 1;
+            const double dbz_dx = (sbz[ic + nx * j] - sbz[im + nx * j]) / dx;
+            sey[u] -= dt * dbz_dx - dt * 0.5 * (sjy[ic + nx * j] + sjy[ic + nx * jp]);
+        }
+    }
+    __syncthreads();
+    double s = 0.0;
+    for (int t = tid; t < mex; t += FT) s += sex[t] * sex[t];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    if (next) {
+        for (int t = tid; t < m; t += FT) {
+            jx[t] = 0.0;
+            jy[t] = 0.0;
+        }
+        faraday_lds();  // its barrier also orders red[]
+    } else {
+        __syncthreads();
+    }
+    if (tid == 0) {
+        double acc = 0.0;
+        for (int w = 0; w < FT / 64; ++w) acc += red[w];
+        const int q = *counter;
+        *counter = q + 1;
+        energy[q] = 0.5 * log(acc * (dx * dy));
+    }
+    for (int t = tid; t < mex; t += FT) fex[t] = sex[t];
+    for (int t = tid; t < mey; t += FT) fey[t] = sey[t];
+    for (int t = tid; t < m; t += FT) fbz[t] = sbz[t];
+}
+
+inline size_t vm_field_lds(const Ctx *c) {
+    const size_t nx = c->nx, ny = c->ny;
+    return (nx * (ny + 1) + (nx + 1) * ny + 3 * nx * ny) * 8;
+}
+
+inline size_t vm_pass_lds(const Ctx *c) { return (size_t)5 * c->nx * c->ny * 8; }
+
 inline dim3 blocks_for(int64_t n) { return dim3((unsigned)((n + MT - 1) / MT)); }
 
 int ensure_fdtd(Ctx *c) {
@@ -274,6 +508,8 @@ int update_fields(Ctx *c) {
     return PIC_OK;
 }
 
+int64_t vm_tiled_blocks(const Ctx *c) { return (int64_t)c->ncu / 2; }
+
 VmArgs vm_args(Ctx *c, double dt) {
     VmArgs a;
     a.n = c->n;
@@ -291,20 +527,68 @@ VmArgs vm_args(Ctx *c, double dt) {
 }
 
 // LDS bytes of the whole-grid current copy (jx and jy), if it fits.
-constexpr size_t VM_LDS_MAX = 48 * 1024;
+constexpr size_t VM_LDS_MAX = 64 * 1024;
 
 int vm_pass(Ctx *c, double dt) {
     if (c->n == 0) return PIC_OK;
     VmArgs a = vm_args(c, dt);
     const size_t grid_bytes = (size_t)2 * c->nx * c->ny * 8;
     a.tiled = grid_bytes <= VM_LDS_MAX;
-    // tiled: enough particles per block that the flush stays small against
-    // the particle work, and about two blocks per compute unit
-    int64_t nb = (c->n + MT - 1) / MT;
-    if (a.tiled) nb = std::min<int64_t>(nb, 2 * (int64_t)c->ncu);
-    nb = std::max<int64_t>(1, std::min<int64_t>(nb, 65535));
-    hipLaunchKernelGGL(k_vm_pass, dim3((unsigned)nb), dim3(MT), a.tiled ? grid_bytes : 0,
-                       c->stream, a);
+    if (a.tiled) {
+        // every block flushes its whole LDS grid with global atomics, which
+        // all meet at the same 2 nx ny addresses: few, wide blocks
+        int64_t nb = (c->n + VT - 1) / VT;
+        nb = std::max<int64_t>(1, std::min<int64_t>(nb, vm_tiled_blocks(c)));
+        hipLaunchKernelGGL(k_vm_pass<VT>, dim3((unsigned)nb), dim3(VT), grid_bytes, c->stream, a);
+    } else {
+        const int64_t nb = std::max<int64_t>(1, std::min<int64_t>((c->n + MT - 1) / MT, 65535));
+        hipLaunchKernelGGL(k_vm_pass<MT>, dim3((unsigned)nb), dim3(MT), 0, c->stream, a);
+    }
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+constexpr size_t LDS_DEFAULT_MAX = 64 * 1024;
+
+// Dynamic LDS above the default 64 KB needs the kernels' opt-in.
+int vm_lds_attributes(size_t flds, size_t plds) {
+    if (flds > LDS_DEFAULT_MAX)
+        PIC_HIP(hipFuncSetAttribute((const void *)k_vm_fields,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));
+    if (plds > LDS_DEFAULT_MAX) {
+        const void *fns[3] = {(const void *)k_vm_pass_lds<VT, 1>, (const void *)k_vm_pass_lds<VT, 2>,
+                              (const void *)k_vm_pass_lds<VT, 4>};
+        for (const void *f : fns)
+            PIC_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds));
+    }
+    return PIC_OK;
+}
+
+// k_vm_pass_lds over the particles: one or two blocks per compute unit (as
+// the LDS allows), each thread taking the fewest particles (1, 2 or 4 per
+// round) for which one round of resident blocks covers them all.
+int vm_pass_lds_launch(Ctx *c, double dt) {
+    const VmArgs a = vm_args(c, dt);
+    const size_t lds = vm_pass_lds(c);
+    const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(2, (int64_t)(160 * 1024 / lds)));
+    const int64_t resident = (int64_t)c->ncu * per_cu;
+    int P = 1;
+    int64_t nb = (c->n + VT - 1) / VT;
+    while (nb > resident && P < 4) {
+        P *= 2;
+        nb = (c->n + (int64_t)VT * P - 1) / ((int64_t)VT * P);
+    }
+    nb = std::max<int64_t>(1, std::min<int64_t>(nb, resident));
+    const double *fx = c->fd_ex, *fy = c->fd_ey, *fb = c->fd_bz;
+    if (P == 1)
+        hipLaunchKernelGGL((k_vm_pass_lds<VT, 1>), dim3((unsigned)nb), dim3(VT), lds, c->stream, a,
+                           fx, fy, fb, c->ex, c->ey, c->bz);
+    else if (P == 2)
+        hipLaunchKernelGGL((k_vm_pass_lds<VT, 2>), dim3((unsigned)nb), dim3(VT), lds, c->stream, a,
+                           fx, fy, fb, c->ex, c->ey, c->bz);
+    else
+        hipLaunchKernelGGL((k_vm_pass_lds<VT, 4>), dim3((unsigned)nb), dim3(VT), lds, c->stream, a,
+                           fx, fy, fb, c->ex, c->ey, c->bz);
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
@@ -456,29 +740,48 @@ int pic_run_maxwell(pic_ctx *ctx, int kernel, double dt, int nsteps, double *ene
     PIC_TRY(ensure_energy(c, nsteps));
     PIC_HIP(hipMemsetAsync(c->step_counter, 0, sizeof(int), c->stream));
     const int64_t m = (int64_t)c->nx * c->ny;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    if (c->timing) {
-        PIC_HIP(hipEventCreate(&ev[0]));
-        PIC_HIP(hipEventCreate(&ev[1]));
-    }
+    // timing: events around every particle pass, read after the loop
+    std::vector<hipEvent_t> ev;
+    auto mark = [&]() -> int {
+        hipEvent_t e;
+        PIC_HIP(hipEventCreate(&e));
+        ev.push_back(e);
+        PIC_HIP(hipEventRecord(e, c->stream));
+        return PIC_OK;
+    };
     int rc = PIC_OK;
-    for (int s = 1; s <= nsteps && rc == PIC_OK; ++s) {
+    const size_t flds = vm_field_lds(c), plds = vm_pass_lds(c);
+    const bool fused = flds <= VM_LDS_BYTES_MAX && plds <= VM_LDS_BYTES_MAX;
+    auto timed_pass = [&](bool lds) {
+        int r = c->timing ? mark() : PIC_OK;
+        if (r == PIC_OK) r = lds ? vm_pass_lds_launch(c, dt) : vm_pass(c, dt);
+        if (r == PIC_OK && c->timing) r = mark();
+        return r;
+    };
+    if (fused) {
+        // iteration s: the pass (with update_fields! in every block), then
+        // faraday!(dt/2), ampere_maxwell!, the energy and, but after the
+        // last iteration, the next one's faraday!(dt/2) in one field launch
+        rc = vm_lds_attributes(flds, plds);
+        if (rc == PIC_OK) rc = launch_fill(c, c->jx, 0.0, m);
+        if (rc == PIC_OK) rc = launch_fill(c, c->jy, 0.0, m);
+        c->bz_mode = PIC_BZ_GRID;
+        for (int s = 1; s <= nsteps && rc == PIC_OK; ++s) {
+            rc = timed_pass(true);
+            if (rc == PIC_OK) {
+                hipLaunchKernelGGL(k_vm_fields, dim3(1), dim3(FT), flds, c->stream, c->nx, c->ny,
+                                   c->dx, c->dy, dt, c->fd_ex, c->fd_ey, c->fd_bz, c->jx, c->jy,
+                                   c->energy_hist, c->step_counter, s < nsteps ? 1 : 0);
+                if (hipGetLastError() != hipSuccess) rc = fail(PIC_ERR_HIP, "field kernel launch");
+            }
+        }
+    }
+    for (int s = 1; s <= nsteps && rc == PIC_OK && !fused; ++s) {
         if (s > 1) rc = faraday(c, 0.5 * dt);
         if (rc == PIC_OK) rc = update_fields(c);
         if (rc == PIC_OK) rc = launch_fill(c, c->jx, 0.0, m);
         if (rc == PIC_OK) rc = launch_fill(c, c->jy, 0.0, m);
-        if (rc == PIC_OK && c->timing) rc = hipEventRecord(ev[0], c->stream) == hipSuccess ? PIC_OK : PIC_ERR_HIP;
-        if (rc == PIC_OK) rc = vm_pass(c, dt);
-        if (rc == PIC_OK && c->timing) {
-            rc = hipEventRecord(ev[1], c->stream) == hipSuccess &&
-                         hipEventSynchronize(ev[1]) == hipSuccess
-                     ? PIC_OK
-                     : PIC_ERR_HIP;
-            float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
-            c->t_pass_ms += ms;
-            c->t_passes += 1;
-        }
+        if (rc == PIC_OK) rc = timed_pass(false);
         if (rc == PIC_OK) rc = faraday(c, 0.5 * dt);
         if (rc == PIC_OK) rc = ampere(c, dt);
         if (rc == PIC_OK) {
@@ -488,8 +791,6 @@ int pic_run_maxwell(pic_ctx *ctx, int kernel, double dt, int nsteps, double *ene
             if (hipGetLastError() != hipSuccess) rc = fail(PIC_ERR_HIP, "energy kernel launch");
         }
     }
-    for (hipEvent_t e : ev)
-        if (e) (void)hipEventDestroy(e);
     if (rc == PIC_OK && energy)
         rc = hipMemcpyAsync(energy, c->energy_hist, (size_t)nsteps * 8, hipMemcpyDeviceToHost,
                             c->stream) == hipSuccess
@@ -497,6 +798,15 @@ int pic_run_maxwell(pic_ctx *ctx, int kernel, double dt, int nsteps, double *ene
                  : fail(PIC_ERR_HIP, "energy copy");
     const hipError_t se = hipStreamSynchronize(c->stream);
     if (rc == PIC_OK && se != hipSuccess) rc = fail(PIC_ERR_HIP, "pic_run_maxwell: %s", hipGetErrorString(se));
+    if (rc == PIC_OK && c->timing) {
+        for (size_t k = 0; k + 1 < ev.size(); k += 2) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+            c->t_pass_ms += ms;
+            c->t_passes += 1;
+        }
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     c->seg_valid = false;
     return rc;
 }

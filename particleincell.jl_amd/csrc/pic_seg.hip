@@ -368,10 +368,8 @@ struct SegF64 {
 // floor(xp)+1 (trunc in compute_rho CIC, pushers_cic.jl:86-87, equal for
 // x >= 0), d = (xp - i) + 1; returns the 0-based wrapped cell.
 __device__ __forceinline__ int ref_cell_s(double x, double dx, int n, bool trunc_, double &d) {
-    double xp = x / dx;
-    long long i1 = (trunc_ ? (long long)xp : (long long)floor(xp)) + 1;
-    d = (xp - (double)i1) + 1;
-    return (int)picd::mod1z(i1, n);
+    const double xp = x / dx;
+    return picd::cell_of(xp, trunc_ ? trunc(xp) : floor(xp), n, d);
 }
 
 // Field at a particle from the nodes (u, v) of its stencil, FP64, with the

@@ -161,10 +161,8 @@ __device__ __forceinline__ int64_t tile_node(const Box &b, int a, int bb, int ir
 // dxp = (xp - i) + 1, 0-based wrapped cell returned.
 __device__ __forceinline__ void ref_cell(double x, double dx, int n, bool trunc_, int &iw,
                                          double &d) {
-    double xp = x / dx;
-    long long i1 = (trunc_ ? (long long)xp : (long long)floor(xp)) + 1;
-    d = (xp - (double)i1) + 1;
-    iw = (int)picd::mod1z(i1, n);
+    const double xp = x / dx;
+    iw = picd::cell_of(xp, trunc_ ? trunc(xp) : floor(xp), n, d);
 }
 
 template <int K, bool GATHER, int NDRIFT, bool DEPOSIT, int BZ, int PPT>

@@ -193,14 +193,16 @@ def test_run_maxwell_first_step_bit_exact(gpu, O):
 
 @pytest.mark.parametrize("nx,ny,npm,steps,midpoint", [(128, 16, 100, 100, False),
                                                       (32, 8, 50, 200, True),
+                                                      (96, 40, 20, 40, True),
                                                       (128, 64, 10, 40, True)])
 def test_run_maxwell_matches_oracle_loop(gpu, O, nx, ny, npm, steps, midpoint):
     """pic_run_maxwell against the oracle's loop: the README's 204,800
     particles on 128 x 16 cells for 100 steps (the loop of
-    examples/landau_damping_maxwell_fdtd.jl), a small grid for 200 steps and
-    a 128 x 64 grid whose current is summed in global memory.  The current's
-    reordered sums are the only difference, so energies and particles agree
-    to 1e-10 relative."""
+    examples/landau_damping_maxwell_fdtd.jl), a small grid for 200 steps, a
+    96 x 40 grid (fields in more than 64 KB of one workgroup's LDS, current
+    summed in global memory) and a 128 x 64 grid (one kernel per field
+    update).  The current's reordered sums are the only difference, so
+    energies and particles agree to 1e-10 relative."""
     pic = gpu
     mesh, p, f, o = _vm_setup(pic, O, nx, ny, npm, midpoint)
     sim = pic.Simulation(mesh, p, pic.CloudInCell())
