@@ -360,6 +360,67 @@ int pic_fdtd_energy(pic_ctx *ctx, double *energy);
  * be PIC_CIC; FP64 state. */
 int pic_run_maxwell(pic_ctx *ctx, int kernel, double dt, int nsteps, double *energy);
 
+/* ------------------------------------------ uniformly accurate (UA) PIC */
+
+/* The UA step of src/ua_steps.jl for the 2D Vlasov-Poisson problem with the
+ * strong inhomogeneous magnetic field b(x) = 1 + sin(x1) sin(x2)/2, run as in
+ * test/bupdate.jl.  The context's particles are the reference's `Particles`
+ * (x, v through pic_set_particles_xv with the common weight w; Particles.e
+ * from pic_interpol_eb_m6 or pic_set_particle_fields); the fields are
+ * MeshFields2D's (rho from these calls, E from pic_solve_fields).  FP64
+ * state.
+ *
+ * Complex arrays cross the boundary as interleaved (re, im) doubles in the
+ * reference's column-major (ntau, 2, n) layout: element (m, c, k), 0-based,
+ * at complex index m + ntau (c + 2 k); pl and ql are (ntau, n); et is a real
+ * (ntau, 2, n) array. */
+
+/* UA(ntau, eps, nbpart): ua_type.jl:16-41.  ntau is a power of two from 2 to
+ * 64 (the tau-FFTs run across ntau lanes of one wave). */
+int pic_ua_init(pic_ctx *ctx, int ntau, double eps);
+
+/* preparation!(ua, dt, particles, xt, yt): ua_steps.jl:3-83.  Keeps b and t
+ * (Particles.b, .t) in the context for the calls below; any output pointer
+ * may be NULL (b, t: n doubles).  The calls below take Particles.b or .t as
+ * an n-vector, or NULL for the context's from the last preparation. */
+int pic_ua_preparation(pic_ctx *ctx, double dt, double *b, double *t, double *pl, double *ql,
+                       double *xt, double *yt);
+
+/* update_particles_e!(particles, et, fields, ua, xt) = interpol_eb_m6!(et,
+ * fields, xt, nbpart, ntau): ua_steps.jl:87-93, interpolation_m6.jl:3-123,
+ * with the context's E (pic_set_grid or pic_solve_fields). */
+int pic_ua_interpol(pic_ctx *ctx, const double *xt, double *et);
+
+/* compute_f!(fx, fy, ua, particles, xt, yt, et): ua_steps.jl:107-148,
+ * including its closing fft!(fx, 1), fft!(fy, 1). */
+int pic_ua_compute_f(pic_ctx *ctx, const double *b, const double *xt, const double *yt,
+                     const double *et, double *fx, double *fy);
+
+/* ua_step!(xt, xtilde, ua, particles, fx) (ua_steps.jl:152-172), or with gx
+ * not NULL ua_step!(xt, xtilde, ua, particles, fx, gx) (:174-202). */
+int pic_ua_step(pic_ctx *ctx, const double *t, const double *xtilde, const double *fx,
+                const double *gx, double *xt);
+
+/* update_particles_x!(particles, fields, ua, xt) = compute_rho_m6!(fields,
+ * particles, xt, ua): compute_rho_m6.jl:29-179.  Writes Particles.x and the
+ * context's rho (read it with pic_get_grid). */
+int pic_ua_compute_rho(pic_ctx *ctx, const double *t, const double *xt);
+
+/* compute_v!(yt, particles, ua): ua_steps.jl:206-224, into Particles.v. */
+int pic_ua_compute_v(pic_ctx *ctx, const double *t, const double *yt);
+
+/* nsteps iterations of the loop of test/bupdate.jl:62-116 (prediction,
+ * solve!, correction, solve!, compute_v!) on the resident particles, from
+ * the state the prologue (:56-60: compute_rho_m6!, solve!, interpol_eb_m6!)
+ * left.  energy (may be NULL) receives the two solve! energies of every
+ * iteration, 2 x nsteps.  Per particle the loop keeps only the positions at
+ * the step's start and the tau-spectra of fx, fy (64 ntau + 32 bytes). */
+int pic_ua_run(pic_ctx *ctx, double dt, int nsteps, double *energy);
+
+/* Particles.e (2 x n) from the host; the counterpart of
+ * pic_get_particle_fields. */
+int pic_set_particle_fields(pic_ctx *ctx, const double *e);
+
 /* ------------------------------------------ Fortran-twin drop-ins */
 
 /* The symbols of src/pic.f90, same arguments (scalars by reference, arrays

@@ -56,6 +56,18 @@ def lib():
             "or_fdtd_energy": (_d, [_i32, _i32, _d, _d, _dp]),
             "or_maxwell_loop": (_i32, [_i64, _dp, _i32, _i32, _d, _d, _d, _i32, _dp, _dp, _dp,
                                        _dpc]),
+            "or_fft_c": (None, [_cp, _i32, _i32]),
+            "or_ua_preparation": (None, [_i32, _d, _d, _i64, _dp, _dp, _dp, _dp, _dp, _cp, _cp,
+                                         _cp, _cp]),
+            "or_ua_interpol": (None, [_i32, _i64, _cp, _i32, _i32, _d, _d, _d, _d, _dp, _dp]),
+            "or_ua_compute_f": (None, [_i32, _d, _i64, _dp, _cp, _cp, _dp, _cp, _cp]),
+            "or_ua_step": (None, [_i32, _d, _i64, _dp, _cp, C.c_void_p, _cp, _cp, C.c_void_p,
+                                  _cp]),
+            "or_ua_compute_rho": (None, [_i32, _d, _i64, _dp, _cp, _d, _i32, _i32, _d, _d, _d,
+                                         _d, _dp, _dp]),
+            "or_ua_compute_v": (None, [_i32, _d, _i64, _dp, _cp, _dp]),
+            "or_ua_run": (_i32, [_i32, _d, _d, _i32, _i64, _dp, _dp, _dp, _d, _i32, _i32, _d, _d,
+                                 _d, _d, _dp, _dp]),
             "or_f_tsc": (_d, [_d]),
             "or_f_m4": (_d, [_d]),
             "or_push_v_ngp": (None, [_i64, _dp, _d, _i32, _i32, _d, _d, _dp, _dp, C.c_void_p, _i32]),
@@ -431,3 +443,83 @@ def maxwell_loop(p, f, nx, ny, dimx, dimy, dt, nsteps):
                                e)
     assert rc == 0
     return e
+
+
+# ------------------------------------------------------------------ UA
+
+def _C(a):
+    return np.asfortranarray(a, dtype=np.complex128)
+
+
+def fft_c(a, sign=-1):
+    """The oracle's radix-2 transform of a 1-D complex array (sign -1
+    forward, +1 backward unnormalised)."""
+    out = _C(np.array(a, dtype=np.complex128))
+    lib().or_fft_c(out, out.shape[0], sign)
+    return out
+
+
+def ua_preparation(ntau, eps, dt, x, v, e):
+    """preparation!(ua, dt, particles, xt, yt): x, v, e are Particles' (2, n)
+    arrays; returns b, t, pl, ql (ntau, n), xt, yt (ntau, 2, n)."""
+    n = x.shape[1]
+    b, t = np.zeros((1, n), order="F"), np.zeros((1, n), order="F")
+    pl, ql = (np.zeros((ntau, n), np.complex128, order="F") for _ in range(2))
+    xt, yt = (np.zeros((ntau, 2, n), np.complex128, order="F") for _ in range(2))
+    lib().or_ua_preparation(ntau, eps, dt, n, _F(x), _F(v), _F(e), b, t, pl, ql, xt, yt)
+    return b[0], t[0], pl, ql, xt, yt
+
+
+def ua_interpol(xt, e_fields, nx, ny, xmin, xmax, ymin, ymax):
+    """update_particles_e!: et (ntau, 2, n) from the real parts of xt."""
+    ntau, _, n = xt.shape
+    et = np.zeros((ntau, 2, n), order="F")
+    lib().or_ua_interpol(ntau, n, _C(xt), nx, ny, xmin, xmax, ymin, ymax, _F(e_fields), et)
+    return et
+
+
+def ua_compute_f(eps, b, xt, yt, et):
+    ntau, _, n = xt.shape
+    fx, fy = (np.zeros((ntau, 2, n), np.complex128, order="F") for _ in range(2))
+    lib().or_ua_compute_f(ntau, eps, n, _F(np.reshape(b, (1, n))), _C(xt), _C(yt), _F(et), fx, fy)
+    return fx, fy
+
+
+def ua_step(eps, t, pl, xtilde, fx, ql=None, gx=None):
+    ntau, _, n = xtilde.shape
+    out = np.zeros((ntau, 2, n), np.complex128, order="F")
+    qlc = _C(ql) if ql is not None else None
+    gxc = _C(gx) if gx is not None else None
+    lib().or_ua_step(ntau, eps, n, _F(np.reshape(t, (1, n))), _C(pl),
+                     qlc.ctypes.data if qlc is not None else None, _C(xtilde), _C(fx),
+                     gxc.ctypes.data if gxc is not None else None, out)
+    return out
+
+
+def ua_compute_rho(eps, t, xt, w, nx, ny, xmin, xmax, ymin, ymax):
+    """update_particles_x!: returns (Particles.x (2, n), fields.rho (nx+1, ny+1))."""
+    ntau, _, n = xt.shape
+    x = np.zeros((2, n), order="F")
+    rho = np.zeros((nx + 1, ny + 1), order="F")
+    lib().or_ua_compute_rho(ntau, eps, n, _F(np.reshape(t, (1, n))), _C(xt), w, nx, ny, xmin,
+                            xmax, ymin, ymax, x, rho)
+    return x, rho
+
+
+def ua_compute_v(eps, t, yt):
+    ntau, _, n = yt.shape
+    v = np.zeros((2, n), order="F")
+    lib().or_ua_compute_v(ntau, eps, n, _F(np.reshape(t, (1, n))), _C(yt), v)
+    return v
+
+
+def ua_run(ntau, eps, dt, nsteps, x, v, e, w, e_fields, xmin, xmax, ymin, ymax):
+    """The loop of test/bupdate.jl; x, v and e_fields are updated in place;
+    returns the solve energies (2, nsteps): prediction, correction."""
+    n = x.shape[1]
+    nx, ny = e_fields.shape[1] - 1, e_fields.shape[2] - 1
+    en = np.zeros((2, nsteps), order="F")
+    rc = lib().or_ua_run(ntau, eps, dt, nsteps, n, x, v, _F(e), w, nx, ny, xmin, xmax, ymin,
+                         ymax, e_fields, en)
+    assert rc == 0
+    return en

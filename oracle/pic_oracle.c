@@ -486,6 +486,10 @@ static void fft_strided(double complex *a, int n, size_t s, int sign)
     }
 }
 
+/* The same transform over a contiguous array, for the other oracle files and
+ * the tests: sign -1 forward, +1 backward (unnormalised). */
+void or_fft_c(double complex *a, int n, int sign) { fft_strided(a, n, 1, sign); }
+
 static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 /* rfft of an nx x ny real array (x fastest): output (nx/2+1) x ny, as Julia's

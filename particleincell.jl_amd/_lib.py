@@ -77,6 +77,15 @@ SIGNATURES = {
     "pic_set_current": [_vp, _vp, _vp],
     "pic_fdtd_energy": [_vp, _pd],
     "pic_run_maxwell": [_vp, _i, _d, _i, _vp],
+    "pic_ua_init": [_vp, _i, _d],
+    "pic_ua_preparation": [_vp, _d, _vp, _vp, _vp, _vp, _vp, _vp],
+    "pic_ua_interpol": [_vp, _vp, _vp],
+    "pic_ua_compute_f": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "pic_ua_step": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "pic_ua_compute_rho": [_vp, _vp, _vp],
+    "pic_ua_compute_v": [_vp, _vp, _vp],
+    "pic_ua_run": [_vp, _d, _i, _vp],
+    "pic_set_particle_fields": [_vp, _vp],
     "pic_f90_push_v": [_pi32, _vp, _pd, _pi32, _pi32, _pd, _pd, _vp, _vp, _vp],
     "pic_f90_push_x": [_pi32, _pd, _pd, _vp, _pd],
     "pic_f90_deposition": [_pi32, _pi32, _pi32, _pd, _pd, _vp, _vp, _vp],

@@ -21,6 +21,8 @@ argument order and array layouts (a Julia `f!` is written `f_` here):
     FDTD(mesh), faraday_, ampere_maxwell_, update_fields_, compute_energy
                                              fdtd.jl:3-80
     compute_current_(jx, jy, mesh, CloudInCell(), p)   cloud_in_cell.jl:10
+    UA(ntau, eps, n), preparation_, update_particles_e_, compute_f_, ua_step_,
+    update_particles_x_, compute_v_             ua_type.jl, ua_steps.jl
 
 Arrays are numpy float64 in Fortran order, so the memory layout is the Julia
 one.  Each call copies its inputs to the GPU and its outputs back (the
@@ -519,6 +521,149 @@ def compute_energy(fdtd, mesh):
     e = C.c_double()
     L.call("pic_fdtd_energy", ctx.h, C.byref(e))
     return e.value
+
+
+# ------------------------------------------------------------------- UA
+
+class UA:
+    """UA(ntau, eps, nbpart) (ua_type.jl:16-41): tau, ltau, and the pl, ql
+    tables ((ntau, nbpart) complex) that preparation_ fills."""
+
+    def __init__(self, ntau, eps, nbpart):
+        self.ntau, self.eps, self.nbpart = int(ntau), float(eps), int(nbpart)
+        dtau = 2 * math.pi / self.ntau
+        self.ltau = np.array(list(range(self.ntau // 2)) + list(range(-(self.ntau // 2), 0)),
+                             dtype=np.float64)
+        self.tau = np.array([i * dtau for i in range(self.ntau)])
+        self.pl = np.zeros((self.ntau, self.nbpart), np.complex128, order="F")
+        self.ql = np.zeros((self.ntau, self.nbpart), np.complex128, order="F")
+
+
+# the context of the UA calls that take no mesh (their kernels use none)
+_PARTICLE_MESH = TwoDGrid(0.0, 1.0, 8, 0.0, 1.0, 8)
+
+
+def _ua_context(ua, particles, mesh=None):
+    ctx = _context(mesh if mesh is not None else _PARTICLE_MESH)
+    _set_xv(ctx, particles)
+    L.call("pic_set_particle_fields", ctx.h, np.asfortranarray(particles.e).ctypes.data)
+    L.call("pic_ua_init", ctx.h, ua.ntau, ua.eps)
+    return ctx
+
+
+def _ua_array(a, ua, n, name, dtype=np.complex128):
+    if a.shape != (ua.ntau, 2, n) or a.dtype != dtype or not a.flags.f_contiguous:
+        raise ValueError(f"{name} must be a {np.dtype(dtype).name} Fortran array of shape "
+                         f"({ua.ntau}, 2, {n})")
+    return a.ctypes.data
+
+
+def preparation_(ua, dt, particles, xt, yt):
+    """preparation!(ua, dt, particles, xt, yt) (ua_steps.jl:3-83): fills
+    particles.b, particles.t, ua.pl, ua.ql, xt and yt."""
+    n = particles.nbpart
+    ctx = _ua_context(ua, particles)
+    b, t = np.zeros(n), np.zeros(n)
+    L.call("pic_ua_preparation", ctx.h, float(dt), b.ctypes.data, t.ctypes.data,
+           ua.pl.ctypes.data, ua.ql.ctypes.data, _ua_array(xt, ua, n, "xt"),
+           _ua_array(yt, ua, n, "yt"))
+    particles.b[...] = b
+    particles.t[...] = t
+
+
+def update_particles_e_(particles, et, fields, ua, xt):
+    """update_particles_e!(particles, et, fields, ua, xt) (ua_steps.jl:87-93):
+    the M6 gather of fields.e at every real(xt[m, :, k])."""
+    n = particles.nbpart
+    ctx = _ua_context(ua, particles, fields.mesh)
+    ctx.set_grid(L.PIC_GRID_EX, fields.e[0])
+    ctx.set_grid(L.PIC_GRID_EY, fields.e[1])
+    L.call("pic_ua_interpol", ctx.h, _ua_array(xt, ua, n, "xt"),
+           _ua_array(et, ua, n, "et", np.float64))
+
+
+def compute_f_(fx, fy, ua, particles, xt, yt, et):
+    """compute_f!(fx, fy, ua, particles, xt, yt, et) (ua_steps.jl:107-148)."""
+    n = particles.nbpart
+    ctx = _ua_context(ua, particles)
+    b = np.ascontiguousarray(particles.b, dtype=np.float64)
+    L.call("pic_ua_compute_f", ctx.h, b.ctypes.data, _ua_array(xt, ua, n, "xt"),
+           _ua_array(yt, ua, n, "yt"), _ua_array(et, ua, n, "et", np.float64),
+           _ua_array(fx, ua, n, "fx"), _ua_array(fy, ua, n, "fy"))
+
+
+def ua_step_(xt, xtilde, ua, particles, fx, gx=None):
+    """ua_step!(xt, x~t, ua, particles, fx[, gx]) (ua_steps.jl:152-202)."""
+    n = particles.nbpart
+    ctx = _ua_context(ua, particles)
+    t = np.ascontiguousarray(particles.t, dtype=np.float64)
+    L.call("pic_ua_step", ctx.h, t.ctypes.data, _ua_array(xtilde, ua, n, "xtilde"),
+           _ua_array(fx, ua, n, "fx"), None if gx is None else _ua_array(gx, ua, n, "gx"),
+           _ua_array(xt, ua, n, "xt"))
+
+
+def update_particles_x_(particles, fields, ua, xt):
+    """update_particles_x!(particles, fields, ua, xt) = compute_rho_m6!(fields,
+    particles, xt, ua) (compute_rho_m6.jl:29-179): particles.x at t and
+    fields.rho."""
+    n = particles.nbpart
+    ctx = _ua_context(ua, particles, fields.mesh)
+    t = np.ascontiguousarray(particles.t, dtype=np.float64)
+    L.call("pic_ua_compute_rho", ctx.h, t.ctypes.data, _ua_array(xt, ua, n, "xt"))
+    fields.rho[...] = ctx.get_grid(L.PIC_GRID_RHO, ghost=True)
+    x = np.zeros((2, n), order="F")
+    L.call("pic_get_particles_xv", ctx.h, x.ctypes.data, None)
+    particles.x[...] = x
+
+
+def compute_v_(yt, particles, ua):
+    """compute_v!(yt, particles, ua) (ua_steps.jl:206-224): particles.v."""
+    n = particles.nbpart
+    ctx = _ua_context(ua, particles)
+    t = np.ascontiguousarray(particles.t, dtype=np.float64)
+    L.call("pic_ua_compute_v", ctx.h, t.ctypes.data, _ua_array(yt, ua, n, "yt"))
+    v = np.zeros((2, n), order="F")
+    L.call("pic_get_particles_xv", ctx.h, None, v.ctypes.data)
+    particles.v[...] = v
+
+
+class UASimulation:
+    """The UA loop of test/bupdate.jl on device-resident Particles: the
+    prologue (compute_rho_m6!, solve!, interpol_eb_m6!, :56-60) at
+    construction, then run(dt, nsteps) (pic_ua_run)."""
+
+    def __init__(self, mesh, particles, ntau, eps, device=0):
+        self.mesh, self.n = mesh, particles.nbpart
+        self.ctx = Context(mesh, capacity=self.n, device=device)
+        _set_xv(self.ctx, particles)
+        L.call("pic_compute_rho_m6_fields", self.ctx.h, 1)
+        en = C.c_double()
+        L.call("pic_solve_fields", self.ctx.h, C.byref(en))
+        L.call("pic_interpol_eb_m6", self.ctx.h)
+        L.call("pic_ua_init", self.ctx.h, int(ntau), float(eps))
+        self.energy0 = en.value
+
+    def run(self, dt, nsteps):
+        """Returns the solve! energies, (2, nsteps): prediction, correction."""
+        e = np.zeros((2, nsteps), order="F")
+        L.call("pic_ua_run", self.ctx.h, float(dt), int(nsteps), e.ctypes.data)
+        return e
+
+    def particles(self):
+        """(x, v), each 2 x n."""
+        x, v = np.zeros((2, self.n), order="F"), np.zeros((2, self.n), order="F")
+        L.call("pic_get_particles_xv", self.ctx.h, x.ctypes.data, v.ctypes.data)
+        return x, v
+
+    def fields_e(self):
+        """MeshFields2D.e (2, nx+1, ny+1) of the last solve."""
+        e = np.zeros((2, self.mesh.nx + 1, self.mesh.ny + 1), order="F")
+        e[0] = self.ctx.get_grid(L.PIC_GRID_EX, ghost=True)
+        e[1] = self.ctx.get_grid(L.PIC_GRID_EY, ghost=True)
+        return e
+
+    def close(self):
+        self.ctx.close()
 
 
 # ------------------------------------------------- device-resident run

@@ -409,6 +409,7 @@ int pic_ctx_destroy(pic_ctx *ctx) {
     dfree(c->slow_chunks); dfree(c->defer_count);
     dfree(c->seg_k); dfree(c->seg_tbl); dfree(c->seg_A); dfree(c->seg_M);
     dfree(c->fd_ex); dfree(c->fd_ey); dfree(c->fd_bz); dfree(c->jx); dfree(c->jy);
+    dfree(c->ua_b); dfree(c->ua_t); dfree(c->ua_x0); dfree(c->ua_y0); dfree(c->ua_fhat);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PIC_OK;
@@ -528,6 +529,22 @@ int pic_get_particle_fields(pic_ctx *ctx, double *e) {
         e[2 * k] = a[k];
         e[2 * k + 1] = b[k];
     }
+    return PIC_OK;
+}
+
+int pic_set_particle_fields(pic_ctx *ctx, const double *e) {
+    PIC_CHECK(ctx && e, PIC_ERR_ARG, "null pointer");
+    Ctx *c = ctx_of(ctx);
+    PIC_HIP(hipSetDevice(c->device));
+    PIC_TRY(ensure_pfields(c));
+    std::vector<double> a((size_t)c->n), b((size_t)c->n);
+    for (int64_t k = 0; k < c->n; ++k) {
+        a[k] = e[2 * k];
+        b[k] = e[2 * k + 1];
+    }
+    PIC_HIP(hipMemcpyAsync(c->epx, a.data(), (size_t)c->n * 8, hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(hipMemcpyAsync(c->epy, b.data(), (size_t)c->n * 8, hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
 }
 

@@ -52,13 +52,7 @@ __global__ void k_interpolate_cic(int64_t n, const double *x, const double *y, i
     epy[k] = a1 * ey[c00] + a2 * ey[c10] + a3 * ey[c11] + a4 * ey[c01];
 }
 
-// Julia mod(x, y) for Float64: fmod shifted into the sign of y.
-__device__ __forceinline__ double jl_fmod(double x, double y) {
-    double r = fmod(x, y);
-    if (r == 0.0) return copysign(r, y);
-    if ((r > 0) != (y > 0)) return r + y;
-    return r;
-}
+using picd::jl_fmod;
 
 // compute_rho_m6!(fields, particles): compute_rho_m6.jl:181-316, raw
 // accumulation into acc (nx*ny, the ghost copies are implied).  compat
@@ -111,21 +105,8 @@ __global__ void k_mf_gather_m6(int64_t n, double *x, double *y, int nx, int ny, 
     double dpx = px - (double)i, dpy = py - (double)j;
     x[k] = xmin + xn;
     y[k] = ymin + yn;
-    double wx[6], wy[6];
-    picd::m6_weights_d(dpx, wx);
-    picd::m6_weights_d(dpy, wy);
-    double e1 = 0.0, e2 = 0.0;
-    for (int a = 0; a < 6; ++a) {
-        int ia = picd::wrapi(i - 2 + a, nx);
-        double t1 = 0.0, t2 = 0.0;
-        for (int b = 0; b < 6; ++b) {
-            size_t g = ia + (size_t)nx * picd::wrapi(j - 2 + b, ny);
-            t1 = __builtin_fma(wy[b], ex[g], t1);
-            t2 = __builtin_fma(wy[b], ey[g], t2);
-        }
-        e1 = __builtin_fma(wx[a], t1, e1);
-        e2 = __builtin_fma(wx[a], t2, e2);
-    }
+    double e1, e2;
+    picd::mf_gather_m6(i, j, dpx, dpy, nx, ny, ex, ey, e1, e2);
     epx[k] = e1;
     epy[k] = e2;
 }

@@ -142,6 +142,15 @@ struct Ctx {
     double *fd_ex = nullptr, *fd_ey = nullptr, *fd_bz = nullptr;
     double *jx = nullptr, *jy = nullptr;
 
+    // UA (ua_type.jl): ntau, eps; per particle b, t (Particles.b, .t), the
+    // positions at the start of a UA step and, between its prediction and
+    // correction, the tau-spectra of fx and fy (ntau x 4 complex)
+    int ua_ntau = 0;
+    double ua_eps = 0.0;
+    int64_t ua_cap = 0;
+    double *ua_b = nullptr, *ua_t = nullptr, *ua_x0 = nullptr, *ua_y0 = nullptr;
+    double *ua_fhat = nullptr;
+
     // adjoint reverse sweep: steps whose positions are kept at once (0 =
     // the most the free device memory allows), and the last sweep's plan
     int adj_block = 0;
@@ -233,6 +242,14 @@ __host__ __device__ inline int wrap2(int i, int n) {
     if (i >= n) i -= n;
     if (i >= n) i -= n;
     return i;
+}
+
+// Julia mod(x, y) for Float64: fmod shifted into the sign of y.
+__device__ __forceinline__ double jl_fmod(double x, double y) {
+    double r = fmod(x, y);
+    if (r == 0.0) return copysign(r, y);
+    if ((r > 0) != (y > 0)) return r + y;
+    return r;
 }
 
 // Julia mod1(x, n) - 1, i.e. the 0-based periodic index of 1-based x.
@@ -361,6 +378,30 @@ __device__ __forceinline__ void ngp_weights(T d, int cell, T w[2]) {
     const bool up = d > T(0.5) || (d == T(0.5) && (cell & 1));
     w[0] = up ? T(0) : T(1);
     w[1] = up ? T(1) : T(0);
+}
+
+// The M6 gather of interpol_eb_m6! (interpolation_m6.jl:125-247, and its
+// UA form :3-123) at cell (i, j) with offsets (dpx, dpy): nodes i-2..i+3 and
+// j-2..j+3, wrapped; ex, ey are nx x ny.  Sums with FMAs, y inner.
+__device__ __forceinline__ void mf_gather_m6(int i, int j, double dpx, double dpy, int nx, int ny,
+                                             const double *ex, const double *ey, double &e1,
+                                             double &e2) {
+    double wx[6], wy[6];
+    m6_weights_d(dpx, wx);
+    m6_weights_d(dpy, wy);
+    e1 = 0.0;
+    e2 = 0.0;
+    for (int a = 0; a < 6; ++a) {
+        int ia = wrapi(i - 2 + a, nx);
+        double t1 = 0.0, t2 = 0.0;
+        for (int b = 0; b < 6; ++b) {
+            size_t g = ia + (size_t)nx * wrapi(j - 2 + b, ny);
+            t1 = __builtin_fma(wy[b], ex[g], t1);
+            t2 = __builtin_fma(wy[b], ey[g], t2);
+        }
+        e1 = __builtin_fma(wx[a], t1, e1);
+        e2 = __builtin_fma(wx[a], t2, e2);
+    }
 }
 
 }  // namespace picd
