@@ -8,6 +8,10 @@ quiet start.  A step is one iteration of the reference's time loop
 (examples/landau_damping_poisson_cic_m6.jl:70-83): Poisson solve, kick,
 drift, deposit, on device-resident particles.
 
+The UA workloads ua_bupdate and ua_large time the uniformly accurate loop
+of test/bupdate.jl (FP64, 128 x 64 cells, ntau 16, eps 0.1, dt pi/16, 8
+steps; 204,800 or 4,096,000 particles).
+
 The Vlasov-Maxwell workloads vm2d and vm2d_npm500 time the FDTD + CIC
 current loop of examples/vm2d.jl (FP64, 128 x 16 cells, 204,800 or
 1,024,000 particles, dt = 0.01, 1000 steps) the same way.
@@ -176,6 +180,15 @@ WORKLOADS = {
     "vm2d_npm500": dict(name="vm2d-fdtd-cic-128x16-1024000-dt0.01-1000steps", grid=(128, 16),
                         dimy=1.0, n=1_024_000, kernel="cic", alpha=0.1, k=0.5, dt=0.01,
                         steps=1000, warmup=20, sampler="quiet", sort=(0, 0), maxwell=True),
+    # the uniformly accurate (UA) loop of test/bupdate.jl: 128 x 64 cells on
+    # [0, 4pi] x [0, 2pi], ntau 16, eps 0.1, dt = pi/16 (8 steps to pi/2),
+    # particles from plasma(mesh, n)'s densities
+    "ua_bupdate": dict(name="ua-bupdate-m6-128x64-204800-ntau16-eps0.1-8steps", grid=(128, 64),
+                       n=204_800, ntau=16, eps=0.1, dt=math.pi / 16, steps=8, warmup=2,
+                       ua=True, sort=(0, 0)),
+    "ua_large": dict(name="ua-bupdate-m6-128x64-4096000-ntau16-eps0.1-8steps", grid=(128, 64),
+                     n=4_096_000, ntau=16, eps=0.1, dt=math.pi / 16, steps=8, warmup=2,
+                     ua=True, sort=(0, 0)),
     # adjoint sweeps: forward loop with checkpoints + reverse sweep, timed as
     # one call; value = particle-steps/s of the whole sweep
     "adjoint_cic": dict(name="adjoint-landau2d2v-cic-128x128-2e6-dt0.1-200steps-ckpt20",
@@ -438,6 +451,120 @@ def bench_maxwell(args, W, pic, entry, steps, warmup):
     print(json.dumps(out), flush=True)
 
 
+def plasma_sample(n, dimx, dimy, seed):
+    """Particles with plasma(mesh, n)'s densities (src/plasma.jl:3-58):
+    x, y by rejection from 1 + sin(y) + 0.05 cos(x/2), v by rejection from
+    the two Maxwellians at (+-2, 0) on [-5, 5]^2; numpy's generator."""
+    rng = np.random.default_rng(seed)
+    kx, alpha = 0.5, 0.05
+    out = np.zeros((2, 0))
+    while out.shape[1] < n:
+        m = 2 * n
+        xi, yi = rng.uniform(0, dimx, m), rng.uniform(0, dimy, m)
+        zi = (2.0 + alpha) * rng.uniform(size=m)
+        keep = 1.0 + np.sin(yi) + alpha * np.cos(kx * xi) >= zi
+        out = np.concatenate([out, np.stack([xi[keep], yi[keep]])], axis=1)
+    x = out[:, :n]
+    vel = np.zeros((2, 0))
+    while vel.shape[1] < n:
+        m = 4 * n
+        xi, yi = (rng.uniform(size=m) - 0.5) * 10, (rng.uniform(size=m) - 0.5) * 10
+        zi = rng.uniform(size=m)
+        tem = (np.exp(-((xi - 2) ** 2 + yi ** 2) / 2) + np.exp(-((xi + 2) ** 2 + yi ** 2) / 2)) / 2
+        keep = tem >= zi
+        vel = np.concatenate([vel, np.stack([xi[keep], yi[keep]])], axis=1)
+    return np.asfortranarray(x), np.asfortranarray(vel[:, :n])
+
+
+def bench_ua(args, W, pic, entry, steps, warmup):
+    """The UA workloads: UASimulation (prologue of test/bupdate.jl) then
+    pic_ua_run over `steps` iterations, timed as one call; the prediction and
+    correction kernels timed with HIP events in a second run."""
+    nx, ny = W["grid"]
+    n = W["n"] if args.particles is None else args.particles
+    ntau, eps, dt = W["ntau"], W["eps"], W["dt"]
+    dimx, dimy = 4 * math.pi, 2 * math.pi
+    mesh = pic.TwoDGrid(0.0, dimx, nx, 0.0, dimy, ny)
+    log(f"bench: {W['name']}: sampling {n} particles (plasma densities)")
+    x, v = plasma_sample(n, dimx, dimy, 1234)
+    p = pic.Particles(n, dimx * dimy / n)
+    p.x[...], p.v[...] = x, v
+
+    def fresh():
+        return pic.UASimulation(mesh, p, ntau, eps)
+
+    if warmup:
+        s0 = fresh()
+        s0.run(dt, warmup)
+        s0.close()
+    sim = fresh()
+    sim.ctx.sync()
+    log(f"bench: UA loop, {steps} steps")
+    t0 = time.perf_counter()
+    en = sim.run(dt, steps)
+    t1 = time.perf_counter()
+    sim.close()
+    sim = fresh()
+    sim.set_timing(True)
+    sim.run(dt, steps)
+    tm = sim.timing()
+    sim.close()
+    seconds = t1 - t0
+    # algorithmic bytes per particle: x, y, vx, vy and Particles.e read by both
+    # kernels; the prediction writes x0, y0, b, t, x, y and the tau-spectra of
+    # fx, fy (4 ntau complex), the correction reads those spectra and writes
+    # x, y, vx, vy
+    kb = {"predict": 48.0 + 48.0 + 64.0 * ntau, "correct": 48.0 + 32.0 + 64.0 * ntau}
+    kern = {}
+    for k, (ms, cnt) in tm.items():
+        if cnt:
+            avg = ms / cnt
+            gbs = kb[k] * n / (avg * 1e-3) / 1e9
+            kern[k] = {"avg_us": avg * 1e3, "launches": cnt, "bytes_per_particle": kb[k],
+                       "GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+    dom = max(kern, key=lambda k: kern[k]["avg_us"])
+    d = kern[dom]
+    out = {
+        "metric": "particle-steps/s of the UA PIC loop (test/bupdate.jl)",
+        "value": n * steps / seconds, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": seconds * 1e3 / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (plasma(mesh, n) densities by rejection, numpy seed 1234)",
+        "config": {"workload": W["name"], "grid": [nx, ny], "particles": n, "ntau": ntau,
+                   "eps": eps, "dt": dt, "kernel": "M6", "state": "fp64 soa",
+                   "parallelism": "single-gpu"},
+        "seconds_total": seconds,
+        "roofline": {"bound": "hbm", "achieved": d["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": d["frac"], "traffic": None,
+                     "kernel": f"k_ua_{dom}<{ntau}> (tau-FFTs across lanes, M6 gather, deposit)",
+                     "bytes_per_particle": d["bytes_per_particle"], "avg_us": d["avg_us"]},
+        "ua_kernels": kern,
+        "energy_last": [float(en[0, -1]), float(en[1, -1])],
+    }
+    if not args.no_cpu and args.cpu_steps > 0:
+        O = entry.load_oracle()
+        m = min(n, 20_000)
+        steps_cpu = 2
+        xs = np.asfortranarray(x[:, :m].copy())
+        vs = np.asfortranarray(v[:, :m].copy())
+        w = dimx * dimy / m
+        rho = O.compute_rho_m6_fields(xs, w, nx, ny, 0.0, dimx, 0.0, dimy, compat=True)
+        e_f, _ = O.solve_fields(rho, 0.0, dimx, 0.0, dimy)
+        eo = O.interpol_eb_m6(xs, e_f, nx, ny, 0.0, dimx, 0.0, dimy)
+        log(f"bench: cpu baseline, oracle UA loop, {m} particles, {steps_cpu} steps")
+        c0 = time.perf_counter()
+        O.ua_run(ntau, eps, dt, steps_cpu, xs, vs, eo, w, e_f, 0.0, dimx, 0.0, dimy)
+        c1 = time.perf_counter()
+        cores, model = cpu_info()
+        out["cpu_baseline"] = {
+            "value": m * steps_cpu / (c1 - c0), "unit": "particle-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"first {m} particles of the same workload (weight dimx dimy / {m}), "
+                      f"{steps_cpu} steps of the UA loop (oracle C restatement, serial), "
+                      f"{model}, nproc={cores}"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -473,6 +600,10 @@ def main():
     from picmi355x import diagnostics
 
     W = WORKLOADS[args.workload]
+    if W.get("ua"):
+        bench_ua(args, W, pic, entry, W["steps"] if args.steps is None else args.steps,
+                 W["warmup"] if args.warmup is None else args.warmup)
+        return
     if W.get("maxwell"):
         bench_maxwell(args, W, pic, entry, W["steps"] if args.steps is None else args.steps,
                       W["warmup"] if args.warmup is None else args.warmup)

@@ -417,9 +417,44 @@ int pic_ua_compute_v(pic_ctx *ctx, const double *t, const double *yt);
  * the step's start and the tau-spectra of fx, fy (64 ntau + 32 bytes). */
 int pic_ua_run(pic_ctx *ctx, double dt, int nsteps, double *energy);
 
+/* Summed durations (ms, HIP events) and launch counts of pic_ua_run's two
+ * particle kernels, [0] prediction and [1] correction, since timing was
+ * enabled (pic_set_timing); either pointer may be NULL.  No reference
+ * counterpart. */
+int pic_get_ua_timing(pic_ctx *ctx, double *ms, int64_t *launches);
+
 /* Particles.e (2 x n) from the host; the counterpart of
  * pic_get_particle_fields. */
 int pic_set_particle_fields(pic_ctx *ctx, const double *e);
+
+/* ------------------------------------- initial conditions and files */
+
+/* sample!(pg, mesh, LandauDamping(alpha, kx)): landau_damping.jl:77-97, the
+ * quiet start, made on the device: n particles with x = newton(r1) (:19-28),
+ * y = ymin + r2 dimy, |v| = sqrt(-2 log((i - 1/2)/n)), theta = 2 pi r3,
+ * w = dimx dimy / n, where (r1, r2, r3) is the i-th point (1-based) of the
+ * unscrambled 3-dimensional Sobol sequence (Joe-Kuo direction numbers, 30
+ * bits; the all-zero point 0 is not used).  The mesh must have dimx =
+ * 2 pi / kx (:81).  The particles replace the context's, in `state`
+ * (PIC_STATE_F64 or PIC_STATE_F32). */
+int pic_sample_landau(pic_ctx *ctx, int64_t n, double alpha, double kx, int state);
+
+/* read_particles(filename, mesh): read_particles.jl:3-35.  One particle per
+ * line, "ix iy dpx dpy vx vy" (cell indices, offsets in the cell,
+ * velocity); blank lines are skipped.  In PIC_STATE_F64 the positions are
+ * (dpx + ix) dx, (dpy + iy) dy as in the reference; PIC_STATE_F32 stores the
+ * cell ix + nx iy and the offsets rounded to FP32 (cells must lie in the
+ * mesh and offsets in [0, 1); an offset that rounds up to 1 moves to the
+ * next cell with offset 0).  w = dimx dimy / n.  *n_out (may be NULL)
+ * receives n. */
+int pic_read_particles(pic_ctx *ctx, const char *path, int state, int64_t *n_out);
+
+/* Writes the context's particles in the same format: from the FP32 state
+ * the cells and offsets as stored (9 significant digits: read back into the
+ * FP32 state they are bit-identical); from the FP64 state ix = floor(x/dx),
+ * dpx = x/dx - ix with 17 digits.  No reference counterpart (the reference
+ * only reads). */
+int pic_write_particles(pic_ctx *ctx, const char *path);
 
 /* ------------------------------------------ Fortran-twin drop-ins */
 

@@ -23,6 +23,8 @@ argument order and array layouts (a Julia `f!` is written `f_` here):
     compute_current_(jx, jy, mesh, CloudInCell(), p)   cloud_in_cell.jl:10
     UA(ntau, eps, n), preparation_, update_particles_e_, compute_f_, ua_step_,
     update_particles_x_, compute_v_             ua_type.jl, ua_steps.jl
+    read_particles(filename, mesh)           read_particles.jl:3
+    Simulation.landau(mesh, n, alpha, kx, kernel)   landau_damping.jl:77 (on the GPU)
 
 Arrays are numpy float64 in Fortran order, so the memory layout is the Julia
 one.  Each call copies its inputs to the GPU and its outputs back (the
@@ -31,6 +33,7 @@ fields resident for the time loop.
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
@@ -523,6 +526,36 @@ def compute_energy(fdtd, mesh):
     return e.value
 
 
+# ---------------------------------------------------------- particle files
+
+def read_particles(filename, mesh):
+    """read_particles(filename, mesh) (read_particles.jl:3-35): Particles
+    with x = (dpx + ix) dx, (dpy + iy) dy, v and w = dimx dimy / n."""
+    ctx = Context(mesh)
+    try:
+        n = C.c_int64()
+        L.call("pic_read_particles", ctx.h, os.fsencode(filename), L.PIC_STATE_F64, C.byref(n))
+        p = Particles(n.value, (mesh.dimx * mesh.dimy) / n.value if n.value else 0.0)
+        if n.value:
+            x = np.zeros((2, n.value), order="F")
+            v = np.zeros((2, n.value), order="F")
+            L.call("pic_get_particles_xv", ctx.h, x.ctypes.data, v.ctypes.data)
+            p.x[...], p.v[...] = x, v
+        return p
+    finally:
+        ctx.close()
+
+
+def write_particles(filename, particles, mesh):
+    """Writes Particles in read_particles' format (no reference counterpart)."""
+    ctx = Context(mesh, capacity=particles.nbpart)
+    try:
+        _set_xv(ctx, particles)
+        L.call("pic_write_particles", ctx.h, os.fsencode(filename))
+    finally:
+        ctx.close()
+
+
 # ------------------------------------------------------------------- UA
 
 class UA:
@@ -649,6 +682,17 @@ class UASimulation:
         L.call("pic_ua_run", self.ctx.h, float(dt), int(nsteps), e.ctypes.data)
         return e
 
+    def set_timing(self, on=True):
+        L.call("pic_set_timing", self.ctx.h, 1 if on else 0)
+
+    def timing(self):
+        """{kernel: (total_ms, launches)} of the prediction and correction
+        kernels since timing was enabled (pic_get_ua_timing)."""
+        ms = (C.c_double * 2)()
+        cnt = (C.c_int64 * 2)()
+        L.call("pic_get_ua_timing", self.ctx.h, ms, cnt)
+        return {"predict": (ms[0], cnt[0]), "correct": (ms[1], cnt[1])}
+
     def particles(self):
         """(x, v), each 2 x n."""
         x, v = np.zeros((2, self.n), order="F"), np.zeros((2, self.n), order="F")
@@ -675,6 +719,11 @@ class Simulation:
     def __init__(self, mesh, p, kernel, state=L.PIC_STATE_F64, bz=None, device=0):
         self.mesh = mesh
         self.kernel = _kcode(kernel)
+        if p is None:  # the classmethods below fill the particles
+            self.ctx = Context(mesh, capacity=0, device=device)
+            self.ctx.set_bz(L.PIC_BZ_NONE)
+            self.n = 0
+            return
         arr = p.array if isinstance(p, ParticleGroup) else p
         n = arr.shape[0] if _is_device_array(arr) else arr.shape[1]
         self.ctx = Context(mesh, capacity=n, device=device)
@@ -688,6 +737,29 @@ class Simulation:
             self.ctx.set_bz(L.PIC_BZ_GRID)
         self.ctx.set_state(state)
         self.n = n
+
+    @classmethod
+    def landau(cls, mesh, n, alpha, kx, kernel, state=L.PIC_STATE_F64, device=0):
+        """The quiet start of sample!(pg, mesh, LandauDamping(alpha, kx))
+        (landau_damping.jl:77-97) made on the device (pic_sample_landau)."""
+        sim = cls(mesh, None, kernel, device=device)
+        L.call("pic_sample_landau", sim.ctx.h, int(n), float(alpha), float(kx), int(state))
+        sim.n = int(n)
+        return sim
+
+    @classmethod
+    def from_file(cls, mesh, path, kernel, state=L.PIC_STATE_F64, device=0):
+        """Particles of a read_particles file (read_particles.jl:3-35),
+        parsed by the library straight into `state` (pic_read_particles)."""
+        sim = cls(mesh, None, kernel, device=device)
+        n = C.c_int64()
+        L.call("pic_read_particles", sim.ctx.h, os.fsencode(path), int(state), C.byref(n))
+        sim.n = n.value
+        return sim
+
+    def write_particles(self, path):
+        """The particles in read_particles' format (pic_write_particles)."""
+        L.call("pic_write_particles", self.ctx.h, os.fsencode(path))
 
     def sort(self, lanes=False):
         """Stable sort by cell (pic_sort), or that followed by the lane-spread

@@ -160,6 +160,9 @@ struct Ctx {
     // forward passes, gather-adjoint, deposit-adjoint and recompute passes
     double t_adj_ms[4] = {0.0, 0.0, 0.0, 0.0};
     int64_t t_adj_n[4] = {0, 0, 0, 0};
+    // UA loop kernels: 0 prediction, 1 correction
+    double t_ua_ms[2] = {0.0, 0.0};
+    int64_t t_ua_n[2] = {0, 0};
 
     // pic_run kernel timing (HIP events on the context stream)
     bool timing = false;
@@ -250,6 +253,19 @@ __device__ __forceinline__ double jl_fmod(double x, double y) {
     if (r == 0.0) return copysign(r, y);
     if ((r > 0) != (y > 0)) return r + y;
     return r;
+}
+
+// jl_fmod(x, y) for y > 0, with the common cases -y <= x < 2y done without
+// fmod: there fmod(x, y) is x, x or x - y (exact by Sterbenz's lemma), so the
+// result is the same bits.
+__device__ __forceinline__ double jl_mod_pos(double x, double y) {
+    if (x > 0.0 && x < y) return x;
+    if (x < 0.0 && x >= -y) return x + y;
+    if (x >= y && x < y + y) {
+        const double r = x - y;
+        return r == 0.0 ? 0.0 : r;
+    }
+    return jl_fmod(x, y);
 }
 
 // Julia mod1(x, n) - 1, i.e. the 0-based periodic index of 1-based x.
@@ -382,7 +398,8 @@ __device__ __forceinline__ void ngp_weights(T d, int cell, T w[2]) {
 
 // The M6 gather of interpol_eb_m6! (interpolation_m6.jl:125-247, and its
 // UA form :3-123) at cell (i, j) with offsets (dpx, dpy): nodes i-2..i+3 and
-// j-2..j+3, wrapped; ex, ey are nx x ny.  Sums with FMAs, y inner.
+// j-2..j+3, wrapped; ex, ey are nx x ny; i in [0, nx], j in [0, ny] (cells of
+// a coordinate brought into the domain).  Sums with FMAs, y inner.
 __device__ __forceinline__ void mf_gather_m6(int i, int j, double dpx, double dpy, int nx, int ny,
                                              const double *ex, const double *ey, double &e1,
                                              double &e2) {
@@ -392,10 +409,10 @@ __device__ __forceinline__ void mf_gather_m6(int i, int j, double dpx, double dp
     e1 = 0.0;
     e2 = 0.0;
     for (int a = 0; a < 6; ++a) {
-        int ia = wrapi(i - 2 + a, nx);
+        int ia = wrap2(i - 2 + a, nx);
         double t1 = 0.0, t2 = 0.0;
         for (int b = 0; b < 6; ++b) {
-            size_t g = ia + (size_t)nx * wrapi(j - 2 + b, ny);
+            size_t g = ia + (size_t)nx * wrap2(j - 2 + b, ny);
             t1 = __builtin_fma(wy[b], ex[g], t1);
             t2 = __builtin_fma(wy[b], ey[g], t2);
         }

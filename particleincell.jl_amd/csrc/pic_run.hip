@@ -128,6 +128,10 @@ extern "C" int pic_set_timing(pic_ctx *ctx, int enable) {
         c->t_adj_ms[k] = 0.0;
         c->t_adj_n[k] = 0;
     }
+    for (int k = 0; k < 2; ++k) {
+        c->t_ua_ms[k] = 0.0;
+        c->t_ua_n[k] = 0;
+    }
     return PIC_OK;
 }
 

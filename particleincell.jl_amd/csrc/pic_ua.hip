@@ -50,21 +50,34 @@ __device__ __forceinline__ int bitrev(int m) {
     return r;
 }
 
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+// This lane's twiddle of every FFT stage: stage h = NT/2, NT/4, ..., 1 uses
+// cos(pi j / h) and sin(pi j / h), j = m mod h (computed once per kernel).
+template <int NT>
+struct Twiddles {
+    double c[ilog2(NT)], s[ilog2(NT)];
+    __device__ explicit Twiddles(int m) {
+        int q = 0;
+#pragma unroll
+        for (int h = NT / 2; h >= 1; h >>= 1, ++q)
+            sincospi((double)(m & (h - 1)) / (double)h, &s[q], &c[q]);
+    }
+};
+
 // Radix-2 decimation-in-frequency FFT over the NT lanes of a group (lane m
 // holds element m in and out): forward exp(-2 pi i jk / NT), or the inverse
 // with exp(+...) scaled by 1/NT.
 template <int NT, bool INV>
-__device__ __forceinline__ cd fft_lanes(cd z, int m, int base) {
+__device__ __forceinline__ cd fft_lanes(cd z, int m, int base, const Twiddles<NT> &w) {
+    int q = 0;
 #pragma unroll
-    for (int h = NT / 2; h >= 1; h >>= 1) {
+    for (int h = NT / 2; h >= 1; h >>= 1, ++q) {
+        // lower lane: z + p; upper lane: (p - z) w, without a branch
         const cd p = shfl_xor(z, h);
-        if (m & h) {
-            double s, c;
-            sincospi((double)(m & (h - 1)) / (double)h, &s, &c);
-            z = (p - z) * mk(c, INV ? s : -s);
-        } else {
-            z = z + p;
-        }
+        const bool up = (m & h) != 0;
+        const cd d = up ? p - z : z + p;
+        z = up ? d * mk(w.c[q], INV ? w.s[q] : -w.s[q]) : d;
     }
     z = shfl(z, base + bitrev<NT>(m));
     if (INV) z = mk(z.re * (1.0 / NT), z.im * (1.0 / NT));
@@ -79,37 +92,37 @@ __device__ __forceinline__ cd lane_sum(cd z) {
     return z;
 }
 
-// Per-lane constants of UA(ntau, eps): tau = m dtau, ltau (ua_type.jl:18-24).
+// Per-lane constants of UA(ntau, eps): tau = m dtau, ltau (ua_type.jl:18-24),
+// and the lane's FFT twiddles.
+template <int NT>
 struct UaLane {
     int m;
     double l, c, s;  // ltau[m], cos(tau[m]), sin(tau[m])
+    Twiddles<NT> w;
+    __device__ explicit UaLane(int m_) : m(m_), w(m_) {
+        l = m < NT / 2 ? (double)m : (double)(m - NT);
+        const double tau = (double)m * (2 * M_PI / NT);
+        sincos(tau, &s, &c);
+    }
 };
-
-template <int NT>
-__device__ __forceinline__ UaLane ua_lane(int m) {
-    UaLane L;
-    L.m = m;
-    L.l = m < NT / 2 ? (double)m : (double)(m - NT);
-    const double tau = (double)m * (2 * M_PI / NT);
-    L.c = cos(tau);
-    L.s = sin(tau);
-    return L;
-}
 
 // exp(sg 1im l t / eps) as Julia evaluates it: cis(((sg l) t) / eps).
 __device__ __forceinline__ cd ua_cis(int sg, double l, double t, double eps) {
     const double th = (((double)sg * l) * t) / eps;
-    return mk(cos(th), sin(th));
+    double s, c;
+    sincos(th, &s, &c);
+    return mk(c, s);
 }
 
-// pl[m], ql[m] of preparation! (ua_steps.jl:26-33).
-__device__ __forceinline__ void ua_plql(const UaLane &L, double t, double eps, cd &pl, cd &ql) {
+// pl[m], ql[m] of preparation! (ua_steps.jl:26-33); elt = exp(-1im l t / eps).
+template <int NT>
+__device__ __forceinline__ void ua_plql(const UaLane<NT> &L, double t, double eps, cd elt, cd &pl,
+                                        cd &ql) {
     if (L.m == 0) {
         pl = mk(t);
         ql = mk(t * t / 2);
         return;
     }
-    const cd elt = ua_cis(-1, L.l, t, eps);
     pl = (mk(0.0, eps) * (elt - mk(1.0))) / L.l;                         // eps*1im*(elt-1)/l
     ql = (eps * (eps * (mk(1.0) - elt) - mk(0.0 * t, L.l * t))) / (L.l * L.l);  // eps*(eps*(1-elt)-1im*l*t)/l^2
 }
@@ -117,9 +130,10 @@ __device__ __forceinline__ void ua_plql(const UaLane &L, double t, double eps, c
 // preparation! for one particle (ua_steps.jl:3-83): b, t, xt (real) and yt
 // at this lane's tau.
 template <int NT>
-__device__ __forceinline__ void ua_prep(const UaLane &L, int base, double x1, double x2,
+__device__ __forceinline__ void ua_prep(const UaLane<NT> &L, int base, double x1, double x2,
                                         double vx, double vy, double ex, double ey, double dt,
-                                        double eps, double &b, double &t, double xt[2], cd yt[2]) {
+                                        double eps, double &b, double &t, double xt[2], cd yt[2],
+                                        double &interv_out) {
     b = 1 + 0.5 * sin(x1) * sin(x2);
     t = dt * b;
     const double vxb = vx / b, vyb = vy / b;
@@ -129,27 +143,34 @@ __device__ __forceinline__ void ua_prep(const UaLane &L, int base, double x1, do
     xt[0] = x1 + h1 + eps * vyb;
     xt[1] = x2 + h2 - eps * vxb;
     const double interv = (1 + 0.5 * sin(xt[0]) * sin(xt[1]) - b) / eps;
+    interv_out = interv;
     const double exb = ((c * vy - s * vx) * interv + ex) / b;
     const double eyb = ((-c * vx - s * vy) * interv + ey) / b;
     cd r[2] = {mk(c * exb - s * eyb), mk(s * exb + c * eyb)};
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        cd R = fft_lanes<NT, false>(r[q], L.m, base);
+        cd R = fft_lanes<NT, false>(r[q], L.m, base, L.w);
         if (L.m != 0) R = mk(R.im / L.l, -R.re / L.l);  // -1im * R / l
-        r[q] = fft_lanes<NT, true>(R, L.m, base);
+        r[q] = fft_lanes<NT, true>(R, L.m, base, L.w);
     }
     const cd r10 = shfl(r[0], base), r20 = shfl(r[1], base);
     yt[0] = vx + eps * (r[0] - r10);
     yt[1] = vy + eps * (r[1] - r20);
 }
 
-// compute_f! at this lane's tau (ua_steps.jl:117-145), before its FFTs.
-__device__ __forceinline__ void ua_f(const UaLane &L, double b, double eps, const double xt[2],
+// (1 + sin(x1) sin(x2)/2 - b)/eps at the lane's position (ua_steps.jl:55, :130).
+__device__ __forceinline__ double ua_interv(const double xt[2], double b, double eps) {
+    return (1 + 0.5 * sin(xt[0]) * sin(xt[1]) - b) / eps;
+}
+
+// compute_f! at this lane's tau (ua_steps.jl:117-145), before its FFTs;
+// interv from ua_interv at xt.
+template <int NT>
+__device__ __forceinline__ void ua_f(const UaLane<NT> &L, double b, double interv,
                                      const cd yt[2], const double et[2], cd fx[2], cd fy[2]) {
     const double c = L.c, s = L.s;
     fx[0] = (c * yt[0] + s * yt[1]) / b;
     fx[1] = (-s * yt[0] + c * yt[1]) / b;
-    const double interv = (1 + 0.5 * sin(xt[0]) * sin(xt[1]) - b) / eps;
     const cd tmp1 = et[0] + interv * (c * yt[1] - s * yt[0]);
     const cd tmp2 = et[1] + interv * (-c * yt[0] - s * yt[1]);
     fy[0] = (c * tmp1 - s * tmp2) / b;
@@ -161,7 +182,8 @@ __device__ __forceinline__ void ua_gather(const double xt[2], int nx, int ny, do
                                           double ymin, double dimx, double dimy, double dx,
                                           double dy, const double *ex, const double *ey,
                                           double et[2]) {
-    const double xn = picd::jl_fmod(xt[0] - xmin, dimx), yn = picd::jl_fmod(xt[1] - ymin, dimy);
+    const double xn = picd::jl_mod_pos(xt[0] - xmin, dimx);
+    const double yn = picd::jl_mod_pos(xt[1] - ymin, dimy);
     const double px = xn / dx, py = yn / dy;
     const int i = (int)px, j = (int)py;
     picd::mf_gather_m6(i, j, px - (double)i, py - (double)j, nx, ny, ex, ey, et[0], et[1]);
@@ -178,17 +200,19 @@ struct UaGrid {
 // update_particles_x! for one particle (compute_rho_m6.jl:37-164): x(t) from
 // its tau-samples, into the domain, stored; the M6 deposit (nodes i-2..i+3,
 // wrapped, as in :70-81) spread over the group's lanes.
+// eplus = exp(+1im l t / eps), the conjugate of elt (cos is even and sin odd
+// in the device library, and the two angles differ only in sign).
 template <int NT>
-__device__ __forceinline__ void ua_reconstruct_deposit(const UaLane &L, int base, double t,
-                                                       double eps, const cd xt[2],
-                                                       const UaGrid &g, bool live, double *xo,
-                                                       double *yo) {
+__device__ __forceinline__ void ua_reconstruct_deposit(const UaLane<NT> &L, int base, cd eplus,
+                                                       const cd xt[2], const UaGrid &g, bool live,
+                                                       double *xo, double *yo) {
     double pos[2];
-    const cd e = ua_cis(+1, L.l, t, eps) / (double)NT;
+    const cd e = eplus / (double)NT;
 #pragma unroll
-    for (int q = 0; q < 2; ++q) pos[q] = lane_sum<NT>(fft_lanes<NT, false>(xt[q], L.m, base) * e).re;
-    const double xt1 = picd::jl_fmod(pos[0] - g.xmin, g.dimx);
-    const double xt2 = picd::jl_fmod(pos[1] - g.ymin, g.dimy);
+    for (int q = 0; q < 2; ++q)
+        pos[q] = lane_sum<NT>(fft_lanes<NT, false>(xt[q], L.m, base, L.w) * e).re;
+    const double xt1 = picd::jl_mod_pos(pos[0] - g.xmin, g.dimx);
+    const double xt2 = picd::jl_mod_pos(pos[1] - g.ymin, g.dimy);
     if (!live) return;
     if (L.m == 0) {
         *xo = xt1 + g.xmin;
@@ -201,16 +225,16 @@ __device__ __forceinline__ void ua_reconstruct_deposit(const UaLane &L, int base
     picd::m6_weights_d(py - (double)j, wy);
     for (int q = L.m; q < 36; q += NT) {
         const int a = q / 6, bb = q % 6;
-        const int ii = picd::wrapi(i - 2 + a, g.nx), jj = picd::wrapi(j - 2 + bb, g.ny);
+        const int ii = picd::wrap2(i - 2 + a, g.nx), jj = picd::wrap2(j - 2 + bb, g.ny);
         unsafeAtomicAdd(&g.rho_acc[ii + (size_t)g.nx * jj], wx[a] * wy[bb] * g.w);
     }
 }
 
 // compute_v! for one particle (ua_steps.jl:206-224) from the tau-spectrum yt.
 template <int NT>
-__device__ __forceinline__ void ua_v(const UaLane &L, double t, double eps, const cd yt[2],
-                                     double &v1, double &v2) {
-    const cd e = ua_cis(+1, L.l, t, eps);
+__device__ __forceinline__ void ua_v(const UaLane<NT> &L, double t, double eps, cd eplus,
+                                     const cd yt[2], double &v1, double &v2) {
+    const cd e = eplus;
     const cd px = lane_sum<NT>((yt[0] / (double)NT) * e);
     const cd py = lane_sum<NT>((yt[1] / (double)NT) * e);
     const double ct = cos(t / eps), st = sin(t / eps);
@@ -263,28 +287,28 @@ struct UaArgs {
 template <int NT>
 __global__ __launch_bounds__(UB) void k_ua_predict(UaArgs a) {
     const Slot<NT> S(a.n);
-    const UaLane L = ua_lane<NT>(S.m);
+    const UaLane<NT> L(S.m);
     const int64_t k = S.k;
     const double x1 = a.x[k], x2 = a.y[k];
-    double b, t, xt[2], et[2];
+    double b, t, xt[2], et[2], interv;
     cd yt[2], fx[2], fy[2];
     ua_prep<NT>(L, S.base, x1, x2, a.vx[k], a.vy[k], a.epx[k], a.epy[k], a.dt, a.eps, b, t, xt,
-                yt);
+                yt, interv);
     ua_gather(xt, a.g.nx, a.g.ny, a.g.xmin, a.g.ymin, a.g.dimx, a.g.dimy, a.g.dx, a.g.dy, a.g.ex,
               a.g.ey, et);
-    ua_f(L, b, a.eps, xt, yt, et, fx, fy);
-    cd pl, ql;
-    ua_plql(L, t, a.eps, pl, ql);
+    ua_f<NT>(L, b, interv, yt, et, fx, fy);
     const cd elt = ua_cis(-1, L.l, t, a.eps);
+    cd pl, ql;
+    ua_plql<NT>(L, t, a.eps, elt, pl, ql);
     cd xn[2], yn[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        fx[q] = fft_lanes<NT, false>(fx[q], L.m, S.base);
-        fy[q] = fft_lanes<NT, false>(fy[q], L.m, S.base);
-        const cd xs = fft_lanes<NT, false>(mk(xt[q]), L.m, S.base);
-        const cd ys = fft_lanes<NT, false>(yt[q], L.m, S.base);
-        xn[q] = fft_lanes<NT, true>(elt * xs + pl * fx[q], L.m, S.base);
-        yn[q] = fft_lanes<NT, true>(elt * ys + pl * fy[q], L.m, S.base);
+        fx[q] = fft_lanes<NT, false>(fx[q], L.m, S.base, L.w);
+        fy[q] = fft_lanes<NT, false>(fy[q], L.m, S.base, L.w);
+        const cd xs = fft_lanes<NT, false>(mk(xt[q]), L.m, S.base, L.w);
+        const cd ys = fft_lanes<NT, false>(yt[q], L.m, S.base, L.w);
+        xn[q] = fft_lanes<NT, true>(elt * xs + pl * fx[q], L.m, S.base, L.w);
+        yn[q] = fft_lanes<NT, true>(elt * ys + pl * fy[q], L.m, S.base, L.w);
     }
     if (S.live) {
         if (S.m == 0) {
@@ -299,7 +323,8 @@ __global__ __launch_bounds__(UB) void k_ua_predict(UaArgs a) {
             a.fhat[uf(S.m, 2 + q, k, NT)] = fy[q];
         }
     }
-    ua_reconstruct_deposit<NT>(L, S.base, t, a.eps, xn, a.g, S.live, &a.x[k], &a.y[k]);
+    ua_reconstruct_deposit<NT>(L, S.base, mk(elt.re, -elt.im), xn, a.g, S.live, &a.x[k],
+                               &a.y[k]);
 }
 
 // Correction of test/bupdate.jl:90-114: the predicted xt, yt rebuilt from the
@@ -308,44 +333,45 @@ __global__ __launch_bounds__(UB) void k_ua_predict(UaArgs a) {
 template <int NT>
 __global__ __launch_bounds__(UB) void k_ua_correct(UaArgs a) {
     const Slot<NT> S(a.n);
-    const UaLane L = ua_lane<NT>(S.m);
+    const UaLane<NT> L(S.m);
     const int64_t k = S.k;
-    double b, t, xt[2], et[2];
+    double b, t, xt[2], et[2], interv;
     cd yt[2];
     ua_prep<NT>(L, S.base, a.x0[k], a.y0[k], a.vx[k], a.vy[k], a.epx[k], a.epy[k], a.dt, a.eps,
-                b, t, xt, yt);
-    cd pl, ql;
-    ua_plql(L, t, a.eps, pl, ql);
+                b, t, xt, yt, interv);
     const cd elt = ua_cis(-1, L.l, t, a.eps);
+    cd pl, ql;
+    ua_plql<NT>(L, t, a.eps, elt, pl, ql);
     cd fx[2], fy[2], xs[2], ys[2], xp[2], yp[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         fx[q] = a.fhat[uf(S.m, q, k, NT)];
         fy[q] = a.fhat[uf(S.m, 2 + q, k, NT)];
-        xs[q] = fft_lanes<NT, false>(mk(xt[q]), L.m, S.base);
-        ys[q] = fft_lanes<NT, false>(yt[q], L.m, S.base);
-        xp[q] = fft_lanes<NT, true>(elt * xs[q] + pl * fx[q], L.m, S.base);
-        yp[q] = fft_lanes<NT, true>(elt * ys[q] + pl * fy[q], L.m, S.base);
+        xs[q] = fft_lanes<NT, false>(mk(xt[q]), L.m, S.base, L.w);
+        ys[q] = fft_lanes<NT, false>(yt[q], L.m, S.base, L.w);
+        xp[q] = fft_lanes<NT, true>(elt * xs[q] + pl * fx[q], L.m, S.base, L.w);
+        yp[q] = fft_lanes<NT, true>(elt * ys[q] + pl * fy[q], L.m, S.base, L.w);
     }
     const double xr[2] = {xp[0].re, xp[1].re};
     ua_gather(xr, a.g.nx, a.g.ny, a.g.xmin, a.g.ymin, a.g.dimx, a.g.dimy, a.g.dx, a.g.dy, a.g.ex,
               a.g.ey, et);
     cd gx[2], gy[2];
-    ua_f(L, b, a.eps, xr, yp, et, gx, gy);
+    ua_f<NT>(L, b, ua_interv(xr, b, a.eps), yp, et, gx, gy);
     cd xn[2], yn[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        gx[q] = fft_lanes<NT, false>(gx[q], L.m, S.base);
-        gy[q] = fft_lanes<NT, false>(gy[q], L.m, S.base);
+        gx[q] = fft_lanes<NT, false>(gx[q], L.m, S.base, L.w);
+        gy[q] = fft_lanes<NT, false>(gy[q], L.m, S.base, L.w);
         xn[q] = elt * xs[q] + pl * fx[q];
         xn[q] = xn[q] + (ql * (gx[q] - fx[q])) / t;
-        xn[q] = fft_lanes<NT, true>(xn[q], L.m, S.base);
+        xn[q] = fft_lanes<NT, true>(xn[q], L.m, S.base, L.w);
         yn[q] = elt * ys[q] + pl * fy[q];
         yn[q] = yn[q] + (ql * (gy[q] - fy[q])) / t;
     }
-    ua_reconstruct_deposit<NT>(L, S.base, t, a.eps, xn, a.g, S.live, &a.x[k], &a.y[k]);
+    const cd eplus = mk(elt.re, -elt.im);
+    ua_reconstruct_deposit<NT>(L, S.base, eplus, xn, a.g, S.live, &a.x[k], &a.y[k]);
     double v1, v2;
-    ua_v<NT>(L, t, a.eps, yn, v1, v2);
+    ua_v<NT>(L, t, a.eps, eplus, yn, v1, v2);
     if (S.live && S.m == 0) {
         a.vx[k] = v1;
         a.vy[k] = v2;
@@ -358,14 +384,14 @@ __global__ __launch_bounds__(UB) void k_ua_correct(UaArgs a) {
 template <int NT>
 __global__ __launch_bounds__(UB) void k_ua_prep_call(UaArgs a, cd *pl, cd *ql, cd *xt, cd *yt) {
     const Slot<NT> S(a.n);
-    const UaLane L = ua_lane<NT>(S.m);
+    const UaLane<NT> L(S.m);
     const int64_t k = S.k;
-    double b, t, x[2];
+    double b, t, x[2], interv;
     cd y[2];
     ua_prep<NT>(L, S.base, a.x[k], a.y[k], a.vx[k], a.vy[k], a.epx[k], a.epy[k], a.dt, a.eps, b,
-                t, x, y);
+                t, x, y, interv);
     cd p, q;
-    ua_plql(L, t, a.eps, p, q);
+    ua_plql<NT>(L, t, a.eps, ua_cis(-1, L.l, t, a.eps), p, q);
     if (!S.live) return;
     if (S.m == 0) {
         a.b[k] = b;
@@ -398,16 +424,17 @@ template <int NT>
 __global__ __launch_bounds__(UB) void k_ua_f_call(UaArgs a, const cd *xt, const cd *yt,
                                                   const double *et, cd *fx, cd *fy) {
     const Slot<NT> S(a.n);
-    const UaLane L = ua_lane<NT>(S.m);
+    const UaLane<NT> L(S.m);
     const int64_t k = S.k;
     const double x[2] = {xt[ux(S.m, 0, k, NT)].re, xt[ux(S.m, 1, k, NT)].re};
     const cd y[2] = {yt[ux(S.m, 0, k, NT)], yt[ux(S.m, 1, k, NT)]};
     const double e[2] = {et[ux(S.m, 0, k, NT)], et[ux(S.m, 1, k, NT)]};
+    const double b = a.b[k];
     cd f[2], h[2];
-    ua_f(L, a.b[k], a.eps, x, y, e, f, h);
+    ua_f<NT>(L, b, ua_interv(x, b, a.eps), y, e, f, h);
     for (int c = 0; c < 2; ++c) {
-        f[c] = fft_lanes<NT, false>(f[c], L.m, S.base);
-        h[c] = fft_lanes<NT, false>(h[c], L.m, S.base);
+        f[c] = fft_lanes<NT, false>(f[c], L.m, S.base, L.w);
+        h[c] = fft_lanes<NT, false>(h[c], L.m, S.base, L.w);
     }
     if (!S.live) return;
     for (int c = 0; c < 2; ++c) {
@@ -421,12 +448,12 @@ template <int NT>
 __global__ __launch_bounds__(UB) void k_ua_step_call(UaArgs a, const cd *xtil, const cd *fx,
                                                      const cd *gx, cd *xt) {
     const Slot<NT> S(a.n);
-    const UaLane L = ua_lane<NT>(S.m);
+    const UaLane<NT> L(S.m);
     const int64_t k = S.k;
     const double t = a.t[k];
-    cd pl, ql;
-    ua_plql(L, t, a.eps, pl, ql);
     const cd elt = ua_cis(-1, L.l, t, a.eps);
+    cd pl, ql;
+    ua_plql<NT>(L, t, a.eps, elt, pl, ql);
     if (!S.live) return;
     for (int c = 0; c < 2; ++c) {
         const size_t u = ux(S.m, c, k, NT);
@@ -441,21 +468,23 @@ __global__ __launch_bounds__(UB) void k_ua_step_call(UaArgs a, const cd *xtil, c
 template <int NT>
 __global__ __launch_bounds__(UB) void k_ua_rho_call(UaArgs a, const cd *xt) {
     const Slot<NT> S(a.n);
-    const UaLane L = ua_lane<NT>(S.m);
+    const UaLane<NT> L(S.m);
     const int64_t k = S.k;
     const cd x[2] = {xt[ux(S.m, 0, k, NT)], xt[ux(S.m, 1, k, NT)]};
-    ua_reconstruct_deposit<NT>(L, S.base, a.t[k], a.eps, x, a.g, S.live, &a.x[k], &a.y[k]);
+    ua_reconstruct_deposit<NT>(L, S.base, ua_cis(+1, L.l, a.t[k], a.eps), x, a.g, S.live,
+                               &a.x[k], &a.y[k]);
 }
 
 // compute_v!.
 template <int NT>
 __global__ __launch_bounds__(UB) void k_ua_v_call(UaArgs a, const cd *yt) {
     const Slot<NT> S(a.n);
-    const UaLane L = ua_lane<NT>(S.m);
+    const UaLane<NT> L(S.m);
     const int64_t k = S.k;
     const cd y[2] = {yt[ux(S.m, 0, k, NT)], yt[ux(S.m, 1, k, NT)]};
+    const double t = a.t[k];
     double v1, v2;
-    ua_v<NT>(L, a.t[k], a.eps, y, v1, v2);
+    ua_v<NT>(L, t, a.eps, ua_cis(+1, L.l, t, a.eps), y, v1, v2);
     if (S.live && S.m == 0) {
         a.vx[k] = v1;
         a.vy[k] = v2;
@@ -742,11 +771,24 @@ int pic_ua_run(pic_ctx *ctx, double dt, int nsteps, double *energy) {
     const int nt = c->ua_ntau;
     const int64_t m = (int64_t)c->nx * c->ny;
     double en = 0.0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (c->timing) {
+        PIC_HIP(hipEventCreate(&ev[0]));
+        PIC_HIP(hipEventCreate(&ev[1]));
+    }
+    struct EvFree {
+        hipEvent_t *e;
+        ~EvFree() {
+            for (int k = 0; k < 2; ++k)
+                if (e[k]) (void)hipEventDestroy(e[k]);
+        }
+    } evfree{ev};
     for (int s = 0; s < nsteps; ++s) {
         for (int half = 0; half < 2; ++half) {
             PIC_TRY(launch_fill(c, c->rho_acc, 0.0, m));
             if (c->n) {
                 const UaArgs a = ua_args(c, dt);
+                if (c->timing) PIC_HIP(hipEventRecord(ev[0], c->stream));
                 if (half == 0) {
                     UA_NT_SWITCH(nt, hipLaunchKernelGGL(k_ua_predict<NT>, ua_grid(c->n, NT),
                                                         dim3(UB), 0, c->stream, a));
@@ -755,6 +797,15 @@ int pic_ua_run(pic_ctx *ctx, double dt, int nsteps, double *energy) {
                                                         dim3(UB), 0, c->stream, a));
                 }
                 PIC_HIP(hipGetLastError());
+                if (c->timing) {
+                    // the solve below synchronises the stream anyway
+                    PIC_HIP(hipEventRecord(ev[1], c->stream));
+                    PIC_HIP(hipEventSynchronize(ev[1]));
+                    float ms = 0.0f;
+                    PIC_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+                    c->t_ua_ms[half] += ms;
+                    c->t_ua_n[half] += 1;
+                }
             }
             PIC_TRY(finalize_rho(c, PIC_M6));
             PIC_TRY(pic_solve_fields(ctx, &en));
@@ -763,6 +814,16 @@ int pic_ua_run(pic_ctx *ctx, double dt, int nsteps, double *energy) {
     }
     PIC_HIP(hipStreamSynchronize(c->stream));
     c->seg_valid = false;
+    return PIC_OK;
+}
+
+int pic_get_ua_timing(pic_ctx *ctx, double *ms, int64_t *launches) {
+    PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
+    Ctx *c = ctx_of(ctx);
+    for (int k = 0; k < 2; ++k) {
+        if (ms) ms[k] = c->t_ua_ms[k];
+        if (launches) launches[k] = c->t_ua_n[k];
+    }
     return PIC_OK;
 }
 
