@@ -163,6 +163,10 @@ int pic_push_v(pic_ctx *ctx, int kernel, double dt);
  * wrap).  FP64 state. */
 int pic_push_x(pic_ctx *ctx, double dt);
 
+/* operator_t!(split, dt): poisson_2d.jl:265-277, the drift of the splitting
+ * operator, x += dt v with no periodic wrap.  FP64 state. */
+int pic_operator_t(pic_ctx *ctx, double dt);
+
 /* interpolate_cic!(epx, epy, mesh, p, ex, ey): interpolate_cic.jl:3-33.
  * Writes n values each into host epx, epy (either may be NULL to keep the
  * result on the device for pic_get_particle_fields). */

@@ -18,13 +18,14 @@ from .api import (FDTD, CloudInCell, Context, F90, M4, M6, MeshFields2D, Nearest
                   faraday_, interpol_eb_m6_, interpolate_cic_, push_v_, push_x_, solve_,
                   update_fields_, UA, UASimulation, preparation_, update_particles_e_,
                   compute_f_, ua_step_, update_particles_x_, compute_v_, read_particles,
-                  write_particles)
+                  write_particles, operator_t_)
 from . import sampling, diagnostics
 
 __all__ = [
     "FDTD", "ampere_maxwell_", "compute_current_", "compute_energy", "faraday_",
     "update_fields_", "UA", "UASimulation", "preparation_", "update_particles_e_", "compute_f_",
     "ua_step_", "update_particles_x_", "compute_v_", "read_particles", "write_particles",
+    "operator_t_",
     "CloudInCell", "Context", "F90", "M4", "M6", "MeshFields2D", "NearestGridPoint",
     "ParticleGroup", "Particles", "TriangularShapeCloud",
     "Poisson2D", "Simulation", "TwoDGrid", "TwoDPoissonPeriodic", "compute_rho",

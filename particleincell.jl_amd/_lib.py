@@ -47,6 +47,7 @@ SIGNATURES = {
     "pic_compute_rho": [_vp, _i],
     "pic_push_v": [_vp, _i, _d],
     "pic_push_x": [_vp, _d],
+    "pic_operator_t": [_vp, _d],
     "pic_interpolate_cic": [_vp, _vp, _vp],
     "pic_solve_e": [_vp],
     "pic_solve_phi": [_vp],

@@ -9,6 +9,7 @@ argument order and array layouts (a Julia `f!` is written `f_` here):
     compute_rho(p, kernel, mesh)            pushers_cic.jl:75, pushers_m6.jl:226
     push_v_(p, kernel, mesh, ex, ey, bz, dt) pushers_cic.jl:3, pushers_m6.jl:5
     push_x_(p, mesh, dt)                     pushers_cic.jl:55
+    operator_t_(pg, mesh, dt)                poisson_2d.jl:265 (splitting drift)
     interpolate_cic_(epx, epy, mesh, p, ex, ey)   interpolate_cic.jl:3
     TwoDPoissonPeriodic(grid)                poisson_2d.jl:28
     solve_(phi, poisson, rho)                poisson_2d.jl:78
@@ -279,6 +280,15 @@ def push_x_(p, mesh, dt):
     ctx.set_particles(p.array)
     L.call("pic_push_x", ctx.h, float(dt))
     ctx.get_particles(p.array)
+
+
+def operator_t_(pg, mesh, dt):
+    """operator_t!(split, dt) (poisson_2d.jl:265-277) on the splitting
+    operator's particle group: x += dt v, no wrap, in place on pg.array."""
+    ctx = _context(mesh)
+    ctx.set_particles(pg.array)
+    L.call("pic_operator_t", ctx.h, float(dt))
+    ctx.get_particles(pg.array)
 
 
 def interpolate_cic_(epx, epy, mesh, p, ex, ey):

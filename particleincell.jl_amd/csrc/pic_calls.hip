@@ -31,6 +31,15 @@ __global__ void k_push_x(int64_t n, double *x, double *y, const double *vx, cons
     y[k] = p2;
 }
 
+// operator_t!(split, dt): poisson_2d.jl:265-277, x += dt v with no wrap.
+__global__ void k_operator_t(int64_t n, double *x, double *y, const double *vx, const double *vy,
+                             double dt) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    x[k] += dt * vx[k];
+    y[k] += dt * vy[k];
+}
+
 // interpolate_cic!: interpolate_cic.jl:3-33 (trunc index, mod1 neighbours).
 __global__ void k_interpolate_cic(int64_t n, const double *x, const double *y, int nx, int ny,
                                   double dx, double dy, const double *ex, const double *ey,
@@ -203,6 +212,21 @@ int pic_push_x(pic_ctx *ctx, double dt) {
         PIC_HIP(hipGetLastError());
     }
     PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_operator_t(pic_ctx *ctx, double dt) {
+    PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
+    Ctx *c = ctx_of(ctx);
+    PIC_HIP(hipSetDevice(c->device));
+    PIC_TRY(need_f64(c));
+    if (c->n) {
+        hipLaunchKernelGGL(k_operator_t, grid1(c->n), dim3(256), 0, c->stream, c->n, c->x, c->y,
+                           c->vx, c->vy, dt);
+        PIC_HIP(hipGetLastError());
+    }
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    c->seg_valid = false;
     return PIC_OK;
 }
 

@@ -169,3 +169,21 @@ def test_read_particles_errors(gpu, tmp_path):
     out.write_text("9 0 0.5 0.5 0.0 0.0\n")
     with pytest.raises(pic._lib.PicError):
         pic.Simulation.from_file(mesh, str(out), pic.M6(), state=pic._lib.PIC_STATE_F32)
+
+
+def test_operator_t_drift_without_wrap(gpu):
+    """operator_t!(split, dt) (poisson_2d.jl:265-277): x += dt v, no periodic
+    wrap, bit for bit with numpy."""
+    pic = gpu
+    rng = np.random.default_rng(6)
+    mesh = pic.TwoDGrid(0.0, 1.0, 16, 0.0, 1.0, 16)
+    n = 5000
+    g = pic.ParticleGroup(n)
+    g.array[0:2] = rng.uniform(0, 1, (2, n))
+    g.array[2:4] = rng.normal(size=(2, n)) * 10
+    g.array[4] = 1.0 / n
+    want = g.array.copy(order="F")
+    want[0:2] += 0.37 * want[2:4]
+    pic.operator_t_(g, mesh, 0.37)
+    assert np.array_equal(g.array, want)
+    assert np.max(g.array[0]) > 1.0 and np.min(g.array[0]) < 0.0
