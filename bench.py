@@ -419,12 +419,12 @@ def bench_maxwell(args, W, pic, entry, steps, warmup):
         "seconds_total": seconds,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_vm_pass_lds<CIC> update_fields+push_v+push_x+current pass",
+                     "kernel": "k_vm_step<P> (one launch per iteration: field work, update_fields, push_v, push_x, current)",
                      "bytes_per_particle": VM_BYTES, "avg_us": pass_ms * 1e3},
         "energy_first_last": [float(energy[0]), float(energy[-1])],
     }
     if args.workload == "vm2d" and n == W["n"]:
-        traffic = pmc_traffic("k_vm_pass_lds", "pmc_traffic_vm2d_128x16_204800.txt")
+        traffic = pmc_traffic("k_vm_step", "pmc_traffic_vm2d_128x16_204800.txt")
         if traffic is not None:
             out["roofline"]["traffic"] = traffic[0]
             out["roofline"]["traffic_source"] = traffic[1]

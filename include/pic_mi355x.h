@@ -364,6 +364,13 @@ int pic_fdtd_energy(pic_ctx *ctx, double *energy);
  * be PIC_CIC; FP64 state. */
 int pic_run_maxwell(pic_ctx *ctx, int kernel, double dt, int nsteps, double *energy);
 
+/* How pic_run_maxwell launches an iteration: 0 (the default) the most
+ * fused form the grid allows in LDS (one launch per iteration up to 2048
+ * cells, two up to 3072), 2 at most two launches, 3 one kernel per
+ * reference function.  All compute the same arithmetic; for tests and
+ * measurements.  No reference counterpart. */
+int pic_set_maxwell_path(pic_ctx *ctx, int path);
+
 /* ------------------------------------------ uniformly accurate (UA) PIC */
 
 /* The UA step of src/ua_steps.jl for the 2D Vlasov-Poisson problem with the

@@ -78,6 +78,7 @@ SIGNATURES = {
     "pic_set_current": [_vp, _vp, _vp],
     "pic_fdtd_energy": [_vp, _pd],
     "pic_run_maxwell": [_vp, _i, _d, _i, _vp],
+    "pic_set_maxwell_path": [_vp, _i],
     "pic_ua_init": [_vp, _i, _d],
     "pic_ua_preparation": [_vp, _d, _vp, _vp, _vp, _vp, _vp, _vp],
     "pic_ua_interpol": [_vp, _vp, _vp],

@@ -850,6 +850,12 @@ class Simulation:
         _fdtd_out(self.ctx, f)
         return f
 
+    def set_maxwell_path(self, path):
+        """0: the most fused launch form the grid allows; 2: at most two
+        launches per iteration; 3: one kernel per reference function
+        (pic_set_maxwell_path)."""
+        L.call("pic_set_maxwell_path", self.ctx.h, int(path))
+
     def run_maxwell(self, dt, nsteps):
         """nsteps iterations of the Vlasov-Maxwell loop
         (examples/landau_damping_maxwell_fdtd.jl:58-83, pic_run_maxwell);

@@ -409,6 +409,8 @@ int pic_ctx_destroy(pic_ctx *ctx) {
     dfree(c->slow_chunks); dfree(c->defer_count);
     dfree(c->seg_k); dfree(c->seg_tbl); dfree(c->seg_A); dfree(c->seg_M);
     dfree(c->fd_ex); dfree(c->fd_ey); dfree(c->fd_bz); dfree(c->jx); dfree(c->jy);
+    dfree(c->fd2_ex); dfree(c->fd2_ey); dfree(c->fd2_bz);
+    for (double *&p : c->jb) dfree(p);
     dfree(c->ua_b); dfree(c->ua_t); dfree(c->ua_x0); dfree(c->ua_y0); dfree(c->ua_fhat);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

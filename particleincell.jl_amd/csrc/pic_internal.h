@@ -141,6 +141,11 @@ struct Ctx {
     // folded in)
     double *fd_ex = nullptr, *fd_ey = nullptr, *fd_bz = nullptr;
     double *jx = nullptr, *jy = nullptr;
+    // the one-launch iteration's second set of FDTD fields and two more
+    // current buffers
+    double *fd2_ex = nullptr, *fd2_ey = nullptr, *fd2_bz = nullptr;
+    double *jb[4] = {nullptr, nullptr, nullptr, nullptr};  // J1 x, y, J2 x, y
+    int vm_path = 0;  // pic_set_maxwell_path
 
     // UA (ua_type.jl): ntau, eps; per particle b, t (Particles.b, .t), the
     // positions at the start of a UA step and, between its prediction and

@@ -243,31 +243,106 @@ struct Walk {
     }
 };
 
-// The loop's particle work for grids whose node fields and current fit in
-// one workgroup's LDS.  Each block forms the node fields of update_fields!
-// from the FDTD fields (k_update_fields' arithmetic) in LDS and writes its
-// share of them to the context's node grids, then runs P particles per
-// thread through push_v!, push_x!(dt/2), compute_current!, push_x!(dt/2)
-// (loads of all P first), sums the current in LDS and adds it to jx, jy
-// with global atomics.
+constexpr int FT = 1024;  // threads of a block that works the whole grid in LDS
+constexpr int VT = 1024;  // threads of a pass block that sums the current in LDS
+constexpr size_t VM_LDS_BYTES_MAX = 120 * 1024;
+constexpr size_t VM_STEP_LDS_MAX = 150 * 1024;
+
+// ---- the FDTD updates on LDS copies of the fields (FT threads each)
+
+// faraday!(dt): fdtd.jl:24-36, bz += dt curl e.
+__device__ __forceinline__ void lds_faraday(int nx, int ny, double dx, double dy, double dt,
+                                            const double *sex, const double *sey, double *sbz) {
+    const int m = nx * ny, tid = threadIdx.x;
+    Walk w(tid, nx, FT);
+    for (int t = tid; t < m; t += FT, w.next()) {
+        const int i = w.i, j = w.j;
+        const int ip = i + 1 == nx ? 0 : i + 1, jp = j + 1 == ny ? 0 : j + 1;
+        const double dex_dy = (sex[i + nx * jp] - sex[i + nx * j]) / dy;
+        const double dey_dx = (sey[ip + (nx + 1) * j] - sey[i + (nx + 1) * j]) / dx;
+        sbz[t] += dt * (dex_dy - dey_dx);
+    }
+}
+
+// ampere_maxwell!(dt): fdtd.jl:40-54, with the periodic current sjx, sjy.
+__device__ __forceinline__ void lds_ampere(int nx, int ny, double dx, double dy, double dt,
+                                           double *sex, double *sey, const double *sbz,
+                                           const double *sjx, const double *sjy) {
+    const int mex = nx * (ny + 1), mey = (nx + 1) * ny, tid = threadIdx.x;
+    Walk w(tid, nx, FT);
+    for (int t = tid; t < mex; t += FT, w.next()) {
+        const int i = w.i, j = w.j;  // j in [0, ny]
+        const int jc = j == ny ? 0 : j, jm = j == 0 ? ny - 1 : j - 1;
+        const int ip = i + 1 == nx ? 0 : i + 1;
+        const double dbz_dy = (sbz[i + nx * jc] - sbz[i + nx * jm]) / dy;
+        sex[t] += dt * dbz_dy - dt * 0.5 * (sjx[i + nx * jc] + sjx[ip + nx * jc]);
+    }
+    Walk v(tid, nx + 1, FT);
+    for (int u = tid; u < mey; u += FT, v.next()) {
+        const int i = v.i, j = v.j;  // i in [0, nx]
+        const int ic = i == nx ? 0 : i, im = i == 0 ? nx - 1 : i - 1;
+        const int jp = j + 1 == ny ? 0 : j + 1;
+        const double dbz_dx = (sbz[ic + nx * j] - sbz[im + nx * j]) / dx;
+        sey[u] -= dt * dbz_dx - dt * 0.5 * (sjy[ic + nx * j] + sjy[ic + nx * jp]);
+    }
+}
+
+// compute_energy: fdtd.jl:79-80 in a fixed order (per-thread strided sums,
+// a wave butterfly, the waves in order); every thread of the block calls
+// it; thread 0 writes energy[*counter] and increments the counter.
+__device__ __forceinline__ void lds_energy(const double *sex, int mex, double dxdy, double *red,
+                                           double *energy, int *counter) {
+    const int tid = threadIdx.x;
+    double s = 0.0;
+    for (int t = tid; t < mex; t += FT) s += sex[t] * sex[t];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+        double acc = 0.0;
+        for (int w = 0; w < FT / 64; ++w) acc += red[w];
+        const int q = *counter;
+        *counter = q + 1;
+        energy[q] = 0.5 * log(acc * dxdy);
+    }
+}
+
+// update_fields!: fdtd.jl:60-77 on the nx x ny nodes, from the FDTD fields
+// (global or LDS) into LDS node arrays; nodes [lo, hi) also to global memory.
+template <int BS>
+__device__ __forceinline__ void nodes_from_fdtd(int nx, int ny, const double *fex,
+                                                const double *fey, const double *fbz,
+                                                double *sex, double *sey, double *sbz, int lo,
+                                                int hi, double *nex, double *ney, double *nbz) {
+    const int m = nx * ny, tid = threadIdx.x;
+    Walk w(tid, nx, BS);
+    for (int t = tid; t < m; t += BS, w.next()) {
+        const int i = w.i, j = w.j;
+        const int im = i == 0 ? nx - 1 : i - 1, jm = j == 0 ? ny - 1 : j - 1;
+        const double e1 = 0.5 * (fex[im + nx * j] + fex[i + nx * j]);
+        const double e2 = 0.5 * (fey[i + (nx + 1) * jm] + fey[i + (nx + 1) * j]);
+        const double b3 =
+            0.25 * (((fbz[im + nx * jm] + fbz[i + nx * jm]) + fbz[im + nx * j]) + fbz[i + nx * j]);
+        sex[t] = e1;
+        sey[t] = e2;
+        sbz[t] = b3;
+        if (t >= lo && t < hi) {
+            nex[t] = e1;
+            ney[t] = e2;
+            nbz[t] = b3;
+        }
+    }
+}
+
+// P particles per thread, loads first (kept in registers across calls).
 template <int BS, int P>
-__global__ __launch_bounds__(BS) void k_vm_pass_lds(VmArgs a, const double *__restrict__ fex,
-                                                    const double *__restrict__ fey,
-                                                    const double *__restrict__ fbz,
-                                                    double *__restrict__ nex,
-                                                    double *__restrict__ ney,
-                                                    double *__restrict__ nbz) {
-    extern __shared__ double lds[];
-    const int nx = a.nx, ny = a.ny, m = nx * ny, tid = threadIdx.x;
-    double *sex = lds, *sey = lds + m, *sbz = lds + 2 * m, *sjx = lds + 3 * m, *sjy = lds + 4 * m;
-    const int64_t n = a.n, stride = (int64_t)gridDim.x * (BS * P);
-    int64_t base = (int64_t)blockIdx.x * (BS * P);
+struct VmBatch {
     double px[P], py[P], v1[P], v2[P], wg[P];
-    auto load = [&]() {
+    __device__ void load(const VmArgs &a, int64_t base) {
 #pragma unroll
         for (int r = 0; r < P; ++r) {
-            const int64_t k = base + r * BS + tid;
-            if (k < n) {
+            const int64_t k = base + r * BS + threadIdx.x;
+            if (k < a.n) {
                 px[r] = a.x[k];
                 py[r] = a.y[k];
                 v1[r] = a.vx[k];
@@ -275,38 +350,18 @@ __global__ __launch_bounds__(BS) void k_vm_pass_lds(VmArgs a, const double *__re
                 wg[r] = a.w ? a.w[k] : a.w0;
             }
         }
-    };
-    load();
-    {
-        const int lo = (int)((int64_t)m * blockIdx.x / gridDim.x);
-        const int hi = (int)((int64_t)m * (blockIdx.x + 1) / gridDim.x);
-        Walk w(tid, nx, BS);
-        for (int t = tid; t < m; t += BS, w.next()) {
-            const int i = w.i, j = w.j;
-            const int im = i == 0 ? nx - 1 : i - 1, jm = j == 0 ? ny - 1 : j - 1;
-            const double e1 = 0.5 * (fex[im + nx * j] + fex[i + nx * j]);
-            const double e2 = 0.5 * (fey[i + (nx + 1) * jm] + fey[i + (nx + 1) * j]);
-            const double b3 =
-                0.25 * (((fbz[im + nx * jm] + fbz[i + nx * jm]) + fbz[im + nx * j]) + fbz[i + nx * j]);
-            sex[t] = e1;
-            sey[t] = e2;
-            sbz[t] = b3;
-            sjx[t] = 0.0;
-            sjy[t] = 0.0;
-            if (t >= lo && t < hi) {
-                nex[t] = e1;
-                ney[t] = e2;
-                nbz[t] = b3;
-            }
-        }
     }
-    __syncthreads();
-    const double dt = a.dt, h = 0.5 * dt;
-    while (base < n) {
+    // push_v!(dt) (pushers_cic.jl:3-53), push_x!(dt/2), compute_current!
+    // (cloud_in_cell.jl:31-60), push_x!(dt/2), with the node fields and the
+    // current accumulators in LDS.
+    __device__ void push(const VmArgs &a, int64_t base, const double *sex, const double *sey,
+                         const double *sbz, double *sjx, double *sjy) {
+        const int nx = a.nx, ny = a.ny;
+        const double dt = a.dt, h = 0.5 * dt;
 #pragma unroll
         for (int r = 0; r < P; ++r) {
-            const int64_t k = base + r * BS + tid;
-            if (k >= n) break;
+            const int64_t k = base + r * BS + threadIdx.x;
+            if (k >= a.n) break;
             double x = px[r], y = py[r], u1 = v1[r], u2 = v2[r];
             {  // push_v!: pushers_cic.jl:13-51
                 double dxp, dyp;
@@ -361,14 +416,54 @@ __global__ __launch_bounds__(BS) void k_vm_pass_lds(VmArgs a, const double *__re
             a.vx[k] = u1;
             a.vy[k] = u2;
         }
-        base += stride;
-        if (base < n) load();
+    }
+};
+
+// The block's LDS current added to the global one (non-zero nodes).
+template <int BS>
+__device__ __forceinline__ void flush_current(int m, const double *sjx, const double *sjy,
+                                              double *jx, double *jy) {
+    for (int t = threadIdx.x; t < m; t += BS) {
+        if (sjx[t] != 0.0) unsafeAtomicAdd(&jx[t], sjx[t]);
+        if (sjy[t] != 0.0) unsafeAtomicAdd(&jy[t], sjy[t]);
+    }
+}
+
+// The loop's particle work for grids whose node fields and current fit in
+// one workgroup's LDS.  Each block forms the node fields of update_fields!
+// from the FDTD fields in LDS and writes its share of them to the context's
+// node grids, then runs P particles per thread through the particle half of
+// the iteration (loads of all P first), sums the current in LDS and adds it
+// to jx, jy with global atomics.
+template <int BS, int P>
+__global__ __launch_bounds__(BS) void k_vm_pass_lds(VmArgs a, const double *__restrict__ fex,
+                                                    const double *__restrict__ fey,
+                                                    const double *__restrict__ fbz,
+                                                    double *__restrict__ nex,
+                                                    double *__restrict__ ney,
+                                                    double *__restrict__ nbz) {
+    extern __shared__ double lds[];
+    const int nx = a.nx, ny = a.ny, m = nx * ny, tid = threadIdx.x;
+    double *sex = lds, *sey = lds + m, *sbz = lds + 2 * m, *sjx = lds + 3 * m, *sjy = lds + 4 * m;
+    const int64_t stride = (int64_t)gridDim.x * (BS * P);
+    int64_t base = (int64_t)blockIdx.x * (BS * P);
+    VmBatch<BS, P> B;
+    B.load(a, base);
+    const int lo = (int)((int64_t)m * blockIdx.x / gridDim.x);
+    const int hi = (int)((int64_t)m * (blockIdx.x + 1) / gridDim.x);
+    nodes_from_fdtd<BS>(nx, ny, fex, fey, fbz, sex, sey, sbz, lo, hi, nex, ney, nbz);
+    for (int t = tid; t < m; t += BS) {
+        sjx[t] = 0.0;
+        sjy[t] = 0.0;
     }
     __syncthreads();
-    for (int t = tid; t < m; t += BS) {
-        if (sjx[t] != 0.0) unsafeAtomicAdd(&a.jx[t], sjx[t]);
-        if (sjy[t] != 0.0) unsafeAtomicAdd(&a.jy[t], sjy[t]);
+    while (base < a.n) {
+        B.push(a, base, sex, sey, sbz, sjx, sjy);
+        base += stride;
+        if (base < a.n) B.load(a, base);
     }
+    __syncthreads();
+    flush_current<BS>(m, sjx, sjy, a.jx, a.jy);
 }
 
 // The field work between two such passes, in one workgroup with the FDTD
@@ -377,88 +472,135 @@ __global__ __launch_bounds__(BS) void k_vm_pass_lds(VmArgs a, const double *__re
 //   faraday!(dt/2), ampere_maxwell!(dt) with jx, jy, the energy into
 //   energy[*counter] (then incremented), and with `next` (every iteration
 //   but the last) jx = jy = 0 and the next iteration's faraday!(dt/2).
-constexpr int FT = 1024;
-constexpr int VT = 1024;  // threads of a pass block that sums the current in LDS
-constexpr size_t VM_LDS_BYTES_MAX = 120 * 1024;
-
+// fex/fey/fbz are read from (fin_*) and written to (fex...); the two may be
+// the same arrays.
 __global__ __launch_bounds__(FT) void k_vm_fields(int nx, int ny, double dx, double dy, double dt,
-                                                  double *__restrict__ fex,
-                                                  double *__restrict__ fey,
-                                                  double *__restrict__ fbz, double *__restrict__ jx,
-                                                  double *__restrict__ jy, double *energy,
-                                                  int *counter, int next) {
+                                                  const double *fin_ex, const double *fin_ey,
+                                                  const double *fin_bz, double *fex, double *fey,
+                                                  double *fbz, double *jx, double *jy,
+                                                  double *energy, int *counter, int next) {
     extern __shared__ double lds[];
     __shared__ double red[FT / 64];
     const int m = nx * ny, mex = nx * (ny + 1), mey = (nx + 1) * ny, tid = threadIdx.x;
     double *sex = lds, *sey = lds + mex, *sbz = sey + mey, *sjx = sbz + m, *sjy = sjx + m;
-    for (int t = tid; t < mex; t += FT) sex[t] = fex[t];
-    for (int t = tid; t < mey; t += FT) sey[t] = fey[t];
+    for (int t = tid; t < mex; t += FT) sex[t] = fin_ex[t];
+    for (int t = tid; t < mey; t += FT) sey[t] = fin_ey[t];
     for (int t = tid; t < m; t += FT) {
-        sbz[t] = fbz[t];
+        sbz[t] = fin_bz[t];
         sjx[t] = jx[t];
         sjy[t] = jy[t];
     }
     __syncthreads();
     const double h = 0.5 * dt;
-    auto faraday_lds = [&]() {  // fdtd.jl:24-36
-        Walk w(tid, nx, FT);
-        for (int t = tid; t < m; t += FT, w.next()) {
-            const int i = w.i, j = w.j;
-            const int ip = i + 1 == nx ? 0 : i + 1, jp = j + 1 == ny ? 0 : j + 1;
-            const double dex_dy = (sex[i + nx * jp] - sex[i + nx * j]) / dy;
-            const double dey_dx = (sey[ip + (nx + 1) * j] - sey[i + (nx + 1) * j]) / dx;
-            sbz[t] += h * (dex_dy - dey_dx);
-        }
-        __syncthreads();
-    };
-    faraday_lds();
-    {  // ampere_maxwell!: fdtd.jl:40-54
-        Walk w(tid, nx, FT);
-        for (int t = tid; t < mex; t += FT, w.next()) {
-            const int i = w.i, j = w.j;  // j in [0, ny]
-            const int jc = j == ny ? 0 : j, jm = j == 0 ? ny - 1 : j - 1;
-            const int ip = i + 1 == nx ? 0 : i + 1;
-            const double dbz_dy = (sbz[i + nx * jc] - sbz[i + nx * jm]) / dy;
-            sex[t] += dt * dbz_dy - dt * 0.5 * (sjx[i + nx * jc] + sjx[ip + nx * jc]);
-        }
-        Walk v(tid, nx + 1, FT);
-        for (int u = tid; u < mey; u += FT, v.next()) {
-            const int i = v.i, j = v.j;  // i in [0, nx]
-            const int ic = i == nx ? 0 : i, im = i == 0 ? nx - 1 : i - 1;
-            const int jp = j + 1 == ny ? 0 : j + 1;
-            const double dbz_dx = (sbz[ic + nx * j] - sbz[im + nx * j]) / dx;
-            sey[u] -= dt * dbz_dx - dt * 0.5 * (sjy[ic + nx * j] + sjy[ic + nx * jp]);
-        }
-    }
+    lds_faraday(nx, ny, dx, dy, h, sex, sey, sbz);
     __syncthreads();
-    double s = 0.0;
-    for (int t = tid; t < mex; t += FT) s += sex[t] * sex[t];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if ((tid & 63) == 0) red[tid >> 6] = s;
-    if (next) {
+    lds_ampere(nx, ny, dx, dy, dt, sex, sey, sbz, sjx, sjy);
+    __syncthreads();
+    lds_energy(sex, mex, dx * dy, red, energy, counter);
+    if (next) {  // the current is consumed: clear it for the next pass
         for (int t = tid; t < m; t += FT) {
             jx[t] = 0.0;
             jy[t] = 0.0;
         }
-        faraday_lds();  // its barrier also orders red[]
-    } else {
+        lds_faraday(nx, ny, dx, dy, h, sex, sey, sbz);
         __syncthreads();
-    }
-    if (tid == 0) {
-        double acc = 0.0;
-        for (int w = 0; w < FT / 64; ++w) acc += red[w];
-        const int q = *counter;
-        *counter = q + 1;
-        energy[q] = 0.5 * log(acc * (dx * dy));
     }
     for (int t = tid; t < mex; t += FT) fex[t] = sex[t];
     for (int t = tid; t < mey; t += FT) fey[t] = sey[t];
     for (int t = tid; t < m; t += FT) fbz[t] = sbz[t];
 }
 
+// One whole iteration s of the loop in one launch, for grids small enough
+// that every block holds the FDTD fields, the current and the node fields in
+// LDS (up to 2048 cells).  Every block first finishes iteration s-1's field
+// work on its own LDS copy, all with the same operations on the same inputs:
+// faraday!(dt/2), ampere_maxwell!(dt) with the current j(s-1), then
+// iteration s's leading faraday!(dt/2) and update_fields!; block 0 also
+// records the energy of iteration s-1 and stores the fields for the next
+// launch.  Then the particles as in k_vm_pass_lds, into j(s).  The FDTD
+// fields alternate between two buffers and the current between three (the
+// one read, the one summed into, the one cleared for the next launch), so no
+// launch writes what another block of it reads.  For s = 1 the field work is
+// only update_fields! of the initial fields.
+struct VmStepIo {
+    const double *bin_ex, *bin_ey, *bin_bz;  // the fields the field work starts from
+    double *bout_ex, *bout_ey, *bout_bz;     // after it (block 0 writes)
+    const double *jin_x, *jin_y;             // j(s-1)
+    double *jacc_x, *jacc_y;                 // j(s), zero on entry
+    double *jclr_x, *jclr_y;                 // cleared for the next launch
+    double *nex, *ney, *nbz;                 // the context's node grids
+    double *energy;
+    int *counter;
+    int first;
+};
+
+template <int P>
+__global__ __launch_bounds__(VT) void k_vm_step(VmArgs a, VmStepIo io) {
+    extern __shared__ double lds[];
+    __shared__ double red[FT / 64];
+    const int nx = a.nx, ny = a.ny, m = nx * ny, mex = nx * (ny + 1), mey = (nx + 1) * ny;
+    const int tid = threadIdx.x;
+    double *fex = lds, *fey = fex + mex, *fbz = fey + mey, *fjx = fbz + m, *fjy = fjx + m;
+    double *sex = fjy + m, *sey = sex + m, *sbz = sey + m;  // node fields
+    double *sjx = lds, *sjy = lds + m;  // the current, over the FDTD region once it is free
+    const int64_t stride = (int64_t)gridDim.x * (VT * P);
+    int64_t base = (int64_t)blockIdx.x * (VT * P);
+    VmBatch<VT, P> B;
+    B.load(a, base);
+    for (int t = tid; t < mex; t += VT) fex[t] = io.bin_ex[t];
+    for (int t = tid; t < mey; t += VT) fey[t] = io.bin_ey[t];
+    for (int t = tid; t < m; t += VT) fbz[t] = io.bin_bz[t];
+    if (!io.first)
+        for (int t = tid; t < m; t += VT) {
+            fjx[t] = io.jin_x[t];
+            fjy[t] = io.jin_y[t];
+        }
+    __syncthreads();
+    const double h = 0.5 * a.dt;
+    if (!io.first) {
+        lds_faraday(nx, ny, a.dx, a.dy, h, fex, fey, fbz);
+        __syncthreads();
+        lds_ampere(nx, ny, a.dx, a.dy, a.dt, fex, fey, fbz, fjx, fjy);
+        __syncthreads();
+        if (blockIdx.x == 0) lds_energy(fex, mex, a.dx * a.dy, red, io.energy, io.counter);
+        lds_faraday(nx, ny, a.dx, a.dy, h, fex, fey, fbz);
+        __syncthreads();
+    }
+    if (blockIdx.x == 0) {
+        for (int t = tid; t < mex; t += VT) io.bout_ex[t] = fex[t];
+        for (int t = tid; t < mey; t += VT) io.bout_ey[t] = fey[t];
+        for (int t = tid; t < m; t += VT) io.bout_bz[t] = fbz[t];
+    }
+    const int lo = (int)((int64_t)m * blockIdx.x / gridDim.x);
+    const int hi = (int)((int64_t)m * (blockIdx.x + 1) / gridDim.x);
+    nodes_from_fdtd<VT>(nx, ny, fex, fey, fbz, sex, sey, sbz, lo, hi, io.nex, io.ney, io.nbz);
+    for (int t = lo + tid; t < hi; t += VT) {
+        io.jclr_x[t] = 0.0;
+        io.jclr_y[t] = 0.0;
+    }
+    __syncthreads();
+    for (int t = tid; t < m; t += VT) {
+        sjx[t] = 0.0;
+        sjy[t] = 0.0;
+    }
+    __syncthreads();
+    while (base < a.n) {
+        B.push(a, base, sex, sey, sbz, sjx, sjy);
+        base += stride;
+        if (base < a.n) B.load(a, base);
+    }
+    __syncthreads();
+    flush_current<VT>(m, sjx, sjy, io.jacc_x, io.jacc_y);
+}
+
 inline size_t vm_field_lds(const Ctx *c) {
     const size_t nx = c->nx, ny = c->ny;
     return (nx * (ny + 1) + (nx + 1) * ny + 3 * nx * ny) * 8;
+}
+
+inline size_t vm_step_lds(const Ctx *c) {
+    const size_t nx = c->nx, ny = c->ny;
+    return (nx * (ny + 1) + (nx + 1) * ny + 6 * nx * ny) * 8;
 }
 
 inline size_t vm_pass_lds(const Ctx *c) { return (size_t)5 * c->nx * c->ny * 8; }
@@ -478,6 +620,17 @@ int ensure_fdtd(Ctx *c) {
         PIC_HIP(hipMemsetAsync(c->fd_bz, 0, nx * ny * 8, c->stream));
         PIC_HIP(hipMemsetAsync(c->jx, 0, nx * ny * 8, c->stream));
         PIC_HIP(hipMemsetAsync(c->jy, 0, nx * ny * 8, c->stream));
+    }
+    return PIC_OK;
+}
+
+int ensure_step_buffers(Ctx *c) {
+    const size_t nx = c->nx, ny = c->ny;
+    if (!c->fd2_ex) {
+        PIC_HIP(hipMalloc(&c->fd2_ex, nx * (ny + 1) * 8));
+        PIC_HIP(hipMalloc(&c->fd2_ey, (nx + 1) * ny * 8));
+        PIC_HIP(hipMalloc(&c->fd2_bz, nx * ny * 8));
+        for (double *&p : c->jb) PIC_HIP(hipMalloc(&p, nx * ny * 8));
     }
     return PIC_OK;
 }
@@ -556,11 +709,42 @@ int vm_lds_attributes(size_t flds, size_t plds) {
         PIC_HIP(hipFuncSetAttribute((const void *)k_vm_fields,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));
     if (plds > LDS_DEFAULT_MAX) {
-        const void *fns[3] = {(const void *)k_vm_pass_lds<VT, 1>, (const void *)k_vm_pass_lds<VT, 2>,
-                              (const void *)k_vm_pass_lds<VT, 4>};
+        const void *fns[6] = {(const void *)k_vm_pass_lds<VT, 1>, (const void *)k_vm_pass_lds<VT, 2>,
+                              (const void *)k_vm_pass_lds<VT, 4>, (const void *)k_vm_step<1>,
+                              (const void *)k_vm_step<2>, (const void *)k_vm_step<4>};
         for (const void *f : fns)
             PIC_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds));
     }
+    return PIC_OK;
+}
+
+// Particles per thread (1, 2 or 4) and blocks of a launch whose blocks need
+// `lds` bytes each: the fewest particles per thread for which one round of
+// resident blocks covers them all.
+void vm_shape(const Ctx *c, size_t lds, int &P, int64_t &nb) {
+    const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(2, (int64_t)(160 * 1024 / lds)));
+    const int64_t resident = (int64_t)c->ncu * per_cu;
+    P = 1;
+    nb = (c->n + VT - 1) / VT;
+    while (nb > resident && P < 4) {
+        P *= 2;
+        nb = (c->n + (int64_t)VT * P - 1) / ((int64_t)VT * P);
+    }
+    nb = std::max<int64_t>(1, std::min<int64_t>(nb, resident));
+}
+
+int vm_step_launch(Ctx *c, double dt, const VmStepIo &io, size_t lds) {
+    const VmArgs a = vm_args(c, dt);
+    int P;
+    int64_t nb;
+    vm_shape(c, lds, P, nb);
+    if (P == 1)
+        hipLaunchKernelGGL(k_vm_step<1>, dim3((unsigned)nb), dim3(VT), lds, c->stream, a, io);
+    else if (P == 2)
+        hipLaunchKernelGGL(k_vm_step<2>, dim3((unsigned)nb), dim3(VT), lds, c->stream, a, io);
+    else
+        hipLaunchKernelGGL(k_vm_step<4>, dim3((unsigned)nb), dim3(VT), lds, c->stream, a, io);
+    PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
 
@@ -570,15 +754,9 @@ int vm_lds_attributes(size_t flds, size_t plds) {
 int vm_pass_lds_launch(Ctx *c, double dt) {
     const VmArgs a = vm_args(c, dt);
     const size_t lds = vm_pass_lds(c);
-    const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(2, (int64_t)(160 * 1024 / lds)));
-    const int64_t resident = (int64_t)c->ncu * per_cu;
-    int P = 1;
-    int64_t nb = (c->n + VT - 1) / VT;
-    while (nb > resident && P < 4) {
-        P *= 2;
-        nb = (c->n + (int64_t)VT * P - 1) / ((int64_t)VT * P);
-    }
-    nb = std::max<int64_t>(1, std::min<int64_t>(nb, resident));
+    int P;
+    int64_t nb;
+    vm_shape(c, lds, P, nb);
     const double *fx = c->fd_ex, *fy = c->fd_ey, *fb = c->fd_bz;
     if (P == 1)
         hipLaunchKernelGGL((k_vm_pass_lds<VT, 1>), dim3((unsigned)nb), dim3(VT), lds, c->stream, a,
@@ -730,6 +908,13 @@ int pic_fdtd_energy(pic_ctx *ctx, double *out) {
     return PIC_OK;
 }
 
+int pic_set_maxwell_path(pic_ctx *ctx, int path) {
+    PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
+    PIC_CHECK(path >= 0 && path <= 3 && path != 1, PIC_ERR_ARG, "path %d", path);
+    ctx_of(ctx)->vm_path = path;
+    return PIC_OK;
+}
+
 int pic_run_maxwell(pic_ctx *ctx, int kernel, double dt, int nsteps, double *energy) {
     PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
     PIC_CHECK(nsteps >= 0, PIC_ERR_ARG, "nsteps %d", nsteps);
@@ -750,14 +935,62 @@ int pic_run_maxwell(pic_ctx *ctx, int kernel, double dt, int nsteps, double *ene
         return PIC_OK;
     };
     int rc = PIC_OK;
-    const size_t flds = vm_field_lds(c), plds = vm_pass_lds(c);
-    const bool fused = flds <= VM_LDS_BYTES_MAX && plds <= VM_LDS_BYTES_MAX;
+    const size_t flds = vm_field_lds(c), plds = vm_pass_lds(c), slds = vm_step_lds(c);
+    const bool one = slds <= VM_STEP_LDS_MAX && flds <= VM_STEP_LDS_MAX && c->vm_path != 2;
+    const bool fused = !one && flds <= VM_LDS_BYTES_MAX && plds <= VM_LDS_BYTES_MAX &&
+                       c->vm_path != 3;
     auto timed_pass = [&](bool lds) {
         int r = c->timing ? mark() : PIC_OK;
         if (r == PIC_OK) r = lds ? vm_pass_lds_launch(c, dt) : vm_pass(c, dt);
         if (r == PIC_OK && c->timing) r = mark();
         return r;
     };
+    if (one) {
+        // one launch per iteration (k_vm_step), then the last iteration's
+        // field work; FDTD fields alternate between B0 (the context's) and B1,
+        // the current among J0 (the context's), J1, J2
+        rc = ensure_step_buffers(c);
+        if (rc == PIC_OK) rc = vm_lds_attributes(flds, slds);
+        double *B[2][3] = {{c->fd_ex, c->fd_ey, c->fd_bz}, {c->fd2_ex, c->fd2_ey, c->fd2_bz}};
+        double *J[3][2] = {{c->jx, c->jy}, {c->jb[0], c->jb[1]}, {c->jb[2], c->jb[3]}};
+        if (rc == PIC_OK) rc = launch_fill(c, J[1][0], 0.0, m);
+        if (rc == PIC_OK) rc = launch_fill(c, J[1][1], 0.0, m);
+        c->bz_mode = PIC_BZ_GRID;
+        for (int s = 1; s <= nsteps && rc == PIC_OK; ++s) {
+            VmStepIo io;
+            double **bi = B[(s - 1) & 1], **bo = B[s & 1];
+            io.bin_ex = bi[0]; io.bin_ey = bi[1]; io.bin_bz = bi[2];
+            io.bout_ex = bo[0]; io.bout_ey = bo[1]; io.bout_bz = bo[2];
+            io.jin_x = J[(s - 1) % 3][0]; io.jin_y = J[(s - 1) % 3][1];
+            io.jacc_x = J[s % 3][0]; io.jacc_y = J[s % 3][1];
+            io.jclr_x = J[(s + 1) % 3][0]; io.jclr_y = J[(s + 1) % 3][1];
+            io.nex = c->ex; io.ney = c->ey; io.nbz = c->bz;
+            io.energy = c->energy_hist;
+            io.counter = c->step_counter;
+            io.first = s == 1;
+            if (c->timing) rc = mark();
+            if (rc == PIC_OK) rc = vm_step_launch(c, dt, io, slds);
+            if (rc == PIC_OK && c->timing) rc = mark();
+        }
+        if (rc == PIC_OK) {
+            double **bi = B[nsteps & 1];
+            double **jn = J[nsteps % 3];
+            hipLaunchKernelGGL(k_vm_fields, dim3(1), dim3(FT), flds, c->stream, c->nx, c->ny,
+                               c->dx, c->dy, dt, (const double *)bi[0], (const double *)bi[1],
+                               (const double *)bi[2], c->fd_ex, c->fd_ey, c->fd_bz, jn[0], jn[1],
+                               c->energy_hist, c->step_counter, 0);
+            if (hipGetLastError() != hipSuccess) rc = fail(PIC_ERR_HIP, "field kernel launch");
+        }
+        if (rc == PIC_OK && nsteps % 3 != 0) {  // the context keeps the last current
+            const size_t bytes = (size_t)m * 8;
+            double **jn = J[nsteps % 3];
+            if (hipMemcpyAsync(c->jx, jn[0], bytes, hipMemcpyDeviceToDevice, c->stream) !=
+                    hipSuccess ||
+                hipMemcpyAsync(c->jy, jn[1], bytes, hipMemcpyDeviceToDevice, c->stream) !=
+                    hipSuccess)
+                rc = fail(PIC_ERR_HIP, "current copy");
+        }
+    }
     if (fused) {
         // iteration s: the pass (with update_fields! in every block), then
         // faraday!(dt/2), ampere_maxwell!, the energy and, but after the
@@ -770,13 +1003,15 @@ int pic_run_maxwell(pic_ctx *ctx, int kernel, double dt, int nsteps, double *ene
             rc = timed_pass(true);
             if (rc == PIC_OK) {
                 hipLaunchKernelGGL(k_vm_fields, dim3(1), dim3(FT), flds, c->stream, c->nx, c->ny,
-                                   c->dx, c->dy, dt, c->fd_ex, c->fd_ey, c->fd_bz, c->jx, c->jy,
-                                   c->energy_hist, c->step_counter, s < nsteps ? 1 : 0);
+                                   c->dx, c->dy, dt, (const double *)c->fd_ex,
+                                   (const double *)c->fd_ey, (const double *)c->fd_bz, c->fd_ex,
+                                   c->fd_ey, c->fd_bz, c->jx, c->jy, c->energy_hist,
+                                   c->step_counter, s < nsteps ? 1 : 0);
                 if (hipGetLastError() != hipSuccess) rc = fail(PIC_ERR_HIP, "field kernel launch");
             }
         }
     }
-    for (int s = 1; s <= nsteps && rc == PIC_OK && !fused; ++s) {
+    for (int s = 1; s <= nsteps && rc == PIC_OK && !fused && !one; ++s) {
         if (s > 1) rc = faraday(c, 0.5 * dt);
         if (rc == PIC_OK) rc = update_fields(c);
         if (rc == PIC_OK) rc = launch_fill(c, c->jx, 0.0, m);
