@@ -360,8 +360,10 @@ int pic_fdtd_energy(pic_ctx *ctx, double *energy);
  * second iteration, update_fields!, push_v!(dt), push_x!(dt/2),
  * compute_current!, push_x!(dt/2), faraday!(dt/2), ampere_maxwell!(dt);
  * energy[s] = compute_energy after iteration s+1 (examples/vm2d.jl:55; may
- * be NULL).  The particle part is one fused pass per iteration.  kernel must
- * be PIC_CIC; FP64 state. */
+ * be NULL).  One launch per iteration when 8 (nx(ny+1) + (nx+1)ny + 6 nx ny)
+ * bytes fit in 150 KiB of LDS (132 KiB at 128 x 16), else two up to 3072
+ * cells, else one kernel per reference function.  kernel must be PIC_CIC;
+ * FP64 state. */
 int pic_run_maxwell(pic_ctx *ctx, int kernel, double dt, int nsteps, double *energy);
 
 /* How pic_run_maxwell launches an iteration: 0 (the default) the most
