@@ -26,8 +26,13 @@
 //  2 fixed-point int32 LDS tiles: every contribution is rounded to a multiple
 //    of a power-of-two quantum u (per block, 2^-30 of the largest sum a node
 //    of the block can reach) and summed exactly with ds_add_u32; the flush
-//    converts to FP64.  Four-byte words halve the LDS bank pressure of the
-//    deposit, whose lanes spread over neighbouring rows once particles drift.
+//    converts to FP64.  Passes that drift before they deposit keep two
+//    copies of the tile, one for the even and one for the odd lanes: a
+//    ds_add_u32 whose lanes address one word serialises them, and once
+//    particles drift two neighbouring lanes often sit in one cell.  Measured on MI355X with the 6x6 stencil's address pattern, a
+//    64-lane ds_add_u32 takes ~4.1 clocks per CU with the lanes in 64
+//    adjacent cells, ~8.3 after a drift of half a cell (sigma), ~6.3 with the
+//    two copies; bank conflicts alone (rows apart) cost little (~4.9).
 //  1 FP64 LDS tiles: every FP32 contribution is added exactly converted to
 //    FP64 (ds_add_f64);
 //  0 compensated (hi, lo) FP32 tile pairs.  Every contribution q is split as
@@ -70,6 +75,7 @@ struct SegF32 {
     unsigned long long *slow;
     uint32_t *dlist;  // deferred particles and their count
     unsigned *dcount;
+    int tiles;  // fixed-point charge tiles (1 or 2, see k_seg_f32)
 };
 
 // C = 1.5 * 2^23 * u of the (hi, lo) split for a block of `count`
@@ -161,8 +167,12 @@ __device__ __forceinline__ void charge32_fixed(float fx, float fy, int ci, int c
 // the block's LDS window is not touched here: it goes to the deferred list
 // whole (gather outside) or for its deposit only (drifted outside), and
 // k_defer_f32 completes it from global memory after this kernel.
+//
+// At most 64 VGPRs (eight waves per SIMD): four 40 KiB blocks then share a
+// CU, against three at the 66 registers the compiler picks unbounded (the M6
+// pass with one charge tile ran ~1.5 % faster so).
 template <int K, bool GATHER, bool DRIFT, bool DEPOSIT, int BZ, int ACC, bool PERM>
-__global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
+__global__ __launch_bounds__(SEG_T, 8) void k_seg_f32(SegF32 a) {
     using St = SegStencil<K>;
     constexpr int S = St::S, HL = St::HL;
     extern __shared__ __align__(16) char smem_s32[];
@@ -188,6 +198,11 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
     float *hi = reinterpret_cast<float *>(dep), *lo = hi + cap;
     double *dacc = reinterpret_cast<double *>(dep);
     int *iacc = reinterpret_cast<int *>(dep);
+    // fixed-point tiles: with a.tiles == 2 even lanes add into iacc[0, cap)
+    // and odd lanes into iacc[cap, 2 cap), so that two neighbouring lanes
+    // whose particles drifted into one cell do not add to one LDS word in
+    // the same instruction
+    int *itile = iacc + ((a.tiles == 2 && (tid & 1)) ? cap : 0);
     if (fits) {
         const int nt = W.tw * W.th;
         for (int t = tid; t < nt; t += SEG_T) {
@@ -205,6 +220,7 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
                     dacc[t] = 0.0;
                 } else {
                     iacc[t] = 0;
+                    if (a.tiles == 2) iacc[t + cap] = 0;
                 }
             }
         }
@@ -246,7 +262,7 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
             if (!in) continue;
             if (ACC == 2) {
                 charge32_fixed<K>(fx, fy, ci, cj, wu, [&](int u, int v, int qi) {
-                    __hip_atomic_fetch_add(&iacc[(lx + u) + W.tw * (ly + v)], qi,
+                    __hip_atomic_fetch_add(&itile[(lx + u) + W.tw * (ly + v)], qi,
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 });
                 continue;
@@ -273,7 +289,9 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f32(SegF32 a) {
     for (int t = tid; t < nt; t += SEG_T) {
         const double q = ACC == 0   ? (double)hi[t] + (double)lo[t]
                          : ACC == 1 ? dacc[t]
-                                    : ldexp((double)iacc[t], qe);
+                                    : ldexp((double)((int64_t)iacc[t] +
+                                                     (a.tiles == 2 ? iacc[t + cap] : 0)),
+                                            qe);
         if (q != 0.0) {
             const int u = t % W.tw, v = t / W.tw;
             unsafeAtomicAdd(&a.acc[(int64_t)((W.x0 + u) & mx) + (int64_t)nx * ((W.y0 + v) & my)],
@@ -725,9 +743,18 @@ int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     a.margin = 32;
     const bool gather = mode == 1 || mode == 2;
     const int acc = c->f32_acc;
+    // two fixed-point tiles for the M6 passes that drift and then deposit.
+    // Measured: the M6 fused pass 216 -> 208 us (1e7 particles, 128^2) and
+    // 12.8 -> 11.6 ms (5e8, 512^2); the deposit-only pass ~10 % slower with
+    // two (its particles are in their sorted lanes after a sort); the CIC,
+    // NGP, TSC and M4 passes 4-8 % slower (4 or 16 atomics per particle do
+    // not pay for the fourth block per CU that the larger LDS window costs).
+    a.tiles = acc == 2 && kernel == PIC_M6 && (mode == 0 || mode == 1) ? 2 : 1;
     const size_t per_node = (gather ? (c->bz_mode == PIC_BZ_GRID ? 12 : 8) : 0) +
-                            (mode == 2 ? 0 : acc == 2 ? 4 : 8);
-    a.cap = (int)(SEG_LDS_BYTES / (per_node ? per_node : 8)) & ~15;
+                            (mode == 2 ? 0 : acc == 2 ? 4 * a.tiles : 8);
+    a.cap = (int)((a.tiles == 2 ? SEG32_FIXED_LDS_BYTES : SEG_LDS_BYTES) /
+                  (per_node ? per_node : 8)) &
+            ~15;
     a.slow = c->slow_chunks;
     PIC_TRY(ensure_defer(c));
     a.dlist = c->defer;

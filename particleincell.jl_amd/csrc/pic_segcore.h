@@ -95,6 +95,11 @@ constexpr int DEFER_T = 256, DEFER_BLOCKS = 256;
 // 512-thread blocks share a CU's 160 KiB; an M6 window of a 64-cell row
 // segment then keeps a drift margin of 11 cells (FP32) or 7 cells (FP64).
 constexpr size_t SEG_LDS_BYTES = 40 * 1024;
+// The FP32 M6 passes with two fixed-point charge tiles hold the field (8 B)
+// and two int32 tiles (8 B) per node: 53 KiB keeps the M6 window of a 64-cell
+// row segment at the 14-cell drift margin of 40 KiB with one tile, at three
+// blocks per CU.
+constexpr size_t SEG32_FIXED_LDS_BYTES = 53 * 1024;
 
 // Blocks per segment: as many as keep every CU busy in one round of
 // resident blocks (the occupancy of this kernel instantiation at this LDS
