@@ -7,9 +7,10 @@
 //                  (LDS counters, one atomic per run of equal digits)
 //   k_scan_*       exclusive scan of the histograms in digit-major order,
 //                  giving each (digit, tile) its first output slot
-//   k_digit_scatter  per tile, in item order: the rank of each key among the
-//                  equal digits before it (wave ballots for the lanes of one
-//                  wave, LDS counts for earlier waves and rounds), which
+//   k_digit_scatter  per tile, each wave ranking its own quarter of the
+//                  tile: the rank of each key among the equal digits before it
+//                  (wave ballots within a round of 64, a wave-private count
+//                  per digit across rounds, one prefix over the waves), which
 //                  places it in the tile's digit-ordered copy in LDS; the copy
 //                  is then written out, each digit's run to consecutive slots
 // The first pass reads the keys where they are (the FP32 state's cell array)
@@ -150,10 +151,12 @@ __global__ __launch_bounds__(SORT_T) void k_digit_scatter(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, int64_t n, int shift,
     int bits, int ntiles, const uint32_t *__restrict__ offs, uint32_t *__restrict__ kout,
     uint32_t *__restrict__ vout) {
+    constexpr int NW = SORT_T / 64, WITEMS = SORT_TILE / NW;
     __shared__ uint32_t gbase[SORT_MAXR];   // first output slot of each digit of this tile
     __shared__ uint32_t dstart[SORT_MAXR];  // its first position in the tile's digit order
-    __shared__ uint32_t run[SORT_MAXR];     // items of each digit placed so far
-    __shared__ uint32_t wcnt[SORT_T / 64][SORT_MAXR];
+    // per wave: its items of each digit, then (after the prefix over waves)
+    // the position of its first item of each digit in the tile's digit order
+    __shared__ uint32_t wcnt[NW][SORT_MAXR];
     __shared__ uint32_t lk[SORT_TILE], lv[SORT_TILE];
     __shared__ uint32_t sh[SCAN_T];
     const int R = 1 << bits;
@@ -175,52 +178,63 @@ __global__ __launch_bounds__(SORT_T) void k_digit_scatter(
             const uint32_t g1 = e + 1 < total ? offs[e + 1] : (uint32_t)n;
             gbase[d] = g0;
             cd[q] = g1 - g0;
-            run[d] = 0;
-            for (int w = 0; w < SORT_T / 64; ++w) wcnt[w][d] = 0;
+            for (int w = 0; w < NW; ++w) wcnt[w][d] = 0;
         }
     }
     uint32_t tot;
     const uint32_t ex = block_excl_scan(cd[0] + cd[1], sh, tot);
     if (2 * tid < R) dstart[2 * tid] = ex;
     if (2 * tid + 1 < R) dstart[2 * tid + 1] = ex + cd[0];
-    // every (key, value) of this thread first, so the loads are in flight
-    // together rather than one round at a time between the barriers
-    uint32_t kr[SORT_R], vr[SORT_R];
+    // wave wv owns the tile's items [wv WITEMS, (wv + 1) WITEMS), 64 per
+    // round; every (key, value) of this thread first, so the loads are in
+    // flight together
+    const int64_t wbase = base + (int64_t)wv * WITEMS + lane;
+    uint32_t kr[SORT_R], vr[SORT_R], lpos[SORT_R];
 #pragma unroll
     for (int r = 0; r < SORT_R; ++r) {
-        const int64_t p = base + (int64_t)r * SORT_T + tid;
+        const int64_t p = wbase + 64 * r;
         kr[r] = p < n ? kin[p] : 0u;
         vr[r] = IDX ? (uint32_t)p : (p < n ? vin[p] : 0u);
     }
     __syncthreads();
+    // rank of every item among the equal digits before it in its wave: the
+    // wave's earlier rounds (a wave-private counter per digit, advanced by
+    // the lowest lane of each digit group) plus the lower lanes of its round
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
 #pragma unroll
     for (int r = 0; r < SORT_R; ++r) {
-        const int64_t p = base + (int64_t)r * SORT_T + tid;
-        const bool valid = p < n;
-        const uint32_t kk = kr[r], vv = vr[r];
-        uint32_t d = (kk >> shift) & (uint32_t)(R - 1);
+        const bool valid = wbase + 64 * r < n;
+        const uint32_t d = (kr[r] >> shift) & (uint32_t)(R - 1);
         uint64_t peers = __ballot(valid);
         for (int b = 0; b < bits; ++b) {
-            uint64_t bal = __ballot((d >> b) & 1u);
+            const uint64_t bal = __ballot((d >> b) & 1u);
             peers &= ((d >> b) & 1u) ? bal : ~bal;
         }
-        const int rank_w = __popcll(peers & lt);
         const bool leader = valid && (peers & lt) == 0;
-        if (leader) wcnt[wv][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t before = 0;
-            for (int w = 0; w < wv; ++w) before += wcnt[w][d];
-            const uint32_t lp = dstart[d] + run[d] + before + (uint32_t)rank_w;
-            lk[lp] = kk;
-            lv[lp] = vv;
+        uint32_t before = 0;
+        if (leader) before = atomicAdd(&wcnt[wv][d], (uint32_t)__popcll(peers));
+        before = (uint32_t)__shfl((int)before, peers ? __ffsll((long long)peers) - 1 : 0);
+        lpos[r] = before + (uint32_t)__popcll(peers & lt);
+    }
+    __syncthreads();
+    // prefix over the waves, from the digit's first position in the tile
+    for (int d = tid; d < R; d += SORT_T) {
+        uint32_t a = dstart[d];
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t t = wcnt[w][d];
+            wcnt[w][d] = a;
+            a += t;
         }
-        __syncthreads();
-        if (leader) atomicAdd(&run[d], wcnt[wv][d]);
-        __syncthreads();
-        if (leader) wcnt[wv][d] = 0;
-        // the next round reads wcnt only after its own __syncthreads
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SORT_R; ++r) {
+        if (wbase + 64 * r < n) {
+            const uint32_t d = (kr[r] >> shift) & (uint32_t)(R - 1);
+            const uint32_t lp = wcnt[wv][d] + lpos[r];
+            lk[lp] = kr[r];
+            lv[lp] = vr[r];
+        }
     }
     __syncthreads();
     // the tile in digit order: each digit's items go to consecutive slots
@@ -266,24 +280,28 @@ __global__ void k_seg_max(const int64_t *__restrict__ off, int ncells, int nseg,
     if (l == 0) kmax[s] = (uint32_t)cnt;
 }
 
-// One wave per segment: tables A_s(k), M_s(k) at tbl[s] + k.
-__global__ void k_seg_tables(const int64_t *__restrict__ off, int ncells, int nseg,
-                             const uint32_t *__restrict__ kmax, const uint32_t *__restrict__ tbl,
-                             uint32_t *__restrict__ A, unsigned long long *__restrict__ M) {
-    const int s = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    const int l = threadIdx.x & 63;
-    if (s >= nseg) return;
+// Tables A_s(k), M_s(k) at tbl[s] + k, in chunks of 64 entries: block
+// (s, y) of SEGT_W waves fills the chunks y SEGT_W + w, advancing by
+// gridDim.y SEGT_W, of segment s.  A chunk starts from
+// A_s(k0) = sum_l min(count_l, k0) (a wave sum) and steps through its 64
+// ranks with one ballot each.
+constexpr int SEGT_W = 4, SEGT_Y = 8;
+__global__ __launch_bounds__(64 * SEGT_W) void k_seg_tables(
+    const int64_t *__restrict__ off, int ncells, const uint32_t *__restrict__ kmax,
+    const uint32_t *__restrict__ tbl, uint32_t *__restrict__ A, unsigned long long *__restrict__ M) {
+    const int s = blockIdx.x;
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int c = 64 * s + l;
     const uint32_t cnt = c < ncells ? (uint32_t)(off[c + 1] - off[c]) : 0u;
     const uint32_t K = kmax[s], base = tbl[s];
-    uint32_t acc = 0;
-    for (uint32_t k0 = 0; k0 < K; k0 += 64) {
-        // lane j writes entry k0 + j: gather the 64 masks of this block of k
+    for (uint32_t k0 = 64u * (blockIdx.y * SEGT_W + w); k0 < K; k0 += 64u * SEGT_W * gridDim.y) {
+        uint32_t acc = min(cnt, k0);
+        for (int o = 32; o > 0; o >>= 1) acc += (uint32_t)__shfl_xor((int)acc, o);
         unsigned long long mine = 0;
         uint32_t acc_mine = 0;
         const uint32_t kend = min(K - k0, 64u);
         for (uint32_t j = 0; j < kend; ++j) {
-            unsigned long long m = __ballot(cnt > k0 + j);
+            const unsigned long long m = __ballot(cnt > k0 + j);
             if ((uint32_t)l == j) {
                 mine = m;
                 acc_mine = acc;
@@ -597,8 +615,8 @@ static int lane_spread(Ctx *c) {
     PIC_HIP(hipMemcpyAsync(c->seg_tbl, c->seg_k, (size_t)nseg * 4, hipMemcpyDeviceToDevice,
                            c->stream));
     PIC_TRY(scan(c, c->seg_tbl, nseg));
-    hipLaunchKernelGGL(k_seg_tables, gs, bs, 0, c->stream, c->cell_off, ncells, nseg, c->seg_k,
-                       c->seg_tbl, c->seg_A, c->seg_M);
+    hipLaunchKernelGGL(k_seg_tables, dim3((unsigned)nseg, SEGT_Y), dim3(64 * SEGT_W), 0,
+                       c->stream, c->cell_off, ncells, c->seg_k, c->seg_tbl, c->seg_A, c->seg_M);
     PIC_HIP(hipGetLastError());
     (void)n;
     return PIC_OK;
