@@ -396,6 +396,11 @@ int ensure_sort(Ctx *c) {
 }
 
 static int scan(Ctx *c, uint32_t *a, int64_t n) {
+    if (n <= 2 * SCAN_B) {  // short arrays (segment tables): one block, one launch
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, c->stream, a, (int)n);
+        PIC_HIP(hipGetLastError());
+        return PIC_OK;
+    }
     const int nb = (int)((n + SCAN_B - 1) / SCAN_B);
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, c->stream, a, n, c->sort_blk);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, c->stream, c->sort_blk, nb);
