@@ -322,7 +322,7 @@ def test_field_energy(gpu, O):
 
 @pytest.mark.parametrize("state", ["f64", "f32"])
 @pytest.mark.parametrize("nx,ny,n", [(64, 64, 100000), (128, 128, 300001), (512, 512, 70000),
-                                     (8, 8, 5)])
+                                     (8, 8, 5), (8, 8, 150000)])
 def test_sort_bitwise(gpu, O, state, nx, ny, n):
     pic = gpu
     rng = np.random.default_rng(nx + n)
@@ -347,10 +347,13 @@ def test_sort_bitwise(gpu, O, state, nx, ny, n):
 
 
 @pytest.mark.parametrize("nx,ny,n", [(64, 64, 200000), (128, 128, 300001), (32, 16, 9000),
-                                     (512, 512, 50000), (8, 8, 3)])
+                                     (512, 512, 50000), (8, 8, 3), (8, 8, 150000)])
 def test_sort_lanes_bitwise(gpu, O, nx, ny, n):
     """pic_sort_lanes: the stable cell sort followed by the lane-spread order
-    within 64-cell segments, against the oracle's lexsort definition."""
+    within 64-cell segments, against the oracle's lexsort definition.  On the
+    8 x 8 grid with 150,000 particles every cell holds ~2,300, so the
+    segment's rank tables span more than one round of their chunked fill
+    (2,048 ranks) and every sort tile holds long runs of equal keys."""
     pic = gpu
     rng = np.random.default_rng(nx * 7 + n)
     mesh = pic.TwoDGrid(4 * np.pi, nx, 4 * np.pi, ny)
