@@ -179,14 +179,9 @@ __global__ __launch_bounds__(SEG_T, 8) void k_seg_f32(SegF32 a) {
     const int tid = threadIdx.x;
     const int nx = a.nx, ny = a.ny;
     const int mx = nx - 1, my = ny - 1;  // nx, ny are powers of two
-    // XCD-aware order: workgroups are dealt to the 8 XCDs round-robin, so
-    // block b takes slice (b % 8) nb/8 + b/8; each XCD then works on one
-    // contiguous band of segments, whose overlapping field windows and
-    // source slots share its L2 (the pass ran ~1 % faster on the M6
-    // workload)
-    const int nbk = (int)gridDim.x;
-    const int lb = nbk % 8 == 0 ? (int)(blockIdx.x % 8) * (nbk / 8) + (int)(blockIdx.x / 8)
-                                : (int)blockIdx.x;
+    // XCD-aware block order (the pass ran ~1 % faster on the M6 workload,
+    // the pass that applies a sort's permutation ~7 %)
+    const int lb = seg_block_order();
     const int s = lb / a.bps, bi = lb - s * a.bps;
     const int ce = min(64 * s + 64, a.ncells);
     const int64_t s0 = a.off[64 * s], s1 = a.off[ce];

@@ -55,6 +55,16 @@ __device__ Win seg_window(int s, int nx, int ny, int ncells, int margin, int cap
     return w;
 }
 
+// XCD-aware block order of the segment passes: workgroups are dealt to the
+// 8 XCDs round-robin, so block b takes slice (b % 8) nb/8 + b/8 of the
+// nb = gridDim.x slices; each XCD then works on one contiguous band of
+// segments, whose overlapping windows and source slots share its L2.
+__device__ __forceinline__ int seg_block_order() {
+    const int nbk = (int)gridDim.x;
+    return nbk % 8 == 0 ? (int)(blockIdx.x % 8) * (nbk / 8) + (int)(blockIdx.x / 8)
+                        : (int)blockIdx.x;
+}
+
 // End of a deferred-list kernel: block 0 adds the list length to the
 // slow-path counter; the last block to arrive (every block has read the
 // count by then) clears count and arrivals for the next pass.
