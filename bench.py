@@ -159,7 +159,7 @@ WORKLOADS = {
                      warmup=10, sampler="maxwellian_device", bz=20.0, sort=(32, 16)),
     "large_m6": dict(name="landau2d2v-m6-512x512-5e8-dt0.05-200steps", grid=512,
                      n=500_000_000, kernel="m6", alpha=0.05, k=0.5, dt=0.05, steps=200,
-                     warmup=5, sampler="landau_device", sort=(2, 1)),
+                     warmup=5, sampler="landau_device", sort=(3, 1)),
     # the other shapes of the reference's Landau example
     # (examples/landau_damping_poisson_cic_m6.jl:88-103) on the M6 workload
     "landau_ngp": dict(name="landau2d2v-ngp-128x128-1e7-dt0.05-400steps", grid=128,

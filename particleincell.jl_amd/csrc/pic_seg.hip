@@ -674,6 +674,11 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f64(SegF64 a) {
 // kernel `kd`.
 template <class Kern, class Defer>
 static void seg32_launch(const Ctx *c, SegF32 a, Kern k, Defer kd, size_t sm) {
+    // windows above 64 KiB need the kernel's dynamic LDS limit raised; a
+    // failure here fails the launch, which hipGetLastError reports
+    if (sm > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     a.bps = seg_bps(c, reinterpret_cast<const void *>(k), sm);
     dim3 grid((unsigned)(((c->nx * c->ny + 63) / 64) * a.bps));
     hipLaunchKernelGGL(k, grid, dim3(SEG_T), sm, c->stream, a);
@@ -751,7 +756,7 @@ int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     // 12.8 -> 11.6 ms (5e8, 512^2); the deposit-only pass ~10 % slower with
     // two (its particles are in their sorted lanes after a sort); the CIC,
     // NGP, TSC and M4 passes 4-8 % slower (4 or 16 atomics per particle do
-    // not pay for the fourth block per CU that the larger LDS window costs).
+    // not pay for the blocks per CU that the larger LDS window costs).
     a.tiles = acc == 2 && kernel == PIC_M6 && (mode == 0 || mode == 1) ? 2 : 1;
     const size_t per_node = (gather ? (c->bz_mode == PIC_BZ_GRID ? 12 : 8) : 0) +
                             (mode == 2 ? 0 : acc == 2 ? 4 * a.tiles : 8);

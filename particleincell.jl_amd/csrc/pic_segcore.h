@@ -106,10 +106,13 @@ constexpr int DEFER_T = 256, DEFER_BLOCKS = 256;
 // segment then keeps a drift margin of 11 cells (FP32) or 7 cells (FP64).
 constexpr size_t SEG_LDS_BYTES = 40 * 1024;
 // The FP32 M6 passes with two fixed-point charge tiles hold the field (8 B)
-// and two int32 tiles (8 B) per node: 53 KiB keeps the M6 window of a 64-cell
-// row segment at the 14-cell drift margin of 40 KiB with one tile, at three
-// blocks per CU.
-constexpr size_t SEG32_FIXED_LDS_BYTES = 53 * 1024;
+// and two int32 tiles (8 B) per node.  80 KiB (two blocks per CU) gives the
+// M6 window of a 64-cell row segment a 20-cell drift margin.  Measured on
+// MI355X against 53 KiB (three blocks per CU, margin 14): the same pass time
+// at 1e7 particles on 128^2 with ~1000x fewer deferred particles, and on
+// 512^2 (5e8 particles, ~2 cells of drift per step) a sort every 3 steps
+// instead of every 2, 15 % more particle-steps/s.
+constexpr size_t SEG32_FIXED_LDS_BYTES = 80 * 1024;
 
 // Blocks per segment: as many as keep every CU busy in one round of
 // resident blocks (the occupancy of this kernel instantiation at this LDS
