@@ -197,7 +197,7 @@ WORKLOADS = {
                         adjoint=dict(ckpt=20, fd=True, f64=True)),
     "adjoint_large_m6": dict(name="adjoint-landau2d2v-m6-512x512-5e8-dt0.05-200steps-ckpt25",
                              grid=512, n=500_000_000, kernel="m6", alpha=0.05, k=0.5, dt=0.05,
-                             steps=200, warmup=0, sampler="landau_device", sort=(2, 0),
+                             steps=200, warmup=0, sampler="landau_device", sort=(3, 0),
                              adjoint=dict(ckpt=25, fd=False, f64=False)),
 }
 

