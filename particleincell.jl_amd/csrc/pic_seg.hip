@@ -168,11 +168,22 @@ __device__ __forceinline__ void charge32_fixed(float fx, float fy, int ci, int c
 // whole (gather outside) or for its deposit only (drifted outside), and
 // k_defer_f32 completes it from global memory after this kernel.
 //
-// At most 64 VGPRs (eight waves per SIMD): four 40 KiB blocks then share a
-// CU, against three at the 66 registers the compiler picks unbounded (the M6
-// pass with one charge tile ran ~1.5 % faster so).
+// Register budget: the passes with 40 KiB windows are held to 64 VGPRs
+// (eight waves per SIMD), so that four blocks share a CU instead of the three
+// that the 66 registers the compiler picks unbounded allow (the M6 pass with
+// one charge tile ran ~1.5 % faster so).  The M6 drift-and-deposit passes
+// with two charge tiles have 80 KiB windows, two blocks per CU: they are
+// bounded at four waves per SIMD only (measured ~1.5 % faster at 1e7
+// particles on 128^2 and ~4 % at 5e8 on 512^2 than under the 64-register
+// bound).
+template <int K, bool DRIFT, bool DEPOSIT, int ACC>
+constexpr int seg32_waves() {
+    return K == PIC_M6 && ACC == 2 && DRIFT && DEPOSIT ? 4 : 8;
+}
+
 template <int K, bool GATHER, bool DRIFT, bool DEPOSIT, int BZ, int ACC, bool PERM>
-__global__ __launch_bounds__(SEG_T, 8) void k_seg_f32(SegF32 a) {
+__global__ __launch_bounds__(SEG_T, (seg32_waves<K, DRIFT, DEPOSIT, ACC>())) void k_seg_f32(
+    SegF32 a) {
     using St = SegStencil<K>;
     constexpr int S = St::S, HL = St::HL;
     extern __shared__ __align__(16) char smem_s32[];
