@@ -172,18 +172,28 @@ __device__ __forceinline__ void charge32_fixed(float fx, float fy, int ci, int c
 // (eight waves per SIMD), so that four blocks share a CU instead of the three
 // that the 66 registers the compiler picks unbounded allow (the M6 pass with
 // one charge tile ran ~1.5 % faster so).  The M6 drift-and-deposit passes
-// with two charge tiles have 80 KiB windows, two blocks per CU: they are
-// bounded at four waves per SIMD only (measured ~1.5 % faster at 1e7
-// particles on 128^2 and ~4 % at 5e8 on 512^2 than under the 64-register
-// bound).
+// with two charge tiles have 80 KiB windows, at most two blocks per CU: they
+// are bounded at four waves per SIMD only (with 512-thread blocks this was
+// ~1.5 % faster at 1e7 particles on 128^2 and ~4 % at 5e8 on 512^2 than the
+// 64-register bound).
 template <int K, bool DRIFT, bool DEPOSIT, int ACC>
 constexpr int seg32_waves() {
     return K == PIC_M6 && ACC == 2 && DRIFT && DEPOSIT ? 4 : 8;
 }
 
+// Block size: the M6 two-tile passes run 1024 threads per block, so that
+// their two 80 KiB blocks per CU hold 32 waves instead of 16 (measured 1.5 %
+// faster at 1e7 particles on 128^2, 4 % at 5e8 on 512^2; the pass after a
+// sort, at 74 VGPRs, then runs one such block per CU); the others SEG_T.
+template <int K, bool DRIFT, bool DEPOSIT, int ACC>
+constexpr int seg32_threads() {
+    return K == PIC_M6 && ACC == 2 && DRIFT && DEPOSIT ? 1024 : SEG_T;
+}
+
 template <int K, bool GATHER, bool DRIFT, bool DEPOSIT, int BZ, int ACC, bool PERM>
-__global__ __launch_bounds__(SEG_T, (seg32_waves<K, DRIFT, DEPOSIT, ACC>())) void k_seg_f32(
-    SegF32 a) {
+__global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
+                             (seg32_waves<K, DRIFT, DEPOSIT, ACC>())) void k_seg_f32(SegF32 a) {
+    constexpr int BT = seg32_threads<K, DRIFT, DEPOSIT, ACC>();
     using St = SegStencil<K>;
     constexpr int S = St::S, HL = St::HL;
     extern __shared__ __align__(16) char smem_s32[];
@@ -219,7 +229,7 @@ __global__ __launch_bounds__(SEG_T, (seg32_waves<K, DRIFT, DEPOSIT, ACC>())) voi
     int *itile = iacc + ((a.tiles == 2 && (tid & 1)) ? cap : 0);
     if (fits) {
         const int nt = W.tw * W.th;
-        for (int t = tid; t < nt; t += SEG_T) {
+        for (int t = tid; t < nt; t += BT) {
             const int u = t % W.tw, v = t / W.tw;
             const int64_t g = (int64_t)((W.x0 + u) & mx) + (int64_t)nx * ((W.y0 + v) & my);
             if (GATHER) {
@@ -241,7 +251,7 @@ __global__ __launch_bounds__(SEG_T, (seg32_waves<K, DRIFT, DEPOSIT, ACC>())) voi
     }
     __syncthreads();
 
-    for (uint32_t p = (uint32_t)p0 + tid; p < (uint32_t)p1; p += SEG_T) {
+    for (uint32_t p = (uint32_t)p0 + tid; p < (uint32_t)p1; p += BT) {
         const uint32_t src = PERM ? a.perm[p] : p;
         const int c = a.cell_in[src];
         int ci = c & mx, cj = c >> a.lognx;
@@ -300,7 +310,7 @@ __global__ __launch_bounds__(SEG_T, (seg32_waves<K, DRIFT, DEPOSIT, ACC>())) voi
     if (!DEPOSIT || !fits) return;
     __syncthreads();
     const int nt = W.tw * W.th;
-    for (int t = tid; t < nt; t += SEG_T) {
+    for (int t = tid; t < nt; t += BT) {
         const double q = ACC == 0   ? (double)hi[t] + (double)lo[t]
                          : ACC == 1 ? dacc[t]
                                     : ldexp((double)((int64_t)iacc[t] +
@@ -684,32 +694,33 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f64(SegF64 a) {
 // Launches the pass kernel `k` over every segment, then its deferred-list
 // kernel `kd`.
 template <class Kern, class Defer>
-static void seg32_launch(const Ctx *c, SegF32 a, Kern k, Defer kd, size_t sm) {
+static void seg32_launch(const Ctx *c, SegF32 a, Kern k, Defer kd, size_t sm, int bt) {
     // windows above 64 KiB need the kernel's dynamic LDS limit raised; a
     // failure here fails the launch, which hipGetLastError reports
     if (sm > 65536)
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-    a.bps = seg_bps(c, reinterpret_cast<const void *>(k), sm);
+    a.bps = seg_bps(c, reinterpret_cast<const void *>(k), sm, bt);
     dim3 grid((unsigned)(((c->nx * c->ny + 63) / 64) * a.bps));
-    hipLaunchKernelGGL(k, grid, dim3(SEG_T), sm, c->stream, a);
+    hipLaunchKernelGGL(k, grid, dim3(bt), sm, c->stream, a);
     hipLaunchKernelGGL(kd, dim3(DEFER_BLOCKS), dim3(DEFER_T), 0, c->stream, a);
 }
 
 template <int K, bool G, bool DR, bool D, int ACC, bool P>
 static void seg32_bz(const Ctx *c, const SegF32 &a, int bz, size_t sm) {
+    constexpr int bt = seg32_threads<K, DR, D, ACC>();
     switch (bz) {
         case PIC_BZ_NONE:
             seg32_launch(c, a, k_seg_f32<K, G, DR, D, PIC_BZ_NONE, ACC, P>,
-                         k_defer_f32<K, G, DR, D, PIC_BZ_NONE, P>, sm);
+                         k_defer_f32<K, G, DR, D, PIC_BZ_NONE, P>, sm, bt);
             break;
         case PIC_BZ_UNIFORM:
             seg32_launch(c, a, k_seg_f32<K, G, DR, D, PIC_BZ_UNIFORM, ACC, P>,
-                         k_defer_f32<K, G, DR, D, PIC_BZ_UNIFORM, P>, sm);
+                         k_defer_f32<K, G, DR, D, PIC_BZ_UNIFORM, P>, sm, bt);
             break;
         default:
             seg32_launch(c, a, k_seg_f32<K, G, DR, D, PIC_BZ_GRID, ACC, P>,
-                         k_defer_f32<K, G, DR, D, PIC_BZ_GRID, P>, sm);
+                         k_defer_f32<K, G, DR, D, PIC_BZ_GRID, P>, sm, bt);
     }
 }
 

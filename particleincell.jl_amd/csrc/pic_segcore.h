@@ -119,10 +119,10 @@ constexpr size_t SEG32_FIXED_LDS_BYTES = 80 * 1024;
 // size), at least one.  A larger grid than one round leaves a partial last
 // round; a grid with more segments than resident blocks takes several full
 // rounds.
-inline int seg_bps(const Ctx *c, const void *kernel, size_t sm) {
+inline int seg_bps(const Ctx *c, const void *kernel, size_t sm, int threads = SEG_T) {
     const int nseg = (c->nx * c->ny + 63) / 64;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, SEG_T, sm) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, sm) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
     int bps = c->ncu * per_cu / nseg;
