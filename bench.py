@@ -97,10 +97,12 @@ def copy_probe(mib, reps=5):
                       "GBs": 2 * n * 4 / dt / 1e9}), flush=True)
 
 
-def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_acc=2, bz=None):
+def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_acc=2, bz=None,
+             tiles=4):
     """Warmup, then the timed run from the initial state. Returns dict."""
     sim = pic.Simulation(mesh, p0, kernel, state=state, bz=bz)
     sim.set_f32_accumulation(f32_acc)
+    sim.set_f32_tiles(tiles)
     sim.sort(lanes=True)
     if warmup:
         sim.run(dt, warmup, sort_every=sort_every)
@@ -159,7 +161,7 @@ WORKLOADS = {
                      warmup=10, sampler="maxwellian_device", bz=20.0, sort=(32, 16)),
     "large_m6": dict(name="landau2d2v-m6-512x512-5e8-dt0.05-200steps", grid=512,
                      n=500_000_000, kernel="m6", alpha=0.05, k=0.5, dt=0.05, steps=200,
-                     warmup=5, sampler="landau_device", sort=(3, 1)),
+                     warmup=5, sampler="landau_device", sort=(3, 1), tiles=2),
     # the other shapes of the reference's Landau example
     # (examples/landau_damping_poisson_cic_m6.jl:88-103) on the M6 workload
     "landau_ngp": dict(name="landau2d2v-ngp-128x128-1e7-dt0.05-400steps", grid=128,
@@ -197,7 +199,7 @@ WORKLOADS = {
                         adjoint=dict(ckpt=20, fd=True, f64=True)),
     "adjoint_large_m6": dict(name="adjoint-landau2d2v-m6-512x512-5e8-dt0.05-200steps-ckpt25",
                              grid=512, n=500_000_000, kernel="m6", alpha=0.05, k=0.5, dt=0.05,
-                             steps=200, warmup=0, sampler="landau_device", sort=(3, 0),
+                             steps=200, warmup=0, sampler="landau_device", sort=(3, 0), tiles=2,
                              adjoint=dict(ckpt=25, fd=False, f64=False)),
 }
 
@@ -243,6 +245,7 @@ def bench_adjoint(args, W, pic, entry, mesh, held, n, steps, warmup):
         if args.only not in ("both", name):
             continue
         sim = pic.Simulation(mesh, p0, kernel, state=state)
+        sim.set_f32_tiles(W.get("tiles", 4))
         if on_device:
             del p0
             torch.cuda.empty_cache()
@@ -644,7 +647,7 @@ def main():
         log(f"bench: {name} path, warmup {warmup}, timed {steps} steps")
         sort_every = args.sort_every_f32 if name == "f32" else args.sort_every_f64
         res[name] = run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every,
-                             args.f32_acc, bz)
+                             args.f32_acc, bz, W.get("tiles", 4))
         r = res[name]
         log(f"bench: {name} {r['seconds']*1e3:.2f} ms total, pass {r['pass_ms_avg']*1e3:.1f} us,"
             f" solve {r['solve_ms_avg']*1e3:.1f} us, deferred {r['slow_chunks']}")
@@ -688,7 +691,8 @@ def main():
                    "alpha": alpha, "k": kx, "dt": dt, "bz": bz,
                    "state": "int32 cell + fp32 offset + fp32 velocity" if head == "f32" else
                    "fp64 soa", "sort_every": args.sort_every_f32 if head == "f32" else
-                   args.sort_every_f64, "parallelism": "single-gpu"},
+                   args.sort_every_f64, "f32_charge_tiles": W.get("tiles", 4),
+                   "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": f"k_seg_{head}<{kname}> gather+kick+drift+deposit pass" + tiles,

@@ -878,6 +878,11 @@ class Simulation:
         sums (pic_set_f32_accumulation)."""
         L.call("pic_set_f32_accumulation", self.ctx.h, int(mode))
 
+    def set_f32_tiles(self, tiles):
+        """Copies (1, 2 or 4) of the fixed-point charge tile in the M6 FP32
+        drift-and-deposit passes (pic_set_f32_tiles)."""
+        L.call("pic_set_f32_tiles", self.ctx.h, int(tiles))
+
     def timing(self):
         """(pass_ms_total, passes, solve_ms_total) of pic_run since timing
         was enabled (HIP events on the context stream)."""

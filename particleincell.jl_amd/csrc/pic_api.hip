@@ -782,3 +782,11 @@ extern "C" int pic_set_f32_accumulation(pic_ctx *ctx, int mode) {
     ctx_of(ctx)->f32_acc = mode;
     return PIC_OK;
 }
+
+extern "C" int pic_set_f32_tiles(pic_ctx *ctx, int tiles) {
+    PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
+    PIC_CHECK(tiles == 1 || tiles == 2 || tiles == 4, PIC_ERR_ARG, "tile copies %d (1, 2 or 4)",
+              tiles);
+    ctx_of(ctx)->f32_tiles = tiles;
+    return PIC_OK;
+}

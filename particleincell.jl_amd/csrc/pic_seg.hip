@@ -75,7 +75,7 @@ struct SegF32 {
     unsigned long long *slow;
     uint32_t *dlist;  // deferred particles and their count
     unsigned *dcount;
-    int tiles;  // fixed-point charge tiles (1 or 2, see k_seg_f32)
+    int tiles;  // fixed-point charge tile copies (1, 2 or 4, see k_seg_f32)
 };
 
 // C = 1.5 * 2^23 * u of the (hi, lo) split for a block of `count`
@@ -105,6 +105,12 @@ __device__ __forceinline__ int fixed_exponent(float w, double qmax, int64_t coun
     e1 = (m1 == 0.5 ? e1 - 1 : e1) - 30;  // ceil(log2(.)) - 30
     e2 = (m2 == 0.5 ? e2 - 1 : e2) - 22;
     return e1 > e2 ? e1 : e2;
+}
+
+__device__ __forceinline__ int64_t tile_sum(const int *iacc, int t, int cap, int tiles) {
+    int64_t q = 0;
+    for (int k = 0; k < tiles; ++k) q += iacc[t + k * cap];
+    return q;
 }
 
 // Calls add(u, v, q) with the FP32 charge q of stencil node (u, v) of a
@@ -222,11 +228,11 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
     float *hi = reinterpret_cast<float *>(dep), *lo = hi + cap;
     double *dacc = reinterpret_cast<double *>(dep);
     int *iacc = reinterpret_cast<int *>(dep);
-    // fixed-point tiles: with a.tiles == 2 even lanes add into iacc[0, cap)
-    // and odd lanes into iacc[cap, 2 cap), so that two neighbouring lanes
-    // whose particles drifted into one cell do not add to one LDS word in
-    // the same instruction
-    int *itile = iacc + ((a.tiles == 2 && (tid & 1)) ? cap : 0);
+    // fixed-point tiles: lane l adds into copy l mod a.tiles,
+    // iacc[k cap, (k + 1) cap), so that neighbouring lanes whose particles
+    // drifted into one cell do not add to one LDS word in the same
+    // instruction
+    int *itile = iacc + (tid & (a.tiles - 1)) * cap;
     if (fits) {
         const int nt = W.tw * W.th;
         for (int t = tid; t < nt; t += BT) {
@@ -244,7 +250,7 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
                     dacc[t] = 0.0;
                 } else {
                     iacc[t] = 0;
-                    if (a.tiles == 2) iacc[t + cap] = 0;
+                    for (int k = 1; k < a.tiles; ++k) iacc[t + k * cap] = 0;
                 }
             }
         }
@@ -313,9 +319,7 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
     for (int t = tid; t < nt; t += BT) {
         const double q = ACC == 0   ? (double)hi[t] + (double)lo[t]
                          : ACC == 1 ? dacc[t]
-                                    : ldexp((double)((int64_t)iacc[t] +
-                                                     (a.tiles == 2 ? iacc[t + cap] : 0)),
-                                            qe);
+                                    : ldexp((double)tile_sum(iacc, t, cap, a.tiles), qe);
         if (q != 0.0) {
             const int u = t % W.tw, v = t / W.tw;
             unsafeAtomicAdd(&a.acc[(int64_t)((W.x0 + u) & mx) + (int64_t)nx * ((W.y0 + v) & my)],
@@ -773,16 +777,18 @@ int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     a.margin = 32;
     const bool gather = mode == 1 || mode == 2;
     const int acc = c->f32_acc;
-    // two fixed-point tiles for the M6 passes that drift and then deposit.
-    // Measured: the M6 fused pass 216 -> 208 us (1e7 particles, 128^2) and
-    // 12.8 -> 11.6 ms (5e8, 512^2); the deposit-only pass ~10 % slower with
-    // two (its particles are in their sorted lanes after a sort); the CIC,
-    // NGP, TSC and M4 passes 4-8 % slower (4 or 16 atomics per particle do
-    // not pay for the blocks per CU that the larger LDS window costs).
-    a.tiles = acc == 2 && kernel == PIC_M6 && (mode == 0 || mode == 1) ? 2 : 1;
+    // copies of the fixed-point tile (pic_set_f32_tiles, default 4) for the
+    // M6 passes that drift and then deposit.  Measured with two: the M6
+    // fused pass 216 -> 208 us (1e7 particles, 128^2) and 12.8 -> 11.6 ms
+    // (5e8, 512^2); four: 200 -> 190 us at 1e7; the deposit-only pass ~10 %
+    // slower with two (its particles are in their sorted lanes after a
+    // sort); the CIC, NGP, TSC and M4 passes 4-8 % slower (4 or 16 atomics
+    // per particle do not pay for the blocks per CU that the larger LDS
+    // window costs).
+    a.tiles = acc == 2 && kernel == PIC_M6 && (mode == 0 || mode == 1) ? c->f32_tiles : 1;
     const size_t per_node = (gather ? (c->bz_mode == PIC_BZ_GRID ? 12 : 8) : 0) +
                             (mode == 2 ? 0 : acc == 2 ? 4 * a.tiles : 8);
-    a.cap = (int)((a.tiles == 2 ? SEG32_FIXED_LDS_BYTES : SEG_LDS_BYTES) /
+    a.cap = (int)((a.tiles > 1 ? SEG32_FIXED_LDS_BYTES : SEG_LDS_BYTES) /
                   (per_node ? per_node : 8)) &
             ~15;
     a.slow = c->slow_chunks;

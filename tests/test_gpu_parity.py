@@ -485,6 +485,27 @@ def test_f32_segment_pass_bitwise(gpu, O, kernel, acc, nx, ny):
     of the oracle's FP64 sum of the same FP32 contributions; that of the
     int32 fixed-point windows (acc 2) within the rounding bound of every
     node.  Particles are spread wide enough that some leave their window."""
+    _check_f32_segment_pass(gpu, O, kernel, acc, nx, ny, None)
+
+
+@pytest.mark.parametrize("tiles", [1, 2, 4])
+def test_f32_segment_pass_tile_copies(gpu, O, tiles):
+    """The M6 fixed-point pass with 1, 2 or 4 copies of the charge tile
+    (pic_set_f32_tiles): the same bound against the oracle for every count."""
+    _check_f32_segment_pass(gpu, O, "m6", 2, 128, 128, tiles)
+
+
+def test_f32_tiles_rejects_other_counts(gpu):
+    pic = gpu
+    mesh = pic.TwoDGrid(4 * np.pi, 16, 4 * np.pi, 16)
+    ctx = pic.Context(mesh, capacity=16)
+    for bad in (0, 3, 8):
+        with pytest.raises(pic._lib.PicError, match="tile copies"):
+            pic._lib.call("pic_set_f32_tiles", ctx.h, bad)
+    ctx.close()
+
+
+def _check_f32_segment_pass(gpu, O, kernel, acc, nx, ny, tiles):
     pic = gpu
     rng = np.random.default_rng(41 + nx)
     n = 200000
@@ -503,6 +524,8 @@ def test_f32_segment_pass_bitwise(gpu, O, kernel, acc, nx, ny):
     ctx.set_grid(pic._lib.PIC_GRID_EY, ey)
     ctx.set_bz(pic._lib.PIC_BZ_NONE)
     pic._lib.call("pic_set_f32_accumulation", ctx.h, acc)
+    if tiles is not None:
+        pic._lib.call("pic_set_f32_tiles", ctx.h, tiles)
     e32 = np.empty(2 * nx * ny, np.float32)
     e32[0::2] = ex.ravel(order="F")
     e32[1::2] = ey.ravel(order="F")
