@@ -26,13 +26,15 @@
 //  2 fixed-point int32 LDS tiles: every contribution is rounded to a multiple
 //    of a power-of-two quantum u (per block, 2^-30 of the largest sum a node
 //    of the block can reach) and summed exactly with ds_add_u32; the flush
-//    converts to FP64.  Passes that drift before they deposit keep two
-//    copies of the tile, one for the even and one for the odd lanes: a
-//    ds_add_u32 whose lanes address one word serialises them, and once
-//    particles drift two neighbouring lanes often sit in one cell.  Measured on MI355X with the 6x6 stencil's address pattern, a
+//    converts to FP64.  The M6 passes that drift before they deposit keep
+//    2 or 4 copies of the tile (Ctx::f32_tiles), lane l adding into copy
+//    l mod copies: a ds_add_u32 whose lanes address one word serialises
+//    them, and once particles drift neighbouring lanes often sit in one
+//    cell.  Measured on MI355X with the 6x6 stencil's address pattern, a
 //    64-lane ds_add_u32 takes ~4.1 clocks per CU with the lanes in 64
-//    adjacent cells, ~8.3 after a drift of half a cell (sigma), ~6.3 with the
-//    two copies; bank conflicts alone (rows apart) cost little (~4.9).
+//    adjacent cells, ~8.3 after a drift of half a cell (sigma), ~6.3 with
+//    two copies, ~5.6 with four; bank conflicts alone (rows apart) cost
+//    little (~4.9).
 //  1 FP64 LDS tiles: every FP32 contribution is added exactly converted to
 //    FP64 (ds_add_f64);
 //  0 compensated (hi, lo) FP32 tile pairs.  Every contribution q is split as
@@ -177,20 +179,21 @@ __device__ __forceinline__ void charge32_fixed(float fx, float fy, int ci, int c
 // Register budget: the passes with 40 KiB windows are held to 64 VGPRs
 // (eight waves per SIMD), so that four blocks share a CU instead of the three
 // that the 66 registers the compiler picks unbounded allow (the M6 pass with
-// one charge tile ran ~1.5 % faster so).  The M6 drift-and-deposit passes
-// with two charge tiles have 80 KiB windows, at most two blocks per CU: they
-// are bounded at four waves per SIMD only (with 512-thread blocks this was
-// ~1.5 % faster at 1e7 particles on 128^2 and ~4 % at 5e8 on 512^2 than the
-// 64-register bound).
+// one charge tile ran ~1.5 % faster so).  The M6 fixed-point drift-and-
+// deposit passes (80 KiB windows with 2 or 4 copies of the charge tile, at
+// most two blocks per CU) are bounded at four waves per SIMD only (with two
+// copies and 512-thread blocks this was ~1.5 % faster at 1e7 particles on
+// 128^2 and ~4 % at 5e8 on 512^2 than the 64-register bound).
 template <int K, bool DRIFT, bool DEPOSIT, int ACC>
 constexpr int seg32_waves() {
     return K == PIC_M6 && ACC == 2 && DRIFT && DEPOSIT ? 4 : 8;
 }
 
-// Block size: the M6 two-tile passes run 1024 threads per block, so that
-// their two 80 KiB blocks per CU hold 32 waves instead of 16 (measured 1.5 %
-// faster at 1e7 particles on 128^2, 4 % at 5e8 on 512^2; the pass after a
-// sort, at 74 VGPRs, then runs one such block per CU); the others SEG_T.
+// Block size: the M6 fixed-point drift-and-deposit passes run 1024 threads
+// per block, so that their two 80 KiB blocks per CU hold 32 waves instead of
+// 16 (measured with two tile copies: 1.5 % faster at 1e7 particles on 128^2,
+// 4 % at 5e8 on 512^2; the pass after a sort, at 74 VGPRs, then runs one
+// such block per CU); the others SEG_T.
 template <int K, bool DRIFT, bool DEPOSIT, int ACC>
 constexpr int seg32_threads() {
     return K == PIC_M6 && ACC == 2 && DRIFT && DEPOSIT ? 1024 : SEG_T;

@@ -105,9 +105,10 @@ constexpr int DEFER_T = 256, DEFER_BLOCKS = 256;
 // 512-thread blocks share a CU's 160 KiB; an M6 window of a 64-cell row
 // segment then keeps a drift margin of 11 cells (FP32) or 7 cells (FP64).
 constexpr size_t SEG_LDS_BYTES = 40 * 1024;
-// The FP32 M6 passes with two fixed-point charge tiles hold the field (8 B)
-// and two int32 tiles (8 B) per node.  80 KiB (two blocks per CU) gives the
-// M6 window of a 64-cell row segment a 20-cell drift margin.  Measured on
+// The FP32 M6 passes with several fixed-point charge tiles hold the field
+// (8 B) and 2 or 4 int32 tiles (8 or 16 B) per node.  80 KiB (two blocks per
+// CU) gives the M6 window of a 64-cell row segment a 20-cell drift margin
+// with two tiles, 14 with four.  Measured with two tiles on
 // MI355X against 53 KiB (three blocks per CU, margin 14): the same pass time
 // at 1e7 particles on 128^2 with ~1000x fewer deferred particles, and on
 // 512^2 (5e8 particles, ~2 cells of drift per step) a sort every 3 steps
