@@ -582,6 +582,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["f32", "f64", "both"], default="both")
+    ap.add_argument("--f32-tiles", type=int, choices=[1, 2, 4], default=None,
+                    help="copies of the FP32 M6 fixed-point charge tile (default: the "
+                         "workload's, else 4)")
     ap.add_argument("--f32-acc", type=int, choices=[0, 1, 2], default=2,
                     help="FP32 charge tiles: 0 compensated FP32 pairs, 1 FP64 sums, "
                          "2 int32 fixed point")
@@ -602,7 +605,9 @@ def main():
     pic = entry.load_package()
     from picmi355x import diagnostics
 
-    W = WORKLOADS[args.workload]
+    W = dict(WORKLOADS[args.workload])
+    if args.f32_tiles is not None:
+        W["tiles"] = args.f32_tiles
     if W.get("ua"):
         bench_ua(args, W, pic, entry, W["steps"] if args.steps is None else args.steps,
                  W["warmup"] if args.warmup is None else args.warmup)
