@@ -26,9 +26,10 @@
 //  2 fixed-point int32 LDS tiles: every contribution is rounded to a multiple
 //    of a power-of-two quantum u (per block, 2^-30 of the largest sum a node
 //    of the block can reach) and summed exactly with ds_add_u32; the flush
-//    converts to FP64.  The M6 passes that drift before they deposit keep
-//    2 or 4 copies of the tile (Ctx::f32_tiles), lane l adding into copy
-//    l mod copies: a ds_add_u32 whose lanes address one word serialises
+//    converts to FP64.  The M6, TSC and M4 passes that drift before they
+//    deposit keep 2 or 4 copies of the tile (Ctx::f32_tiles), lane l adding
+//    into copy l mod copies: a ds_add_u32 whose lanes address one word
+//    serialises
 //    them, and once particles drift neighbouring lanes often sit in one
 //    cell.  Measured on MI355X with the 6x6 stencil's address pattern, a
 //    64-lane ds_add_u32 takes ~4.1 clocks per CU with the lanes in 64
@@ -184,9 +185,20 @@ __device__ __forceinline__ void charge32_fixed(float fx, float fy, int ci, int c
 // most two blocks per CU) are bounded at four waves per SIMD only (with two
 // copies and 512-thread blocks this was ~1.5 % faster at 1e7 particles on
 // 128^2 and ~4 % at 5e8 on 512^2 than the 64-register bound).
+// Shapes whose fixed-point drift-and-deposit passes keep several copies of
+// the charge tile (36 or 16 LDS atomics per particle; measured on MI355X
+// with four copies: M6 +4 %, TSC +3 %, M4 +5 % on 1e7 particles, 128^2).
+template <int K>
+constexpr bool multi_tile_shape() {
+    return K == PIC_M6 || K == PIC_TSC || K == PIC_M4;
+}
+inline bool multi_tile_shape(int kernel) {
+    return kernel == PIC_M6 || kernel == PIC_TSC || kernel == PIC_M4;
+}
+
 template <int K, bool DRIFT, bool DEPOSIT, int ACC>
 constexpr int seg32_waves() {
-    return K == PIC_M6 && ACC == 2 && DRIFT && DEPOSIT ? 4 : 8;
+    return multi_tile_shape<K>() && ACC == 2 && DRIFT && DEPOSIT ? 4 : 8;
 }
 
 // Block size: the M6 fixed-point drift-and-deposit passes run 1024 threads
@@ -196,7 +208,7 @@ constexpr int seg32_waves() {
 // such block per CU); the others SEG_T.
 template <int K, bool DRIFT, bool DEPOSIT, int ACC>
 constexpr int seg32_threads() {
-    return K == PIC_M6 && ACC == 2 && DRIFT && DEPOSIT ? 1024 : SEG_T;
+    return multi_tile_shape<K>() && ACC == 2 && DRIFT && DEPOSIT ? 1024 : SEG_T;
 }
 
 template <int K, bool GATHER, bool DRIFT, bool DEPOSIT, int BZ, int ACC, bool PERM>
@@ -781,14 +793,15 @@ int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     const bool gather = mode == 1 || mode == 2;
     const int acc = c->f32_acc;
     // copies of the fixed-point tile (pic_set_f32_tiles, default 4) for the
-    // M6 passes that drift and then deposit.  Measured with two: the M6
+    // M6, TSC and M4 passes that drift and then deposit.  Measured with two: the M6
     // fused pass 216 -> 208 us (1e7 particles, 128^2) and 12.8 -> 11.6 ms
     // (5e8, 512^2); four: 200 -> 190 us at 1e7; the deposit-only pass ~10 %
     // slower with two (its particles are in their sorted lanes after a
-    // sort); the CIC, NGP, TSC and M4 passes 4-8 % slower (4 or 16 atomics
-    // per particle do not pay for the blocks per CU that the larger LDS
-    // window costs).
-    a.tiles = acc == 2 && kernel == PIC_M6 && (mode == 0 || mode == 1) ? c->f32_tiles : 1;
+    // sort); CIC and NGP 4-8 % slower (4 atomics per particle do not pay for
+    // the blocks per CU that the larger LDS window costs), and so were TSC
+    // and M4 with two copies in 53 KiB, 512-thread blocks; with four copies,
+    // 80 KiB and 1024-thread blocks TSC and M4 gain 3-5 %.
+    a.tiles = acc == 2 && multi_tile_shape(kernel) && (mode == 0 || mode == 1) ? c->f32_tiles : 1;
     const size_t per_node = (gather ? (c->bz_mode == PIC_BZ_GRID ? 12 : 8) : 0) +
                             (mode == 2 ? 0 : acc == 2 ? 4 * a.tiles : 8);
     a.cap = (int)((a.tiles > 1 ? SEG32_FIXED_LDS_BYTES : SEG_LDS_BYTES) /

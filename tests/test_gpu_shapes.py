@@ -106,3 +106,23 @@ def test_shape_f32_loop_tracks_f64(gpu, name):
     e64, e32 = out[0][:, 0], out[1][:, 0]
     tol = 2e-3 if name == "ngp" else 1e-4
     assert np.max(np.abs(e32 - e64) / e64) < tol
+
+
+@pytest.mark.parametrize("name", ["tsc", "m4"])
+def test_shape_f32_tile_copies_agree(gpu, name):
+    """The FP32 loop with 1 and with 4 copies of the fixed-point charge tile
+    (pic_set_f32_tiles): the copies sum exactly, so the runs differ only by
+    the order of the FP64 flush atomics (which can flip the FP32 rounding of
+    a field value, and so move a particle by an ulp); field energies over 10
+    steps agree to 1e-6, where a lost or doubled copy would be off by more
+    than 1e-2."""
+    pic = gpu
+    mesh, p = landau_particles(pic, 200000, 64, 64)
+    out = []
+    for tiles in (1, 4):
+        sim = pic.Simulation(mesh, p, _shape(pic, name), state=pic._lib.PIC_STATE_F32)
+        sim.set_f32_tiles(tiles)
+        out.append(sim.run(0.1, 10, 4))
+        sim.close()
+    e1, e4 = out[0][:, 0], out[1][:, 0]
+    assert np.max(np.abs(e4 - e1) / e1) < 1e-6
