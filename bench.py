@@ -20,7 +20,8 @@ Other workloads, measured the same way: landau_cic (CIC, 64 x 64, 204,800
 particles, alpha 0.05, dt 0.1, 100 steps), strong_b (CIC, 256 x 256, 1e8
 particles, density 1 + 0.05 cos(x/2) cos(y/2), Maxwellian v, uniform bz =
 1/eps = 20, dt = eps/4, 1000 steps) and large_m6 (M6, 512 x 512, 5e8
-particles, alpha 0.05, 200 steps); the last two are sampled on the GPU.
+particles, alpha 0.05, 200 steps); the last two are sampled on the GPU by
+the library (pic_sample_maxwellian, pic_sample_landau).
 
 Prints one JSON line.  value = particle-steps/s of the FP32 path over the
 timed steps; roofline = the fused FP32 particle pass (the dominant kernel),
@@ -97,10 +98,60 @@ def copy_probe(mib, reps=5):
                       "GBs": 2 * n * 4 / dt / 1e9}), flush=True)
 
 
+class DeviceLandau:
+    """The Landau quiet start of sample!(pg, mesh, LandauDamping(alpha, kx))
+    made on the device (pic_sample_landau) whenever a run needs it, so that
+    the particle array never exists in host memory."""
+
+    def __init__(self, n, alpha, kx):
+        self.n, self.alpha, self.kx = n, alpha, kx
+
+    def simulation(self, pic, mesh, kernel, state, bz=None):
+        sim = pic.Simulation.landau(mesh, self.n, self.alpha, self.kx, kernel, state=state)
+        sim._set_bz(bz)
+        return sim
+
+    def reload(self, sim, state):
+        sim.sample_landau(self.n, self.alpha, self.kx, state)
+
+
+class DeviceMaxwellian:
+    """Density 1 + alpha cos(kx x) cos(ky y), Maxwellian velocities of width
+    1, made on the device from a seed (pic_sample_maxwellian)."""
+
+    def __init__(self, n, alpha, kx, ky, seed):
+        self.n, self.alpha, self.kx, self.ky, self.seed = n, alpha, kx, ky, seed
+
+    def simulation(self, pic, mesh, kernel, state, bz=None):
+        return pic.Simulation.maxwellian(mesh, self.n, self.alpha, self.kx, self.ky, kernel,
+                                         seed=self.seed, state=state, bz=bz)
+
+    def reload(self, sim, state):
+        sim.sample_maxwellian(self.n, self.alpha, self.kx, self.ky, self.seed, 1.0, state)
+
+
+DEVICE_SAMPLES = (DeviceLandau, DeviceMaxwellian)
+
+
+def _new_sim(pic, mesh, p0, kernel, state, bz=None):
+    if isinstance(p0, DEVICE_SAMPLES):
+        return p0.simulation(pic, mesh, kernel, state, bz)
+    return pic.Simulation(mesh, p0, kernel, state=state, bz=bz)
+
+
+def _reload(sim, p0, state):
+    if isinstance(p0, DEVICE_SAMPLES):
+        p0.reload(sim, state)
+    else:
+        sim.set_particles(p0, state=state)
+
+
 def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_acc=2, bz=None,
-             tiles=4):
-    """Warmup, then the timed run from the initial state. Returns dict."""
-    sim = pic.Simulation(mesh, p0, kernel, state=state, bz=bz)
+             tiles=4, acc_steps=0):
+    """Warmup, then the timed run from the initial state, then (acc_steps >
+    0) an untimed accuracy run of acc_steps from the same initial state,
+    whose energy history the damping rate is fitted on. Returns dict."""
+    sim = _new_sim(pic, mesh, p0, kernel, state, bz)
     sim.set_f32_accumulation(f32_acc)
     sim.set_f32_tiles(tiles)
     sim.sort(lanes=True)
@@ -108,7 +159,7 @@ def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_ac
         sim.run(dt, warmup, sort_every=sort_every)
     # restart from the initial state for the timed (and fitted) run; the
     # initial ordering (sort by cell, lane spread) is part of the set-up
-    sim.set_particles(p0, state=state)
+    _reload(sim, p0, state)
     sim.sort(lanes=True)
     sim.set_timing(True)
     slow0 = sim.slow_chunks()
@@ -142,6 +193,30 @@ def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_ac
         split[name] = ms_sum / 3
     sim.set_timing(False)
     out["split_ms"] = split
+    if state == 1:
+        # the fused pass under each FP32 charge accumulation: 0 compensated
+        # FP32 (hi, lo) pair tiles, 1 FP64 tiles, 2 int32 fixed-point tiles
+        accs = {}
+        for acc in (0, 1, 2):
+            sim.set_f32_accumulation(acc)
+            ms_sum = 0.0
+            for _ in range(3):
+                sim.sort(lanes=True)
+                sim.set_timing(True)
+                sim.step_pass(1, dt)
+                ms, cnt, _ = sim.timing()
+                ms_sum += ms / max(cnt, 1)
+            accs[acc] = ms_sum / 3
+        sim.set_timing(False)
+        sim.set_f32_accumulation(f32_acc)
+        out["acc_ms"] = accs
+    if acc_steps:
+        if acc_steps == steps:
+            out["energy_acc"] = e
+        else:
+            _reload(sim, p0, state)
+            sim.sort(lanes=True)
+            out["energy_acc"] = sim.run(dt, acc_steps, sort_every=sort_every)
     sim.close()
     return out
 
@@ -229,28 +304,20 @@ def bench_adjoint(args, W, pic, entry, mesh, held, n, steps, warmup):
     kernel = pic.M6() if W["kernel"] == "m6" else pic.CloudInCell()
     kname = "M6" if W["kernel"] == "m6" else "CIC"
     se = args.sort_every_f32
-    # dx0/dalpha of the sampler at fixed uniform numbers
-    x0 = p0[0] if isinstance(p0, np.ndarray) else p0[:, 0].cpu().numpy()
-    dxa = pic.sampling.dnewton_dalpha(x0, alpha, kx)
-    del x0
-    on_device = not isinstance(p0, np.ndarray)
-    cpu_sample = None
-    if on_device:  # what the CPU baseline needs, then free the device copy
-        import torch
-        cpu_sample = np.asfortranarray(p0[:200_000].cpu().numpy().T)
+    on_device = isinstance(p0, DeviceLandau)
+    dxa = None
+    if not on_device:  # dx0/dalpha of the sampler at fixed uniform numbers
+        dxa = pic.sampling.dnewton_dalpha(p0[0], alpha, kx)
     res = {}
     for name, state in (("f32", 1), ("f64", 0)):
         if name == "f64" and not A["f64"]:
             continue
         if args.only not in ("both", name):
             continue
-        sim = pic.Simulation(mesh, p0, kernel, state=state)
+        sim = _new_sim(pic, mesh, p0, kernel, state)
         sim.set_f32_tiles(W.get("tiles", 4))
-        if on_device:
-            del p0
-            torch.cuda.empty_cache()
-            free, total = torch.cuda.mem_get_info()
-            log(f"bench: device memory free {free / 2**30:.1f} GiB of {total / 2**30:.1f} GiB")
+        if dxa is None:  # the start made on the device: x0 read back once
+            dxa = pic.sampling.dnewton_dalpha(sim.particles()[0], alpha, kx)
         if warmup:
             log(f"bench: {name} adjoint warmup, {warmup} steps")
             sim.adjoint(dt, warmup, min(ckpt, warmup), se)
@@ -301,7 +368,7 @@ def bench_adjoint(args, W, pic, entry, mesh, held, n, steps, warmup):
         "warmup": warmup, "ms_per_step": r["seconds"] * 1e3 / steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": head,
         "data": "synthetic (Landau quiet start, Sobol)" if W["sampler"] == "quiet" else
-                "synthetic (Landau start sampled on the GPU, seed 1234)",
+                "synthetic (Landau quiet start, Sobol, made on the GPU)",
         "config": {"workload": W["name"], "grid": [nx, ny], "particles": n, "kernel": kname,
                    "alpha": alpha, "k": kx, "dt": dt, "checkpoint_every": ckpt,
                    "sort_every": se, "sweep": "forward with checkpoints + reverse adjoint",
@@ -348,7 +415,8 @@ def bench_adjoint(args, W, pic, entry, mesh, held, n, steps, warmup):
         O = entry.load_oracle()
         m = min(n, 200_000)
         steps_cpu = max(args.cpu_steps, 4)
-        q = np.asfortranarray(p0[:, :m]) if not on_device else np.asfortranarray(cpu_sample)
+        q = (np.asfortranarray(p0[:, :m]) if not on_device else
+             np.asfortranarray(pic.sampling.landau_damping(m, mesh, alpha, kx)))
         q[4] = mesh.dimx * mesh.dimy / m
         log(f"bench: cpu baseline, oracle adjoint ({kname}), {m} particles, {steps_cpu} steps")
         c0 = time.perf_counter()
@@ -591,6 +659,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run plain device copies of MIB MiB (byte-counter calibration)")
     ap.add_argument("--dump", default="", help="write the energy histories to this .npz")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the untimed full-length run the damping rate is fitted on")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -635,9 +705,9 @@ def main():
     if W["sampler"] == "quiet":
         p0 = pic.sampling.landau_damping(n, mesh, alpha, kx)
     elif W["sampler"] == "landau_device":
-        p0 = pic.sampling.landau_damping_device(n, mesh, alpha, kx, seed=1234)
+        p0 = DeviceLandau(n, alpha, kx)
     else:
-        p0 = pic.sampling.perturbed_maxwellian_device(n, mesh, alpha, kx, kx, seed=1234)
+        p0 = DeviceMaxwellian(n, alpha, kx, kx, seed=1234)
 
     if "adjoint" in W:
         held = [p0]
@@ -645,30 +715,37 @@ def main():
         bench_adjoint(args, W, pic, entry, mesh, held, n, steps, warmup)
         return
 
+    # the damping rate is fitted on an untimed run of the workload's full
+    # length (the reference's loop, examples/landau_damping_poisson_cic_m6.jl
+    # :69-91), whatever --steps is: a short timed run sees no maxima
+    landau = W["sampler"] in ("quiet", "landau_device")
+    acc_steps = W["steps"] if (landau and not args.no_accuracy) else 0
     res = {}
     for name, state in (("f32", 1), ("f64", 0)):
         if args.only not in ("both", name):
             continue
-        log(f"bench: {name} path, warmup {warmup}, timed {steps} steps")
+        log(f"bench: {name} path, warmup {warmup}, timed {steps} steps"
+            + (f", accuracy run {acc_steps} steps" if acc_steps else ""))
         sort_every = args.sort_every_f32 if name == "f32" else args.sort_every_f64
         res[name] = run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every,
-                             args.f32_acc, bz, W.get("tiles", 4))
+                             args.f32_acc, bz, W.get("tiles", 4), acc_steps)
         r = res[name]
         log(f"bench: {name} {r['seconds']*1e3:.2f} ms total, pass {r['pass_ms_avg']*1e3:.1f} us,"
             f" solve {r['solve_ms_avg']*1e3:.1f} us, deferred {r['slow_chunks']}")
 
-    t = np.arange(1, steps + 1) * dt
     if args.dump:
-        np.savez(args.dump, t=t, **{k: v["energy"] for k, v in res.items()})
+        np.savez(args.dump, t=np.arange(1, steps + 1) * dt,
+                 **{k: v["energy"] for k, v in res.items()},
+                 **{k + "_acc": v["energy_acc"] for k, v in res.items() if "energy_acc" in v})
     gam = {}
-    if W["sampler"] in ("quiet", "landau_device"):
+    if acc_steps:
+        t_acc = np.arange(1, acc_steps + 1) * dt
         for name, r in res.items():
-            # ey carries no signal in this problem: its level is the noise floor
-            noise = float(np.median(np.sqrt(r["energy"][:, 1])))
-            g, w, nm = diagnostics.damping_rate(t, np.sqrt(r["energy"][:, 0]), tmin=0.5,
-                                                noise=noise)
-            gam[name] = (g, w, nm)
-            log(f"bench: {name} gamma {g:.5f} omega {w:.4f} from {nm} maxima")
+            ea = r["energy_acc"]
+            g, w, nm, sig = diagnostics.landau_damping_rate(t_acc, ea[:, 0], ea[:, 1])
+            gam[name] = (g, w, nm, sig)
+            log(f"bench: {name} gamma {g:.5f} omega {w:.4f} from {nm} maxima "
+                f"({acc_steps}-step accuracy run, noise {sig:.3e})")
 
     head = "f32" if "f32" in res else "f64"
     r = res[head]
@@ -690,8 +767,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if head == "f32" else "f64",
-        "data": "synthetic (Landau quiet start, Sobol)" if W["sampler"] == "quiet" else
-                "synthetic (random start sampled on the GPU, seed 1234)",
+        "data": {"quiet": "synthetic (Landau quiet start, Sobol)",
+                 "landau_device": "synthetic (Landau quiet start, Sobol, made on the GPU)"}.get(
+                     W["sampler"], "synthetic (density 1 + 0.05 cos(x/2) cos(y/2), Maxwellian v, "
+                                   "Philox seed 1234, sampled on the GPU)"),
         "config": {"workload": W["name"], "grid": [nx, ny], "particles": n, "kernel": kname,
                    "alpha": alpha, "k": kx, "dt": dt, "bz": bz,
                    "state": "int32 cell + fp32 offset + fp32 velocity" if head == "f32" else
@@ -713,8 +792,16 @@ def main():
         out["speedup_fp32_vs_fp64"] = value / v64
         if gam:
             g32, g64 = gam["f32"][0], gam["f64"][0]
-            out["gamma"] = {"fp32": g32, "fp64": g64, "rel_err": abs(g32 - g64) / abs(g64),
-                            "analytic": -0.153359}
+            rel = abs(g32 - g64) / abs(g64)
+            out["gamma"] = {"fp32": g32, "fp64": g64, "rel_err": rel, "analytic": -0.153359,
+                            "fp64_vs_analytic": abs(g64 + 0.153359) / 0.153359,
+                            "omega": {"fp32": gam["f32"][1], "fp64": gam["f64"][1]},
+                            "maxima": {"fp32": gam["f32"][2], "fp64": gam["f64"][2]},
+                            "noise": gam["f64"][3], "accuracy_steps": acc_steps,
+                            "within_1pct": bool(rel <= 0.01),
+                            "fit": "ln of the local maxima of sqrt(sum ex^2 dx dy), noise "
+                                   "(median of sum ey^2 dx dy) removed in quadrature, up to the "
+                                   "first below 2x noise, at least 4"}
         e32, e64 = res["f32"]["energy"][:, 0], res["f64"]["energy"][:, 0]
         out["energy_rel_diff_fp32_fp64"] = float(np.max(np.abs(e32 - e64)) / np.max(e64))
         # normwise relative difference of the last step's rho and E
@@ -730,6 +817,12 @@ def main():
     # FP64 x, y 16 B); gather + kick + drift reads and writes it
     split_bytes = {"f32": {"deposit": 20.0, "gather_push": 40.0, "fused": 40.0},
                    "f64": {"deposit": 16.0, "gather_push": 64.0, "fused": 64.0}}
+    if "acc_ms" in res.get("f32", {}):
+        names = {0: "fp32_pairs", 1: "fp64_tiles", 2: "int32_fixed_point"}
+        out["f32_accumulation_pass_us"] = {
+            names[k]: {"avg_us": ms * 1e3,
+                       "frac": BYTES_F32 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            for k, ms in res["f32"]["acc_ms"].items()}
     out["passes_split"] = {}
     for name, rr in res.items():
         out["passes_split"][name] = {
@@ -747,8 +840,16 @@ def main():
         O = entry.load_oracle()
         steps_cpu = args.cpu_steps
         m = min(n, 10_000_000)
-        q = (np.asfortranarray(p0[:, :m]) if isinstance(p0, np.ndarray)
-             else np.asfortranarray(p0[:m].cpu().numpy().T))
+        if isinstance(p0, DeviceLandau):  # the same start made on the host at m particles
+            q = np.asfortranarray(pic.sampling.landau_damping(m, mesh, alpha, kx))
+            sample = f"a {m}-particle quiet start of the same configuration"
+        elif isinstance(p0, DeviceMaxwellian):
+            q = np.asfortranarray(pic.sampling.perturbed_maxwellian(m, mesh, alpha, kx, kx,
+                                                                    seed=1234))
+            sample = f"a {m}-particle sample of the same density (numpy, seed 1234)"
+        else:
+            q = np.asfortranarray(p0[:, :m])
+            sample = f"first {m} particles of the same workload"
         q[4] = mesh.dimx * mesh.dimy / m
         log(f"bench: cpu baseline, oracle loop ({kname}), {m} particles, {steps_cpu} steps, "
             f"{O.num_threads()} threads")
@@ -759,7 +860,7 @@ def main():
         out["cpu_baseline"] = {
             "value": m * steps_cpu / (c1 - c0), "unit": "particle-steps/s",
             "cores": O.num_threads(), "kind": "port",
-            "sample": f"first {m} particles of the same workload, first {steps_cpu} steps of "
+            "sample": f"{sample}, first {steps_cpu} steps of "
                       f"the reference loop without magnetic field (oracle C restatement: "
                       f"serial deposit, OpenMP push), {model}, nproc={cores}"}
     print(json.dumps(out), flush=True)

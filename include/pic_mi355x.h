@@ -85,8 +85,10 @@ int pic_device_count(int *count);
  * room for `capacity` particles.  Replaces the host allocations the reference
  * makes per call (compute_rho's zeros(nx, ny), pushers_cic.jl:80) and the
  * Poisson tables TwoDPoissonPeriodic(grid) (poisson_2d.jl:28-61) and
- * Poisson2D(mesh) (meshfields.jl:27-73).  nx and ny must be powers of two
- * >= 8 (every reference configuration is). */
+ * Poisson2D(mesh) (meshfields.jl:27-73).  nx, ny >= 4 and nx * ny <=
+ * 2^24; any sizes for the FP64 calls (the Fortran twin's 100 x 100 too), but
+ * the FP32 state (pic_set_state) and the E solve's LDS-FFT form need powers
+ * of two, as every reference configuration has. */
 int pic_ctx_create(pic_ctx **out, int device, int nx, int ny, double xmin, double xmax,
                    double ymin, double ymax, int64_t capacity);
 int pic_ctx_destroy(pic_ctx *ctx);
@@ -464,6 +466,19 @@ int pic_set_particle_fields(pic_ctx *ctx, const double *e);
  * 2 pi / kx (:81).  The particles replace the context's, in `state`
  * (PIC_STATE_F64 or PIC_STATE_F32). */
 int pic_sample_landau(pic_ctx *ctx, int64_t n, double alpha, double kx, int state);
+
+/* n particles of density 1 + alpha cos(kx (x - xmin)) cos(ky (y - ymin))
+ * and Maxwellian velocities of width vth, w = dimx dimy / n, seeded and
+ * reproducible: random numbers from the counter-based Philox4x32-10
+ * generator (key = seed), x uniform, y by inverting its conditional
+ * distribution (Newton), velocities by the Box-Muller transform.  The
+ * mesh's dimy must hold whole periods 2 pi / ky; |alpha| < 1.  The build's
+ * start for the strong-field workload (1 + 0.05 cos(x/2) cos(y/2)), in
+ * place of the rejection loop of plasma(mesh, n) (src/plasma.jl:17-31),
+ * which samples a density of the same kind.  The particles replace the
+ * context's, in `state`. */
+int pic_sample_maxwellian(pic_ctx *ctx, int64_t n, double alpha, double kx, double ky,
+                          double vth, uint64_t seed, int state);
 
 /* read_particles(filename, mesh): read_particles.jl:3-35.  One particle per
  * line, "ix iy dpx dpy vx vy" (cell indices, offsets in the cell,

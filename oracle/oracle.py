@@ -87,6 +87,7 @@ def lib():
             "or_landau_loop": (_i32, [_i64, _dp, _i32, _i32, _d, _d, _d, _i32, _i32, _d, _dpc]),
             "or_num_threads": (_i32, []),
             "orf_m6_weights": (None, [_f, _fp]),
+            "orf_shape_weights": (None, [_i32, _i32, _f, _i32, _fp]),
             "orf_step": (None, [_i64, _ip, _fp, _fp, _fp, _fp, _i32, _i32, _i32, _fp, _i32, _f,
                                 C.c_void_p, _f, _f, _f, _f, _i32, _dp]),
             "orf_deposit": (None, [_i64, _ip, _fp, _fp, _i32, _i32, _i32, _f, _dp]),
@@ -282,6 +283,28 @@ def f32_m6_weights(t):
     return w
 
 
+# shape codes of the FP32 step (the library's PIC_CIC .. PIC_M4)
+F32_SHAPES = {"cic": 0, "m6": 1, "ngp": 2, "tsc": 3, "m4": 4}
+
+
+def f32_shape_code(kernel):
+    try:
+        return F32_SHAPES[kernel]
+    except KeyError:
+        raise ValueError(f"unknown shape {kernel!r}: one of {sorted(F32_SHAPES)}") from None
+
+
+def f32_shape_weights(kernel, t, cell=0, gather=True):
+    """FP32 weights of the S nodes (i - HL ..) of a separable shape at
+    in-cell offset t of 0-based cell `cell` (NGP's tie rule reads its
+    parity); gather=False gives compute_rho's (TSC: f_tsc)."""
+    s = {"m6": 6, "ngp": 2, "tsc": 4, "m4": 4}[kernel]
+    w = np.zeros(6, dtype=np.float32)
+    lib().orf_shape_weights(f32_shape_code(kernel), 1 if gather else 0, np.float32(t), int(cell),
+                            w)
+    return w[:s]
+
+
 def f32_from_f64(p, nx, ny, dx, dy):
     n = p.shape[1]
     cell = np.zeros(n, np.int32)
@@ -305,7 +328,7 @@ def f32_step(state, nx, ny, kernel, e32, hdt, dtdx, dtdy, w, bz_mode=0, bz0=0.0,
     cell, ox, oy, vx, vy = state
     rho = np.zeros((nx, ny), order="F")
     b = None if bz32 is None else np.ascontiguousarray(bz32, dtype=np.float32)
-    lib().orf_step(cell.size, cell, ox, oy, vx, vy, nx, ny, 0 if kernel == "cic" else 1,
+    lib().orf_step(cell.size, cell, ox, oy, vx, vy, nx, ny, f32_shape_code(kernel),
                    np.ascontiguousarray(e32, dtype=np.float32), bz_mode, np.float32(bz0),
                    _ptr_or_none(b), np.float32(hdt), np.float32(dtdx), np.float32(dtdy),
                    np.float32(w), 1 if deposit else 0, rho)
@@ -315,7 +338,7 @@ def f32_step(state, nx, ny, kernel, e32, hdt, dtdx, dtdy, w, bz_mode=0, bz0=0.0,
 def f32_deposit(state, nx, ny, kernel, w):
     cell, ox, oy = state[:3]
     rho = np.zeros((nx, ny), order="F")
-    lib().orf_deposit(cell.size, cell, ox, oy, nx, ny, 0 if kernel == "cic" else 1,
+    lib().orf_deposit(cell.size, cell, ox, oy, nx, ny, f32_shape_code(kernel),
                       np.float32(w), rho)
     return rho
 

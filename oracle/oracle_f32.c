@@ -76,6 +76,125 @@ static inline int wrapi(int i, int n)
     return r < 0 ? r + n : r;
 }
 
+/* FP32 weights of the other separable shapes, build additions beside
+ * the reference's FP64 f_tsc (pushers_tsc.jl:3-12) and f_m4
+ * (pushers_m4.jl:8-19):
+ *   f_tsc(x) = 3/4 - x^2 (x <= 1/2), (3/2 - x)^2 / 2 (x <= 3/2), else 0;
+ *   f_m4(x)  = (1 - 3/2 x^2 + 3/4 x^3) 2/3 = 2/3 + x^2 (x/2 - 1) (x < 1),
+ *              (2 - x)^3 / 6 (1 <= x <= 2), else 0,
+ * the second written with one fused multiply-add in that expanded form. */
+static inline float orf_f_tsc(float x)
+{
+    if (x <= 0.5f) return 0.75f - x * x;
+    if (x <= 1.5f) return 0.5f * ((1.5f - x) * (1.5f - x));
+    return 0.0f;
+}
+
+static inline float orf_f_m4(float x)
+{
+    if (x > 2.0f) return 0.0f;
+    if (x >= 1.0f) {
+        const float t = 2.0f - x;
+        return (t * t) * t * QF(1, 6);
+    }
+    return fmaf(x * x, fmaf(0.5f, x, -1.0f), QF(2, 3));
+}
+
+/* Stencil of shape `kernel` (1 M6, 2 NGP, 3 TSC, 4 M4): S nodes per
+ * direction starting HL nodes before the particle's cell. */
+static inline int shape_s(int kernel) { return kernel == 1 ? 6 : kernel == 2 ? 2 : 4; }
+static inline int shape_hl(int kernel) { return kernel == 1 ? 2 : kernel == 2 ? 0 : 1; }
+
+/* Weights of the S nodes at in-cell offset t of a particle in (0-based)
+ * cell `cell` of one direction.  gather = 1: push_v!'s weights, 0:
+ * compute_rho's.  TSC gathers with f_m4 and deposits with f_tsc, as the
+ * reference's pushers_tsc.jl:46-57 and :165-176 do.  NGP puts the whole
+ * weight on the node of round(Int, xp) (ties to even, pushers_ngp.jl:22-23),
+ * xp = cell + t: the lower node unless t > 1/2, or t == 1/2 in an odd cell. */
+void orf_shape_weights(int kernel, int gather, float t, int cell, float w[6])
+{
+    if (kernel == 1) {
+        orf_m6_weights(t, w);
+    } else if (kernel == 2) {
+        const int up = t > 0.5f || (t == 0.5f && (cell & 1));
+        w[0] = up ? 0.0f : 1.0f;
+        w[1] = up ? 1.0f : 0.0f;
+    } else if (kernel == 3 && !gather) {
+        w[0] = orf_f_tsc(1.0f + t);
+        w[1] = orf_f_tsc(t);
+        w[2] = orf_f_tsc(1.0f - t);
+        w[3] = orf_f_tsc(2.0f - t);
+    } else {
+        w[0] = orf_f_m4(1.0f + t);
+        w[1] = orf_f_m4(t);
+        w[2] = orf_f_m4(1.0f - t);
+        w[3] = orf_f_m4(2.0f - t);
+    }
+}
+
+/* FP32 field of a separable shape at a particle of cell (i, j) and offsets
+ * (fx, fy): x offset outer, y offset inner, each inner sum started with a
+ * product and continued with fused multiply-adds, the outer likewise. */
+static void sep_gather(int kernel, int i, int j, float fx, float fy, int nx, int ny,
+                       const float *e32, int bz_mode, const float *bz32, float *e1o, float *e2o,
+                       float *b3o)
+{
+    const int S = shape_s(kernel), HL = shape_hl(kernel);
+    float wx[6], wy[6];
+    orf_shape_weights(kernel, 1, fx, i, wx);
+    orf_shape_weights(kernel, 1, fy, j, wy);
+    int ii[6], jj[6];
+    for (int a = 0; a < S; ++a) {
+        ii[a] = wrapi(i - HL + a, nx);
+        jj[a] = wrapi(j - HL + a, ny);
+    }
+    float e1 = 0.0f, e2 = 0.0f, b = 0.0f;
+    for (int a = 0; a < S; ++a) {
+        float t1 = wy[0] * e32[2 * (ii[a] + nx * jj[0])];
+        float t2 = wy[0] * e32[2 * (ii[a] + nx * jj[0]) + 1];
+        for (int bb = 1; bb < S; ++bb) {
+            int node = ii[a] + nx * jj[bb];
+            t1 = fmaf(wy[bb], e32[2 * node], t1);
+            t2 = fmaf(wy[bb], e32[2 * node + 1], t2);
+        }
+        if (a == 0) {
+            e1 = wx[0] * t1;
+            e2 = wx[0] * t2;
+        } else {
+            e1 = fmaf(wx[a], t1, e1);
+            e2 = fmaf(wx[a], t2, e2);
+        }
+    }
+    if (bz_mode == 2) {
+        for (int a = 0; a < S; ++a) {
+            float t3 = wy[0] * bz32[ii[a] + nx * jj[0]];
+            for (int bb = 1; bb < S; ++bb) t3 = fmaf(wy[bb], bz32[ii[a] + nx * jj[bb]], t3);
+            b = a == 0 ? wx[0] * t3 : fmaf(wx[a], t3, b);
+        }
+    }
+    *e1o = e1;
+    *e2o = e2;
+    *b3o = b;
+}
+
+/* FP32 charges of a separable shape, (w_x[u] * (w_y[v] * w)), added in FP64
+ * to the S x S nodes from (i - HL, j - HL). */
+static void sep_deposit(int kernel, int i, int j, float fx, float fy, int nx, int ny, float w,
+                        double *rho_acc)
+{
+    const int S = shape_s(kernel), HL = shape_hl(kernel);
+    float wx[6], wy[6];
+    orf_shape_weights(kernel, 0, fx, i, wx);
+    orf_shape_weights(kernel, 0, fy, j, wy);
+    for (int bb = 0; bb < S; ++bb) wy[bb] = wy[bb] * w;
+    for (int a = 0; a < S; ++a) {
+        int ia = wrapi(i - HL + a, nx);
+        for (int bb = 0; bb < S; ++bb)
+            rho_acc[ia + nx * wrapi(j - HL + bb, ny)] += (double)(wx[a] * wy[bb]);
+    }
+}
+
+
 /* One offset component drifted by u = v * dt/d (in cells): returns the integer
  * cell shift and the new offset in [0,1).  An offset that rounds up to 1.0
  * (t in [-2^-25, 0) after the floor) is moved to 0 of the next cell. */
@@ -99,7 +218,8 @@ static inline int drift1(float o, float v, float dtd, float *onew)
  *   drift by dt_drift (the two half drifts of consecutive loop iterations,
  *   pushers_cic.jl:55-72, taken as one),
  *   deposit w at the new position into rho_acc (FP64, nx*ny, ADDED to).
- * kernel 0 = CIC, 1 = M6.  e32 holds (ex, ey) interleaved per node, nx*ny.
+ * kernel 0 = CIC, 1 = M6, 2 = NGP, 3 = TSC, 4 = M4 (the library's shape codes).
+ * e32 holds (ex, ey) interleaved per node, nx*ny.
  * bz_mode 0 = no magnetic field, 1 = uniform bz0, 2 = grid bz32 (nx*ny).
  * hdt = (float)(dt/2), dtdx = (float)(dt_drift/dx), dtdy likewise, all
  * rounded once on the host.  deposit = 0 skips the deposit.
@@ -134,40 +254,7 @@ void orf_step(int64_t n, int32_t *cell, float *ox, float *oy, float *vx, float *
                 b = fmaf(a[3], bz32[n01], b);
             }
         } else {
-            float wx[6], wy[6];
-            orf_m6_weights(fx, wx);
-            orf_m6_weights(fy, wy);
-            int ii[6], jj[6];
-            for (int a = 0; a < 6; ++a) {
-                ii[a] = wrapi(i - 2 + a, nx);
-                jj[a] = wrapi(j - 2 + a, ny);
-            }
-            e1 = 0.0f;
-            e2 = 0.0f;
-            for (int a = 0; a < 6; ++a) {
-                float t1 = wy[0] * e32[2 * (ii[a] + nx * jj[0])];
-                float t2 = wy[0] * e32[2 * (ii[a] + nx * jj[0]) + 1];
-                for (int bb = 1; bb < 6; ++bb) {
-                    int node = ii[a] + nx * jj[bb];
-                    t1 = fmaf(wy[bb], e32[2 * node], t1);
-                    t2 = fmaf(wy[bb], e32[2 * node + 1], t2);
-                }
-                if (a == 0) {
-                    e1 = wx[0] * t1;
-                    e2 = wx[0] * t2;
-                } else {
-                    e1 = fmaf(wx[a], t1, e1);
-                    e2 = fmaf(wx[a], t2, e2);
-                }
-            }
-            if (bz_mode == 2) {
-                for (int a = 0; a < 6; ++a) {
-                    float t3 = wy[0] * bz32[ii[a] + nx * jj[0]];
-                    for (int bb = 1; bb < 6; ++bb)
-                        t3 = fmaf(wy[bb], bz32[ii[a] + nx * jj[bb]], t3);
-                    b = a == 0 ? wx[0] * t3 : fmaf(wx[a], t3, b);
-                }
-            }
+            sep_gather(kernel, i, j, fx, fy, nx, ny, e32, bz_mode, bz32, &e1, &e2, &b);
         }
         if (bz_mode == 1) b = bz0;
         float v1 = fmaf(hdt, e1, vx[k]);
@@ -202,15 +289,7 @@ void orf_step(int64_t n, int32_t *cell, float *ox, float *oy, float *vx, float *
             rho_acc[ip + nx * jp] += (double)(a[2] * w);
             rho_acc[i + nx * jp] += (double)(a[3] * w);
         } else {
-            float wx[6], wy[6];
-            orf_m6_weights(nfx, wx);
-            orf_m6_weights(nfy, wy);
-            for (int bb = 0; bb < 6; ++bb) wy[bb] = wy[bb] * w;
-            for (int a = 0; a < 6; ++a) {
-                int ia = wrapi(i - 2 + a, nx);
-                for (int bb = 0; bb < 6; ++bb)
-                    rho_acc[ia + nx * wrapi(j - 2 + bb, ny)] += (double)(wx[a] * wy[bb]);
-            }
+            sep_deposit(kernel, i, j, nfx, nfy, nx, ny, w, rho_acc);
         }
     }
 }
@@ -233,15 +312,7 @@ void orf_deposit(int64_t n, const int32_t *cell, const float *ox, const float *o
             rho_acc[ip + nx * jp] += (double)(a[2] * w);
             rho_acc[i + nx * jp] += (double)(a[3] * w);
         } else {
-            float wx[6], wy[6];
-            orf_m6_weights(ox[k], wx);
-            orf_m6_weights(oy[k], wy);
-            for (int bb = 0; bb < 6; ++bb) wy[bb] = wy[bb] * w;
-            for (int a = 0; a < 6; ++a) {
-                int ia = wrapi(i - 2 + a, nx);
-                for (int bb = 0; bb < 6; ++bb)
-                    rho_acc[ia + nx * wrapi(j - 2 + bb, ny)] += (double)(wx[a] * wy[bb]);
-            }
+            sep_deposit(kernel, i, j, ox[k], oy[k], nx, ny, w, rho_acc);
         }
     }
 }

@@ -91,6 +91,7 @@ SIGNATURES = {
     "pic_set_particle_fields": [_vp, _vp],
     "pic_get_ua_timing": [_vp, _pd, C.POINTER(C.c_int64)],
     "pic_sample_landau": [_vp, _i64, _d, _d, _i],
+    "pic_sample_maxwellian": [_vp, _i64, _d, _d, _d, _d, C.c_uint64, _i],
     "pic_read_particles": [_vp, C.c_char_p, _i, C.POINTER(C.c_int64)],
     "pic_write_particles": [_vp, C.c_char_p],
     "pic_f90_push_v": [_pi32, _vp, _pd, _pi32, _pi32, _pd, _pd, _vp, _vp, _vp],

@@ -26,6 +26,7 @@ argument order and array layouts (a Julia `f!` is written `f_` here):
     update_particles_x_, compute_v_             ua_type.jl, ua_steps.jl
     read_particles(filename, mesh)           read_particles.jl:3
     Simulation.landau(mesh, n, alpha, kx, kernel)   landau_damping.jl:77 (on the GPU)
+    Simulation.maxwellian(mesh, n, alpha, kx, ky, kernel, seed)  (seeded, on the GPU)
 
 Arrays are numpy float64 in Fortran order, so the memory layout is the Julia
 one.  Each call copies its inputs to the GPU and its outputs back (the
@@ -114,18 +115,6 @@ def _kcode(kernel):
 
 # --------------------------------------------------------- device context
 
-def _is_device_array(arr):
-    """A torch float64 tensor of shape (n, 5) in GPU memory, contiguous (the
-    memory of a 5 x n column-major particle array)."""
-    if not (hasattr(arr, "is_cuda") and hasattr(arr, "data_ptr")):
-        return False
-    if not arr.is_cuda:
-        raise ValueError("a torch particle array must be on the GPU")
-    import torch
-    if arr.dtype != torch.float64 or arr.dim() != 2 or arr.shape[1] != 5 or \
-            not arr.is_contiguous():
-        raise ValueError("a device particle array must be a contiguous float64 (n, 5) tensor")
-    return True
 
 
 
@@ -152,15 +141,6 @@ class Context:
 
     # particles
     def set_particles(self, arr):
-        if _is_device_array(arr):
-            # (n, 5) float64 on the GPU: the 5 x n column-major p.array.  The
-            # tensor was written on torch's stream and is read on the
-            # context's: wait for both sides.
-            import torch
-            torch.cuda.synchronize()
-            L.call("pic_set_particles_device", self.h, C.c_void_p(arr.data_ptr()), arr.shape[0])
-            self.sync()
-            return arr
         a = np.asfortranarray(arr, dtype=np.float64)
         assert a.ndim == 2 and a.shape[0] == 5
         L.call("pic_set_particles", self.h, a.ctypes.data, a.shape[1])
@@ -735,9 +715,14 @@ class Simulation:
             self.n = 0
             return
         arr = p.array if isinstance(p, ParticleGroup) else p
-        n = arr.shape[0] if _is_device_array(arr) else arr.shape[1]
+        n = arr.shape[1]
         self.ctx = Context(mesh, capacity=n, device=device)
         self.ctx.set_particles(arr)
+        self._set_bz(bz)
+        self.ctx.set_state(state)
+        self.n = n
+
+    def _set_bz(self, bz):
         if bz is None:
             self.ctx.set_bz(L.PIC_BZ_NONE)
         elif np.ndim(bz) == 0:
@@ -745,17 +730,38 @@ class Simulation:
         else:
             self.ctx.set_grid(L.PIC_GRID_BZ, bz)
             self.ctx.set_bz(L.PIC_BZ_GRID)
-        self.ctx.set_state(state)
-        self.n = n
 
     @classmethod
     def landau(cls, mesh, n, alpha, kx, kernel, state=L.PIC_STATE_F64, device=0):
         """The quiet start of sample!(pg, mesh, LandauDamping(alpha, kx))
         (landau_damping.jl:77-97) made on the device (pic_sample_landau)."""
         sim = cls(mesh, None, kernel, device=device)
-        L.call("pic_sample_landau", sim.ctx.h, int(n), float(alpha), float(kx), int(state))
-        sim.n = int(n)
+        sim.sample_landau(n, alpha, kx, state)
         return sim
+
+    @classmethod
+    def maxwellian(cls, mesh, n, alpha, kx, ky, kernel, seed=0, vth=1.0,
+                   state=L.PIC_STATE_F64, bz=None, device=0):
+        """n particles of density 1 + alpha cos(kx x) cos(ky y) with
+        Maxwellian velocities of width vth, made on the device from a seed
+        (pic_sample_maxwellian); bz as in Simulation()."""
+        sim = cls(mesh, None, kernel, device=device)
+        sim._set_bz(bz)
+        sim.sample_maxwellian(n, alpha, kx, ky, seed, vth, state)
+        return sim
+
+    def sample_maxwellian(self, n, alpha, kx, ky, seed=0, vth=1.0, state=L.PIC_STATE_F64):
+        """Replaces the particles by the device-made sample of
+        Simulation.maxwellian (pic_sample_maxwellian)."""
+        L.call("pic_sample_maxwellian", self.ctx.h, int(n), float(alpha), float(kx), float(ky),
+               float(vth), int(seed), int(state))
+        self.n = int(n)
+
+    def sample_landau(self, n, alpha, kx, state=L.PIC_STATE_F64):
+        """Replaces the particles by the device-made quiet start of
+        Simulation.landau (pic_sample_landau)."""
+        L.call("pic_sample_landau", self.ctx.h, int(n), float(alpha), float(kx), int(state))
+        self.n = int(n)
 
     @classmethod
     def from_file(cls, mesh, path, kernel, state=L.PIC_STATE_F64, device=0):
@@ -894,7 +900,7 @@ class Simulation:
         arr = p.array if isinstance(p, ParticleGroup) else p
         self.ctx.set_particles(arr)
         self.ctx.set_state(state)
-        self.n = arr.shape[0] if _is_device_array(arr) else arr.shape[1]
+        self.n = arr.shape[1]
 
     def slow_chunks(self):
         v = C.c_uint64()

@@ -1,6 +1,7 @@
 // pic_io.hip -- particle initial conditions made on the device and particle
 // files: the Landau quiet start of sample!(pg, mesh, LandauDamping(alpha, kx))
-// (src/landau_damping.jl:77-97) as one kernel, and read_particles
+// (src/landau_damping.jl:77-97) as one kernel, a seeded perturbed
+// Maxwellian (the strong-field workload's start), and read_particles
 // (src/read_particles.jl:3-35) with its "ix iy dpx dpy vx vy" line format,
 // which maps onto the FP32 cell-relative state one to one.
 #include <algorithm>
@@ -79,6 +80,71 @@ __global__ void k_sample_landau(int64_t n, SobolDirs d, double alpha, double kx,
     y[k] = ymin + r2 * dimy;
     vx[k] = v * cos(theta);
     vy[k] = v * sin(theta);
+}
+
+// ------------------------------------------- perturbed Maxwellian, seeded
+
+// Philox4x32-10 (Salmon, Moraes, Dror and Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC 2011): ten rounds of two 32 x 32 -> 64-bit products
+// on a 128-bit counter under a 64-bit key bumped by the Weyl constants.
+// Counter-based: particle k draws from counters (k, 0, j, 0), so the sample
+// does not depend on the launch shape.
+struct U4 {
+    uint32_t a, b, c, d;
+};
+
+__device__ __forceinline__ U4 philox4x32_10(U4 ctr, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * ctr.a;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * ctr.c;
+        ctr = U4{(uint32_t)(p1 >> 32) ^ ctr.b ^ k0, (uint32_t)p1,
+                 (uint32_t)(p0 >> 32) ^ ctr.d ^ k1, (uint32_t)p0};
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return ctr;
+}
+
+// A double in [0, 1) from 53 bits of two words.
+__device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
+    return (double)(((uint64_t)(hi >> 5) << 26) | (lo >> 6)) * 0x1.0p-53;
+}
+
+// n particles of density 1 + alpha cos(kx (x - xmin)) cos(ky (y - ymin)) on
+// the mesh and Maxwellian velocities of width vth.  Integrated over whole
+// periods in y the density is uniform in x, so x is uniform; y then follows
+// its conditional density 1 + a cos(ky y'), a = alpha cos(kx (x - xmin)),
+// whose distribution function y' + a sin(ky y') / ky = u dimy is inverted
+// by Newton's method (|a| < 1 keeps it monotone).  Velocities by the
+// Box-Muller transform.  Particle k takes the words of counters (k, 0, 0, 0)
+// and (k, 0, 1, 0): x, y from the first, the velocity from the second.
+__global__ void k_sample_maxwellian(int64_t n, uint32_t k0, uint32_t k1, double alpha, double kx,
+                                    double ky, double xmin, double dimx, double ymin, double dimy,
+                                    double vth, double *x, double *y, double *vx, double *vy) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t lo = (uint32_t)k, hi = (uint32_t)((uint64_t)k >> 32);
+    const U4 r = philox4x32_10(U4{lo, hi, 0u, 0u}, k0, k1);
+    const U4 s = philox4x32_10(U4{lo, hi, 1u, 0u}, k0, k1);
+    const double xr = u53(r.a, r.b) * dimx;
+    const double a = alpha * cos(kx * xr);
+    const double target = u53(r.c, r.d) * dimy;
+    double yr = target;
+    for (int it = 0; it < 60; ++it) {
+        const double f = yr + a * sin(ky * yr) / ky - target;
+        const double step = f / (1 + a * cos(ky * yr));
+        yr -= step;
+        if (fabs(step) <= 1e-15 * dimy) break;
+    }
+    yr = fmin(fmax(yr, 0.0), dimy);
+    if (yr >= dimy) yr = 0.0;
+    const double rad = vth * sqrt(-2 * log(1.0 - u53(s.a, s.b)));  // 1 - u in (0, 1]
+    const double th = 2 * M_PI * u53(s.c, s.d);
+    x[k] = xmin + xr;
+    y[k] = ymin + yr;
+    vx[k] = rad * cos(th);
+    vy[k] = rad * sin(th);
 }
 
 // ------------------------------------------------------------ text files
@@ -203,6 +269,36 @@ int pic_sample_landau(pic_ctx *ctx, int64_t n, double alpha, double kx, int stat
         hipLaunchKernelGGL(k_sample_landau, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                            c->stream, n, sobol_dirs(), alpha, kx, c->ymin, c->dimy, c->x, c->y,
                            c->vx, c->vy);
+        PIC_HIP(hipGetLastError());
+    }
+    PIC_TRY(pic_set_state(ctx, state));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_sample_maxwellian(pic_ctx *ctx, int64_t n, double alpha, double kx, double ky,
+                          double vth, uint64_t seed, int state) {
+    PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
+    PIC_CHECK(n >= 0 && n <= INT32_MAX, PIC_ERR_ARG, "particle count %lld", (long long)n);
+    PIC_CHECK(fabs(alpha) < 1.0, PIC_ERR_ARG, "|alpha| must be below 1");
+    PIC_CHECK(kx > 0.0 && ky > 0.0 && vth >= 0.0, PIC_ERR_ARG, "kx, ky > 0 and vth >= 0");
+    PIC_CHECK(state == PIC_STATE_F64 || state == PIC_STATE_F32, PIC_ERR_ARG, "state %d", state);
+    Ctx *c = ctx_of(ctx);
+    // the conditional inversion needs whole periods of cos(ky y) in the mesh
+    const double periods = ky * c->dimy / (2 * M_PI);
+    PIC_CHECK(periods >= 0.5 && fabs(periods - nearbyint(periods)) <= 1e-9 * periods,
+              PIC_ERR_ARG, "the mesh's dimy must hold whole periods 2pi/ky");
+    PIC_HIP(hipSetDevice(c->device));
+    PIC_TRY(ensure_particles(c, n));
+    c->n = n;
+    c->state = PIC_STATE_F64;
+    c->seg_valid = false;
+    c->uniform_w = true;
+    c->w0 = n ? (c->dimx * c->dimy) / (double)n : 0.0;
+    if (n) {
+        hipLaunchKernelGGL(k_sample_maxwellian, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           c->stream, n, (uint32_t)seed, (uint32_t)(seed >> 32), alpha, kx, ky,
+                           c->xmin, c->dimx, c->ymin, c->dimy, vth, c->x, c->y, c->vx, c->vy);
         PIC_HIP(hipGetLastError());
     }
     PIC_TRY(pic_set_state(ctx, state));

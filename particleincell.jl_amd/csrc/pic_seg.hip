@@ -26,7 +26,10 @@
 //  2 fixed-point int32 LDS tiles: every contribution is rounded to a multiple
 //    of a power-of-two quantum u (per block, 2^-30 of the largest sum a node
 //    of the block can reach) and summed exactly with ds_add_u32; the flush
-//    converts to FP64.  The M6, TSC and M4 passes that drift before they
+//    converts to FP64.  Each particle adds exactly R = round(w/u) units: the
+//    node of its own cell takes R minus the rounded shares of the others, and
+//    the flush scales units by w/R, so a block deposits exactly its count
+//    times w (see charge32_fixed).  The M6, TSC and M4 passes that drift before they
 //    deposit keep 2 or 4 copies of the tile (Ctx::f32_tiles), lane l adding
 //    into copy l mod copies: a ds_add_u32 whose lanes address one word
 //    serialises
@@ -110,6 +113,14 @@ __device__ __forceinline__ int fixed_exponent(float w, double qmax, int64_t coun
     return e1 > e2 ? e1 : e2;
 }
 
+// Largest block for which the fixed-point tiles conserve charge particle by
+// particle (charge32_fixed): the node that takes a particle's remainder
+// receives at most its own share plus the S^2 / 2 roundings of the others
+// and the 1e-6 by which the FP32 weights may miss a sum of 1 (at most ~50
+// units with w/u < 2^24), so a node's sum stays below 2^30 u + 50 count,
+// within int32 for count <= 2^24.  Larger blocks round every share.
+constexpr int64_t CONSERVE_MAX_COUNT = (int64_t)1 << 24;
+
 __device__ __forceinline__ int64_t tile_sum(const int *iacc, int t, int cap, int tiles) {
     int64_t q = 0;
     for (int k = 0; k < tiles; ++k) q += iacc[t + k * cap];
@@ -145,30 +156,49 @@ __device__ __forceinline__ void charge32(float fx, float fy, int ci, int cj, flo
 // shape factors times w/u, rounded to the nearest integer by one fused
 // multiply-add onto 1.5 * 2^23 (the float grid there is the integers; the
 // products stay below 2^22 by the choice of u).
+//
+// With `conserve` the node (HL, HL), the lower-left node of the particle's
+// own cell, takes R minus the sum of the other nodes' rounded shares, R =
+// round(w/u) the same for every particle of the block, so that every
+// particle adds exactly R units and the flush's scale w/R gives the block
+// exactly its count times w.  Rounding every share to nearest instead drops
+// the many shares below u/2 at a stencil's far nodes (M6: products of two
+// weights of ~1e-3): measured on 5e8 particles on 512^2, ~2e4 units per
+// particle, the deposit lost 2.2e-5 of the charge.  The remainder node
+// absorbs those roundings (at most S^2/2 units) and the ~1e-6 by which the
+// FP32 weights miss a sum of 1.
 template <int K, class F>
-__device__ __forceinline__ void charge32_fixed(float fx, float fy, int ci, int cj, float wu,
-                                               F &&add) {
-    constexpr int S = Shape<K>::S;
+__device__ __forceinline__ void charge32_fixed(float fx, float fy, int ci, int cj, float wu, int R,
+                                               bool conserve, F &&add) {
+    constexpr int S = Shape<K>::S, HL = Shape<K>::HL;
     constexpr float C = 12582912.0f;  // 1.5 * 2^23
     auto n = [&](float a, float b) {
         return __float_as_int(__builtin_fmaf(a, b, C)) - __float_as_int(C);
     };
     if (K == PIC_CIC) {
         const float sx = 1.0f - fx, sy = (1.0f - fy) * wu, ty = fy * wu;
-        add(0, 0, n(sx, sy));
-        add(1, 0, n(fx, sy));
-        add(1, 1, n(fx, ty));
-        add(0, 1, n(sx, ty));
+        const int n10 = n(fx, sy), n11 = n(fx, ty), n01 = n(sx, ty);
+        add(1, 0, n10);
+        add(1, 1, n11);
+        add(0, 1, n01);
+        add(0, 0, conserve ? R - (n10 + n11 + n01) : n(sx, sy));
     } else {
         float wx[6], wy[6];
         deposit_w<K>(fx, ci, wx);
         deposit_w<K>(fy, cj, wy);
 #pragma unroll
         for (int v = 0; v < S; ++v) wy[v] = wy[v] * wu;
+        int s = 0;
 #pragma unroll
         for (int u = 0; u < S; ++u)
 #pragma unroll
-            for (int v = 0; v < S; ++v) add(u, v, n(wx[u], wy[v]));
+            for (int v = 0; v < S; ++v) {
+                if (u == HL && v == HL) continue;
+                const int q = n(wx[u], wy[v]);
+                s += q;
+                add(u, v, q);
+            }
+        add(HL, HL, conserve ? R - s : n(wx[HL], wy[HL]));
     }
 }
 
@@ -237,6 +267,10 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
     const float cq = ACC == 0 ? split_constant(a.w, p1 - p0) : 0.0f;
     const int qe = ACC == 2 ? fixed_exponent(a.w, Shape<K>::QMAX, p1 - p0) : 0;
     const float wu = ACC == 2 ? ldexpf(a.w, -qe) : 0.0f;
+    // charge-conserving fixed point: R units per particle, worth w/R each
+    const bool conserve = ACC == 2 && p1 - p0 <= CONSERVE_MAX_COUNT;
+    const int R = conserve ? (int)rintf(wu) : 0;
+    const double unit = ACC != 2 ? 0.0 : (conserve && R != 0) ? (double)a.w / R : ldexp(1.0, qe);
     float2 *te = reinterpret_cast<float2 *>(smem_s32);
     float *tb = reinterpret_cast<float *>(smem_s32 + 8 * (size_t)cap);
     char *dep = smem_s32 + (GATHER ? (BZ == PIC_BZ_GRID ? 12 : 8) * (size_t)cap : 0);
@@ -306,7 +340,7 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
             defer_push(!in, (uint32_t)p | DEFER_DEPOSIT_ONLY, a.dlist, a.dcount);
             if (!in) continue;
             if (ACC == 2) {
-                charge32_fixed<K>(fx, fy, ci, cj, wu, [&](int u, int v, int qi) {
+                charge32_fixed<K>(fx, fy, ci, cj, wu, R, conserve, [&](int u, int v, int qi) {
                     __hip_atomic_fetch_add(&itile[(lx + u) + W.tw * (ly + v)], qi,
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 });
@@ -334,7 +368,7 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
     for (int t = tid; t < nt; t += BT) {
         const double q = ACC == 0   ? (double)hi[t] + (double)lo[t]
                          : ACC == 1 ? dacc[t]
-                                    : ldexp((double)tile_sum(iacc, t, cap, a.tiles), qe);
+                                    : (double)tile_sum(iacc, t, cap, a.tiles) * unit;
         if (q != 0.0) {
             const int u = t % W.tw, v = t / W.tw;
             unsafeAtomicAdd(&a.acc[(int64_t)((W.x0 + u) & mx) + (int64_t)nx * ((W.y0 + v) & my)],

@@ -94,61 +94,9 @@ def perturbed_maxwellian(n, mesh, alpha, kx, ky, seed, vth=1.0):
     return p
 
 
-# ------------------------------------------------------ device-side sampling
-#
-# Samplers for the large configurations, run on the GPU with torch so that the
-# particle array never exists in host memory.  They return a torch float64
-# tensor of shape (n, 5) on cuda:0, memory-identical to the reference's 5 x n
-# column-major p.array; Simulation accepts it directly.  Random (Philox,
-# seeded) rather than quiet starts: the statistics of these runs are not
-# compared with the CPU, only their throughput and the properties the tests
-# check.
-
-def landau_damping_device(n, mesh, alpha, kx, seed=0):
-    """Landau initial condition sampled on the GPU: x by Newton inversion of
-    P(x) = x + alpha sin(kx x)/kx at uniform random r (landau_damping.jl:19-28
-    with random instead of Sobol numbers), y uniform, |v| = sqrt(-2 log u),
-    theta = 2 pi u', w = dimx dimy / n."""
-    import torch
-    g = torch.Generator(device="cuda")
-    g.manual_seed(seed)
-    p = torch.empty((n, 5), dtype=torch.float64, device="cuda")
-    r = torch.rand(n, generator=g, dtype=torch.float64, device="cuda") * (2 * math.pi / kx)
-    x = r.clone()
-    for _ in range(40):  # |alpha| < 1: the iteration contracts quickly
-        x = x - (x + alpha * torch.sin(kx * x) / kx - r) / (1 + alpha * torch.cos(kx * x))
-    p[:, 0] = x + mesh.xmin
-    del r, x
-    p[:, 1] = mesh.ymin + torch.rand(n, generator=g, dtype=torch.float64, device="cuda") * mesh.dimy
-    vn = torch.sqrt(-2.0 * torch.log1p(-torch.rand(n, generator=g, dtype=torch.float64,
-                                                    device="cuda")))
-    th = 2 * math.pi * torch.rand(n, generator=g, dtype=torch.float64, device="cuda")
-    p[:, 2] = vn * torch.cos(th)
-    p[:, 3] = vn * torch.sin(th)
-    p[:, 4] = mesh.dimx * mesh.dimy / n
-    return p
-
-
-def perturbed_maxwellian_device(n, mesh, alpha, kx, ky, seed=0, vth=1.0):
-    """perturbed_maxwellian() on the GPU: rejection sample of the density
-    1 + alpha cos(kx x) cos(ky y), Maxwellian velocities of width vth."""
-    import torch
-    g = torch.Generator(device="cuda")
-    g.manual_seed(seed)
-    p = torch.empty((n, 5), dtype=torch.float64, device="cuda")
-    done = 0
-    while done < n:
-        m = min(int((n - done) * 1.1) + 4096, 1 << 27)
-        x = torch.rand(m, generator=g, dtype=torch.float64, device="cuda") * mesh.dimx + mesh.xmin
-        y = torch.rand(m, generator=g, dtype=torch.float64, device="cuda") * mesh.dimy + mesh.ymin
-        z = (1.0 + alpha) * torch.rand(m, generator=g, dtype=torch.float64, device="cuda")
-        keep = (1.0 + alpha * torch.cos(kx * x) * torch.cos(ky * y)) >= z
-        x, y = x[keep], y[keep]
-        k = min(x.numel(), n - done)
-        p[done:done + k, 0] = x[:k]
-        p[done:done + k, 1] = y[:k]
-        done += k
-    p[:, 2] = vth * torch.randn(n, generator=g, dtype=torch.float64, device="cuda")
-    p[:, 3] = vth * torch.randn(n, generator=g, dtype=torch.float64, device="cuda")
-    p[:, 4] = mesh.dimx * mesh.dimy / n
-    return p
+# The large workloads sample on the device, through the library:
+# Simulation.landau (pic_sample_landau, the same quiet start as
+# landau_damping above, made on the GPU) and Simulation.maxwellian
+# (pic_sample_maxwellian, a seeded perturbed Maxwellian like
+# perturbed_maxwellian above), so that their particle arrays never exist in
+# host memory.

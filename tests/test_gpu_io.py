@@ -55,6 +55,40 @@ def test_sample_landau_into_fp32_state(gpu):
         assert np.max(np.abs(u[same] - w[same])) <= 2e-7
 
 
+@pytest.mark.parametrize("n,seed", [(1, 5), (1000, 1234), (300_000, 2**40 + 7)])
+def test_sample_maxwellian_matches_restatement(gpu, n, seed):
+    """pic_sample_maxwellian against its numpy restatement (tests/philox.py,
+    whose Philox4x32-10 meets the generator's known answers): x is a linear
+    map of the same 53-bit uniform, so it agrees bit for bit; y (Newton with
+    sin, cos) and v (log, sqrt, cos, sin) agree to the device library's
+    rounding.  Offset mesh origin, two periods in y."""
+    from philox import maxwellian
+    pic = gpu
+    mesh = pic.TwoDGrid(-1.0, -1.0 + 4 * math.pi, 64, 0.5, 0.5 + 4 * math.pi, 32)
+    sim = pic.Simulation.maxwellian(mesh, n, 0.05, 0.5, 1.0, pic.CloudInCell(), seed=seed,
+                                    vth=1.5)
+    q = sim.particles()
+    sim.close()
+    x, y, vx, vy = maxwellian(n, 0.05, 0.5, 1.0, 1.5, seed, mesh.xmin, mesh.dimx, mesh.ymin,
+                              mesh.dimy)
+    assert np.array_equal(q[0], x)
+    assert np.max(np.abs(q[1] - y)) < 1e-13 * mesh.dimy
+    assert np.max(np.abs(q[2] - vx)) < 1e-14 * max(1.0, np.max(np.abs(vx)))
+    assert np.max(np.abs(q[3] - vy)) < 1e-14 * max(1.0, np.max(np.abs(vy)))
+    assert np.all(q[4] == mesh.dimx * mesh.dimy / n)
+
+
+def test_sample_maxwellian_rejects_partial_periods(gpu):
+    pic = gpu
+    mesh = pic.TwoDGrid(0.0, 4 * math.pi, 32, 0.0, 10.0, 32)
+    sim = pic.Simulation(mesh, None, pic.CloudInCell())
+    with pytest.raises(pic._lib.PicError, match="whole periods"):
+        sim.sample_maxwellian(100, 0.05, 0.5, 0.5)
+    with pytest.raises(pic._lib.PicError, match="alpha"):
+        sim.sample_maxwellian(100, 1.5, 0.5, 2 * math.pi / 10.0)
+    sim.close()
+
+
 def test_sample_landau_rejects_a_wrong_box(gpu):
     pic = gpu
     mesh = pic.TwoDGrid(0.0, 10.0, 32, 0.0, 10.0, 32)

@@ -454,25 +454,44 @@ def test_f32_step_bitwise(gpu, O, kernel):
     ctx.close()
 
 
+# stencil of each shape: nodes per direction S, nodes left of the cell HL,
+# and QMAX, the bound on the product of two deposit weights the library's
+# fixed-point quantum is sized by (pic_internal.h Shape<K>)
+SHAPE_STENCIL = {"cic": (2, 0, 1.0), "ngp": (2, 0, 1.0), "m6": (6, 2, 0.31),
+                 "tsc": (4, 1, 0.5625), "m4": (4, 1, 0.445)}
+SHAPE_CODE = {"cic": 0, "m6": 1, "ngp": 2, "tsc": 3, "m4": 4}
+
+
 def fixed_point_bound(cell, nx, ny, kernel, w, n):
     """Largest error the int32 fixed-point tiles (accumulation mode 2) may
-    add to each node, against the oracle's FP64 sum of FP32 contributions:
-    (contributions to the node) * (u / 2 + 2^-22 qmax |w|), with u the
-    quantum of a block holding all n particles (an upper bound on every
-    block's quantum: 2^(max(ceil(log2(qmax w (n + 5))) - 30,
-    ceil(log2(qmax w)) - 22)), qmax = 1 for CIC, 0.31 for M6).  u / 2 is the
-    rounding to the quantum; 2^-22 qmax |w| covers the different FP32
-    rounding of the product of the two shape factors and w."""
-    aw = (1.0 if kernel == "cic" else 0.31) * abs(float(w))
+    add to each node, against the oracle's FP64 sum of FP32 contributions.
+
+    A block of m particles works in units of u = 2^e, e = max(ceil(log2(qmax
+    w (m + 5))) - 30, ceil(log2(qmax w)) - 22), qmax the shape's bound on a
+    product of two weights; the block of all n particles gives an upper
+    bound U on every block's u.  Each particle adds R = round(w/u) units,
+    worth w/R each (a scale off u by eps <= u/(2w)): every node but one of
+    its stencil takes its share rounded to the nearest unit, and the node of
+    its own cell takes the remainder.  So each share is off by at most
+    a = U/2 (1 + eps) + eps qmax w + 2^-22 qmax w (the last term for the
+    FP32 roundings of the products), and each remainder by the S^2 - 1
+    others' errors plus eps w plus 4e-6 w (the FP32 weights' sum misses 1 by
+    ~1e-6).  Returns a per node times the shares it receives plus that for
+    the particles whose own cell is the node."""
+    s, hl, qmax = SHAPE_STENCIL[kernel]
+    aw = qmax * abs(float(w))
     e = max(math.ceil(math.log2(aw * (n + 5))) - 30, math.ceil(math.log2(aw)) - 22)
+    U = 2.0 ** e
+    eps = U / (2 * abs(float(w)) - U)
     counts = np.bincount(cell, minlength=nx * ny).reshape(ny, nx).T  # (i, j)
-    s, hl = (2, 0) if kernel == "cic" else (6, 2)
     contrib = np.zeros((nx, ny))
     for du in range(s):
         for dv in range(s):
             # a particle in cell (i, j) reaches node (i - hl + du, j - hl + dv)
             contrib += np.roll(np.roll(counts, du - hl, axis=0), dv - hl, axis=1)
-    return contrib * (2.0 ** e / 2 + 2.0 ** -22 * aw)
+    a = U / 2 * (1 + eps) + eps * aw + 2.0 ** -22 * aw
+    b = (s * s - 1) * a + (eps + 4e-6) * abs(float(w))
+    return contrib * a + counts * b
 
 
 @pytest.mark.parametrize("acc", [0, 1, 2])
@@ -493,6 +512,17 @@ def test_f32_segment_pass_tile_copies(gpu, O, tiles):
     """The M6 fixed-point pass with 1, 2 or 4 copies of the charge tile
     (pic_set_f32_tiles): the same bound against the oracle for every count."""
     _check_f32_segment_pass(gpu, O, "m6", 2, 128, 128, tiles)
+
+
+@pytest.mark.parametrize("tiles", [1, 4])
+@pytest.mark.parametrize("kernel", ["ngp", "tsc", "m4"])
+def test_f32_segment_pass_shapes(gpu, O, kernel, tiles):
+    """The NGP, TSC and M4 fixed-point passes on 128 x 128, where the
+    windows are narrower than the grid and the deferred-particle kernel
+    runs, with one and four tile copies: cells, offsets and velocities
+    bit-identical to the oracle's FP32 step of the same shape, charge within
+    the fixed-point bound of every node."""
+    _check_f32_segment_pass(gpu, O, kernel, 2, 128, 128, tiles)
 
 
 def test_f32_tiles_rejects_other_counts(gpu):
@@ -529,10 +559,11 @@ def _check_f32_segment_pass(gpu, O, kernel, acc, nx, ny, tiles):
     e32 = np.empty(2 * nx * ny, np.float32)
     e32[0::2] = ex.ravel(order="F")
     e32[1::2] = ey.ravel(order="F")
-    kc = pic._lib.PIC_CIC if kernel == "cic" else pic._lib.PIC_M6
+    kc = SHAPE_CODE[kernel]
     raw = np.zeros((nx, ny), order="F")
     dt = 0.5
     slow0 = slow_count(pic, ctx)
+    slow_before = slow0
     for _ in range(3):  # the slots stay; particles drift out of their windows
         pic._lib.call("pic_step_pass", ctx.h, kc, 1, dt, raw.ctypes.data)
         acc_o = O.f32_step(st, nx, ny, kernel, e32, np.float32(dt / 2),
@@ -544,8 +575,13 @@ def _check_f32_segment_pass(gpu, O, kernel, acc, nx, ny, tiles):
             bound = fixed_point_bound(st[0], nx, ny, kernel, w, n)
             assert np.all(np.abs(raw - acc_o) <= bound + 1e-12 * np.abs(acc_o))
             assert nrel(raw, acc_o) < 1e-6
+            # every particle on the LDS path deposits exactly w; the deferred
+            # ones add their FP32 shares (within 4e-6 w of w each)
+            ndef = slow_count(pic, ctx) - slow_before
+            assert abs(raw.sum() - n * float(w)) <= 4e-6 * float(w) * ndef + 1e-12 * n * float(w)
         else:
             assert nrel(raw, acc_o) < 1e-12
+        slow_before = slow_count(pic, ctx)
     if nx * ny > 64 * 64:
         # windows smaller than the grid: the global-memory fallback of
         # out-of-window particles ran
