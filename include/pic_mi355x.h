@@ -155,6 +155,21 @@ int pic_set_bz(pic_ctx *ctx, int mode, double bz0);
  * FP64 state. */
 int pic_compute_rho(pic_ctx *ctx, int kernel);
 
+/* compute_rho(p, kernel, mesh) as pic_compute_rho computes it, on
+ * caller-owned device memory and allocating nothing: no counterpart in the
+ * reference, for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * d_p is the reference's p.array (5 x n doubles: x, y, vx, vy, w) and d_rho
+ * the nx x ny result, both in device memory of the current device, on the
+ * mesh TwoDGrid(xmin, xmax, nx, ymin, ymax, ny); any particle weights.  The
+ * deposit accumulates in a static device array of the library (one per
+ * GPU, held until the process ends) that limits the grid to nx * ny <=
+ * 512 * 512 nodes (PIC_ERR_ARG beyond).  The kernels are enqueued on
+ * `stream` (a hipStream_t, NULL for the null stream) and the call returns
+ * without waiting; calls that overlap on one GPU share the array and race. */
+int pic_compute_rho_noalloc(int kernel, const double *d_p, int64_t n, int nx, int ny, double xmin,
+                            double xmax, double ymin, double ymax, double *d_rho, void *stream);
+
 /* push_v!(p, kernel, mesh, ex, ey, bz, dt): pushers_cic.jl:3-53 (CIC) and
  * pushers_m6.jl:5-224 (M6), with the grids EX, EY and the field chosen by
  * pic_set_bz.  FP64 state; CIC results are bit-identical to the reference's

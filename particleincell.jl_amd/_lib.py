@@ -45,6 +45,7 @@ SIGNATURES = {
     "pic_grid_device_ptr": [_vp, _i, C.POINTER(_vp)],
     "pic_set_bz": [_vp, _i, _d],
     "pic_compute_rho": [_vp, _i],
+    "pic_compute_rho_noalloc": [_i, _vp, _i64, _i, _i, _d, _d, _d, _d, _vp, _vp],
     "pic_push_v": [_vp, _i, _d],
     "pic_push_x": [_vp, _d],
     "pic_operator_t": [_vp, _d],

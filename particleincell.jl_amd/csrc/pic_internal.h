@@ -234,6 +234,17 @@ struct DevBuf {
 constexpr int RED_BLOCKS = 256;
 constexpr int RED_THREADS = 256;
 
+// Charge reductions of pic_poisson.hip, also launched by the allocation-free
+// compute_rho (pic_noalloc.hip): the fixed-order two-level sum of a grid
+// (partials per block, the last block to arrive sums them and clears
+// *count), and rho = acc / (dx dy) minus the reference's shift.
+__global__ __launch_bounds__(RED_THREADS) void k_sum(const double *__restrict__ a, int64_t m,
+                                                     double *part, unsigned *count,
+                                                     double *out);
+__global__ void k_finalize_rho(const double *__restrict__ acc, double *__restrict__ rho, int64_t m,
+                               double dxdy, const double *total, int kernel, double dx, double dy,
+                               double dimx, double dimy);
+
 }  // namespace pic
 
 // ------------------------------------------------------- device helpers
