@@ -147,7 +147,7 @@ def _reload(sim, p0, state):
 
 
 def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_acc=2, bz=None,
-             tiles=4, acc_steps=0):
+             tiles=2, acc_steps=0):
     """Warmup, then the timed run from the initial state, then (acc_steps >
     0) an untimed accuracy run of acc_steps from the same initial state,
     whose energy history the damping rate is fitted on. Returns dict."""
@@ -315,7 +315,7 @@ def bench_adjoint(args, W, pic, entry, mesh, held, n, steps, warmup):
         if args.only not in ("both", name):
             continue
         sim = _new_sim(pic, mesh, p0, kernel, state)
-        sim.set_f32_tiles(W.get("tiles", 4))
+        sim.set_f32_tiles(W.get("tiles", 2))
         if dxa is None:  # the start made on the device: x0 read back once
             dxa = pic.sampling.dnewton_dalpha(sim.particles()[0], alpha, kx)
         if warmup:
@@ -651,8 +651,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["f32", "f64", "both"], default="both")
     ap.add_argument("--f32-tiles", type=int, choices=[1, 2, 4], default=None,
-                    help="copies of the FP32 M6 fixed-point charge tile (default: the "
-                         "workload's, else 4)")
+                    help="copies of the fixed-point charge tile of the FP32 M6, TSC and M4 "
+                         "drift-and-deposit passes (default: the workload's, else 2)")
     ap.add_argument("--f32-acc", type=int, choices=[0, 1, 2], default=2,
                     help="FP32 charge tiles: 0 compensated FP32 pairs, 1 FP64 sums, "
                          "2 int32 fixed point")
@@ -728,7 +728,7 @@ def main():
             + (f", accuracy run {acc_steps} steps" if acc_steps else ""))
         sort_every = args.sort_every_f32 if name == "f32" else args.sort_every_f64
         res[name] = run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every,
-                             args.f32_acc, bz, W.get("tiles", 4), acc_steps)
+                             args.f32_acc, bz, W.get("tiles", 2), acc_steps)
         r = res[name]
         log(f"bench: {name} {r['seconds']*1e3:.2f} ms total, pass {r['pass_ms_avg']*1e3:.1f} us,"
             f" solve {r['solve_ms_avg']*1e3:.1f} us, deferred {r['slow_chunks']}")
@@ -775,7 +775,7 @@ def main():
                    "alpha": alpha, "k": kx, "dt": dt, "bz": bz,
                    "state": "int32 cell + fp32 offset + fp32 velocity" if head == "f32" else
                    "fp64 soa", "sort_every": args.sort_every_f32 if head == "f32" else
-                   args.sort_every_f64, "f32_charge_tiles": W.get("tiles", 4),
+                   args.sort_every_f64, "f32_charge_tiles": W.get("tiles", 2),
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -258,14 +258,14 @@ int pic_set_f32_accumulation(pic_ctx *ctx, int mode);
 
 /* Copies of the fixed-point charge tile in the LDS window of pic_run's M6,
  * TSC and M4 FP32 passes that drift and then deposit (accumulation mode 2):
- * 1, 2 or 4 (default 4).  Lane l adds into copy l mod tiles, so that neighbouring
+ * 1, 2 or 4 (default 2).  Lane l adds into copy l mod tiles, so that neighbouring
  * lanes whose particles drifted into one cell do not add to one LDS word in
  * one instruction; the copies are summed exactly at the flush, so the
  * result does not depend on the count.  More copies leave a smaller drift
- * margin in the 80 KiB window, for M6 20 cells with 2, 14 with 4 (1 uses
- * 40 KiB, margin 14).  Measured on MI355X: 4 is fastest at ~0.5 cell of drift per
- * step (Landau M6 on 128^2, dt 0.05), 2 at ~2 cells (512^2), where the
- * larger margin lets the lane sort wait 3 steps.  No reference
+ * margin in the 80 KiB window, which also holds the field twice: for M6 14
+ * cells with 2 copies, 10 with 4.  Measured on MI355X (Landau M6, 1e7
+ * particles on 128^2, dt 0.05): 2 copies 180 us per pass, 4 copies 210 us
+ * (30x more particles leave the narrower window).  No reference
  * counterpart. */
 int pic_set_f32_tiles(pic_ctx *ctx, int tiles);
 

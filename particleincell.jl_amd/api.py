@@ -885,8 +885,8 @@ class Simulation:
         L.call("pic_set_f32_accumulation", self.ctx.h, int(mode))
 
     def set_f32_tiles(self, tiles):
-        """Copies (1, 2 or 4) of the fixed-point charge tile in the M6 FP32
-        drift-and-deposit passes (pic_set_f32_tiles)."""
+        """Copies (1, 2 or 4, default 2) of the fixed-point charge tile in the
+        M6, TSC and M4 FP32 drift-and-deposit passes (pic_set_f32_tiles)."""
         L.call("pic_set_f32_tiles", self.ctx.h, int(tiles))
 
     def timing(self):

@@ -82,6 +82,76 @@ __device__ __forceinline__ void interp32(float fx, float fy, int ci, int cj, FE 
     }
 }
 
+// interp32 with the field read two nodes at a time: pair(u, v) returns the
+// float4 (ex, ey) of nodes (u, v) and (u + 1, v), u even, which an LDS
+// window laid out for it reads with one 16-byte ds_read_b128 (on MI355X
+// twice the bytes per clock of the two 8-byte halves of a ds_read2_b64).
+// The same operations in the same order as interp32, so the results are
+// bit-identical; the S of every shape is even.
+template <int K, int BZ, class FP, class FB>
+__device__ __forceinline__ void interp32_pairs(float fx, float fy, int ci, int cj, FP pair,
+                                               FB node_b, float &e1, float &e2, float &b3) {
+    constexpr int S = Shape<K>::S;
+    static_assert(S % 2 == 0, "pairs of nodes");
+    if (K == PIC_CIC) {
+        float sx = 1.0f - fx, sy = 1.0f - fy;
+        float a0 = sx * sy, a1 = fx * sy, a2 = fx * fy, a3 = sx * fy;
+        const float4 R0 = pair(0, 0), R1 = pair(0, 1);  // (E00, E10), (E01, E11)
+        e1 = a0 * R0.x;
+        e1 = __builtin_fmaf(a1, R0.z, e1);
+        e1 = __builtin_fmaf(a2, R1.z, e1);
+        e1 = __builtin_fmaf(a3, R1.x, e1);
+        e2 = a0 * R0.y;
+        e2 = __builtin_fmaf(a1, R0.w, e2);
+        e2 = __builtin_fmaf(a2, R1.w, e2);
+        e2 = __builtin_fmaf(a3, R1.y, e2);
+        if (BZ == PIC_BZ_GRID) {
+            b3 = a0 * node_b(0, 0);
+            b3 = __builtin_fmaf(a1, node_b(1, 0), b3);
+            b3 = __builtin_fmaf(a2, node_b(1, 1), b3);
+            b3 = __builtin_fmaf(a3, node_b(0, 1), b3);
+        }
+    } else {
+        float wx[6], wy[6];
+        gather_w<K>(fx, ci, wx);
+        gather_w<K>(fy, cj, wy);
+        e1 = 0.0f;
+        e2 = 0.0f;
+#pragma unroll
+        for (int u = 0; u < S; u += 2) {
+            const float4 P0 = pair(u, 0);
+            float t1 = wy[0] * P0.x, t2 = wy[0] * P0.y;
+            float q1 = wy[0] * P0.z, q2 = wy[0] * P0.w;
+#pragma unroll
+            for (int v = 1; v < S; ++v) {
+                const float4 Pv = pair(u, v);
+                t1 = __builtin_fmaf(wy[v], Pv.x, t1);
+                t2 = __builtin_fmaf(wy[v], Pv.y, t2);
+                q1 = __builtin_fmaf(wy[v], Pv.z, q1);
+                q2 = __builtin_fmaf(wy[v], Pv.w, q2);
+            }
+            if (u == 0) {
+                e1 = wx[0] * t1;
+                e2 = wx[0] * t2;
+            } else {
+                e1 = __builtin_fmaf(wx[u], t1, e1);
+                e2 = __builtin_fmaf(wx[u], t2, e2);
+            }
+            e1 = __builtin_fmaf(wx[u + 1], q1, e1);
+            e2 = __builtin_fmaf(wx[u + 1], q2, e2);
+        }
+        if (BZ == PIC_BZ_GRID) {
+#pragma unroll
+            for (int u = 0; u < S; ++u) {
+                float t3 = wy[0] * node_b(u, 0);
+#pragma unroll
+                for (int v = 1; v < S; ++v) t3 = __builtin_fmaf(wy[v], node_b(u, v), t3);
+                b3 = u == 0 ? wx[0] * t3 : __builtin_fmaf(wx[u], t3, b3);
+            }
+        }
+    }
+}
+
 // Velocity update of the FP32 step from the gathered field: half kick,
 // rotation by bz, half kick (pushers_cic.jl:38-49 in FP32 with FMAs).
 template <int BZ>

@@ -226,9 +226,9 @@ inline bool multi_tile_shape(int kernel) {
     return kernel == PIC_M6 || kernel == PIC_TSC || kernel == PIC_M4;
 }
 
-template <int K, bool DRIFT, bool DEPOSIT, int ACC>
+template <int K, bool DRIFT, bool DEPOSIT, int ACC, bool PERM>
 constexpr int seg32_waves() {
-    return multi_tile_shape<K>() && ACC == 2 && DRIFT && DEPOSIT ? 4 : 8;
+    return multi_tile_shape<K>() && ACC == 2 && DRIFT && DEPOSIT && PERM ? 4 : 8;
 }
 
 // Block size: the M6 fixed-point drift-and-deposit passes run 1024 threads
@@ -243,7 +243,7 @@ constexpr int seg32_threads() {
 
 template <int K, bool GATHER, bool DRIFT, bool DEPOSIT, int BZ, int ACC, bool PERM>
 __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
-                             (seg32_waves<K, DRIFT, DEPOSIT, ACC>())) void k_seg_f32(SegF32 a) {
+                             (seg32_waves<K, DRIFT, DEPOSIT, ACC, PERM>())) void k_seg_f32(SegF32 a) {
     constexpr int BT = seg32_threads<K, DRIFT, DEPOSIT, ACC>();
     using St = SegStencil<K>;
     constexpr int S = St::S, HL = St::HL;
@@ -259,9 +259,16 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
     const int64_t s0 = a.off[64 * s], s1 = a.off[ce];
     const int64_t len = s1 - s0;
     const int64_t p0 = s0 + len * bi / a.bps, p1 = s0 + len * (bi + 1) / a.bps;
-    const Win W = seg_window<S, HL>(s, nx, ny, a.ncells, a.margin, a.cap);
-    const bool fits = W.tw * W.th <= a.cap;
+    // The field window is held twice, as float2 (ex, ey) rows of pitch P
+    // (even): copy A with node u at position u, copy B with node u at u + 1.
+    // A particle whose stencil starts at an even column reads A, one that
+    // starts at an odd column B, both at an even position, so every pair of
+    // nodes (u, u + 1) it needs is one aligned 16-byte read (interp32_pairs).
+    constexpr int PAD = GATHER ? 2 : 0;
+    const Win W = seg_window<S, HL>(s, nx, ny, a.ncells, a.margin, a.cap, PAD);
+    const bool fits = (W.tw + PAD) * W.th <= a.cap;
     const int cap = a.cap;
+    const int P = GATHER ? (W.tw + 2) & ~1 : W.tw;
     // ACC 0: the (hi, lo) split constant; ACC 2: the fixed-point quantum
     // 2^qe and w in units of it (exact: a power-of-two scaling)
     const float cq = ACC == 0 ? split_constant(a.w, p1 - p0) : 0.0f;
@@ -271,9 +278,10 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
     const bool conserve = ACC == 2 && p1 - p0 <= CONSERVE_MAX_COUNT;
     const int R = conserve ? (int)rintf(wu) : 0;
     const double unit = ACC != 2 ? 0.0 : (conserve && R != 0) ? (double)a.w / R : ldexp(1.0, qe);
-    float2 *te = reinterpret_cast<float2 *>(smem_s32);
-    float *tb = reinterpret_cast<float *>(smem_s32 + 8 * (size_t)cap);
-    char *dep = smem_s32 + (GATHER ? (BZ == PIC_BZ_GRID ? 12 : 8) * (size_t)cap : 0);
+    float2 *teA = reinterpret_cast<float2 *>(smem_s32);
+    float2 *teB = teA + cap;  // cap is a multiple of 16: 16-byte aligned
+    float *tb = reinterpret_cast<float *>(smem_s32 + 16 * (size_t)cap);
+    char *dep = smem_s32 + (GATHER ? (BZ == PIC_BZ_GRID ? 20 : 16) * (size_t)cap : 0);
     float *hi = reinterpret_cast<float *>(dep), *lo = hi + cap;
     double *dacc = reinterpret_cast<double *>(dep);
     int *iacc = reinterpret_cast<int *>(dep);
@@ -288,7 +296,9 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
             const int u = t % W.tw, v = t / W.tw;
             const int64_t g = (int64_t)((W.x0 + u) & mx) + (int64_t)nx * ((W.y0 + v) & my);
             if (GATHER) {
-                te[t] = a.e32[g];
+                const float2 ev = a.e32[g];
+                teA[u + P * v] = ev;
+                teB[u + 1 + P * v] = ev;
                 if (BZ == PIC_BZ_GRID) tb[t] = a.bz[g];
             }
             if (DEPOSIT) {
@@ -318,8 +328,12 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
             defer_push(!in, (uint32_t)p, a.dlist, a.dcount);
             if (!in) continue;
             float e1, e2, b3 = 0.0f;
-            interp32<K, BZ>(
-                fx, fy, ci, cj, [&](int u, int v) { return te[(lx + u) + W.tw * (ly + v)]; },
+            const int odd = lx & 1;
+            const float4 *row =
+                reinterpret_cast<const float4 *>((odd ? teB : teA) + (lx + odd) + P * ly);
+            const int P2 = P >> 1;
+            interp32_pairs<K, BZ>(
+                fx, fy, ci, cj, [&](int u, int v) { return row[(u >> 1) + P2 * v]; },
                 [&](int u, int v) { return tb[(lx + u) + W.tw * (ly + v)]; }, e1, e2, b3);
             kick32<BZ>(a.hdt, a.bz0, e1, e2, b3, ux, uy);
         }
@@ -836,7 +850,8 @@ int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     // and M4 with two copies in 53 KiB, 512-thread blocks; with four copies,
     // 80 KiB and 1024-thread blocks TSC and M4 gain 3-5 %.
     a.tiles = acc == 2 && multi_tile_shape(kernel) && (mode == 0 || mode == 1) ? c->f32_tiles : 1;
-    const size_t per_node = (gather ? (c->bz_mode == PIC_BZ_GRID ? 12 : 8) : 0) +
+    // field window twice (16 B per node), bz (4 B), charge tiles
+    const size_t per_node = (gather ? (c->bz_mode == PIC_BZ_GRID ? 20 : 16) : 0) +
                             (mode == 2 ? 0 : acc == 2 ? 4 * a.tiles : 8);
     a.cap = (int)((a.tiles > 1 ? SEG32_FIXED_LDS_BYTES : SEG_LDS_BYTES) /
                   (per_node ? per_node : 8)) &

@@ -24,11 +24,12 @@ struct Win {
 };
 
 // Window of segment s: its home cells widened by the stencil and a drift
-// margin, the margin reduced until the window fits `cap` nodes.  A window as
-// wide as the periodic grid plus the stencil covers every cell of that
-// direction (x0 = -HL then), duplicating S-1 columns.
+// margin, the margin reduced until the window, with `pad` spare columns per
+// row, fits `cap` nodes.  A window as wide as the periodic grid plus the
+// stencil covers every cell of that direction (x0 = -HL then), duplicating
+// S-1 columns.
 template <int S, int HL>
-__device__ Win seg_window(int s, int nx, int ny, int ncells, int margin, int cap) {
+__device__ Win seg_window(int s, int nx, int ny, int ncells, int margin, int cap, int pad = 0) {
     const int cf = 64 * s, cl = min(64 * s + 63, ncells - 1);
     const int jh0 = cf / nx, jh1 = cl / nx;
     int ih0, ih1;
@@ -45,7 +46,7 @@ __device__ Win seg_window(int s, int nx, int ny, int ncells, int margin, int cap
     for (;; --M) {
         tw = min(hw + S - 1 + 2 * M, nx + S - 1);
         th = min(hh + S - 1 + 2 * M, ny + S - 1);
-        if (tw * th <= cap || M == 0) break;
+        if ((tw + pad) * th <= cap || M == 0) break;
     }
     Win w;
     w.tw = tw;
@@ -105,10 +106,12 @@ constexpr int DEFER_T = 256, DEFER_BLOCKS = 256;
 // 512-thread blocks share a CU's 160 KiB; an M6 window of a 64-cell row
 // segment then keeps a drift margin of 11 cells (FP32) or 7 cells (FP64).
 constexpr size_t SEG_LDS_BYTES = 40 * 1024;
-// The FP32 M6 passes with several fixed-point charge tiles hold the field
-// (8 B) and 2 or 4 int32 tiles (8 or 16 B) per node.  80 KiB (two blocks per
-// CU) gives the M6 window of a 64-cell row segment a 20-cell drift margin
-// with two tiles, 14 with four.  Measured with two tiles on
+// The FP32 M6, TSC and M4 passes with several fixed-point charge tiles hold
+// the field twice (16 B, for 16-byte paired reads) and 2 or 4 int32 tiles
+// (8 or 16 B) per node.  80 KiB (two blocks per CU) gives the M6 window of a
+// 64-cell row segment a 14-cell drift margin with two tiles, 10 with four
+// (with the field held once: 20 and 14).  Measured with two tiles and one
+// field copy on
 // MI355X against 53 KiB (three blocks per CU, margin 14): the same pass time
 // at 1e7 particles on 128^2 with ~1000x fewer deferred particles, and on
 // 512^2 (5e8 particles, ~2 cells of drift per step) a sort every 3 steps
