@@ -80,6 +80,29 @@ def pmc_traffic(kernel, profile="pmc_traffic_landau_m6_128x128_1e7.txt"):
     return None
 
 
+# MI355X issue rates behind the pass's non-HBM roofs: 1024 SIMDs issuing one
+# wave64 VALU instruction per 4 clocks each (FP32 without packed math), LDS
+# 128 bytes per clock per CU on 256 CUs (the ds_read_b128 rate measured on
+# the card), at the 2.4 GHz peak engine clock.
+CLOCK_HZ = 2.4e9
+VALU_WAVE_INSTS_PER_S = 1024 * CLOCK_HZ / 4
+LDS_BYTES_PER_S = 256 * 128 * CLOCK_HZ
+
+
+def issue_counts(kernel, profile="sq_counters_dominant_kernels.txt"):
+    """(VALU wave instructions, LDS bytes) per particle of `kernel` from the
+    committed SQ-counter profile's 'issue' lines, or None."""
+    path = os.path.join(ROOT, "profiles", profile)
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) >= 4 and f[0] == "issue" and f[1] == kernel:
+                return float(f[2]), float(f[3])
+    except OSError:
+        pass
+    return None
+
+
 def copy_probe(mib, reps=5):
     """Plain device copies of `mib` MiB (torch's elementwise copy kernel), the
     calibration workload for the HBM byte counters: a copy of B bytes reads B
@@ -147,13 +170,14 @@ def _reload(sim, p0, state):
 
 
 def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_acc=2, bz=None,
-             tiles=2, acc_steps=0):
+             tiles=4, acc_steps=0, window=160):
     """Warmup, then the timed run from the initial state, then (acc_steps >
     0) an untimed accuracy run of acc_steps from the same initial state,
     whose energy history the damping rate is fitted on. Returns dict."""
     sim = _new_sim(pic, mesh, p0, kernel, state, bz)
     sim.set_f32_accumulation(f32_acc)
     sim.set_f32_tiles(tiles)
+    sim.set_f32_window(window)
     sim.sort(lanes=True)
     if warmup:
         sim.run(dt, warmup, sort_every=sort_every)
@@ -227,7 +251,7 @@ def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_ac
 WORKLOADS = {
     "landau_m6": dict(name="landau2d2v-m6-128x128-1e7-dt0.05-400steps", grid=128, n=10_000_000,
                       kernel="m6", alpha=0.01, k=0.5, dt=0.05, steps=400, warmup=20,
-                      sampler="quiet", sort=(8, 4)),
+                      sampler="quiet", sort=(12, 4)),
     "landau_cic": dict(name="landau2d2v-cic-64x64-204800-dt0.1-100steps", grid=64, n=204_800,
                        kernel="cic", alpha=0.05, k=0.5, dt=0.1, steps=100, warmup=10,
                        sampler="quiet", sort=(8, 4)),
@@ -236,7 +260,7 @@ WORKLOADS = {
                      warmup=10, sampler="maxwellian_device", bz=20.0, sort=(32, 16)),
     "large_m6": dict(name="landau2d2v-m6-512x512-5e8-dt0.05-200steps", grid=512,
                      n=500_000_000, kernel="m6", alpha=0.05, k=0.5, dt=0.05, steps=200,
-                     warmup=5, sampler="landau_device", sort=(3, 1), tiles=2),
+                     warmup=5, sampler="landau_device", sort=(4, 1), tiles=2),
     # the other shapes of the reference's Landau example
     # (examples/landau_damping_poisson_cic_m6.jl:88-103) on the M6 workload
     "landau_ngp": dict(name="landau2d2v-ngp-128x128-1e7-dt0.05-400steps", grid=128,
@@ -244,10 +268,10 @@ WORKLOADS = {
                        warmup=20, sampler="quiet", sort=(8, 4)),
     "landau_tsc": dict(name="landau2d2v-tsc-128x128-1e7-dt0.05-400steps", grid=128,
                        n=10_000_000, kernel="tsc", alpha=0.01, k=0.5, dt=0.05, steps=400,
-                       warmup=20, sampler="quiet", sort=(8, 4)),
+                       warmup=20, sampler="quiet", sort=(12, 4)),
     "landau_m4": dict(name="landau2d2v-m4-128x128-1e7-dt0.05-400steps", grid=128,
                       n=10_000_000, kernel="m4", alpha=0.01, k=0.5, dt=0.05, steps=400,
-                      warmup=20, sampler="quiet", sort=(8, 4)),
+                      warmup=20, sampler="quiet", sort=(12, 4)),
     # the Vlasov-Maxwell loop of examples/vm2d.jl (FDTD fields + CIC current
     # deposit, FP64): 100 particles per cell (the README's 204,800
     # particles) and 500 per cell (vm2d.jl's timed run), 1000 steps
@@ -315,7 +339,8 @@ def bench_adjoint(args, W, pic, entry, mesh, held, n, steps, warmup):
         if args.only not in ("both", name):
             continue
         sim = _new_sim(pic, mesh, p0, kernel, state)
-        sim.set_f32_tiles(W.get("tiles", 2))
+        sim.set_f32_tiles(W.get("tiles", 4))
+        sim.set_f32_window(W.get("window", 160))
         if dxa is None:  # the start made on the device: x0 read back once
             dxa = pic.sampling.dnewton_dalpha(sim.particles()[0], alpha, kx)
         if warmup:
@@ -652,7 +677,10 @@ def main():
     ap.add_argument("--only", choices=["f32", "f64", "both"], default="both")
     ap.add_argument("--f32-tiles", type=int, choices=[1, 2, 4], default=None,
                     help="copies of the fixed-point charge tile of the FP32 M6, TSC and M4 "
-                         "drift-and-deposit passes (default: the workload's, else 2)")
+                         "drift-and-deposit passes (default: the workload's, else 4)")
+    ap.add_argument("--f32-window", type=int, default=None, metavar="KIB",
+                    help="LDS window per block of the FP32 M6, TSC and M4 drift-and-deposit "
+                         "passes, 40 to 160 KiB (default: the workload's, else 160)")
     ap.add_argument("--f32-acc", type=int, choices=[0, 1, 2], default=2,
                     help="FP32 charge tiles: 0 compensated FP32 pairs, 1 FP64 sums, "
                          "2 int32 fixed point")
@@ -678,6 +706,8 @@ def main():
     W = dict(WORKLOADS[args.workload])
     if args.f32_tiles is not None:
         W["tiles"] = args.f32_tiles
+    if args.f32_window is not None:
+        W["window"] = args.f32_window
     if W.get("ua"):
         bench_ua(args, W, pic, entry, W["steps"] if args.steps is None else args.steps,
                  W["warmup"] if args.warmup is None else args.warmup)
@@ -728,7 +758,8 @@ def main():
             + (f", accuracy run {acc_steps} steps" if acc_steps else ""))
         sort_every = args.sort_every_f32 if name == "f32" else args.sort_every_f64
         res[name] = run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every,
-                             args.f32_acc, bz, W.get("tiles", 2), acc_steps)
+                             args.f32_acc, bz, W.get("tiles", 4), acc_steps,
+                             W.get("window", 160))
         r = res[name]
         log(f"bench: {name} {r['seconds']*1e3:.2f} ms total, pass {r['pass_ms_avg']*1e3:.1f} us,"
             f" solve {r['solve_ms_avg']*1e3:.1f} us, deferred {r['slow_chunks']}")
@@ -775,7 +806,8 @@ def main():
                    "alpha": alpha, "k": kx, "dt": dt, "bz": bz,
                    "state": "int32 cell + fp32 offset + fp32 velocity" if head == "f32" else
                    "fp64 soa", "sort_every": args.sort_every_f32 if head == "f32" else
-                   args.sort_every_f64, "f32_charge_tiles": W.get("tiles", 2),
+                   args.sort_every_f64, "f32_charge_tiles": W.get("tiles", 4),
+                   "f32_window_kib": W.get("window", 160),
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -830,8 +862,22 @@ def main():
                 "GBs": split_bytes[name][k] * n / (ms * 1e-3) / 1e9,
                 "frac": split_bytes[name][k] * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
             for k, ms in rr["split_ms"].items()}
-    if args.workload == "landau_m6":
-        traffic = pmc_traffic(f"k_seg_{head}")
+    # the pass against its VALU-issue and LDS roofs (instruction counts of the
+    # committed SQ profile over the live pass time)
+    ic = issue_counts(f"k_seg_{head}<{kname}>")
+    if ic is not None:
+        valu_us = n * ic[0] / VALU_WAVE_INSTS_PER_S * 1e6
+        lds_us = n * ic[1] / LDS_BYTES_PER_S * 1e6
+        pass_us = r["pass_ms_avg"] * 1e3
+        out["roofline"]["issue"] = {
+            "valu": {"wave_insts_per_particle": ic[0], "floor_us": valu_us,
+                     "frac": valu_us / pass_us},
+            "lds": {"bytes_per_particle": ic[1], "floor_us": lds_us, "frac": lds_us / pass_us},
+            "source": "profiles/sq_counters_dominant_kernels.txt"}
+    traffic_file = {"landau_m6": "pmc_traffic_landau_m6_128x128_1e7.txt",
+                    "strong_b": "pmc_traffic_strong_b_256x256_1e8.txt"}.get(args.workload)
+    if traffic_file and n == W["n"]:
+        traffic = pmc_traffic(f"k_seg_{head}", traffic_file)
         if traffic is not None:
             out["roofline"]["traffic"] = traffic[0]
             out["roofline"]["traffic_source"] = traffic[1]

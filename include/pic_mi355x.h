@@ -258,16 +258,28 @@ int pic_set_f32_accumulation(pic_ctx *ctx, int mode);
 
 /* Copies of the fixed-point charge tile in the LDS window of pic_run's M6,
  * TSC and M4 FP32 passes that drift and then deposit (accumulation mode 2):
- * 1, 2 or 4 (default 2).  Lane l adds into copy l mod tiles, so that neighbouring
+ * 1, 2 or 4 (default 4).  Lane l adds into copy l mod tiles, so that neighbouring
  * lanes whose particles drifted into one cell do not add to one LDS word in
  * one instruction; the copies are summed exactly at the flush, so the
  * result does not depend on the count.  More copies leave a smaller drift
- * margin in the 80 KiB window, which also holds the field twice: for M6 14
- * cells with 2 copies, 10 with 4.  Measured on MI355X (Landau M6, 1e7
- * particles on 128^2, dt 0.05): 2 copies 180 us per pass, 4 copies 210 us
- * (30x more particles leave the narrower window).  No reference
- * counterpart. */
+ * margin in the window (pic_set_f32_window), which also holds the field
+ * twice: for M6 in 160 KiB 25 cells with 2 copies, 20 with 4; in 80 KiB 14
+ * and 10.  Measured on MI355X with 160 KiB windows: Landau M6, 1e7
+ * particles on 128^2 (~0.5 cell of drift per step), 4 copies and a lane
+ * sort every 12 steps, 4.48e10 particle-steps/s (2 copies 4.31e10); 5e8
+ * particles on 512^2 (~2 cells per step), 2 copies and a sort every 4 steps,
+ * 3.93e10 (4 copies, every 3 steps, 3.82e10).  No reference counterpart. */
 int pic_set_f32_tiles(pic_ctx *ctx, int tiles);
+
+/* LDS window per block, in KiB (40 to 160, default 160), of the same passes
+ * as pic_set_f32_tiles: 160 runs one 1024-thread block per CU, 80 two.  A
+ * larger window keeps a wider drift margin (M6 with two tile copies: 25 cells
+ * at 160 KiB, 14 at 80), so fewer particles leave it between lane sorts and
+ * take the global-memory path, and the sorts can be further apart.
+ * Measured on MI355X (M6, two copies, 5e8 particles on 512^2): 80 KiB with a
+ * sort every 3 steps 2.76e10 particle-steps/s, 160 KiB every 4 steps 3.93e10.
+ * No reference counterpart. */
+int pic_set_f32_window(pic_ctx *ctx, int kib);
 
 /* One particle pass of pic_run with the fields currently in the context
  * (EX, EY as last set or solved, and the pic_set_bz field), for callers that

@@ -68,6 +68,7 @@ SIGNATURES = {
     "pic_set_timing": [_vp, _i],
     "pic_set_f32_accumulation": [_vp, _i],
     "pic_set_f32_tiles": [_vp, _i],
+    "pic_set_f32_window": [_vp, _i],
     "pic_set_fused_solve": [_vp, _i],
     "pic_get_timing": [_vp, _pd, C.POINTER(C.c_int64), _pd],
     "pic_slow_chunks": [_vp, C.POINTER(C.c_uint64)],

@@ -885,9 +885,14 @@ class Simulation:
         L.call("pic_set_f32_accumulation", self.ctx.h, int(mode))
 
     def set_f32_tiles(self, tiles):
-        """Copies (1, 2 or 4, default 2) of the fixed-point charge tile in the
+        """Copies (1, 2 or 4, default 4) of the fixed-point charge tile in the
         M6, TSC and M4 FP32 drift-and-deposit passes (pic_set_f32_tiles)."""
         L.call("pic_set_f32_tiles", self.ctx.h, int(tiles))
+
+    def set_f32_window(self, kib):
+        """LDS window per block (KiB, 40 to 160, default 160) of the M6, TSC and
+        M4 FP32 drift-and-deposit passes (pic_set_f32_window)."""
+        L.call("pic_set_f32_window", self.ctx.h, int(kib))
 
     def timing(self):
         """(pass_ms_total, passes, solve_ms_total) of pic_run since timing

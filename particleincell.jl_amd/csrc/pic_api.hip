@@ -783,6 +783,13 @@ extern "C" int pic_set_f32_accumulation(pic_ctx *ctx, int mode) {
     return PIC_OK;
 }
 
+extern "C" int pic_set_f32_window(pic_ctx *ctx, int kib) {
+    PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
+    PIC_CHECK(kib >= 40 && kib <= 160, PIC_ERR_ARG, "window of %d KiB (40 to 160)", kib);
+    ctx_of(ctx)->f32_window_kib = kib;
+    return PIC_OK;
+}
+
 extern "C" int pic_set_f32_tiles(pic_ctx *ctx, int tiles) {
     PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
     PIC_CHECK(tiles == 1 || tiles == 2 || tiles == 4, PIC_ERR_ARG, "tile copies %d (1, 2 or 4)",

@@ -853,9 +853,12 @@ int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     // field window twice (16 B per node), bz (4 B), charge tiles
     const size_t per_node = (gather ? (c->bz_mode == PIC_BZ_GRID ? 20 : 16) : 0) +
                             (mode == 2 ? 0 : acc == 2 ? 4 * a.tiles : 8);
-    a.cap = (int)((a.tiles > 1 ? SEG32_FIXED_LDS_BYTES : SEG_LDS_BYTES) /
-                  (per_node ? per_node : 8)) &
-            ~15;
+    // the M6, TSC and M4 drift-and-deposit passes with fixed-point tiles run
+    // 1024-thread blocks with the context's window (pic_set_f32_window, 160
+    // KiB by default: one block per CU); the others 40 KiB
+    const bool wide = acc == 2 && multi_tile_shape(kernel) && (mode == 0 || mode == 1);
+    const size_t win = wide ? (size_t)c->f32_window_kib * 1024 : SEG_LDS_BYTES;
+    a.cap = (int)(win / (per_node ? per_node : 8)) & ~15;
     a.slow = c->slow_chunks;
     PIC_TRY(ensure_defer(c));
     a.dlist = c->defer;

@@ -525,6 +525,24 @@ def test_f32_segment_pass_shapes(gpu, O, kernel, tiles):
     _check_f32_segment_pass(gpu, O, kernel, 2, 128, 128, tiles)
 
 
+@pytest.mark.parametrize("window", [40, 80, 160])
+def test_f32_segment_pass_windows(gpu, O, window):
+    """The M6 fixed-point pass with LDS windows of 40, 80 and 160 KiB
+    (pic_set_f32_window): the same bound against the oracle for every size
+    (the narrower windows send more particles to the global-memory path)."""
+    _check_f32_segment_pass(gpu, O, "m6", 2, 128, 128, 4, window)
+
+
+def test_f32_window_rejects_other_sizes(gpu):
+    pic = gpu
+    mesh = pic.TwoDGrid(4 * np.pi, 16, 4 * np.pi, 16)
+    ctx = pic.Context(mesh, capacity=16)
+    for bad in (0, 39, 161, 1024):
+        with pytest.raises(pic._lib.PicError, match="window"):
+            pic._lib.call("pic_set_f32_window", ctx.h, bad)
+    ctx.close()
+
+
 def test_f32_tiles_rejects_other_counts(gpu):
     pic = gpu
     mesh = pic.TwoDGrid(4 * np.pi, 16, 4 * np.pi, 16)
@@ -535,7 +553,7 @@ def test_f32_tiles_rejects_other_counts(gpu):
     ctx.close()
 
 
-def _check_f32_segment_pass(gpu, O, kernel, acc, nx, ny, tiles):
+def _check_f32_segment_pass(gpu, O, kernel, acc, nx, ny, tiles, window=None):
     pic = gpu
     rng = np.random.default_rng(41 + nx)
     n = 200000
@@ -556,6 +574,8 @@ def _check_f32_segment_pass(gpu, O, kernel, acc, nx, ny, tiles):
     pic._lib.call("pic_set_f32_accumulation", ctx.h, acc)
     if tiles is not None:
         pic._lib.call("pic_set_f32_tiles", ctx.h, tiles)
+    if window is not None:
+        pic._lib.call("pic_set_f32_window", ctx.h, window)
     e32 = np.empty(2 * nx * ny, np.float32)
     e32[0::2] = ex.ravel(order="F")
     e32[1::2] = ey.ravel(order="F")
