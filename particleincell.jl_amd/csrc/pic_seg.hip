@@ -157,11 +157,12 @@ __device__ __forceinline__ void charge32(float fx, float fy, int ci, int cj, flo
 // multiply-add onto 1.5 * 2^23 (the float grid there is the integers; the
 // products stay below 2^22 by the choice of u).
 //
-// With `conserve` the node (HL, HL), the lower-left node of the particle's
-// own cell, takes R minus the sum of the other nodes' rounded shares, R =
-// round(w/u) the same for every particle of the block, so that every
-// particle adds exactly R units and the flush's scale w/R gives the block
-// exactly its count times w.  Rounding every share to nearest instead drops
+// With `conserve` the node (HL | 1, HL) (the cell's own node for TSC and M4,
+// the next one for CIC, NGP and M6: always an odd column of the stencil, so
+// that in the packed tiles below it is the high half of its pair) takes R
+// minus the sum of the other nodes' rounded shares, R = round(|w|/u) the same
+// for every particle of the block, so that every particle adds exactly R
+// units and the flush's scale w/R gives the block exactly its count times w.  Rounding every share to nearest instead drops
 // the many shares below u/2 at a stencil's far nodes (M6: products of two
 // weights of ~1e-3): measured on 5e8 particles on 512^2, ~2e4 units per
 // particle, the deposit lost 2.2e-5 of the charge.  The remainder node
@@ -170,18 +171,18 @@ __device__ __forceinline__ void charge32(float fx, float fy, int ci, int cj, flo
 template <int K, class F>
 __device__ __forceinline__ void charge32_fixed(float fx, float fy, int ci, int cj, float wu, int R,
                                                bool conserve, F &&add) {
-    constexpr int S = Shape<K>::S, HL = Shape<K>::HL;
+    constexpr int S = Shape<K>::S, HL = Shape<K>::HL, UR = HL | 1, VR = HL;
     constexpr float C = 12582912.0f;  // 1.5 * 2^23
     auto n = [&](float a, float b) {
         return __float_as_int(__builtin_fmaf(a, b, C)) - __float_as_int(C);
     };
     if (K == PIC_CIC) {
         const float sx = 1.0f - fx, sy = (1.0f - fy) * wu, ty = fy * wu;
-        const int n10 = n(fx, sy), n11 = n(fx, ty), n01 = n(sx, ty);
-        add(1, 0, n10);
+        const int n00 = n(sx, sy), n11 = n(fx, ty), n01 = n(sx, ty);
+        add(0, 0, n00);
         add(1, 1, n11);
         add(0, 1, n01);
-        add(0, 0, conserve ? R - (n10 + n11 + n01) : n(sx, sy));
+        add(1, 0, conserve ? R - (n00 + n11 + n01) : n(fx, sy));
     } else {
         float wx[6], wy[6];
         deposit_w<K>(fx, ci, wx);
@@ -193,13 +194,69 @@ __device__ __forceinline__ void charge32_fixed(float fx, float fy, int ci, int c
         for (int u = 0; u < S; ++u)
 #pragma unroll
             for (int v = 0; v < S; ++v) {
-                if (u == HL && v == HL) continue;
+                if (u == UR && v == VR) continue;
                 const int q = n(wx[u], wy[v]);
                 s += q;
                 add(u, v, q);
             }
-        add(HL, HL, conserve ? R - s : n(wx[HL], wy[HL]));
+        add(UR, VR, conserve ? R - s : n(wx[UR], wy[VR]));
     }
+}
+
+// The same shares two nodes at a time: add(u, v, q) for u even with the
+// 64-bit q = n(u, v) + 2^32 n(u + 1, v), one ds_add_u64 into a packed tile
+// whose 8-byte words hold the int32 sums of nodes (u, u + 1) (on MI355X one
+// 64-lane ds_add_u64 takes 6.8 clocks per CU against 5.3 for a ds_add_u32,
+// both with lanes on distinct words).  The low halves are never negative
+// (|w| units; the remainder is a high half), so the word's 64-bit sum
+// carries nothing between its halves that the flush cannot undo: with L and
+// H the sums of the low and high halves, both below 2^31 in magnitude, the
+// word holds L + 2^32 H modulo 2^64.
+template <int K, class F>
+__device__ __forceinline__ void charge32_packed(float fx, float fy, int ci, int cj, float wu, int R,
+                                                bool conserve, F &&add) {
+    constexpr int S = Shape<K>::S, HL = Shape<K>::HL, UR = HL | 1, VR = HL;
+    constexpr float C = 12582912.0f;  // 1.5 * 2^23
+    auto n = [&](float a, float b) {
+        return __float_as_int(__builtin_fmaf(a, b, C)) - __float_as_int(C);
+    };
+    auto pack = [](int lo, int hi) {
+        return (unsigned long long)(unsigned)lo | ((unsigned long long)(unsigned)hi << 32);
+    };
+    float wx[6], wy[6];
+    if (K == PIC_CIC) {
+        wx[0] = 1.0f - fx;
+        wx[1] = fx;
+        wy[0] = (1.0f - fy) * wu;
+        wy[1] = fy * wu;
+    } else {
+        deposit_w<K>(fx, ci, wx);
+        deposit_w<K>(fy, cj, wy);
+#pragma unroll
+        for (int v = 0; v < S; ++v) wy[v] = wy[v] * wu;
+    }
+    int s = 0;
+#pragma unroll
+    for (int v = 0; v < S; ++v)
+#pragma unroll
+        for (int u = 0; u < S; u += 2) {
+            const int a = n(wx[u], wy[v]);
+            s += a;
+            if (u + 1 == UR && v == VR) continue;
+            const int b = n(wx[u + 1], wy[v]);
+            s += b;
+            add(u, v, pack(a, b));
+        }
+    const int a = n(wx[UR - 1], wy[VR]);
+    add(UR - 1, VR, pack(a, conserve ? R - s : n(wx[UR], wy[VR])));
+}
+
+// Signed int32 sum of the half at position pos of a packed tile.
+__device__ __forceinline__ int64_t packed_half(const unsigned long long *tile, int pos) {
+    const unsigned long long w = tile[pos >> 1];
+    const int lo = (int)(unsigned)w;
+    if (!(pos & 1)) return lo;
+    return (int)(unsigned)((w - (unsigned long long)(long long)lo) >> 32);
 }
 
 // The particles of one segment slice.  A particle whose stencil lies outside
@@ -210,11 +267,8 @@ __device__ __forceinline__ void charge32_fixed(float fx, float fy, int ci, int c
 // Register budget: the passes with 40 KiB windows are held to 64 VGPRs
 // (eight waves per SIMD), so that four blocks share a CU instead of the three
 // that the 66 registers the compiler picks unbounded allow (the M6 pass with
-// one charge tile ran ~1.5 % faster so).  The M6 fixed-point drift-and-
-// deposit passes (80 KiB windows with 2 or 4 copies of the charge tile, at
-// most two blocks per CU) are bounded at four waves per SIMD only (with two
-// copies and 512-thread blocks this was ~1.5 % faster at 1e7 particles on
-// 128^2 and ~4 % at 5e8 on 512^2 than the 64-register bound).
+// one charge tile ran ~1.5 % faster so).  The multi-tile fixed-point drift-
+// and-deposit passes are bounded at four waves per SIMD (seg32_waves below).
 // Shapes whose fixed-point drift-and-deposit passes keep several copies of
 // the charge tile (36 or 16 LDS atomics per particle; measured on MI355X
 // with four copies: M6 +4 %, TSC +3 %, M4 +5 % on 1e7 particles, 128^2).
@@ -226,9 +280,20 @@ inline bool multi_tile_shape(int kernel) {
     return kernel == PIC_M6 || kernel == PIC_TSC || kernel == PIC_M4;
 }
 
+// The multi-tile fixed-point drift-and-deposit passes run one 1024-thread
+// block per CU with the default 160 KiB window, four waves per SIMD, so they
+// are bounded at four waves per SIMD (128 VGPRs) and load each thread's next
+// particle while processing the current one.  Measured on MI355X (Landau M6,
+// 1e7 particles, 128^2): 166.7 us per pass bounded at eight waves per SIMD,
+// 164.2 at four, 161.2 at four with the early load (at eight with the early
+// load the pass spilled: 165.7).
+template <int K, bool DRIFT, bool DEPOSIT, int ACC>
+constexpr bool seg32_wide() {
+    return multi_tile_shape<K>() && ACC >= 2 && DRIFT && DEPOSIT;
+}
 template <int K, bool DRIFT, bool DEPOSIT, int ACC, bool PERM>
 constexpr int seg32_waves() {
-    return multi_tile_shape<K>() && ACC == 2 && DRIFT && DEPOSIT && PERM ? 4 : 8;
+    return seg32_wide<K, DRIFT, DEPOSIT, ACC>() ? 4 : 8;
 }
 
 // Block size: the M6 fixed-point drift-and-deposit passes run 1024 threads
@@ -238,7 +303,7 @@ constexpr int seg32_waves() {
 // such block per CU); the others SEG_T.
 template <int K, bool DRIFT, bool DEPOSIT, int ACC>
 constexpr int seg32_threads() {
-    return multi_tile_shape<K>() && ACC == 2 && DRIFT && DEPOSIT ? 1024 : SEG_T;
+    return multi_tile_shape<K>() && ACC >= 2 && DRIFT && DEPOSIT ? 1024 : SEG_T;
 }
 
 template <int K, bool GATHER, bool DRIFT, bool DEPOSIT, int BZ, int ACC, bool PERM>
@@ -264,20 +329,26 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
     // A particle whose stencil starts at an even column reads A, one that
     // starts at an odd column B, both at an even position, so every pair of
     // nodes (u, u + 1) it needs is one aligned 16-byte read (interp32_pairs).
-    constexpr int PAD = GATHER ? 2 : 0;
+    // ACC 3, the packed fixed-point tiles, keeps its charge tiles in the same
+    // two shifted layouts as the field (tile pairs A, B): pairs of stencil
+    // nodes are aligned 8-byte words in one of them.
+    constexpr bool FIX = ACC == 2 || ACC == 3, PACK = ACC == 3 && DEPOSIT;
+    constexpr int PAD = GATHER || PACK ? 2 : 0;
     const Win W = seg_window<S, HL>(s, nx, ny, a.ncells, a.margin, a.cap, PAD);
     const bool fits = (W.tw + PAD) * W.th <= a.cap;
     const int cap = a.cap;
-    const int P = GATHER ? (W.tw + 2) & ~1 : W.tw;
-    // ACC 0: the (hi, lo) split constant; ACC 2: the fixed-point quantum
-    // 2^qe and w in units of it (exact: a power-of-two scaling)
+    const int P = GATHER || PACK ? (W.tw + 2) & ~1 : W.tw;
+    // ACC 0: the (hi, lo) split constant; ACC 2, 3: the fixed-point quantum
+    // 2^qe and |w| in units of it (exact: a power-of-two scaling)
     const float cq = ACC == 0 ? split_constant(a.w, p1 - p0) : 0.0f;
-    const int qe = ACC == 2 ? fixed_exponent(a.w, Shape<K>::QMAX, p1 - p0) : 0;
-    const float wu = ACC == 2 ? ldexpf(a.w, -qe) : 0.0f;
+    const int qe = FIX ? fixed_exponent(a.w, Shape<K>::QMAX, p1 - p0) : 0;
+    const float wu = FIX ? ldexpf(fabsf(a.w), -qe) : 0.0f;
     // charge-conserving fixed point: R units per particle, worth w/R each
-    const bool conserve = ACC == 2 && p1 - p0 <= CONSERVE_MAX_COUNT;
+    const bool conserve = FIX && p1 - p0 <= CONSERVE_MAX_COUNT;
     const int R = conserve ? (int)rintf(wu) : 0;
-    const double unit = ACC != 2 ? 0.0 : (conserve && R != 0) ? (double)a.w / R : ldexp(1.0, qe);
+    const double unit = !FIX                      ? 0.0
+                        : (conserve && R != 0) ? (double)a.w / R
+                                               : copysign(ldexp(1.0, qe), (double)a.w);
     float2 *teA = reinterpret_cast<float2 *>(smem_s32);
     float2 *teB = teA + cap;  // cap is a multiple of 16: 16-byte aligned
     float *tb = reinterpret_cast<float *>(smem_s32 + 16 * (size_t)cap);
@@ -290,6 +361,11 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
     // drifted into one cell do not add to one LDS word in the same
     // instruction
     int *itile = iacc + (tid & (a.tiles - 1)) * cap;
+    if (PACK && fits) {
+        const int np = P * W.th;
+        for (int t = tid; t < np; t += BT)
+            for (int k = 0; k < a.tiles; ++k) iacc[t + k * cap] = 0;
+    }
     if (fits) {
         const int nt = W.tw * W.th;
         for (int t = tid; t < nt; t += BT) {
@@ -307,7 +383,7 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
                     lo[t] = 0.0f;
                 } else if (ACC == 1) {
                     dacc[t] = 0.0;
-                } else {
+                } else if (ACC == 2) {
                     iacc[t] = 0;
                     for (int k = 1; k < a.tiles; ++k) iacc[t + k * cap] = 0;
                 }
@@ -316,11 +392,32 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
     }
     __syncthreads();
 
+    // the next particle's state is loaded while this one is processed
+    int c_next = 0;
+    float4 o_next = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    constexpr bool PREFETCH = seg32_wide<K, DRIFT, DEPOSIT, ACC>();
+    if (PREFETCH && (uint32_t)p0 + tid < (uint32_t)p1) {
+        const uint32_t src = PERM ? a.perm[(uint32_t)p0 + tid] : (uint32_t)p0 + tid;
+        c_next = a.cell_in[src];
+        o_next = a.ov_in[src];
+    }
     for (uint32_t p = (uint32_t)p0 + tid; p < (uint32_t)p1; p += BT) {
-        const uint32_t src = PERM ? a.perm[p] : p;
-        const int c = a.cell_in[src];
+        int c;
+        float4 o;
+        if (PREFETCH) {
+            c = c_next;
+            o = o_next;
+            if (p + BT < (uint32_t)p1) {
+                const uint32_t src = PERM ? a.perm[p + BT] : p + BT;
+                c_next = a.cell_in[src];
+                o_next = a.ov_in[src];
+            }
+        } else {
+            const uint32_t src = PERM ? a.perm[p] : p;
+            c = a.cell_in[src];
+            o = a.ov_in[src];
+        }
         int ci = c & mx, cj = c >> a.lognx;
-        const float4 o = a.ov_in[src];
         float fx = o.x, fy = o.y, ux = o.z, uy = o.w;
         if (GATHER) {
             const int lx = (ci - HL - W.x0) & mx, ly = (cj - HL - W.y0) & my;
@@ -360,6 +457,25 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
                 });
                 continue;
             }
+            if (ACC == 3) {
+                // tile pair (A, B) of lane copy tid mod (tiles / 2); B for a
+                // stencil that starts at an odd column
+                const int odd = lx & 1;
+                // word (pair) index of stencil node (0, 0): (lx + odd + P ly) / 2,
+                // P even; node (u, v), u even, is word w0 + u / 2 + v P / 2
+                unsigned long long *w0 =
+                    reinterpret_cast<unsigned long long *>(
+                        iacc + ((a.tiles >= 4 ? 2 * (tid & 1) : 0) + odd) * cap) +
+                    ((lx + odd) >> 1) + (P >> 1) * ly;
+                const int P2 = P >> 1;
+                charge32_packed<K>(fx, fy, ci, cj, wu, R, conserve,
+                                   [&](int u, int v, unsigned long long q) {
+                                       __hip_atomic_fetch_add(&w0[(u >> 1) + P2 * v], q,
+                                                              __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+                                   });
+                continue;
+            }
             charge32<K>(fx, fy, ci, cj, a.w, [&](int u, int v, float q) {
                 const int t = (lx + u) + W.tw * (ly + v);
                 if (ACC == 0) {
@@ -380,11 +496,25 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
     __syncthreads();
     const int nt = W.tw * W.th;
     for (int t = tid; t < nt; t += BT) {
-        const double q = ACC == 0   ? (double)hi[t] + (double)lo[t]
-                         : ACC == 1 ? dacc[t]
-                                    : (double)tile_sum(iacc, t, cap, a.tiles) * unit;
+        const int u = t % W.tw, v = t / W.tw;
+        double q;
+        if (ACC == 0) {
+            q = (double)hi[t] + (double)lo[t];
+        } else if (ACC == 1) {
+            q = dacc[t];
+        } else if (ACC == 2) {
+            q = (double)tile_sum(iacc, t, cap, a.tiles) * unit;
+        } else {
+            // node u of the window: position u of tile A, u + 1 of tile B
+            int64_t qi = 0;
+            for (int k = 0; k + 1 < a.tiles; k += 2) {
+                const auto *A = reinterpret_cast<const unsigned long long *>(iacc + k * cap);
+                const auto *B = reinterpret_cast<const unsigned long long *>(iacc + (k + 1) * cap);
+                qi += packed_half(A, u + P * v) + packed_half(B, u + 1 + P * v);
+            }
+            q = (double)qi * unit;
+        }
         if (q != 0.0) {
-            const int u = t % W.tw, v = t / W.tw;
             unsafeAtomicAdd(&a.acc[(int64_t)((W.x0 + u) & mx) + (int64_t)nx * ((W.y0 + v) & my)],
                             q);
         }
@@ -810,6 +940,26 @@ static int seg32_perm(const Ctx *c, const SegF32 &a, int mode, int bz, size_t sm
                   : seg32_mode<K, ACC, false>(c, a, mode, bz, sm);
 }
 
+// Fixed-point passes: the packed tiles (ACC 3) for the shapes that keep
+// several tile copies, in the passes that drift and deposit; else ACC 2.
+template <int K>
+static int seg32_fixed(const Ctx *c, const SegF32 &a, int mode, int bz, size_t sm, bool packed) {
+    if constexpr (multi_tile_shape<K>()) {
+        if (packed && (mode == 0 || mode == 1)) {
+            if (a.perm) {
+                if (mode == 0) seg32_bz<K, false, true, true, 3, true>(c, a, PIC_BZ_NONE, sm);
+                else seg32_bz<K, true, true, true, 3, true>(c, a, bz, sm);
+            } else {
+                if (mode == 0) seg32_bz<K, false, true, true, 3, false>(c, a, PIC_BZ_NONE, sm);
+                else seg32_bz<K, true, true, true, 3, false>(c, a, bz, sm);
+            }
+            PIC_HIP(hipGetLastError());
+            return PIC_OK;
+        }
+    }
+    return seg32_perm<K, 2>(c, a, mode, bz, sm);
+}
+
 // One segment-persistent pass in FP32 state.  The particles are in the lane
 // order of the last sort_by_cell(..., lanes = true), or that sort left its
 // permutation pending (c->pending_perm): then this pass reads every particle
@@ -865,9 +1015,12 @@ int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     a.dcount = c->defer_count;
     const size_t sm = (size_t)a.cap * (per_node ? per_node : 8);
     int rc;
+    // two or more tile copies: packed pairs of int32 sums (ds_add_u64)
+    const bool packed = acc == 2 && a.tiles >= 2;
     PIC_SHAPE_SWITCH(kernel, rc = acc == 0   ? seg32_perm<KK, 0>(c, a, mode, c->bz_mode, sm)
                                   : acc == 1 ? seg32_perm<KK, 1>(c, a, mode, c->bz_mode, sm)
-                                             : seg32_perm<KK, 2>(c, a, mode, c->bz_mode, sm));
+                                             : seg32_fixed<KK>(c, a, mode, c->bz_mode, sm,
+                                                               packed));
     if (rc == PIC_OK && a.perm) {
         swap_alt_f32(c);
         c->pending_perm = nullptr;

@@ -471,13 +471,14 @@ def fixed_point_bound(cell, nx, ny, kernel, w, n):
     product of two weights; the block of all n particles gives an upper
     bound U on every block's u.  Each particle adds R = round(w/u) units,
     worth w/R each (a scale off u by eps <= u/(2w)): every node but one of
-    its stencil takes its share rounded to the nearest unit, and the node of
-    its own cell takes the remainder.  So each share is off by at most
+    its stencil takes its share rounded to the nearest unit, and stencil node
+    (HL | 1, HL) takes the remainder (the cell's own node for TSC and M4, the
+    next one along x for CIC, NGP and M6).  So each share is off by at most
     a = U/2 (1 + eps) + eps qmax w + 2^-22 qmax w (the last term for the
     FP32 roundings of the products), and each remainder by the S^2 - 1
     others' errors plus eps w plus 4e-6 w (the FP32 weights' sum misses 1 by
     ~1e-6).  Returns a per node times the shares it receives plus that for
-    the particles whose own cell is the node."""
+    the particles whose remainder node it is."""
     s, hl, qmax = SHAPE_STENCIL[kernel]
     aw = qmax * abs(float(w))
     e = max(math.ceil(math.log2(aw * (n + 5))) - 30, math.ceil(math.log2(aw)) - 22)
@@ -491,7 +492,8 @@ def fixed_point_bound(cell, nx, ny, kernel, w, n):
             contrib += np.roll(np.roll(counts, du - hl, axis=0), dv - hl, axis=1)
     a = U / 2 * (1 + eps) + eps * aw + 2.0 ** -22 * aw
     b = (s * s - 1) * a + (eps + 4e-6) * abs(float(w))
-    return contrib * a + counts * b
+    rem = np.roll(counts, (hl | 1) - hl, axis=0)  # remainder node of each cell's particles
+    return contrib * a + rem * b
 
 
 @pytest.mark.parametrize("acc", [0, 1, 2])
