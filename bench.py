@@ -17,7 +17,9 @@ current loop of examples/vm2d.jl (FP64, 128 x 16 cells, 204,800 or
 1,024,000 particles, dt = 0.01, 1000 steps) the same way.
 
 Other workloads, measured the same way: landau_cic (CIC, 64 x 64, 204,800
-particles, alpha 0.05, dt 0.1, 100 steps), strong_b (CIC, 256 x 256, 1e8
+particles, alpha 0.05, dt 0.1, 100 steps; on a grid this small pic_run runs
+one launch per step, and the line also gives the loop as separate solve,
+pass and sort kernels), strong_b (CIC, 256 x 256, 1e8
 particles, density 1 + 0.05 cos(x/2) cos(y/2), Maxwellian v, uniform bz =
 1/eps = 20, dt = eps/4, 1000 steps) and large_m6 (M6, 512 x 512, 5e8
 particles, alpha 0.05, 200 steps); the last two are sampled on the GPU by
@@ -170,7 +172,7 @@ def _reload(sim, p0, state):
 
 
 def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_acc=2, bz=None,
-             tiles=4, acc_steps=0, window=160):
+             tiles=4, acc_steps=0, window=160, fused_step=True):
     """Warmup, then the timed run from the initial state, then (acc_steps >
     0) an untimed accuracy run of acc_steps from the same initial state,
     whose energy history the damping rate is fitted on. Returns dict."""
@@ -178,6 +180,7 @@ def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_ac
     sim.set_f32_accumulation(f32_acc)
     sim.set_f32_tiles(tiles)
     sim.set_f32_window(window)
+    sim.set_fused_step(fused_step)
     sim.sort(lanes=True)
     if warmup:
         sim.run(dt, warmup, sort_every=sort_every)
@@ -687,6 +690,9 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run plain device copies of MIB MiB (byte-counter calibration)")
     ap.add_argument("--dump", default="", help="write the energy histories to this .npz")
+    ap.add_argument("--no-fused-step", action="store_true",
+                    help="grids up to 64 x 64: time the separate solve, pass and sort kernels "
+                         "instead of the one-launch iteration")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the untimed full-length run the damping rate is fitted on")
     args = ap.parse_args()
@@ -759,7 +765,7 @@ def main():
         sort_every = args.sort_every_f32 if name == "f32" else args.sort_every_f64
         res[name] = run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every,
                              args.f32_acc, bz, W.get("tiles", 4), acc_steps,
-                             W.get("window", 160))
+                             W.get("window", 160), fused_step=not args.no_fused_step)
         r = res[name]
         log(f"bench: {name} {r['seconds']*1e3:.2f} ms total, pass {r['pass_ms_avg']*1e3:.1f} us,"
             f" solve {r['solve_ms_avg']*1e3:.1f} us, deferred {r['slow_chunks']}")
@@ -769,17 +775,28 @@ def main():
                  **{k: v["energy"] for k, v in res.items()},
                  **{k + "_acc": v["energy_acc"] for k, v in res.items() if "energy_acc" in v})
     gam = {}
+    gam_skipped = None
     if acc_steps:
         t_acc = np.arange(1, acc_steps + 1) * dt
         for name, r in res.items():
             ea = r["energy_acc"]
             g, w, nm, sig = diagnostics.landau_damping_rate(t_acc, ea[:, 0], ea[:, 1])
-            gam[name] = (g, w, nm, sig)
             log(f"bench: {name} gamma {g:.5f} omega {w:.4f} from {nm} maxima "
                 f"({acc_steps}-step accuracy run, noise {sig:.3e})")
+            if math.isfinite(g):
+                gam[name] = (g, w, nm, sig)
+            else:
+                gam_skipped = (f"fewer than {diagnostics.MIN_MAXIMA} maxima of the mode above "
+                               f"2x the particle noise ({nm} found): no damping-rate fit")
 
     head = "f32" if "f32" in res else "f64"
     r = res[head]
+    # pic_run's one-launch iteration (power-of-two grids up to 64 x 64, bz none
+    # or uniform, FP64 or fixed-point FP32 charge): one kernel per step, whose
+    # launch time is the pass time (the E solve inside it)
+    pow2 = lambda v: v & (v - 1) == 0
+    one_launch = (not args.no_fused_step and nx <= 64 and ny <= 64 and pow2(nx) and pow2(ny)
+                  and (bz is None or np.ndim(bz) == 0) and (head == "f64" or args.f32_acc == 2))
     value = n * steps / r["seconds"]
     bytes_pp = BYTES_F32 if head == "f32" else BYTES_F64
     achieved = bytes_pp * n / (r["pass_ms_avg"] * 1e-3) / 1e9
@@ -811,7 +828,9 @@ def main():
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_seg_{head}<{kname}> gather+kick+drift+deposit pass" + tiles,
+                     "kernel": (f"k_es_step<{kname}> one launch per step: E solve in LDS + "
+                                "gather+kick+drift+deposit" if one_launch else
+                                f"k_seg_{head}<{kname}> gather+kick+drift+deposit pass") + tiles,
                      "bytes_per_particle": bytes_pp, "avg_us": r["pass_ms_avg"] * 1e3},
     }
     if "f32" in res and "f64" in res:
@@ -822,7 +841,9 @@ def main():
                        "sort_every": args.sort_every_f64,
                        "deferred_particles": res["f64"]["slow_chunks"]}
         out["speedup_fp32_vs_fp64"] = value / v64
-        if gam:
+        if gam_skipped:
+            out["gamma"] = {"fit": None, "reason": gam_skipped}
+        if "f32" in gam and "f64" in gam:
             g32, g64 = gam["f32"][0], gam["f64"][0]
             rel = abs(g32 - g64) / abs(g64)
             out["gamma"] = {"fp32": g32, "fp64": g64, "rel_err": rel, "analytic": -0.153359,
@@ -844,6 +865,22 @@ def main():
                                / np.sum(g64["ex"] ** 2 + g64["ey"] ** 2)))}
     out["solve_us"] = r["solve_ms_avg"] * 1e3
     out["deferred_particles"] = r["slow_chunks"]
+    if one_launch:
+        # the same loop as separate kernels (solve, segment pass, lane sorts)
+        sep = {}
+        for name, state in (("f32", 1), ("f64", 0)):
+            if name not in res:
+                continue
+            sort_every = args.sort_every_f32 if name == "f32" else args.sort_every_f64
+            log(f"bench: {name} path as separate kernels, timed {steps} steps")
+            rr = run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every,
+                          args.f32_acc, bz, W.get("tiles", 4), 0, W.get("window", 160),
+                          fused_step=False)
+            sep[name] = {"value": n * steps / rr["seconds"],
+                         "ms_per_step": rr["seconds"] * 1e3 / steps,
+                         "pass_us": rr["pass_ms_avg"] * 1e3, "solve_us": rr["solve_ms_avg"] * 1e3,
+                         "sort_every": sort_every}
+        out["separate_kernels"] = sep
     # the halves of the fused pass timed on their own (HIP events), against
     # their own algorithmic bytes: the deposit reads the state (FP32 20 B,
     # FP64 x, y 16 B); gather + kick + drift reads and writes it
@@ -864,7 +901,7 @@ def main():
             for k, ms in rr["split_ms"].items()}
     # the pass against its VALU-issue and LDS roofs (instruction counts of the
     # committed SQ profile over the live pass time)
-    ic = issue_counts(f"k_seg_{head}<{kname}>")
+    ic = None if one_launch else issue_counts(f"k_seg_{head}<{kname}>")
     if ic is not None:
         valu_us = n * ic[0] / VALU_WAVE_INSTS_PER_S * 1e6
         lds_us = n * ic[1] / LDS_BYTES_PER_S * 1e6

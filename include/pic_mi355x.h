@@ -243,8 +243,25 @@ int pic_cell_offsets(pic_ctx *ctx, int64_t *offsets);
  * Works in either state.  energy (host, may be NULL) receives per step
  * out[2*s] = sum(ex^2)*dx*dy and out[2*s+1] = sum(ey^2)*dx*dy.
  * sort_every > 0 re-sorts the particles by cell every that many steps
- * (0 = never).  Leaves RHO, EX, EY of the last step in the context. */
+ * (0 = never).  Leaves RHO, EX, EY of the last step in the context.
+ * On power-of-two grids of at most 64 x 64 nodes (bz none or uniform, FP64
+ * state or FP32 with fixed-point charge) the loop runs one launch per
+ * iteration instead (pic_set_fused_step): every block solves the field of
+ * the previous deposit in LDS, then moves its particles; no sort (sort_every
+ * is ignored, the particles keep their order). */
 int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort_every, double *energy);
+
+/* pic_run on power-of-two grids of at most 64 x 64 nodes: fused = 1 (the
+ * default) one launch per iteration (the E solve redundantly in every
+ * block's LDS, then gather, push and deposit of the block's particles),
+ * fused = 0 the separate solve, particle-pass and sort kernels.  Measured
+ * on MI355X, Landau CIC 64 x 64 with 204,800 particles, 100 steps: one
+ * launch 21.3 us (FP32 state) and 23.8 us (FP64) per step, 26.1 and 28.9 us
+ * per step of wall time; the separate kernels 66.5 and 74.5 us.  The FP32 state's
+ * one-launch charge is int32 fixed point with one quantum for every block,
+ * summed as int64 (exact and run-to-run identical).  No reference
+ * counterpart. */
+int pic_set_fused_step(pic_ctx *ctx, int fused);
 
 /* Charge accumulation of pic_run's FP32 passes in their LDS windows:
  * 0 compensated FP32 (hi, lo) pairs;
@@ -326,8 +343,12 @@ int pic_get_timing(pic_ctx *ctx, double *pass_ms, int64_t *passes, double *solve
  * pic_run's sort_every, which keeps the segment passes of both sweeps on
  * their LDS windows; the FP64 state sorts at checkpoints only.  gx, gv
  * (host, 2 x n, may be NULL) receive dJ/d(x0, y0) and dJ/d(vx0, vy0) in the
- * context's particle order; the particles are left as they were.  No
- * reference counterpart (the Julia rrule of the drop-in wraps this call). */
+ * context's particle order; the particles are left as they were.  The
+ * sweep's forward pass is the separate-kernel form of pic_run's loop
+ * (pic_set_fused_step(ctx, 0)); in the FP32 state the one-launch form of
+ * small grids rounds its fixed-point charge with another quantum, so its
+ * energies differ from J at the 1e-7 level.  No reference counterpart (the
+ * Julia rrule of the drop-in wraps this call). */
 int pic_adjoint(pic_ctx *ctx, int kernel, double dt, int nsteps, int ckpt, int sort_every,
                 double *J, double *gx, double *gv);
 

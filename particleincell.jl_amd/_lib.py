@@ -70,6 +70,7 @@ SIGNATURES = {
     "pic_set_f32_tiles": [_vp, _i],
     "pic_set_f32_window": [_vp, _i],
     "pic_set_fused_solve": [_vp, _i],
+    "pic_set_fused_step": [_vp, _i],
     "pic_get_timing": [_vp, _pd, C.POINTER(C.c_int64), _pd],
     "pic_slow_chunks": [_vp, C.POINTER(C.c_uint64)],
     "pic_fdtd_set": [_vp, _vp, _vp, _vp],

@@ -878,6 +878,12 @@ class Simulation:
         as rocFFT transforms (pic_set_fused_solve)."""
         L.call("pic_set_fused_solve", self.ctx.h, 1 if on else 0)
 
+    def set_fused_step(self, on=True):
+        """pic_run on power-of-two grids of at most 64 x 64 nodes as one
+        launch per iteration (True, the default) or as the separate solve,
+        pass and sort kernels (pic_set_fused_step)."""
+        L.call("pic_set_fused_step", self.ctx.h, 1 if on else 0)
+
     def set_f32_accumulation(self, mode):
         """Charge accumulation of the FP32 state: 0 = compensated FP32
         (hi, lo) tile pairs, 1 = FP64 tile sums, 2 = int32 fixed-point tile

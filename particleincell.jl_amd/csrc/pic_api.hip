@@ -403,7 +403,7 @@ int pic_ctx_destroy(pic_ctx *ctx) {
     poisson_free(c);
     free_particles(c);
     dfree(c->rho); dfree(c->ex); c->ey = nullptr; dfree(c->bz); dfree(c->phi);
-    dfree(c->rho_acc); dfree(c->e32); dfree(c->bz32);
+    dfree(c->rho_acc); dfree(c->e32); dfree(c->bz32); dfree(c->es_buf);
     dfree(c->red_part); dfree(c->red_count); dfree(c->energy_hist); dfree(c->step_counter);
     dfree(c->sort_hist); dfree(c->cell_off);
     dfree(c->slow_chunks); dfree(c->defer_count);

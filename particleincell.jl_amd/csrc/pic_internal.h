@@ -136,6 +136,11 @@ struct Ctx {
     // E solve on power-of-two grids up to 512 x 512: three LDS-transform
     // kernels (true) or the rocFFT sequence (false)
     bool fused_solve = true;
+    // pic_run on power-of-two grids up to 64 x 64: one launch per iteration
+    // (true, pic_set_fused_step) or the separate solve, pass and sort
+    // kernels; es_buf holds the launches' three raw charge buffers
+    bool fused_step = true;
+    double *es_buf = nullptr;
 
     // Vlasov-Maxwell: FDTD fields on the Yee grid (fdtd.jl:3-20; ex nx x
     // (ny+1), ey (nx+1) x ny, bz nx x ny, column-major) and the periodic CIC
@@ -210,6 +215,10 @@ int alt_f64(Ctx *c, double **x, double **y, double **vx, double **vy, double **w
 void swap_alt_f64(Ctx *c);
 
 int launch_fill(Ctx *c, double *p, double v, int64_t n);
+
+// pic_run's loop with one launch per iteration on small grids (pic_small.hip).
+bool es_step_ok(const Ctx *c);
+int es_run(Ctx *c, int kernel, double dt, int nsteps);
 
 // The adjoint sweep of the FP32 cell-relative state (pic_adjoint32.hip).
 int adjoint_f32(Ctx *c, int kernel, double dt, int nsteps, int ckpt, int sort_every, double *J,

@@ -44,21 +44,10 @@ static int step_pass(Ctx *c, int kernel, int mode, double dt) {
     return launch_step32(c, kernel, mode, dt, mode == 1 ? dt : 0.5 * dt);
 }
 
-}  // namespace pic
-
-using namespace pic;
-
-extern "C" int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort_every,
-                       double *energy) {
-    PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
-    PIC_CHECK(valid_shape(kernel), PIC_ERR_ARG, "kernel %d", kernel);
-    PIC_CHECK(nsteps >= 0, PIC_ERR_ARG, "nsteps %d", nsteps);
-    PIC_CHECK(sort_every >= 0, PIC_ERR_ARG, "sort_every %d", sort_every);
-    Ctx *c = ctx_of(ctx);
-    PIC_HIP(hipSetDevice(c->device));
-    if (nsteps == 0) return PIC_OK;
-    PIC_CHECK(c->state == PIC_STATE_F64 || c->uniform_w, PIC_ERR_STATE, "FP32 state needs uniform w");
-    PIC_TRY(ensure_energy(c, nsteps));
+// pic_run's loop as separate kernels: the E solve, then a segment pass over
+// the lane-sorted particles, a lane sort every sort_every steps.
+static int run_passes(Ctx *c, int kernel, double dt, int nsteps, int sort_every,
+                      std::vector<hipEvent_t> &ev) {
     PIC_HIP(hipMemsetAsync(c->step_counter, 0, sizeof(int), c->stream));
     const int64_t m = (int64_t)c->nx * c->ny;
     const double scale = 1.0 / ((double)c->nx * (double)c->ny * c->dx * c->dy);
@@ -67,8 +56,6 @@ extern "C" int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort
     PIC_TRY(sort_by_cell(c, nullptr, true, false));
     PIC_TRY(launch_fill(c, c->rho_acc, 0.0, m));
     PIC_TRY(seg_pass(c, kernel, 0, dt));
-    // timing: events around every solve and every particle pass
-    std::vector<hipEvent_t> ev;
     auto mark = [&]() -> int {
         hipEvent_t e;
         PIC_HIP(hipEventCreate(&e));
@@ -95,6 +82,33 @@ extern "C" int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort
     if (c->pending_perm) {
         const int rs = settle_state(c);
         if (rc == PIC_OK) rc = rs;
+    }
+    return rc;
+}
+
+}  // namespace pic
+
+using namespace pic;
+
+extern "C" int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort_every,
+                       double *energy) {
+    PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
+    PIC_CHECK(valid_shape(kernel), PIC_ERR_ARG, "kernel %d", kernel);
+    PIC_CHECK(nsteps >= 0, PIC_ERR_ARG, "nsteps %d", nsteps);
+    PIC_CHECK(sort_every >= 0, PIC_ERR_ARG, "sort_every %d", sort_every);
+    Ctx *c = ctx_of(ctx);
+    PIC_HIP(hipSetDevice(c->device));
+    if (nsteps == 0) return PIC_OK;
+    PIC_CHECK(c->state == PIC_STATE_F64 || c->uniform_w, PIC_ERR_STATE, "FP32 state needs uniform w");
+    PIC_TRY(ensure_energy(c, nsteps));
+    // timing: events around every solve and every particle pass
+    std::vector<hipEvent_t> ev;
+    int rc = PIC_OK;
+    if (es_step_ok(c)) {
+        // small grids: one launch per iteration (pic_small.hip)
+        rc = es_run(c, kernel, dt, nsteps);
+    } else {
+        rc = run_passes(c, kernel, dt, nsteps, sort_every, ev);
     }
     if (rc == PIC_OK && energy)
         rc = hipMemcpyAsync(energy, c->energy_hist, (size_t)nsteps * 2 * sizeof(double),

@@ -54,6 +54,7 @@
 
 #include "pic_internal.h"
 #include "pic_push32.h"
+#include "pic_push64.h"
 #include "pic_segcore.h"
 
 namespace pic {
@@ -96,111 +97,10 @@ __device__ __forceinline__ float split_constant(float w, int64_t count) {
     return (float)ldexp(1.5, e + 23);
 }
 
-// Fixed-point tiles: exponent e of the quantum u = 2^e with which a block of
-// `count` particles accumulates its charge as int32 multiples of u.  Every
-// contribution |q| <= qmax |w| (qmax = the largest product of two shape
-// weights: 1 for CIC, (66/120)^2 < 0.31 for M6) is rounded to the nearest
-// multiple of u (error <= u/2); u is large enough that |q| < 2^22 u (so the
-// rounding below is exact integer arithmetic) and that a node's sum stays
-// below 2^31 u (sum |q|/u <= 2^30, plus at most count/2 from the roundings).
-__device__ __forceinline__ int fixed_exponent(float w, double qmax, int64_t count) {
-    const double aw = qmax * fabs((double)w) + 1e-300;
-    int e1, e2;
-    const double m1 = frexp(aw * (double)(count + 5), &e1);
-    const double m2 = frexp(aw, &e2);
-    e1 = (m1 == 0.5 ? e1 - 1 : e1) - 30;  // ceil(log2(.)) - 30
-    e2 = (m2 == 0.5 ? e2 - 1 : e2) - 22;
-    return e1 > e2 ? e1 : e2;
-}
-
-// Largest block for which the fixed-point tiles conserve charge particle by
-// particle (charge32_fixed): the node that takes a particle's remainder
-// receives at most its own share plus the S^2 / 2 roundings of the others
-// and the 1e-6 by which the FP32 weights may miss a sum of 1 (at most ~50
-// units with w/u < 2^24), so a node's sum stays below 2^30 u + 50 count,
-// within int32 for count <= 2^24.  Larger blocks round every share.
-constexpr int64_t CONSERVE_MAX_COUNT = (int64_t)1 << 24;
-
 __device__ __forceinline__ int64_t tile_sum(const int *iacc, int t, int cap, int tiles) {
     int64_t q = 0;
     for (int k = 0; k < tiles; ++k) q += iacc[t + k * cap];
     return q;
-}
-
-// Calls add(u, v, q) with the FP32 charge q of stencil node (u, v) of a
-// particle at offset (fx, fy), in a fixed order.
-template <int K, class F>
-__device__ __forceinline__ void charge32(float fx, float fy, int ci, int cj, float w, F &&add) {
-    constexpr int S = Shape<K>::S;
-    if (K == PIC_CIC) {
-        float sx = 1.0f - fx, sy = 1.0f - fy;
-        add(0, 0, (sx * sy) * w);
-        add(1, 0, (fx * sy) * w);
-        add(1, 1, (fx * fy) * w);
-        add(0, 1, (sx * fy) * w);
-    } else {
-        float wx[6], wy[6];
-        deposit_w<K>(fx, ci, wx);
-        deposit_w<K>(fy, cj, wy);
-#pragma unroll
-        for (int v = 0; v < S; ++v) wy[v] = wy[v] * w;
-#pragma unroll
-        for (int u = 0; u < S; ++u)
-#pragma unroll
-            for (int v = 0; v < S; ++v) add(u, v, wx[u] * wy[v]);
-    }
-}
-
-// The int32 fixed-point charges of a particle's stencil nodes, in units of
-// the quantum u = 2^qe: add(u, v, n) with n = the exact product of the two
-// shape factors times w/u, rounded to the nearest integer by one fused
-// multiply-add onto 1.5 * 2^23 (the float grid there is the integers; the
-// products stay below 2^22 by the choice of u).
-//
-// With `conserve` the node (HL | 1, HL) (the cell's own node for TSC and M4,
-// the next one for CIC, NGP and M6: always an odd column of the stencil, so
-// that in the packed tiles below it is the high half of its pair) takes R
-// minus the sum of the other nodes' rounded shares, R = round(|w|/u) the same
-// for every particle of the block, so that every particle adds exactly R
-// units and the flush's scale w/R gives the block exactly its count times w.  Rounding every share to nearest instead drops
-// the many shares below u/2 at a stencil's far nodes (M6: products of two
-// weights of ~1e-3): measured on 5e8 particles on 512^2, ~2e4 units per
-// particle, the deposit lost 2.2e-5 of the charge.  The remainder node
-// absorbs those roundings (at most S^2/2 units) and the ~1e-6 by which the
-// FP32 weights miss a sum of 1.
-template <int K, class F>
-__device__ __forceinline__ void charge32_fixed(float fx, float fy, int ci, int cj, float wu, int R,
-                                               bool conserve, F &&add) {
-    constexpr int S = Shape<K>::S, HL = Shape<K>::HL, UR = HL | 1, VR = HL;
-    constexpr float C = 12582912.0f;  // 1.5 * 2^23
-    auto n = [&](float a, float b) {
-        return __float_as_int(__builtin_fmaf(a, b, C)) - __float_as_int(C);
-    };
-    if (K == PIC_CIC) {
-        const float sx = 1.0f - fx, sy = (1.0f - fy) * wu, ty = fy * wu;
-        const int n00 = n(sx, sy), n11 = n(fx, ty), n01 = n(sx, ty);
-        add(0, 0, n00);
-        add(1, 1, n11);
-        add(0, 1, n01);
-        add(1, 0, conserve ? R - (n00 + n11 + n01) : n(fx, sy));
-    } else {
-        float wx[6], wy[6];
-        deposit_w<K>(fx, ci, wx);
-        deposit_w<K>(fy, cj, wy);
-#pragma unroll
-        for (int v = 0; v < S; ++v) wy[v] = wy[v] * wu;
-        int s = 0;
-#pragma unroll
-        for (int u = 0; u < S; ++u)
-#pragma unroll
-            for (int v = 0; v < S; ++v) {
-                if (u == UR && v == VR) continue;
-                const int q = n(wx[u], wy[v]);
-                s += q;
-                add(u, v, q);
-            }
-        add(UR, VR, conserve ? R - s : n(wx[UR], wy[VR]));
-    }
 }
 
 // The same shares two nodes at a time: add(u, v, q) for u even with the
@@ -580,150 +480,6 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f32(SegF32 a) {
 }
 
 // ============================================================ FP64 state
-
-struct SegF64 {
-    double *x, *y, *vx, *vy;  // state written by the pass (slot p)
-    // state read by the pass (see SegF32): the same arrays or the unsorted
-    // ones read at perm[p]; weights w (null when uniform) are read at the
-    // same place and, with a permutation, copied to w_out[p]
-    const double *x_in, *y_in, *vx_in, *vy_in;
-    const uint32_t *perm;
-    const double *w;
-    double *w_out;
-    double w0;
-    const double *ex, *ey, *bz;
-    double bz0;
-    int nx, ny, ncells;
-    double dx, dy, dimx, dimy, dt, dtd;
-    double *acc;
-    const int64_t *off;
-    int bps, margin, cap;
-    unsigned long long *slow;
-    uint32_t *dlist;  // deferred particles and their count
-    unsigned *dcount;
-};
-
-// The reference's cell and in-cell offset (pushers_cic.jl:18-29): i =
-// floor(xp)+1 (trunc in compute_rho CIC, pushers_cic.jl:86-87, equal for
-// x >= 0), d = (xp - i) + 1; returns the 0-based wrapped cell.
-__device__ __forceinline__ int ref_cell_s(double x, double dx, int n, bool trunc_, double &d) {
-    const double xp = x / dx;
-    return picd::cell_of(xp, trunc_ ? trunc(xp) : floor(xp), n, d);
-}
-
-// Field at a particle from the nodes (u, v) of its stencil, FP64, with the
-// reference's operation order (pushers_cic.jl:31-44, pushers_m6.jl:63-83).
-template <int K, int BZ, class FE, class FB>
-__device__ __forceinline__ void interp64(double fx, double fy, int ci, int cj, FE node_e,
-                                         FB node_b, double &e1, double &e2, double &b3) {
-    constexpr int S = Shape<K>::S;
-    if (K == PIC_CIC) {
-        double a1 = (1 - fx) * (1 - fy);
-        double a2 = fx * (1 - fy);
-        double a3 = fx * fy;
-        double a4 = (1 - fx) * fy;
-        double2 E00 = node_e(0, 0), E10 = node_e(1, 0), E11 = node_e(1, 1),
-                E01 = node_e(0, 1);
-        e1 = a1 * E00.x + a2 * E10.x + a3 * E11.x + a4 * E01.x;
-        e2 = a1 * E00.y + a2 * E10.y + a3 * E11.y + a4 * E01.y;
-        if (BZ == PIC_BZ_GRID)
-            b3 = a1 * node_b(0, 0) + a2 * node_b(1, 0) + a3 * node_b(1, 1) +
-                 a4 * node_b(0, 1);
-    } else {
-        double wx[6], wy[6];
-        gather_w<K>(fx, ci, wx);
-        gather_w<K>(fy, cj, wy);
-        e1 = 0.0;
-        e2 = 0.0;
-#pragma unroll
-        for (int u = 0; u < S; ++u) {
-            double2 E0 = node_e(u, 0);
-            double t1 = wy[0] * E0.x, t2 = wy[0] * E0.y;
-#pragma unroll
-            for (int v = 1; v < S; ++v) {
-                double2 E = node_e(u, v);
-                t1 = __builtin_fma(wy[v], E.x, t1);
-                t2 = __builtin_fma(wy[v], E.y, t2);
-            }
-            e1 = __builtin_fma(wx[u], t1, e1);
-            e2 = __builtin_fma(wx[u], t2, e2);
-        }
-        if (BZ == PIC_BZ_GRID) {
-#pragma unroll
-            for (int u = 0; u < S; ++u) {
-                double t3 = wy[0] * node_b(u, 0);
-#pragma unroll
-                for (int v = 1; v < S; ++v) t3 = __builtin_fma(wy[v], node_b(u, v), t3);
-                b3 = __builtin_fma(wx[u], t3, b3);
-            }
-        }
-    }
-}
-
-// Half kick, rotation by bz, half kick with the reference's operations
-// (pushers_cic.jl:38-49).
-template <int BZ>
-__device__ __forceinline__ void kick64(const SegF64 &a, double e1, double e2, double b3,
-                                       double &pvx, double &pvy) {
-    if (BZ == PIC_BZ_UNIFORM) b3 = a.bz0;
-    const double dt = a.dt;
-    double v1 = pvx, v2 = pvy;
-    v1 += 0.5 * dt * e1;
-    v2 += 0.5 * dt * e2;
-    if (BZ != PIC_BZ_NONE) {
-        double tantheta = 0.5 * dt * b3;
-        double sintheta = 2 * tantheta / (1 + tantheta * tantheta);
-        v1 += v2 * tantheta;
-        v2 += -v1 * sintheta;
-        v1 += v2 * tantheta;
-    }
-    pvx = v1 + 0.5 * dt * e1;
-    pvy = v2 + 0.5 * dt * e2;
-}
-
-// NDRIFT drifts by dtd with the single conditional wrap of push_x!
-// (pushers_cic.jl:55-72).
-template <int NDRIFT>
-__device__ __forceinline__ void drift64(const SegF64 &a, double pvx, double pvy, double &px,
-                                        double &py) {
-#pragma unroll
-    for (int r = 0; r < NDRIFT; ++r) {
-        double p1 = px + a.dtd * pvx;
-        double p2 = py + a.dtd * pvy;
-        if (p1 > a.dimx) p1 -= a.dimx;
-        if (p2 > a.dimy) p2 -= a.dimy;
-        if (p1 < 0.0) p1 += a.dimx;
-        if (p2 < 0.0) p2 += a.dimy;
-        px = p1;
-        py = p2;
-    }
-}
-
-// Calls add(u, v, q) with the charge of stencil node (u, v)
-// (pushers_cic.jl:91-94, pushers_m6.jl:20-27).
-template <int K, class F>
-__device__ __forceinline__ void charge64(double fx, double fy, int ci, int cj, double wgt,
-                                         F &&add) {
-    constexpr int S = Shape<K>::S;
-    if (K == PIC_CIC) {
-        double a1 = (1 - fx) * (1 - fy);
-        double a2 = fx * (1 - fy);
-        double a3 = fx * fy;
-        double a4 = (1 - fx) * fy;
-        add(0, 0, a1 * wgt);
-        add(1, 0, a2 * wgt);
-        add(1, 1, a3 * wgt);
-        add(0, 1, a4 * wgt);
-    } else {
-        double wx[6], wy[6];
-        deposit_w<K>(fx, ci, wx);
-        deposit_w<K>(fy, cj, wy);
-#pragma unroll
-        for (int u = 0; u < S; ++u)
-#pragma unroll
-            for (int v = 0; v < S; ++v) add(u, v, wx[u] * wy[v] * wgt);
-    }
-}
 
 // FP64 counterpart of k_seg_f32; out-of-window particles go to k_defer_f64.
 template <int K, bool GATHER, int NDRIFT, bool DEPOSIT, int BZ, bool PERM>

@@ -109,8 +109,12 @@ def test_adjoint_f32_against_f64_and_fd(gpu, kernel):
     # the FP32 state is left bit for bit as it was
     for a, b in zip(before, after):
         assert np.array_equal(a, b)
-    # J is the FP32 loop's own field energy (same sorts, same charge sums)
+    # J is the FP32 loop's own field energy (same sorts, same charge sums):
+    # the separate-kernel loop, whose segment passes the sweep's forward
+    # pass runs (on this grid pic_run's default is the one-launch loop, whose
+    # fixed-point charge rounds with another quantum)
     sim = pic.Simulation(mesh, p, k, state=F32)
+    sim.set_fused_step(False)
     e = sim.run(dt, steps, ck)
     sim.close()
     assert J32 == pytest.approx(e[-1, 0] + e[-1, 1], rel=1e-12)

@@ -47,25 +47,31 @@ def test_config1_landau_cic_full(gpu, O):
     """Config 1 at its full size against the oracle loop, FP64: the energy
     of all 100 steps within 1e-9 relative (atomics reorder the deposit sums
     and the FFTs round differently, and 100 steps let those roundings
-    grow); the FP32 state's energies within 1e-4 of the FP64 ones."""
+    grow); the FP32 state's energies within 1e-4 of the FP64 ones.  Both
+    forms of the loop on this grid: one launch per iteration (the default
+    up to 64 x 64) and the separate solve, pass and sort kernels."""
     pic = gpu
     n, nx, dt, steps = 204_800, 64, 0.1, 100
     mesh = _mesh(pic, nx)
     p = pic.sampling.landau_damping(n, mesh, 0.05, 0.5)
-    print(f"config 1: CIC {nx}x{nx}, {n} particles, {steps} steps", flush=True)
-    sim = pic.Simulation(mesh, p, pic.CloudInCell())
-    e64 = sim.run(dt, steps, sort_every=4)
-    sim.close()
     eo = O.landau_loop(p.copy(order="F"), "cic", nx, nx, mesh.dimx, mesh.dimy, dt, steps)
-    rel = np.max(np.abs(np.sqrt(e64[:, 0]) - eo) / eo)
-    print(f"config 1: FP64 energy vs oracle {rel:.3e}", flush=True)
-    assert rel < 1e-9
-    sim = pic.Simulation(mesh, p, pic.CloudInCell(), state=pic._lib.PIC_STATE_F32)
-    e32 = sim.run(dt, steps, sort_every=8)
-    sim.close()
-    rel32 = np.max(np.abs(np.sqrt(e32[:, 0]) - eo) / eo)
-    print(f"config 1: FP32 energy vs oracle {rel32:.3e}", flush=True)
-    assert rel32 < 1e-4
+    for fused in (True, False):
+        print(f"config 1: CIC {nx}x{nx}, {n} particles, {steps} steps, "
+              f"{'one launch per step' if fused else 'separate kernels'}", flush=True)
+        sim = pic.Simulation(mesh, p, pic.CloudInCell())
+        sim.set_fused_step(fused)
+        e64 = sim.run(dt, steps, sort_every=4)
+        sim.close()
+        rel = np.max(np.abs(np.sqrt(e64[:, 0]) - eo) / eo)
+        print(f"config 1: FP64 energy vs oracle {rel:.3e}", flush=True)
+        assert rel < 1e-9
+        sim = pic.Simulation(mesh, p, pic.CloudInCell(), state=pic._lib.PIC_STATE_F32)
+        sim.set_fused_step(fused)
+        e32 = sim.run(dt, steps, sort_every=8)
+        sim.close()
+        rel32 = np.max(np.abs(np.sqrt(e32[:, 0]) - eo) / eo)
+        print(f"config 1: FP32 energy vs oracle {rel32:.3e}", flush=True)
+        assert rel32 < 1e-4
 
 
 def test_config2_landau_m6_damping_rate(gpu):

@@ -735,23 +735,25 @@ def test_f64_fused_pass(gpu, O, kernel):
 
 # ------------------------------------------------------------ time loop
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("kernel", ["cic", "m6"])
-def test_run_matches_reference_loop(gpu, O, kernel):
+def test_run_matches_reference_loop(gpu, O, kernel, fused):
     """pic_run against the oracle's restatement of the reference loop
-    (examples/landau_damping_poisson_cic_m6.jl:70-83), FP64, 10 steps: the
-    energies agree to 1e-10 (atomics reorder the deposit sums, rocFFT and
-    the oracle's FFT round differently, so trajectories are equal to
-    rounding, not bit for bit)."""
+    (examples/landau_damping_poisson_cic_m6.jl:70-83), FP64, 10 steps, in
+    its one-launch form and as separate kernels: the energies agree to 1e-10
+    (atomics reorder the deposit sums, the transforms round differently, so
+    trajectories are equal to rounding, not bit for bit)."""
     pic = gpu
     mesh, p = landau_particles(pic, 100000, 64, 64)
     sim = pic.Simulation(mesh, p, pic.CloudInCell() if kernel == "cic" else pic.M6())
+    sim.set_fused_step(fused)
     e = sim.run(0.1, 10)
     got = sim.particles()
     sim.close()
     q = p.copy(order="F")
     eo = O.landau_loop(q, kernel, 64, 64, mesh.dimx, mesh.dimy, 0.1, 10)
     assert np.max(np.abs(np.sqrt(e[:, 0]) - eo) / eo) < 1e-10
-    # pic_run sorts the particles by cell, so match them as a set: every
+    # the separate-kernel loop sorts the particles by cell, so match them as a set: every
     # GPU particle's nearest oracle particle in (x, y, vx, vy) is within
     # 1e-10 and the matching is one to one
     from scipy.spatial import cKDTree
