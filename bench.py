@@ -435,7 +435,7 @@ def bench_adjoint(args, W, pic, entry, mesh, held, n, steps, warmup):
             abs(res["f64"]["dJ_dalpha"])
         v64 = n * steps / res["f64"]["seconds"]
         out["fp64"] = {"value": v64, "ms_per_step": res["f64"]["seconds"] * 1e3 / steps,
-                       "sort_every": ckpt}
+                       "sort_every": se}
         out["speedup_fp32_vs_fp64"] = value / v64
     out["gradient"] = grad
 

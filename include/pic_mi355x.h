@@ -338,10 +338,10 @@ int pic_get_timing(pic_ctx *ctx, double *pass_ms, int64_t *passes, double *solve
  * positions, as pic_run) while the charge sums, the Poisson solve and its
  * adjoint stay FP64.  The forward pass keeps the solved field of every step
  * and the particle state every `ckpt` steps; the reverse sweep recomputes
- * each segment from its checkpoint.  In the FP32 state the particles are
- * also lane-sorted every `sort_every` steps (0: at checkpoints only), as
- * pic_run's sort_every, which keeps the segment passes of both sweeps on
- * their LDS windows; the FP64 state sorts at checkpoints only.  gx, gv
+ * each segment from its checkpoint.  The particles are also lane-sorted
+ * every `sort_every` steps (0: at checkpoints only), as pic_run's
+ * sort_every, which keeps the segment passes of both sweeps (the deposit,
+ * and in reverse the gather and deposit adjoints) on their LDS windows.  gx, gv
  * (host, 2 x n, may be NULL) receive dJ/d(x0, y0) and dJ/d(vx0, vy0) in the
  * context's particle order; the particles are left as they were.  The
  * sweep's forward pass is the separate-kernel form of pic_run's loop

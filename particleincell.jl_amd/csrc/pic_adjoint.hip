@@ -13,19 +13,24 @@
 //               lx = lx',  lv = lvt + h lx'
 //
 // Forward pass: per step a drift kernel (x' and, in the reverse sweep, its
-// copy), the deposit, the solve (E_s kept, 2 nx*ny doubles per step) and a
-// kick + drift kernel.  Every `ckpt` steps the particles are lane-sorted and
-// the state and the sort permutation are stored.  The reverse sweep
-// recomputes each segment's x'_s from its checkpoint with the same kernels
-// and the stored E_s, so the recomputed trajectory is bit-identical, then
-// runs the adjoint passes backwards and carries lx, lv across segment
-// boundaries (and finally into the caller's particle order) through the
-// stored permutations.
+// copy), the deposit (the time loop's segment pass), the solve (E_s kept,
+// 2 nx*ny doubles per step) and a kick + drift kernel.  The particles are
+// lane-sorted every `ckpt` steps, where the state, the sort permutation and
+// the cell offsets are stored, and every `sort_every` steps between.  The
+// reverse sweep recomputes each segment's x'_s from its checkpoint with the
+// same kernels, sorts and stored E_s, so the recomputed trajectory and its
+// slot orders are bit-identical, then runs the adjoint passes backwards:
+// the gather adjoint (a scatter of mu) and the deposit adjoint (a gather of
+// lrho) over segment windows in LDS, as the time loop's passes, with the
+// particles that left their window completed from global memory.  lx, lv
+// are carried through every sort's permutation (and finally into the
+// caller's particle order).
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
 #include "pic_internal.h"
+#include "pic_segcore.h"
 
 namespace pic {
 
@@ -145,77 +150,298 @@ __global__ void k_adj_kick(int64_t n, double *x, double *y, double *vx, double *
     y[k] = p2;
 }
 
-// Gather adjoint at x' (step s < last): lvt = lv + h lx;
-// lx += dt (lvt . grad G E_s); mu += dt G^T lvt.  Writes lv <- lvt.
-template <int K>
-__global__ void k_adj_gather(int64_t n, const double *__restrict__ xs, const double *__restrict__ ys,
-                             double *lx, double *ly, double *lvx, double *lvy,
-                             const double *__restrict__ ex, const double *__restrict__ ey,
-                             double *mux, double *muy, int nx, int ny, double dx, double dy,
-                             double dt, double h) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    constexpr int S = K == PIC_CIC ? 2 : 6, HL = K == PIC_CIC ? 0 : 2;
+// Gather adjoint at x' for one particle (step s < last): lvt = lv + h lx;
+// lx += dt (lvt . grad G E_s); mu += dt G^T lvt (add_mu(u, v, qx, qy) for
+// stencil node (u, v)); lv <- lvt.  node_e(u, v) is (ex, ey) at the node.
+template <int K, class FE, class FA>
+__device__ __forceinline__ void adj_gather64_one(double x, double y, double &lx, double &ly,
+                                                 double &lvx, double &lvy, int nx, int ny,
+                                                 double dx, double dy, double dt, double h,
+                                                 int &ci, int &cj, FE node_e, FA add_mu) {
+    constexpr int S = K == PIC_CIC ? 2 : 6;
     double fx, fy;
-    const int ci = ref_cell_a(xs[k], dx, nx, fx), cj = ref_cell_a(ys[k], dy, ny, fy);
+    ci = ref_cell_a(x, dx, nx, fx);
+    cj = ref_cell_a(y, dy, ny, fy);
     double wx[6], wy[6], dwx[6], dwy[6];
     shape_d<K>(fx, wx, dwx);
     shape_d<K>(fy, wy, dwy);
-    const double ax = lx[k], ay = ly[k];
-    const double tx = lvx[k] + h * ax, ty = lvy[k] + h * ay;
+    const double ax = lx, ay = ly;
+    const double tx = lvx + h * ax, ty = lvy + h * ay;
     double gxx = 0.0, gxy = 0.0, gyx = 0.0, gyy = 0.0;  // d ex/dx, d ex/dy, d ey/dx, d ey/dy
 #pragma unroll
     for (int u = 0; u < S; ++u) {
-        const int iu = picd::wrap2(ci - HL + u, nx);
 #pragma unroll
         for (int v = 0; v < S; ++v) {
-            const int64_t g = iu + (int64_t)nx * picd::wrap2(cj - HL + v, ny);
-            const double e1 = ex[g], e2 = ey[g];
-            gxx += dwx[u] * wy[v] * e1;
-            gxy += wx[u] * dwy[v] * e1;
-            gyx += dwx[u] * wy[v] * e2;
-            gyy += wx[u] * dwy[v] * e2;
+            const double2 e = node_e(u, v);
+            gxx += dwx[u] * wy[v] * e.x;
+            gxy += wx[u] * dwy[v] * e.x;
+            gyx += dwx[u] * wy[v] * e.y;
+            gyy += wx[u] * dwy[v] * e.y;
             const double wab = wx[u] * wy[v];
-            unsafeAtomicAdd(&mux[g], dt * tx * wab);
-            unsafeAtomicAdd(&muy[g], dt * ty * wab);
+            add_mu(u, v, dt * tx * wab, dt * ty * wab);
         }
     }
-    lx[k] = ax + dt * (tx * gxx + ty * gyx) / dx;
-    ly[k] = ay + dt * (tx * gxy + ty * gyy) / dy;
-    lvx[k] = tx;
-    lvy[k] = ty;
+    lx = ax + dt * (tx * gxx + ty * gyx) / dx;
+    ly = ay + dt * (tx * gxy + ty * gyy) / dy;
+    lvx = tx;
+    lvy = ty;
 }
 
-// Deposit adjoint at x': lx += (w/dxdy) grad W . lrho; then lv += h lx.
-template <int K>
-__global__ void k_adj_deposit(int64_t n, const double *__restrict__ xs,
-                              const double *__restrict__ ys, double *lx, double *ly, double *lvx,
-                              double *lvy, const double *__restrict__ lrho, int nx, int ny,
-                              double dx, double dy, double wdd, double h) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    constexpr int S = K == PIC_CIC ? 2 : 6, HL = K == PIC_CIC ? 0 : 2;
+// Deposit adjoint at x' for one particle: lx += (w/dxdy) grad W . lrho; then
+// lv += h lx.  node_r(u, v) is lrho at the node.
+template <int K, class FR>
+__device__ __forceinline__ void adj_deposit64_one(double x, double y, double &lx, double &ly,
+                                                  double &lvx, double &lvy, int nx, int ny,
+                                                  double dx, double dy, double wdd, double h,
+                                                  int &ci, int &cj, FR node_r) {
+    constexpr int S = K == PIC_CIC ? 2 : 6;
     double fx, fy;
-    const int ci = ref_cell_a(xs[k], dx, nx, fx), cj = ref_cell_a(ys[k], dy, ny, fy);
+    ci = ref_cell_a(x, dx, nx, fx);
+    cj = ref_cell_a(y, dy, ny, fy);
     double wx[6], wy[6], dwx[6], dwy[6];
     shape_d<K>(fx, wx, dwx);
     shape_d<K>(fy, wy, dwy);
     double gx = 0.0, gy = 0.0;
 #pragma unroll
     for (int u = 0; u < S; ++u) {
-        const int iu = picd::wrap2(ci - HL + u, nx);
 #pragma unroll
         for (int v = 0; v < S; ++v) {
-            const double r = lrho[iu + (int64_t)nx * picd::wrap2(cj - HL + v, ny)];
+            const double r = node_r(u, v);
             gx += dwx[u] * wy[v] * r;
             gy += wx[u] * dwy[v] * r;
         }
     }
-    const double ax = lx[k] + wdd * gx / dx, ay = ly[k] + wdd * gy / dy;
-    lx[k] = ax;
-    ly[k] = ay;
-    lvx[k] = lvx[k] + h * ax;
-    lvy[k] = lvy[k] + h * ay;
+    const double ax = lx + wdd * gx / dx, ay = ly + wdd * gy / dy;
+    lx = ax;
+    ly = ay;
+    lvx = lvx + h * ax;
+    lvy = lvy + h * ay;
+}
+
+// Arguments of the reverse passes over the positions x'_s of one step, in
+// the slot order of the lane sort whose offsets `off` are.
+struct AdjSeg64 {
+    const double *xs, *ys;            // x'_s per slot
+    double *lx, *ly, *lvx, *lvy;      // adjoints per slot
+    const double *ex, *ey;            // gather adjoint: E_s
+    const double *lrho;               // deposit adjoint: P^T mu
+    double *mux, *muy;                // gather adjoint: mu accumulators
+    int nx, ny, ncells;
+    double dx, dy, dt, h, wdd;
+    const int64_t *off;
+    int bps, margin, cap;
+    unsigned long long *slow;
+    uint32_t *dlist;
+    unsigned *dcount;
+};
+
+// The slots [p0, p1) of block (s, bi) and the window of segment s.
+struct AdjSlice {
+    int64_t p0, p1;
+    Win W;
+    bool fits;
+};
+
+template <int K>
+__device__ __forceinline__ AdjSlice adj_slice(const AdjSeg64 &a) {
+    constexpr int S = K == PIC_CIC ? 2 : 6, HL = K == PIC_CIC ? 0 : 2;
+    const int s = blockIdx.x / a.bps, bi = blockIdx.x - s * a.bps;
+    const int ce = min(64 * s + 64, a.ncells);
+    const int64_t s0 = a.off[64 * s], s1 = a.off[ce];
+    const int64_t len = s1 - s0;
+    AdjSlice r;
+    r.p0 = s0 + len * bi / a.bps;
+    r.p1 = s0 + len * (bi + 1) / a.bps;
+    r.W = seg_window<S, HL>(s, a.nx, a.ny, a.ncells, a.margin, a.cap);
+    r.fits = r.W.tw * r.W.th <= a.cap;
+    return r;
+}
+
+// Gather adjoint over one segment slice: E_s staged in an LDS window, mu
+// summed in FP64 LDS tiles (ds_add_f64) and flushed once per window node with
+// FP64 global atomics; a particle whose stencil leaves the window goes to
+// the deferred list.
+template <int K>
+__global__ __launch_bounds__(SEG_T) void k_adjseg64_gather(AdjSeg64 a) {
+    constexpr int S = K == PIC_CIC ? 2 : 6, HL = K == PIC_CIC ? 0 : 2;
+    extern __shared__ __align__(16) char smem_adj64[];
+    const int tid = threadIdx.x, nx = a.nx, ny = a.ny;
+    const AdjSlice sl = adj_slice<K>(a);
+    const Win &W = sl.W;
+    double2 *te = reinterpret_cast<double2 *>(smem_adj64);
+    double *tmx = reinterpret_cast<double *>(te + a.cap), *tmy = tmx + a.cap;
+    if (sl.fits) {
+        const int nt = W.tw * W.th;
+        for (int t = tid; t < nt; t += SEG_T) {
+            const int u = t % W.tw, v = t / W.tw;
+            const int64_t g = (int64_t)picd::wrap2(W.x0 + u, nx) +
+                              (int64_t)nx * picd::wrap2(W.y0 + v, ny);
+            te[t] = make_double2(a.ex[g], a.ey[g]);
+            tmx[t] = 0.0;
+            tmy[t] = 0.0;
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = (uint32_t)sl.p0 + tid; p < (uint32_t)sl.p1; p += SEG_T) {
+        const double x = a.xs[p], y = a.ys[p];
+        double dxo;
+        const int ci0 = ref_cell_a(x, a.dx, nx, dxo), cj0 = ref_cell_a(y, a.dy, ny, dxo);
+        const int lx0 = picd::wrap2(ci0 - HL - W.x0, nx), ly0 = picd::wrap2(cj0 - HL - W.y0, ny);
+        const bool in = sl.fits && lx0 + S - 1 < W.tw && ly0 + S - 1 < W.th;
+        defer_push(!in, p, a.dlist, a.dcount);
+        if (!in) continue;
+        double Lx = a.lx[p], Ly = a.ly[p], Lvx = a.lvx[p], Lvy = a.lvy[p];
+        int ci, cj;
+        adj_gather64_one<K>(
+            x, y, Lx, Ly, Lvx, Lvy, nx, ny, a.dx, a.dy, a.dt, a.h, ci, cj,
+            [&](int u, int v) { return te[(lx0 + u) + W.tw * (ly0 + v)]; },
+            [&](int u, int v, double qa, double qb) {
+                const int t = (lx0 + u) + W.tw * (ly0 + v);
+                __hip_atomic_fetch_add(&tmx[t], qa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(&tmy[t], qb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            });
+        a.lx[p] = Lx;
+        a.ly[p] = Ly;
+        a.lvx[p] = Lvx;
+        a.lvy[p] = Lvy;
+    }
+    if (!sl.fits) return;
+    __syncthreads();
+    const int nt = W.tw * W.th;
+    for (int t = tid; t < nt; t += SEG_T) {
+        const double qa = tmx[t], qb = tmy[t];
+        if (qa != 0.0 || qb != 0.0) {
+            const int u = t % W.tw, v = t / W.tw;
+            const int64_t g = (int64_t)picd::wrap2(W.x0 + u, nx) +
+                              (int64_t)nx * picd::wrap2(W.y0 + v, ny);
+            unsafeAtomicAdd(&a.mux[g], qa);
+            unsafeAtomicAdd(&a.muy[g], qb);
+        }
+    }
+}
+
+// The deferred particles of k_adjseg64_gather, from global memory.
+template <int K>
+__global__ __launch_bounds__(DEFER_T) void k_adjdefer64_gather(AdjSeg64 a) {
+    constexpr int HL = K == PIC_CIC ? 0 : 2;
+    const unsigned cnt = __hip_atomic_load(a.dcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int nx = a.nx, ny = a.ny;
+    for (unsigned i = blockIdx.x * DEFER_T + threadIdx.x; i < cnt; i += gridDim.x * DEFER_T) {
+        const uint32_t p = a.dlist[i];
+        double Lx = a.lx[p], Ly = a.ly[p], Lvx = a.lvx[p], Lvy = a.lvy[p];
+        int ci = 0, cj = 0;
+        auto node = [&](int u, int v) {
+            return (int64_t)picd::wrap2(ci - HL + u, nx) + (int64_t)nx * picd::wrap2(cj - HL + v, ny);
+        };
+        adj_gather64_one<K>(
+            a.xs[p], a.ys[p], Lx, Ly, Lvx, Lvy, nx, ny, a.dx, a.dy, a.dt, a.h, ci, cj,
+            [&](int u, int v) {
+                const int64_t g = node(u, v);
+                return make_double2(a.ex[g], a.ey[g]);
+            },
+            [&](int u, int v, double qa, double qb) {
+                const int64_t g = node(u, v);
+                unsafeAtomicAdd(&a.mux[g], qa);
+                unsafeAtomicAdd(&a.muy[g], qb);
+            });
+        a.lx[p] = Lx;
+        a.ly[p] = Ly;
+        a.lvx[p] = Lvx;
+        a.lvy[p] = Lvy;
+    }
+    defer_finish(cnt, a.dcount, a.slow);
+}
+
+// Deposit adjoint over one segment slice: lrho staged in an LDS window.
+template <int K>
+__global__ __launch_bounds__(SEG_T) void k_adjseg64_deposit(AdjSeg64 a) {
+    constexpr int S = K == PIC_CIC ? 2 : 6, HL = K == PIC_CIC ? 0 : 2;
+    extern __shared__ __align__(16) char smem_adj64[];
+    const int tid = threadIdx.x, nx = a.nx, ny = a.ny;
+    const AdjSlice sl = adj_slice<K>(a);
+    const Win &W = sl.W;
+    double *tr = reinterpret_cast<double *>(smem_adj64);
+    if (sl.fits) {
+        const int nt = W.tw * W.th;
+        for (int t = tid; t < nt; t += SEG_T) {
+            const int u = t % W.tw, v = t / W.tw;
+            tr[t] = a.lrho[(int64_t)picd::wrap2(W.x0 + u, nx) +
+                           (int64_t)nx * picd::wrap2(W.y0 + v, ny)];
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = (uint32_t)sl.p0 + tid; p < (uint32_t)sl.p1; p += SEG_T) {
+        const double x = a.xs[p], y = a.ys[p];
+        double dxo;
+        const int ci0 = ref_cell_a(x, a.dx, nx, dxo), cj0 = ref_cell_a(y, a.dy, ny, dxo);
+        const int lx0 = picd::wrap2(ci0 - HL - W.x0, nx), ly0 = picd::wrap2(cj0 - HL - W.y0, ny);
+        const bool in = sl.fits && lx0 + S - 1 < W.tw && ly0 + S - 1 < W.th;
+        defer_push(!in, p, a.dlist, a.dcount);
+        if (!in) continue;
+        double Lx = a.lx[p], Ly = a.ly[p], Lvx = a.lvx[p], Lvy = a.lvy[p];
+        int ci, cj;
+        adj_deposit64_one<K>(x, y, Lx, Ly, Lvx, Lvy, nx, ny, a.dx, a.dy, a.wdd, a.h, ci, cj,
+                             [&](int u, int v) { return tr[(lx0 + u) + W.tw * (ly0 + v)]; });
+        a.lx[p] = Lx;
+        a.ly[p] = Ly;
+        a.lvx[p] = Lvx;
+        a.lvy[p] = Lvy;
+    }
+}
+
+// The deferred particles of k_adjseg64_deposit, from global memory.
+template <int K>
+__global__ __launch_bounds__(DEFER_T) void k_adjdefer64_deposit(AdjSeg64 a) {
+    constexpr int HL = K == PIC_CIC ? 0 : 2;
+    const unsigned cnt = __hip_atomic_load(a.dcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int nx = a.nx, ny = a.ny;
+    for (unsigned i = blockIdx.x * DEFER_T + threadIdx.x; i < cnt; i += gridDim.x * DEFER_T) {
+        const uint32_t p = a.dlist[i];
+        double Lx = a.lx[p], Ly = a.ly[p], Lvx = a.lvx[p], Lvy = a.lvy[p];
+        int ci = 0, cj = 0;
+        adj_deposit64_one<K>(a.xs[p], a.ys[p], Lx, Ly, Lvx, Lvy, nx, ny, a.dx, a.dy, a.wdd, a.h,
+                             ci, cj, [&](int u, int v) {
+                                 return a.lrho[(int64_t)picd::wrap2(ci - HL + u, nx) +
+                                               (int64_t)nx * picd::wrap2(cj - HL + v, ny)];
+                             });
+        a.lx[p] = Lx;
+        a.ly[p] = Ly;
+        a.lvx[p] = Lvx;
+        a.lvy[p] = Lvy;
+    }
+    defer_finish(cnt, a.dcount, a.slow);
+}
+
+// LDS per window node: the gather adjoint holds E (16 B) and two FP64 mu
+// sums (16 B) in 80 KiB (two blocks per CU; an M6 window of a 64-cell row
+// segment keeps a 9-cell drift margin), the deposit adjoint lrho (8 B) in
+// 40 KiB.
+constexpr size_t ADJ64_GATHER_NODE = 32, ADJ64_DEPOSIT_NODE = 8;
+constexpr size_t ADJ64_GATHER_LDS_BYTES = 80 * 1024;
+
+template <int K, bool GATHER>
+static int launch_adjseg64(const Ctx *c, AdjSeg64 a) {
+    const size_t per_node = GATHER ? ADJ64_GATHER_NODE : ADJ64_DEPOSIT_NODE;
+    a.cap = (int)((GATHER ? ADJ64_GATHER_LDS_BYTES : SEG_LDS_BYTES) / per_node) & ~15;
+    a.margin = 32;
+    const size_t sm = (size_t)a.cap * per_node;
+    const void *k = GATHER ? reinterpret_cast<const void *>(k_adjseg64_gather<K>)
+                           : reinterpret_cast<const void *>(k_adjseg64_deposit<K>);
+    if (sm > 65536)
+        (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    a.bps = seg_bps(c, k, sm);
+    const dim3 grid((unsigned)(((c->nx * c->ny + 63) / 64) * a.bps));
+    if (GATHER) {
+        hipLaunchKernelGGL(k_adjseg64_gather<K>, grid, dim3(SEG_T), sm, c->stream, a);
+        hipLaunchKernelGGL(k_adjdefer64_gather<K>, dim3(DEFER_BLOCKS), dim3(DEFER_T), 0,
+                           c->stream, a);
+    } else {
+        hipLaunchKernelGGL(k_adjseg64_deposit<K>, grid, dim3(SEG_T), sm, c->stream, a);
+        hipLaunchKernelGGL(k_adjdefer64_deposit<K>, dim3(DEFER_BLOCKS), dim3(DEFER_T), 0,
+                           c->stream, a);
+    }
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
 }
 
 __global__ void k_adj_seed(int64_t m, const double *ex, const double *ey, double dxdy, double *mux,
@@ -244,28 +470,57 @@ __global__ void k_scatter_back(int64_t n, const uint32_t *__restrict__ perm,
 
 static inline dim3 ga(int64_t n) { return dim3((unsigned)((n + AT - 1) / AT)); }
 
+// Steps at which the sweep lane-sorts the particles: every checkpoint step
+// and, with sort_every > 0, every multiple of it before the last step.  The
+// forward pass and the reverse sweep's recomputation follow the same list,
+// and the stable sort of a bit-identical state is the same permutation.
+struct SortPlan64 {
+    int ckpt, every, nsteps;
+    bool at(int s) const {
+        return s % ckpt == 0 || (every > 0 && s % every == 0 && s < nsteps - 1);
+    }
+};
+
 template <int K>
-static int adjoint_impl(Ctx *c, double dt, int nsteps, int ckpt, double *J, double *gx,
-                        double *gv) {
+static int adjoint_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, double *J,
+                        double *gx, double *gv) {
     const int64_t n = c->n;
     const int nx = c->nx, ny = c->ny;
     const int64_t m = (int64_t)nx * ny;
     const double h = 0.5 * dt, dx = c->dx, dy = c->dy, dxdy = c->dx * c->dy;
     const int nseg = (nsteps + ckpt - 1) / ckpt;
+    const int len = std::min(ckpt, nsteps);
     const double scale_rho = 1.0 / ((double)nx * (double)ny * dxdy);
     const double scale = 1.0 / ((double)nx * (double)ny);
-    DevBuf<double> Es, ck, xs, lam, mu, lrho, tmp;
-    DevBuf<uint32_t> perms;
+    const size_t noff = (size_t)m + 1;
+    const SortPlan64 sp{ckpt, sort_every, nsteps};
+    int nsub = 0;  // the most sorts inside one segment after its checkpoint sort
+    for (int s0 = 0; s0 < nsteps; s0 += ckpt) {
+        int k = 0;
+        for (int s = s0 + 1; s < std::min(nsteps, s0 + ckpt); ++s) k += sp.at(s);
+        nsub = std::max(nsub, k);
+    }
+    PIC_TRY(ensure_energy(c, nsteps));
+    PIC_TRY(ensure_defer(c));
+    if (c->pending_perm) PIC_TRY(settle_state(c));
+    DevBuf<double> Es, ck, xs, lam, mu, lrho;
+    DevBuf<uint32_t> perms, subperm;
+    DevBuf<int64_t> offs, suboff;
     PIC_TRY(Es.alloc((size_t)2 * m * nsteps));
     PIC_TRY(ck.alloc((size_t)4 * n * nseg));
     PIC_TRY(perms.alloc((size_t)n * nseg));
-    PIC_TRY(xs.alloc((size_t)2 * n * ckpt));
-    PIC_TRY(lam.alloc((size_t)4 * n));
+    PIC_TRY(offs.alloc(noff * nseg));
+    PIC_TRY(xs.alloc((size_t)2 * n * len));
+    PIC_TRY(lam.alloc((size_t)5 * n));
     PIC_TRY(mu.alloc((size_t)2 * m));
     PIC_TRY(lrho.alloc((size_t)m));
-    PIC_TRY(tmp.alloc((size_t)n));
-    PIC_TRY(ensure_energy(c, nsteps));
+    PIC_TRY(subperm.alloc((size_t)n * nsub));
+    PIC_TRY(suboff.alloc(noff * nsub));
     PIC_HIP(hipMemsetAsync(c->step_counter, 0, sizeof(int), c->stream));
+    auto dcopy = [&](void *dst, const void *src, size_t bytes) -> int {
+        PIC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+        return PIC_OK;
+    };
     double *st[4] = {nullptr, nullptr, nullptr, nullptr};
     auto state = [&]() {
         st[0] = c->x;
@@ -273,35 +528,35 @@ static int adjoint_impl(Ctx *c, double dt, int nsteps, int ckpt, double *J, doub
         st[2] = c->vx;
         st[3] = c->vy;
     };
+    // checkpoint g: the state right after the sort of step g ckpt, that
+    // sort's permutation and cell offsets
     auto save = [&](int g) -> int {
         state();
-        for (int q = 0; q < 4; ++q)
-            PIC_HIP(hipMemcpyAsync(ck.p + ((size_t)4 * g + q) * n, st[q], (size_t)n * 8,
-                                   hipMemcpyDeviceToDevice, c->stream));
-        PIC_HIP(hipMemcpyAsync(perms.p + (size_t)g * n, c->perm2, (size_t)n * 4,
-                               hipMemcpyDeviceToDevice, c->stream));
+        for (int q = 0; q < 4; ++q) PIC_TRY(dcopy(ck.p + ((size_t)4 * g + q) * n, st[q], (size_t)n * 8));
+        PIC_TRY(dcopy(perms.p + (size_t)g * n, c->perm2, (size_t)n * 4));
+        PIC_TRY(dcopy(offs.p + noff * g, c->cell_off, noff * 8));
         return PIC_OK;
     };
     auto restore = [&](int g) -> int {
         state();
-        for (int q = 0; q < 4; ++q)
-            PIC_HIP(hipMemcpyAsync(st[q], ck.p + ((size_t)4 * g + q) * n, (size_t)n * 8,
-                                   hipMemcpyDeviceToDevice, c->stream));
+        for (int q = 0; q < 4; ++q) PIC_TRY(dcopy(st[q], ck.p + ((size_t)4 * g + q) * n, (size_t)n * 8));
+        PIC_TRY(dcopy(c->cell_off, offs.p + noff * g, noff * 8));
+        c->seg_valid = true;
+        c->pending_perm = nullptr;
         return PIC_OK;
     };
     // ---------------------------------------------------------- forward
     for (int s = 0; s < nsteps; ++s) {
-        if (s % ckpt == 0) {
+        if (sp.at(s)) {
             PIC_TRY(sort_by_cell(c, nullptr, true));
-            PIC_TRY(save(s / ckpt));
+            if (s % ckpt == 0) PIC_TRY(save(s / ckpt));
         }
         hipLaunchKernelGGL(k_adj_drift, ga(n), dim3(AT), 0, c->stream, n, c->x, c->y, c->vx, c->vy,
                            h, c->dimx, c->dimy, (double *)nullptr, (double *)nullptr);
         PIC_TRY(launch_fill(c, c->rho_acc, 0.0, m));
         PIC_TRY(launch_seg64(c, K, 3, 0.0, 0.0));
         PIC_TRY(poisson_solve_e(c, c->rho_acc, scale_rho, 0, -2));
-        PIC_HIP(hipMemcpyAsync(Es.p + (size_t)2 * m * s, c->ex, (size_t)2 * m * 8,
-                               hipMemcpyDeviceToDevice, c->stream));
+        PIC_TRY(dcopy(Es.p + (size_t)2 * m * s, c->ex, (size_t)2 * m * 8));
         if (s == nsteps - 1) break;
         hipLaunchKernelGGL(k_adj_kick<K>, ga(n), dim3(AT), 0, c->stream, n, c->x, c->y, c->vx,
                            c->vy, (const double *)c->ex, (const double *)c->ey, nx, ny, dx, dy, dt,
@@ -316,14 +571,48 @@ static int adjoint_impl(Ctx *c, double dt, int nsteps, int ckpt, double *J, doub
         *J = e2[0] + e2[1];
     }
     // ---------------------------------------------------------- reverse
-    double *lx = lam.p, *ly = lam.p + n, *lvx = lam.p + 2 * n, *lvy = lam.p + 3 * n;
-    hipLaunchKernelGGL(k_zero2, ga(n), dim3(AT), 0, c->stream, n, lx, ly);
-    hipLaunchKernelGGL(k_zero2, ga(n), dim3(AT), 0, c->stream, n, lvx, lvy);
-    const double wdd = c->w0 / dxdy;
+    // L[0..3] = lx, ly, lvx, lvy per slot; L[4] spare for the permutations
+    double *L[5];
+    for (int q = 0; q < 5; ++q) L[q] = lam.p + (size_t)q * n;
+    hipLaunchKernelGGL(k_zero2, ga(n), dim3(AT), 0, c->stream, n, L[0], L[1]);
+    hipLaunchKernelGGL(k_zero2, ga(n), dim3(AT), 0, c->stream, n, L[2], L[3]);
+    // per-slot values into the slot order before the sort of permutation pg
+    auto carry_back = [&](const uint32_t *pg) {
+        for (int q = 0; q < 4; ++q) {
+            hipLaunchKernelGGL(k_scatter_back, ga(n), dim3(AT), 0, c->stream, n, pg,
+                               (const double *)L[q], L[4]);
+            std::swap(L[q], L[4]);
+        }
+    };
+    AdjSeg64 a;
+    a.mux = mu.p;
+    a.muy = mu.p + m;
+    a.lrho = lrho.p;
+    a.nx = nx;
+    a.ny = ny;
+    a.ncells = (int)m;
+    a.dx = dx;
+    a.dy = dy;
+    a.dt = dt;
+    a.h = h;
+    a.wdd = c->w0 / dxdy;
+    a.slow = c->slow_chunks;
+    a.dlist = c->defer;
+    a.dcount = c->defer_count;
+    std::vector<int> sub_of((size_t)len);
     for (int g = nseg - 1; g >= 0; --g) {
         const int s0 = g * ckpt, s1 = std::min(nsteps, s0 + ckpt);
         PIC_TRY(restore(g));
+        // recompute x'_s, s0 <= s < s1, each in the slot order of the sort in force
+        int ns = 0;
         for (int s = s0; s < s1; ++s) {
+            if (s > s0 && sp.at(s)) {
+                PIC_TRY(sort_by_cell(c, nullptr, true));
+                PIC_TRY(dcopy(subperm.p + (size_t)ns * n, c->perm2, (size_t)n * 4));
+                PIC_TRY(dcopy(suboff.p + noff * ns, c->cell_off, noff * 8));
+                ++ns;
+            }
+            sub_of[s - s0] = ns;
             double *xo = xs.p + (size_t)2 * n * (s - s0);
             hipLaunchKernelGGL(k_adj_drift, ga(n), dim3(AT), 0, c->stream, n, c->x, c->y, c->vx,
                                c->vy, h, c->dimx, c->dimy, xo, xo + n);
@@ -333,32 +622,38 @@ static int adjoint_impl(Ctx *c, double dt, int nsteps, int ckpt, double *J, doub
                                c->vy, E, E + m, nx, ny, dx, dy, dt, h, c->dimx, c->dimy);
         }
         for (int s = s1 - 1; s >= s0; --s) {
+            const int k = sub_of[s - s0];
             const double *xo = xs.p + (size_t)2 * n * (s - s0);
             const double *E = Es.p + (size_t)2 * m * s;
+            a.xs = xo;
+            a.ys = xo + n;
+            a.lx = L[0];
+            a.ly = L[1];
+            a.lvx = L[2];
+            a.lvy = L[3];
+            a.ex = E;
+            a.ey = E + m;
+            a.off = k == 0 ? offs.p + noff * g : suboff.p + noff * (k - 1);
             if (s == nsteps - 1) {
                 hipLaunchKernelGGL(k_adj_seed, ga(m), dim3(AT), 0, c->stream, m, E, E + m, dxdy,
                                    mu.p, mu.p + m);
             } else {
                 hipLaunchKernelGGL(k_zero2, ga(m), dim3(AT), 0, c->stream, m, mu.p, mu.p + m);
-                hipLaunchKernelGGL(k_adj_gather<K>, ga(n), dim3(AT), 0, c->stream, n, xo, xo + n,
-                                   lx, ly, lvx, lvy, E, E + m, mu.p, mu.p + m, nx, ny, dx, dy, dt,
-                                   h);
+                PIC_TRY((launch_adjseg64<K, true>(c, a)));
             }
             PIC_TRY(poisson_adjoint(c, mu.p, mu.p + m, lrho.p, scale));
-            hipLaunchKernelGGL(k_adj_deposit<K>, ga(n), dim3(AT), 0, c->stream, n, xo, xo + n, lx,
-                               ly, lvx, lvy, (const double *)lrho.p, nx, ny, dx, dy, wdd, h);
+            PIC_TRY((launch_adjseg64<K, false>(c, a)));
+            // a sort inside the segment at step s: the slots into their order before it
+            if (s > s0 && sp.at(s)) carry_back(subperm.p + (size_t)(k - 1) * n);
         }
         // into the slot order of the previous segment (or the caller's order)
-        const uint32_t *pg = perms.p + (size_t)g * n;
-        for (double *a : {lx, ly, lvx, lvy}) {
-            hipLaunchKernelGGL(k_scatter_back, ga(n), dim3(AT), 0, c->stream, n, pg,
-                               (const double *)a, tmp.p);
-            PIC_HIP(hipMemcpyAsync(a, tmp.p, (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
-        }
+        carry_back(perms.p + (size_t)g * n);
     }
     PIC_HIP(hipGetLastError());
     std::vector<double> h4((size_t)4 * n);
-    PIC_HIP(hipMemcpyAsync(h4.data(), lam.p, (size_t)4 * n * 8, hipMemcpyDeviceToHost, c->stream));
+    for (int q = 0; q < 4; ++q)
+        PIC_HIP(hipMemcpyAsync(h4.data() + (size_t)q * n, L[q], (size_t)n * 8,
+                               hipMemcpyDeviceToHost, c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
     for (int64_t k = 0; k < n; ++k) {
         if (gx) {
@@ -376,8 +671,8 @@ static int adjoint_impl(Ctx *c, double dt, int nsteps, int ckpt, double *J, doub
     state();
     for (int q = 0; q < 4; ++q) {
         hipLaunchKernelGGL(k_scatter_back, ga(n), dim3(AT), 0, c->stream, n,
-                           (const uint32_t *)perms.p, (const double *)st[q], tmp.p);
-        PIC_HIP(hipMemcpyAsync(st[q], tmp.p, (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
+                           (const uint32_t *)perms.p, (const double *)st[q], L[4]);
+        PIC_TRY(dcopy(st[q], L[4], (size_t)n * 8));
     }
     PIC_HIP(hipStreamSynchronize(c->stream));
     c->seg_valid = false;
@@ -401,8 +696,8 @@ extern "C" int pic_adjoint(pic_ctx *ctx, int kernel, double dt, int nsteps, int 
     PIC_CHECK(c->n > 0, PIC_ERR_STATE, "no particles");
     if (c->state == PIC_STATE_F32)
         return adjoint_f32(c, kernel, dt, nsteps, ckpt, sort_every, J, gx, gv);
-    return kernel == PIC_CIC ? adjoint_impl<PIC_CIC>(c, dt, nsteps, ckpt, J, gx, gv)
-                             : adjoint_impl<PIC_M6>(c, dt, nsteps, ckpt, J, gx, gv);
+    return kernel == PIC_CIC ? adjoint_impl<PIC_CIC>(c, dt, nsteps, ckpt, sort_every, J, gx, gv)
+                             : adjoint_impl<PIC_M6>(c, dt, nsteps, ckpt, sort_every, J, gx, gv);
 }
 
 extern "C" int pic_set_adjoint_block(pic_ctx *ctx, int block) {

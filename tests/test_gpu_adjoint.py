@@ -167,7 +167,8 @@ def test_adjoint_f32_windows_and_sorts(gpu, kernel, sort_every):
     them take the deferred global-memory path (sort_every 0), and sorts
     between checkpoints (2, or 3, which does not divide the checkpoint
     interval) are replayed by the recomputation.  Reference: the FP64
-    adjoint, whose kernels read and write global memory only."""
+    adjoint (checkpoint sorts only), itself pinned to the oracle's adjoint
+    on this grid by test_adjoint_f64_windows_and_sorts."""
     pic = gpu
     F32 = pic._lib.PIC_STATE_F32
     n, dt, steps, ck = 65536, 0.1, 20, 5
@@ -203,6 +204,36 @@ def test_adjoint_f32_windows_and_sorts(gpu, kernel, sort_every):
         e = sim.run(dt, steps, sort_every)
         sim.close()
         assert J32 == pytest.approx(e[-1, 0] + e[-1, 1], rel=1e-6)
+
+
+@pytest.mark.parametrize("kernel,sort_every", [("cic", 0), ("cic", 2), ("m6", 0), ("m6", 3)])
+def test_adjoint_f64_windows_and_sorts(gpu, O, kernel, sort_every):
+    """FP64 adjoint on a 256 x 32 grid, where the reverse passes' LDS windows
+    cover a few cells around each 64-cell segment: with sorts at checkpoints
+    only, particles drift past their windows and take the deferred
+    global-memory path; sorts between checkpoints (2, or 3, which does not
+    divide the checkpoint interval) are replayed by the recomputation and
+    the adjoints carried back through their permutations.  Against the
+    oracle's adjoint: J to 1e-10, gradients to 1e-8 normwise; the particles
+    are left as they were."""
+    pic = gpu
+    n, dt, steps, ck = 65536, 0.1, 20, 5
+    k = pic.CloudInCell() if kernel == "cic" else pic.M6()
+    mesh = pic.TwoDGrid(0.0, 4 * math.pi, 256, 0.0, 4 * math.pi, 32)
+    p = pic.sampling.landau_damping(n, mesh, 0.05, 0.5, sobol=pic.sampling.sobol3(n))
+    sim = pic.Simulation(mesh, p, k)
+    slow0 = sim.slow_chunks()
+    J, gx, gv = sim.adjoint(dt, steps, ck, sort_every)
+    deferred = sim.slow_chunks() - slow0
+    after = sim.particles()
+    sim.close()
+    assert np.array_equal(after, p)
+    if sort_every == 0:
+        assert deferred > 0  # the global-memory fallback was exercised
+    Jo, gxo, gvo, _ = O.adjoint(p, kernel, 256, 32, mesh.dimx, mesh.dimy, dt, steps, ck)
+    assert J == pytest.approx(Jo, rel=1e-10)
+    for a, b in ((gx, gxo), (gv, gvo)):
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-8
 
 
 def test_adjoint_f32_two_level_matches_one_level(gpu):
