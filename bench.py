@@ -105,22 +105,57 @@ def issue_counts(kernel, profile="sq_counters_dominant_kernels.txt"):
     return None
 
 
-def copy_probe(mib, reps=5):
+def stream_ceiling():
+    """GB/s of the in-place state stream in profiles/hbm_stream_ceiling.txt."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_stream_ceiling.txt")) as f:
+            for line in f:
+                line = line.strip()
+                if line.startswith('{"state_stream_particles"'):
+                    return float(json.loads(line)["GBs"])
+    except (OSError, ValueError, KeyError):
+        pass
+    return None
+
+
+def copy_probe(mib, reps=5, particles=100_000_000):
     """Plain device copies of `mib` MiB (torch's elementwise copy kernel), the
     calibration workload for the HBM byte counters: a copy of B bytes reads B
-    and writes B."""
+    and writes B.  Then the streaming ceiling of the FP32 passes' access
+    pattern: in-place updates of an int32 array and a float4 array of
+    `particles` entries (the cell and (ox, oy, vx, vy) state: 20 B read and
+    20 B written per particle), no other work."""
     import torch
+
+    def best_of(fn):
+        fn()
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return best
+
     n = mib * (1 << 20) // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / reps
+    dt = best_of(lambda: dst.copy_(src))
     print(json.dumps({"copy_probe_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4,
                       "GBs": 2 * n * 4 / dt / 1e9}), flush=True)
+    del src, dst
+    cell = torch.zeros(particles, dtype=torch.int32, device="cuda")
+    ov = torch.zeros((particles, 4), dtype=torch.float32, device="cuda")
+
+    def state_update():
+        cell.add_(1)
+        ov.add_(1.0)
+
+    dt = best_of(state_update)
+    print(json.dumps({"state_stream_particles": particles, "bytes_read": 20 * particles,
+                      "bytes_written": 20 * particles, "us": dt * 1e6,
+                      "GBs": 40 * particles / dt / 1e9}), flush=True)
 
 
 class DeviceLandau:
@@ -911,6 +946,13 @@ def main():
                      "frac": valu_us / pass_us},
             "lds": {"bytes_per_particle": ic[1], "floor_us": lds_us, "frac": lds_us / pass_us},
             "source": "profiles/sq_counters_dominant_kernels.txt"}
+    # the HBM-bound passes against the measured streaming ceiling of their
+    # access pattern (in-place update of the state arrays)
+    ceiling = stream_ceiling()
+    if ceiling is not None and not one_launch:
+        out["roofline"]["stream_ceiling"] = {
+            "GBs": ceiling, "frac": out["roofline"]["achieved"] / ceiling,
+            "source": "profiles/hbm_stream_ceiling.txt"}
     traffic_file = {"landau_m6": "pmc_traffic_landau_m6_128x128_1e7.txt",
                     "strong_b": "pmc_traffic_strong_b_256x256_1e8.txt"}.get(args.workload)
     if traffic_file and n == W["n"]:
