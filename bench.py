@@ -714,10 +714,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["f32", "f64", "both"], default="both")
     ap.add_argument("--f32-tiles", type=int, choices=[1, 2, 4], default=None,
-                    help="copies of the fixed-point charge tile of the FP32 M6, TSC and M4 "
+                    help="copies of the fixed-point charge tile of the FP32 "
                          "drift-and-deposit passes (default: the workload's, else 4)")
     ap.add_argument("--f32-window", type=int, default=None, metavar="KIB",
-                    help="LDS window per block of the FP32 M6, TSC and M4 drift-and-deposit "
+                    help="LDS window per block of the FP32 drift-and-deposit "
                          "passes, 40 to 160 KiB (default: the workload's, else 160)")
     ap.add_argument("--f32-acc", type=int, choices=[0, 1, 2], default=2,
                     help="FP32 charge tiles: 0 compensated FP32 pairs, 1 FP64 sums, "

@@ -273,8 +273,8 @@ int pic_set_fused_step(pic_ctx *ctx, int fused);
  * The grid totals are FP64 in every mode.  No reference counterpart. */
 int pic_set_f32_accumulation(pic_ctx *ctx, int mode);
 
-/* Copies of the fixed-point charge tile in the LDS window of pic_run's M6,
- * TSC and M4 FP32 passes that drift and then deposit (accumulation mode 2):
+/* Copies of the fixed-point charge tile in the LDS window of pic_run's
+ * FP32 passes that drift and then deposit (accumulation mode 2):
  * 1, 2 or 4 (default 4).  Lane l adds into copy l mod tiles, so that neighbouring
  * lanes whose particles drifted into one cell do not add to one LDS word in
  * one instruction; the copies are summed exactly at the flush, so the

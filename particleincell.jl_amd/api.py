@@ -892,7 +892,7 @@ class Simulation:
 
     def set_f32_tiles(self, tiles):
         """Copies (1, 2 or 4, default 4) of the fixed-point charge tile in the
-        M6, TSC and M4 FP32 drift-and-deposit passes (pic_set_f32_tiles)."""
+        FP32 drift-and-deposit passes (pic_set_f32_tiles)."""
         L.call("pic_set_f32_tiles", self.ctx.h, int(tiles))
 
     def set_f32_window(self, kib):

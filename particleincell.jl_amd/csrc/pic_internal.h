@@ -130,7 +130,7 @@ struct Ctx {
     // on gfx950, whose ds_add_f32 is slow and whose 8-byte LDS atomics
     // conflict on banks twice as often as 4-byte ones)
     int f32_acc = 2;
-    int f32_tiles = 4;  // fixed-point charge tiles of the M6, TSC, M4 drift-and-deposit passes
+    int f32_tiles = 4;  // fixed-point charge tiles of the FP32 drift-and-deposit passes
     int f32_window_kib = 160;  // LDS window per block of those passes (pic_set_f32_window)
 
     // E solve on power-of-two grids up to 512 x 512: three LDS-transform

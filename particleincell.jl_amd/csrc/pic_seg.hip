@@ -29,7 +29,7 @@
 //    converts to FP64.  Each particle adds exactly R = round(w/u) units: the
 //    node of its own cell takes R minus the rounded shares of the others, and
 //    the flush scales units by w/R, so a block deposits exactly its count
-//    times w (see charge32_fixed).  The M6, TSC and M4 passes that drift before they
+//    times w (see charge32_fixed).  The passes that drift before they
 //    deposit keep 2 or 4 copies of the tile (Ctx::f32_tiles), lane l adding
 //    into copy l mod copies: a ds_add_u32 whose lanes address one word
 //    serialises
@@ -170,14 +170,18 @@ __device__ __forceinline__ int64_t packed_half(const unsigned long long *tile, i
 // one charge tile ran ~1.5 % faster so).  The multi-tile fixed-point drift-
 // and-deposit passes are bounded at four waves per SIMD (seg32_waves below).
 // Shapes whose fixed-point drift-and-deposit passes keep several copies of
-// the charge tile (36 or 16 LDS atomics per particle; measured on MI355X
-// with four copies: M6 +4 %, TSC +3 %, M4 +5 % on 1e7 particles, 128^2).
+// the charge tile (packed pairs of int32 sums, 160 KiB windows, 1024-thread
+// blocks): every shape.  Measured on MI355X with four copies: M6 +4 %, TSC
+// +3 %, M4 +5 % on 1e7 particles, 128^2; NGP, whose particles put all their
+// charge on one node and pile onto shared words as they drift, 140 -> 97 us
+// per pass in the loop (1e7 particles, 128^2, a sort every 8 steps); CIC
+// 962 -> 910 us (1e8 particles, 256^2, bz = 20).
 template <int K>
 constexpr bool multi_tile_shape() {
-    return K == PIC_M6 || K == PIC_TSC || K == PIC_M4;
+    return K == PIC_M6 || K == PIC_TSC || K == PIC_M4 || K == PIC_NGP || K == PIC_CIC;
 }
 inline bool multi_tile_shape(int kernel) {
-    return kernel == PIC_M6 || kernel == PIC_TSC || kernel == PIC_M4;
+    return valid_shape(kernel);
 }
 
 // The multi-tile fixed-point drift-and-deposit passes run one 1024-thread
@@ -747,21 +751,21 @@ int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     const bool gather = mode == 1 || mode == 2;
     const int acc = c->f32_acc;
     // copies of the fixed-point tile (pic_set_f32_tiles, default 4) for the
-    // M6, TSC and M4 passes that drift and then deposit.  Measured with two: the M6
-    // fused pass 216 -> 208 us (1e7 particles, 128^2) and 12.8 -> 11.6 ms
-    // (5e8, 512^2); four: 200 -> 190 us at 1e7; the deposit-only pass ~10 %
-    // slower with two (its particles are in their sorted lanes after a
-    // sort); CIC and NGP 4-8 % slower (4 atomics per particle do not pay for
-    // the blocks per CU that the larger LDS window costs), and so were TSC
-    // and M4 with two copies in 53 KiB, 512-thread blocks; with four copies,
-    // 80 KiB and 1024-thread blocks TSC and M4 gain 3-5 %.
+    // passes that drift and then deposit.  Measured with two: the M6 fused
+    // pass 216 -> 208 us (1e7 particles, 128^2) and 12.8 -> 11.6 ms (5e8,
+    // 512^2); four: 200 -> 190 us at 1e7; the deposit-only pass ~10 % slower
+    // with two (its particles are in their sorted lanes after a sort).  CIC
+    // and NGP were 4-8 % slower with two copies in 53 KiB windows of 512-thread
+    // blocks (on freshly sorted particles); with four packed copies, 160 KiB
+    // windows and 1024-thread blocks they gain 5 % (CIC) and 30 % (NGP) in
+    // the loop, as TSC and M4 gain 3-5 %.
     a.tiles = acc == 2 && multi_tile_shape(kernel) && (mode == 0 || mode == 1) ? c->f32_tiles : 1;
     // field window twice (16 B per node), bz (4 B), charge tiles
     const size_t per_node = (gather ? (c->bz_mode == PIC_BZ_GRID ? 20 : 16) : 0) +
                             (mode == 2 ? 0 : acc == 2 ? 4 * a.tiles : 8);
-    // the M6, TSC and M4 drift-and-deposit passes with fixed-point tiles run
-    // 1024-thread blocks with the context's window (pic_set_f32_window, 160
-    // KiB by default: one block per CU); the others 40 KiB
+    // the drift-and-deposit passes with fixed-point tiles run 1024-thread
+    // blocks with the context's window (pic_set_f32_window, 160 KiB by
+    // default: one block per CU); the others 40 KiB
     const bool wide = acc == 2 && multi_tile_shape(kernel) && (mode == 0 || mode == 1);
     const size_t win = wide ? (size_t)c->f32_window_kib * 1024 : SEG_LDS_BYTES;
     a.cap = (int)(win / (per_node ? per_node : 8)) & ~15;

@@ -106,17 +106,6 @@ constexpr int DEFER_T = 256, DEFER_BLOCKS = 256;
 // 512-thread blocks share a CU's 160 KiB; an M6 window of a 64-cell row
 // segment then keeps a drift margin of 11 cells (FP32) or 7 cells (FP64).
 constexpr size_t SEG_LDS_BYTES = 40 * 1024;
-// The FP32 M6, TSC and M4 passes with several fixed-point charge tiles hold
-// the field twice (16 B, for 16-byte paired reads) and 2 or 4 int32 tiles
-// (8 or 16 B) per node.  80 KiB (two blocks per CU) gives the M6 window of a
-// 64-cell row segment a 14-cell drift margin with two tiles, 10 with four
-// (with the field held once: 20 and 14).  Measured with two tiles and one
-// field copy on
-// MI355X against 53 KiB (three blocks per CU, margin 14): the same pass time
-// at 1e7 particles on 128^2 with ~1000x fewer deferred particles, and on
-// 512^2 (5e8 particles, ~2 cells of drift per step) a sort every 3 steps
-// instead of every 2, 15 % more particle-steps/s.
-constexpr size_t SEG32_FIXED_LDS_BYTES = 80 * 1024;
 
 // Blocks per segment: as many as keep every CU busy in one round of
 // resident blocks (the occupancy of this kernel instantiation at this LDS

@@ -517,9 +517,9 @@ def test_f32_segment_pass_tile_copies(gpu, O, tiles):
 
 
 @pytest.mark.parametrize("tiles", [1, 4])
-@pytest.mark.parametrize("kernel", ["ngp", "tsc", "m4"])
+@pytest.mark.parametrize("kernel", ["cic", "ngp", "tsc", "m4"])
 def test_f32_segment_pass_shapes(gpu, O, kernel, tiles):
-    """The NGP, TSC and M4 fixed-point passes on 128 x 128, where the
+    """The CIC, NGP, TSC and M4 fixed-point passes on 128 x 128, where the
     windows are narrower than the grid and the deferred-particle kernel
     runs, with one and four tile copies: cells, offsets and velocities
     bit-identical to the oracle's FP32 step of the same shape, charge within
