@@ -676,6 +676,7 @@ static int adjoint_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every,
     }
     PIC_HIP(hipStreamSynchronize(c->stream));
     c->seg_valid = false;
+    c->sort_fresh = false;
     return PIC_OK;
 }
 

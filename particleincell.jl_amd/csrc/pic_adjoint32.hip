@@ -744,6 +744,7 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, doub
     PIC_HIP(hipStreamSynchronize(c->stream));
     tm.collect();
     c->seg_valid = false;
+    c->sort_fresh = false;
     return PIC_OK;
 }
 

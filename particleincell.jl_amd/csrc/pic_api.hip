@@ -256,6 +256,7 @@ static int set_particles_impl(Ctx *c, const double *d_p, int64_t n) {
     c->n = n;
     c->state = PIC_STATE_F64;
     c->seg_valid = false;
+    c->sort_fresh = false;
     if (n == 0) {
         c->uniform_w = true;
         c->w0 = 0.0;
@@ -616,6 +617,7 @@ int pic_set_state_f32(pic_ctx *ctx, const int32_t *cell, const float *ox, const 
     PIC_HIP(hipStreamSynchronize(c->stream));
     c->state = PIC_STATE_F32;
     c->seg_valid = false;
+    c->sort_fresh = false;
     return PIC_OK;
 }
 

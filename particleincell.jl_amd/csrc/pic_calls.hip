@@ -211,6 +211,7 @@ int pic_push_x(pic_ctx *ctx, double dt) {
                            c->vy, dt, c->dimx, c->dimy);
         PIC_HIP(hipGetLastError());
     }
+    c->sort_fresh = false;
     PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
 }
@@ -227,6 +228,7 @@ int pic_operator_t(pic_ctx *ctx, double dt) {
     }
     PIC_HIP(hipStreamSynchronize(c->stream));
     c->seg_valid = false;
+    c->sort_fresh = false;
     return PIC_OK;
 }
 

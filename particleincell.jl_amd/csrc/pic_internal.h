@@ -111,6 +111,10 @@ struct Ctx {
     // after a lane sort, segment s occupies slots [cell_off[64 s],
     // cell_off[min(64 s + 64, ncells)]) until the next sort
     bool seg_valid = false;             // cell_off describes the current slots
+    // the particles sit in the order the last lane sort left (or its pending
+    // permutation gives) and have not moved since: pic_run can skip its own
+    // first sort
+    bool sort_fresh = false;
     // a lane sort's permutation not yet applied to the state (see sort_by_cell)
     const uint32_t *pending_perm = nullptr;
 

@@ -263,6 +263,7 @@ int pic_sample_landau(pic_ctx *ctx, int64_t n, double alpha, double kx, int stat
     c->n = n;
     c->state = PIC_STATE_F64;
     c->seg_valid = false;
+    c->sort_fresh = false;
     c->uniform_w = true;
     c->w0 = n ? (c->dimx * c->dimy) / (double)n : 0.0;
     if (n) {
@@ -293,6 +294,7 @@ int pic_sample_maxwellian(pic_ctx *ctx, int64_t n, double alpha, double kx, doub
     c->n = n;
     c->state = PIC_STATE_F64;
     c->seg_valid = false;
+    c->sort_fresh = false;
     c->uniform_w = true;
     c->w0 = n ? (c->dimx * c->dimy) / (double)n : 0.0;
     if (n) {

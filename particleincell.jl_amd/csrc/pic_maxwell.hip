@@ -1043,6 +1043,7 @@ int pic_run_maxwell(pic_ctx *ctx, int kernel, double dt, int nsteps, double *ene
     }
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     c->seg_valid = false;
+    c->sort_fresh = false;
     return rc;
 }
 

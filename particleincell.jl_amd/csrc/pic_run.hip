@@ -52,8 +52,9 @@ static int run_passes(Ctx *c, int kernel, double dt, int nsteps, int sort_every,
     const int64_t m = (int64_t)c->nx * c->ny;
     const double scale = 1.0 / ((double)c->nx * (double)c->ny * c->dx * c->dy);
     const int f32 = c->state == PIC_STATE_F32;
-    // the passes work segment by segment on lane-sorted slots
-    PIC_TRY(sort_by_cell(c, nullptr, true, false));
+    // the passes work segment by segment on lane-sorted slots (a lane sort
+    // whose particles have not moved since needs no repeat)
+    if (!(c->seg_valid && c->sort_fresh)) PIC_TRY(sort_by_cell(c, nullptr, true, false));
     PIC_TRY(launch_fill(c, c->rho_acc, 0.0, m));
     PIC_TRY(seg_pass(c, kernel, 0, dt));
     auto mark = [&]() -> int {

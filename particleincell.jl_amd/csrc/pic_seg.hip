@@ -727,6 +727,7 @@ static int seg32_fixed(const Ctx *c, const SegF32 &a, int mode, int bz, size_t s
 // swaps those in.
 int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     if (c->n == 0) return PIC_OK;
+    if (mode != 3) c->sort_fresh = false;  // the pass moves the particles
     PIC_CHECK(c->seg_valid, PIC_ERR_STATE, "segment passes need a lane sort first");
     SegF32 a;
     a.cell = c->cell; a.ov = c->ov32;
@@ -829,6 +830,7 @@ static int seg64_mode(const Ctx *c, const SegF64 &a, int mode, int bz, size_t sm
 // The FP64 counterpart of launch_seg32 (same pending-permutation rule).
 int launch_seg64(Ctx *c, int kernel, int mode, double dt, double dtd) {
     if (c->n == 0) return PIC_OK;
+    if (mode != 3) c->sort_fresh = false;  // the pass moves the particles
     PIC_CHECK(c->seg_valid, PIC_ERR_STATE, "segment passes need a lane sort first");
     SegF64 a;
     a.x = c->x; a.y = c->y; a.vx = c->vx; a.vy = c->vy;

@@ -605,6 +605,7 @@ int es_run(Ctx *c, int kernel, double dt, int nsteps) {
         return rc;
     };
     PIC_TRY(launch_fill(c, buf[0], 0.0, m));
+    c->sort_fresh = false;  // the launches move the particles
     // prologue: drift dt/2 and deposit into buffer 0, buffer 1 cleared
     EsStep e = a;
     e.mode = 0;

@@ -640,6 +640,7 @@ int sort_by_cell(Ctx *c, int32_t *perm_host, bool lanes, bool apply) {
     PIC_TRY(settle_state(c));
     c->lanes_order = lanes;
     c->seg_valid = false;
+    c->sort_fresh = false;
     if (n == 0) return PIC_OK;
     PIC_TRY(ensure_sort(c));
     const int ncells = c->nx * c->ny;
@@ -685,6 +686,7 @@ int sort_by_cell(Ctx *c, int32_t *perm_host, bool lanes, bool apply) {
         PIC_HIP(hipMemcpyAsync(perm_host, perm, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
         PIC_HIP(hipStreamSynchronize(c->stream));
     }
+    c->sort_fresh = lanes;
     return PIC_OK;
 }
 

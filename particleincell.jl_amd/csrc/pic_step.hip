@@ -709,6 +709,7 @@ static int launch64_k(Ctx *c, const ArgsF64 &a, int mode, int bz) {
 
 int launch_step64(Ctx *c, int kernel, int mode, double dt, double dtd) {
     if (c->n == 0) return PIC_OK;
+    if (mode <= 2) c->sort_fresh = false;  // the pass moves the particles
     ArgsF64 a;
     a.n = c->n;
     a.x = c->x; a.y = c->y; a.vx = c->vx; a.vy = c->vy;
@@ -761,6 +762,7 @@ static int launch32_k(Ctx *c, const ArgsF32 &a, int mode, int bz) {
 // dt: kick length; dtd: drift length (one full drift per step in FP32).
 int launch_step32(Ctx *c, int kernel, int mode, double dt, double dtd) {
     if (c->n == 0) return PIC_OK;
+    if (mode <= 2) c->sort_fresh = false;  // the pass moves the particles
     ArgsF32 a;
     a.n = c->n;
     a.cell = c->cell; a.ov = c->ov32;

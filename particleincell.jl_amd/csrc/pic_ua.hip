@@ -742,6 +742,7 @@ int pic_ua_compute_rho(pic_ctx *ctx, const double *t, const double *xt) {
     PIC_TRY(finalize_rho(c, PIC_M6));
     PIC_HIP(hipStreamSynchronize(c->stream));
     c->seg_valid = false;
+    c->sort_fresh = false;
     return PIC_OK;
 }
 
@@ -814,6 +815,7 @@ int pic_ua_run(pic_ctx *ctx, double dt, int nsteps, double *energy) {
     }
     PIC_HIP(hipStreamSynchronize(c->stream));
     c->seg_valid = false;
+    c->sort_fresh = false;
     return PIC_OK;
 }
 
