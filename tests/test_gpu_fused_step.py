@@ -137,3 +137,21 @@ def test_fused_step_one_step_and_rerun(gpu, O):
     assert abs(np.sqrt(e1[0, 0]) - eo1[0]) < 1e-10 * eo1[0]
     assert np.max(np.abs(np.sqrt(e2[:, 0]) - eo2) / eo2) < 1e-10
     assert np.max(_pdist(got[0], q[0], mesh.dimx)) < 1e-9
+
+
+@pytest.mark.parametrize("grid", [(4, 4), (8, 4), (4, 64)])
+def test_fused_step_tiny_grids(gpu, O, grid):
+    """Grids whose transforms span 4 or 8 lanes of a wave (16 or 8
+    transforms per wave at once): FP64 CIC and M6 (a stencil wider than the
+    grid, which wraps onto itself) against the oracle loop, particle by
+    particle."""
+    pic = gpu
+    nx, ny = grid
+    mesh, p = landau_particles(pic, 4096, nx, ny)
+    for name in ("cic", "m6"):
+        e, got, _, _ = _run(pic, mesh, p, name, 10, 0.1)
+        q = p.copy(order="F")
+        eo = O.landau_loop(q, name, nx, ny, mesh.dimx, mesh.dimy, 0.1, 10)
+        assert np.max(np.abs(np.sqrt(e[:, 0]) - eo) / eo) < 1e-10
+        assert np.max(_pdist(got[0], q[0], mesh.dimx)) < 1e-9
+        assert np.max(np.abs(got[2:4] - q[2:4])) < 1e-9

@@ -801,6 +801,41 @@ def test_run_sorted_matches_unsorted(gpu):
     assert np.max(np.abs(ea - eb) / ea) < 1e-9
 
 
+def test_run_after_lane_sort_skips_its_sort(gpu, O):
+    """pic_run skips its first lane sort when the particles were lane-sorted
+    and have not moved since: a run after pic_sort_lanes matches a run that
+    sorts at its start (FP32 state) to the rounding of the FP64 sums of the
+    blocks' charge, which arrive at the grid in any order, with the
+    particles in the same slots.  After pic_push_x the particles have moved,
+    the run sorts again, and its energies still match the oracle loop (FP64)."""
+    pic = gpu
+    F32 = pic._lib.PIC_STATE_F32
+    mesh, p = landau_particles(pic, 200_000, 128, 128)
+    a = pic.Simulation(mesh, p, pic.M6(), state=F32)
+    a.sort(lanes=True)
+    ea = a.run(0.05, 12, sort_every=8)
+    sa = a.state_f32()
+    a.close()
+    b = pic.Simulation(mesh, p, pic.M6(), state=F32)
+    eb = b.run(0.05, 12, sort_every=8)
+    sb = b.state_f32()
+    b.close()
+    assert np.max(np.abs(ea - eb) / np.abs(eb)) < 1e-10
+    # same slots: the cell of (nearly) every slot agrees, offsets to FP32 rounding
+    same = sa[0] == sb[0]
+    assert same.mean() > 0.999
+    assert np.max(np.abs(sa[1][same] - sb[1][same])) < 1e-5
+    c = pic.Simulation(mesh, p, pic.M6())
+    c.sort(lanes=True)
+    pic._lib.call("pic_push_x", c.ctx.h, 0.3)
+    ec = c.run(0.05, 6, sort_every=0)
+    c.close()
+    q = p.copy(order="F")
+    O.push_x(q, mesh.dimx, mesh.dimy, 0.3)
+    eo = O.landau_loop(q, "m6", 128, 128, mesh.dimx, mesh.dimy, 0.05, 6)
+    assert np.max(np.abs(np.sqrt(ec[:, 0]) - eo) / eo) < 1e-10
+
+
 def test_run_empty(gpu):
     pic = gpu
     mesh = pic.TwoDGrid(4 * np.pi, 16, 4 * np.pi, 16)
