@@ -47,6 +47,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# The timed runs record HIP events at every 5th step: coprime to the
+# workloads' sort intervals (3, 4, 8, 12, 16, 32), so the passes that apply a
+# sort's permutation are sampled in proportion
+TIMING_EVERY = 5
 BYTES_F32 = 40.0   # per particle per fused pass: cell, ox, oy, vx, vy read and written
 BYTES_F64 = 64.0   # x, y, vx, vy read and written
 
@@ -223,7 +227,10 @@ def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_ac
     # initial ordering (sort by cell, lane spread) is part of the set-up
     _reload(sim, p0, state)
     sim.sort(lanes=True)
-    sim.set_timing(True)
+    # HIP events around every TIMING_EVERY-th step's solve and pass: each
+    # event costs the stream a few microseconds, which would otherwise be
+    # part of the timed steps
+    sim.set_timing(True, every=TIMING_EVERY)
     slow0 = sim.slow_chunks()
     sim.ctx.sync()
     t0 = time.perf_counter()
@@ -384,7 +391,7 @@ def bench_adjoint(args, W, pic, entry, mesh, held, n, steps, warmup):
         if warmup:
             log(f"bench: {name} adjoint warmup, {warmup} steps")
             sim.adjoint(dt, warmup, min(ckpt, warmup), se)
-        sim.set_timing(True)
+        sim.set_timing(True, every=TIMING_EVERY)
         log(f"bench: {name} adjoint, {steps} steps, checkpoints every {ckpt}, sort every {se}")
         sim.ctx.sync()
         t0 = time.perf_counter()

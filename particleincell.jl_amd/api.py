@@ -672,8 +672,10 @@ class UASimulation:
         L.call("pic_ua_run", self.ctx.h, float(dt), int(nsteps), e.ctypes.data)
         return e
 
-    def set_timing(self, on=True):
-        L.call("pic_set_timing", self.ctx.h, 1 if on else 0)
+    def set_timing(self, on=True, every=1):
+        """HIP events around the loop's kernels, at every `every`-th step
+        (pic_set_timing)."""
+        L.call("pic_set_timing", self.ctx.h, (max(1, int(every)) if on else 0))
 
     def timing(self):
         """{kernel: (total_ms, launches)} of the prediction and correction
@@ -870,8 +872,10 @@ class Simulation:
         L.call("pic_run_maxwell", self.ctx.h, self.kernel, float(dt), int(nsteps), e.ctypes.data)
         return e
 
-    def set_timing(self, on=True):
-        L.call("pic_set_timing", self.ctx.h, 1 if on else 0)
+    def set_timing(self, on=True, every=1):
+        """HIP events around the loop's kernels, at every `every`-th step
+        (pic_set_timing)."""
+        L.call("pic_set_timing", self.ctx.h, (max(1, int(every)) if on else 0))
 
     def set_fused_solve(self, on=True):
         """E solve as three LDS-FFT kernels (power-of-two grids up to 512) or

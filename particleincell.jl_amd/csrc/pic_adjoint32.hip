@@ -400,13 +400,19 @@ __global__ void k_adj32_back(int64_t n, const uint32_t *__restrict__ perm,
 struct PassTimer {
     Ctx *c;
     std::vector<hipEvent_t> ev[4];
+    int64_t calls[4] = {0, 0, 0, 0};  // mark calls: begins and ends alternate
+    bool on[4] = {false, false, false, false};
     explicit PassTimer(Ctx *cc) : c(cc) {}
     ~PassTimer() {
         for (auto &v : ev)
             for (hipEvent_t e : v) (void)hipEventDestroy(e);
     }
+    // the passes of one kind are timed every timing_every-th time
     int mark(int k) {
         if (!c->timing) return PIC_OK;
+        const bool begin = calls[k]++ % 2 == 0;
+        if (begin) on[k] = (calls[k] / 2) % c->timing_every == 0;
+        if (!on[k]) return PIC_OK;
         hipEvent_t e;
         PIC_HIP(hipEventCreate(&e));
         ev[k].push_back(e);

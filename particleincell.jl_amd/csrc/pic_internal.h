@@ -180,8 +180,10 @@ struct Ctx {
     double t_ua_ms[2] = {0.0, 0.0};
     int64_t t_ua_n[2] = {0, 0};
 
-    // pic_run kernel timing (HIP events on the context stream)
+    // pic_run kernel timing (HIP events on the context stream), every
+    // timing_every-th step (pic_set_timing)
     bool timing = false;
+    int timing_every = 1;
     double t_pass_ms = 0.0, t_solve_ms = 0.0;
     int64_t t_passes = 0;
 };

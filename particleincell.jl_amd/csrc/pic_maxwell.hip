@@ -968,9 +968,10 @@ int pic_run_maxwell(pic_ctx *ctx, int kernel, double dt, int nsteps, double *ene
             io.energy = c->energy_hist;
             io.counter = c->step_counter;
             io.first = s == 1;
-            if (c->timing) rc = mark();
+            const bool timed = c->timing && (s - 1) % c->timing_every == 0;
+            if (timed) rc = mark();
             if (rc == PIC_OK) rc = vm_step_launch(c, dt, io, slds);
-            if (rc == PIC_OK && c->timing) rc = mark();
+            if (rc == PIC_OK && timed) rc = mark();
         }
         if (rc == PIC_OK) {
             double **bi = B[nsteps & 1];

@@ -71,11 +71,12 @@ static int run_passes(Ctx *c, int kernel, double dt, int nsteps, int sort_every,
         // post pass: FP32 field copy (FP32 state), accumulator cleared for the
         // next deposit (all but the last step), energy appended
         const int post = (f32 ? 1 : 0) | (last ? 0 : 2);
-        if (rc == PIC_OK && c->timing) rc = mark();
+        const bool timed = c->timing && s % c->timing_every == 0;
+        if (rc == PIC_OK && timed) rc = mark();
         if (rc == PIC_OK) rc = poisson_solve_e(c, c->rho_acc, scale, post, -2);
-        if (rc == PIC_OK && c->timing) rc = mark();
+        if (rc == PIC_OK && timed) rc = mark();
         if (rc == PIC_OK) rc = seg_pass(c, kernel, last ? 2 : 1, dt);
-        if (rc == PIC_OK && c->timing) rc = mark();
+        if (rc == PIC_OK && timed) rc = mark();
         if (rc == PIC_OK && !last && sort_every > 0 && (s + 1) % sort_every == 0)
             rc = sort_by_cell(c, nullptr, true, false);
     }
@@ -137,6 +138,7 @@ extern "C" int pic_set_timing(pic_ctx *ctx, int enable) {
     PIC_CHECK(ctx, PIC_ERR_ARG, "null context");
     Ctx *c = ctx_of(ctx);
     c->timing = enable != 0;
+    c->timing_every = enable > 1 ? enable : 1;
     c->t_pass_ms = c->t_solve_ms = 0.0;
     c->t_passes = 0;
     for (int k = 0; k < 4; ++k) {

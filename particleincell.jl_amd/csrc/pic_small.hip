@@ -632,29 +632,26 @@ int es_run(Ctx *c, int kernel, double dt, int nsteps) {
         e.rin = buf[st % 3];
         e.racc = buf[(st + 1) % 3];
         e.rclr = buf[(st + 2) % 3];
-        if (c->timing) {
+        // events around every timing_every-th launch
+        const bool timed = c->timing && st % c->timing_every == 0;
+        auto mark = [&]() -> int {
             hipEvent_t h;
-            if (hipEventCreate(&h) != hipSuccess) {
-                rc = fail(PIC_ERR_HIP, "event create failed");
-                break;
-            }
+            if (hipEventCreate(&h) != hipSuccess) return fail(PIC_ERR_HIP, "event create failed");
             ev.push_back(h);
             (void)hipEventRecord(h, c->stream);
-        }
-        rc = launch(e);
+            return PIC_OK;
+        };
+        if (timed) rc = mark();
+        if (rc == PIC_OK) rc = launch(e);
+        if (rc == PIC_OK && timed) rc = mark();
     }
-    if (c->timing && rc == PIC_OK && !ev.empty()) {
-        hipEvent_t h;
-        if (hipEventCreate(&h) == hipSuccess) {
-            ev.push_back(h);
-            (void)hipEventRecord(h, c->stream);
-            if (hipEventSynchronize(h) == hipSuccess)
-                for (size_t k = 0; k + 1 < ev.size(); ++k) {
-                    float ms = 0.0f;
-                    (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
-                    c->t_pass_ms += ms;
-                    c->t_passes += 1;
-                }
+    if (c->timing && rc == PIC_OK && !ev.empty() &&
+        hipEventSynchronize(ev.back()) == hipSuccess) {
+        for (size_t k = 0; k + 1 < ev.size(); k += 2) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+            c->t_pass_ms += ms;
+            c->t_passes += 1;
         }
     }
     for (hipEvent_t h : ev) (void)hipEventDestroy(h);

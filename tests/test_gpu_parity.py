@@ -836,6 +836,27 @@ def test_run_after_lane_sort_skips_its_sort(gpu, O):
     assert np.max(np.abs(np.sqrt(ec[:, 0]) - eo) / eo) < 1e-10
 
 
+@pytest.mark.parametrize("grid", [64, 128])
+def test_run_timing_every(gpu, grid):
+    """pic_set_timing(ctx, k): events around every k-th step's pass (the
+    one-launch loop on 64^2, the separate passes on 128^2), so 12 steps count
+    12 timed passes with k = 1 and 3 with k = 4, each of positive duration;
+    the energies do not depend on it."""
+    pic = gpu
+    mesh, p = landau_particles(pic, 100_000, grid, grid)
+    res = {}
+    for every in (1, 4):
+        sim = pic.Simulation(mesh, p, pic.M6(), state=pic._lib.PIC_STATE_F32)
+        sim.set_timing(True, every=every)
+        e = sim.run(0.05, 12, sort_every=4)
+        ms, n, _ = sim.timing()
+        sim.close()
+        res[every] = (e, ms, n)
+    assert res[1][2] == 12 and res[4][2] == 3
+    assert res[1][1] > 0 and res[4][1] > 0
+    assert np.max(np.abs(res[1][0] - res[4][0]) / res[4][0]) < 1e-10
+
+
 def test_run_empty(gpu):
     pic = gpu
     mesh = pic.TwoDGrid(4 * np.pi, 16, 4 * np.pi, 16)
