@@ -97,9 +97,9 @@ __device__ __forceinline__ float split_constant(float w, int64_t count) {
     return (float)ldexp(1.5, e + 23);
 }
 
-__device__ __forceinline__ int64_t tile_sum(const int *iacc, int t, int cap, int tiles) {
+__device__ __forceinline__ int64_t tile_sum(const int *iacc, int t, int stride, int tiles) {
     int64_t q = 0;
-    for (int k = 0; k < tiles; ++k) q += iacc[t + k * cap];
+    for (int k = 0; k < tiles; ++k) q += iacc[t + k * stride];
     return q;
 }
 
@@ -112,16 +112,24 @@ __device__ __forceinline__ int64_t tile_sum(const int *iacc, int t, int cap, int
 // carries nothing between its halves that the flush cannot undo: with L and
 // H the sums of the low and high halves, both below 2^31 in magnitude, the
 // word holds L + 2^32 H modulo 2^64.
+//
+// The share n of a node is read from the bits of fma(a, b, C), which are
+// bits(C) + n; the two raw words of a pair are packed first and bits(C) taken
+// off both halves by one 64-bit subtraction (a single v_lshl_add_u64 on
+// MI355X, where two 32-bit ones cost two instructions).  The low half never
+// borrows (n >= 0), so each half is exactly its share, as with two 32-bit
+// subtractions.  The conserving remainder is formed from the raw sum the
+// same way: sum of the raw words minus bits(C) per share.
 template <int K, class F>
 __device__ __forceinline__ void charge32_packed(float fx, float fy, int ci, int cj, float wu, int R,
                                                 bool conserve, F &&add) {
     constexpr int S = Shape<K>::S, HL = Shape<K>::HL, UR = HL | 1, VR = HL;
     constexpr float C = 12582912.0f;  // 1.5 * 2^23
-    auto n = [&](float a, float b) {
-        return __float_as_int(__builtin_fmaf(a, b, C)) - __float_as_int(C);
-    };
-    auto pack = [](int lo, int hi) {
-        return (unsigned long long)(unsigned)lo | ((unsigned long long)(unsigned)hi << 32);
+    constexpr unsigned CB = 0x4B400000u;  // bits of C
+    constexpr unsigned long long CB2 = (unsigned long long)CB << 32 | CB;
+    auto n = [&](float a, float b) { return __float_as_uint(__builtin_fmaf(a, b, C)); };
+    auto pack = [](unsigned lo, unsigned hi) {
+        return ((unsigned long long)lo | (unsigned long long)hi << 32) - CB2;
     };
     float wx[6], wy[6];
     if (K == PIC_CIC) {
@@ -135,20 +143,21 @@ __device__ __forceinline__ void charge32_packed(float fx, float fy, int ci, int 
 #pragma unroll
         for (int v = 0; v < S; ++v) wy[v] = wy[v] * wu;
     }
-    int s = 0;
+    unsigned sr = 0;  // raw words of the S * S - 1 shares before the last
 #pragma unroll
     for (int v = 0; v < S; ++v)
 #pragma unroll
         for (int u = 0; u < S; u += 2) {
-            const int a = n(wx[u], wy[v]);
-            s += a;
+            const unsigned a = n(wx[u], wy[v]);
+            sr += a;
             if (u + 1 == UR && v == VR) continue;
-            const int b = n(wx[u + 1], wy[v]);
-            s += b;
+            const unsigned b = n(wx[u + 1], wy[v]);
+            sr += b;
             add(u, v, pack(a, b));
         }
-    const int a = n(wx[UR - 1], wy[VR]);
-    add(UR - 1, VR, pack(a, conserve ? R - s : n(wx[UR], wy[VR])));
+    const int s = (int)(sr - (unsigned)(S * S - 1) * CB);
+    const unsigned a = n(wx[UR - 1], wy[VR]);
+    add(UR - 1, VR, pack(a, conserve ? (unsigned)(R - s) + CB : n(wx[UR], wy[VR])));
 }
 
 // Signed int32 sum of the half at position pos of a packed tile.
@@ -242,6 +251,10 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
     const bool fits = (W.tw + PAD) * W.th <= a.cap;
     const int cap = a.cap;
     const int P = GATHER || PACK ? (W.tw + 2) & ~1 : W.tw;
+    // window position of a stencil whose reference cell is (ci, cj):
+    // ((ci - ox) & mx, (cj - oy) & my); the products of window rows below are
+    // 24-bit (full-rate v_mul_u32_u24: a window holds fewer than 2^24 nodes)
+    const int ox = W.x0 + HL, oy = W.y0 + HL;
     // ACC 0: the (hi, lo) split constant; ACC 2, 3: the fixed-point quantum
     // 2^qe and |w| in units of it (exact: a power-of-two scaling)
     const float cq = ACC == 0 ? split_constant(a.w, p1 - p0) : 0.0f;
@@ -311,11 +324,13 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
         if (PREFETCH) {
             c = c_next;
             o = o_next;
-            if (p + BT < (uint32_t)p1) {
-                const uint32_t src = PERM ? a.perm[p + BT] : p + BT;
-                c_next = a.cell_in[src];
-                o_next = a.ov_in[src];
-            }
+            // unconditional: past the end of the slice it re-reads the last
+            // particle (p < p1 here), so no copy of the previous value is
+            // kept for a skipped load (five register moves per particle)
+            const uint32_t q = min(p + (uint32_t)BT, (uint32_t)p1 - 1u);
+            const uint32_t src = PERM ? a.perm[q] : q;
+            c_next = a.cell_in[src];
+            o_next = a.ov_in[src];
         } else {
             const uint32_t src = PERM ? a.perm[p] : p;
             c = a.cell_in[src];
@@ -324,14 +339,14 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
         int ci = c & mx, cj = c >> a.lognx;
         float fx = o.x, fy = o.y, ux = o.z, uy = o.w;
         if (GATHER) {
-            const int lx = (ci - HL - W.x0) & mx, ly = (cj - HL - W.y0) & my;
+            const int lx = (ci - ox) & mx, ly = (cj - oy) & my;
             const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
             defer_push(!in, (uint32_t)p, a.dlist, a.dcount);
             if (!in) continue;
             float e1, e2, b3 = 0.0f;
             const int odd = lx & 1;
             const float4 *row =
-                reinterpret_cast<const float4 *>((odd ? teB : teA) + (lx + odd) + P * ly);
+                reinterpret_cast<const float4 *>((odd ? teB : teA) + (lx + odd) + __umul24(P, ly));
             const int P2 = P >> 1;
             interp32_pairs<K, BZ>(
                 fx, fy, ci, cj, [&](int u, int v) { return row[(u >> 1) + P2 * v]; },
@@ -347,10 +362,10 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
             fx = nfx;
             fy = nfy;
         }
-        if (DRIFT || PERM) a.cell[p] = ci + nx * cj;
+        if (DRIFT || PERM) a.cell[p] = ci + (cj << a.lognx);
         if (GATHER || DRIFT || PERM) a.ov[p] = make_float4(fx, fy, ux, uy);
         if (DEPOSIT) {
-            const int lx = (ci - HL - W.x0) & mx, ly = (cj - HL - W.y0) & my;
+            const int lx = (ci - ox) & mx, ly = (cj - oy) & my;
             const bool in = fits && lx + S - 1 < W.tw && ly + S - 1 < W.th;
             defer_push(!in, (uint32_t)p | DEFER_DEPOSIT_ONLY, a.dlist, a.dcount);
             if (!in) continue;
@@ -369,8 +384,8 @@ __global__ __launch_bounds__((seg32_threads<K, DRIFT, DEPOSIT, ACC>()),
                 // P even; node (u, v), u even, is word w0 + u / 2 + v P / 2
                 unsigned long long *w0 =
                     reinterpret_cast<unsigned long long *>(
-                        iacc + ((a.tiles >= 4 ? 2 * (tid & 1) : 0) + odd) * cap) +
-                    ((lx + odd) >> 1) + (P >> 1) * ly;
+                        iacc + __umul24((a.tiles >= 4 ? 2 * (tid & 1) : 0) + odd, cap)) +
+                    ((lx + odd) >> 1) + __umul24(P >> 1, ly);
                 const int P2 = P >> 1;
                 charge32_packed<K>(fx, fy, ci, cj, wu, R, conserve,
                                    [&](int u, int v, unsigned long long q) {
