@@ -83,6 +83,9 @@ struct Ctx {
     // Poisson: tables (nx/2+1) x ny, complex work grids nx x ny, rocFFT
     double *kx_k2 = nullptr, *ky_k2 = nullptr, *k2 = nullptr;
     double2 *zf = nullptr, *zg = nullptr;
+    // twiddles of the LDS transforms: x forward, x inverse (nx/2 each), y
+    // forward, y inverse (ny/2 each)
+    double2 *fft_tw = nullptr;
     rocfft_plan plan_fwd = nullptr, plan_inv = nullptr;
     rocfft_execution_info fft_info = nullptr;
     void *fft_work = nullptr;

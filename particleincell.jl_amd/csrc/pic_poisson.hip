@@ -19,8 +19,10 @@
 #include <algorithm>
 #include <cstdio>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
+#include "pic_fftlanes.h"
 #include "pic_internal.h"
 
 namespace pic {
@@ -40,6 +42,8 @@ static int make_plan(rocfft_plan *plan, rocfft_transform_type t, int nx, int ny)
                                lengths, 1, nullptr));
     return PIC_OK;
 }
+
+static int fft_tw_init(Ctx *c);
 
 int poisson_init(Ctx *c) {
     if (!g_rocfft_ready) {
@@ -74,6 +78,7 @@ int poisson_init(Ctx *c) {
     PIC_HIP(hipMemcpy(c->kx_k2, kx.data(), m * 8, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(c->ky_k2, ky.data(), m * 8, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(c->k2, k2.data(), m * 8, hipMemcpyHostToDevice));
+    PIC_TRY(fft_tw_init(c));
     PIC_TRY(make_plan(&c->plan_fwd, rocfft_transform_type_complex_forward, nx, ny));
     PIC_TRY(make_plan(&c->plan_inv, rocfft_transform_type_complex_inverse, nx, ny));
     size_t wb = 0;
@@ -99,7 +104,7 @@ void poisson_free(Ctx *c) {
     if (c->fft_info) rocfft_execution_info_destroy(c->fft_info);
     c->fft_info = nullptr;
     for (void **q : {(void **)&c->kx_k2, (void **)&c->ky_k2, (void **)&c->k2, (void **)&c->zf,
-                     (void **)&c->zg, &c->fft_work})
+                     (void **)&c->zg, (void **)&c->fft_tw, &c->fft_work})
         if (*q) {
             (void)hipFree(*q);
             *q = nullptr;
@@ -306,6 +311,8 @@ static inline dim3 grid_of(int64_t m) { return dim3((unsigned)((m + 255) / 256))
 constexpr int FFT_T = 64;
 constexpr int FFT_MAX_LOG = 9;
 
+// w[k] = exp(sign 2 pi i k / N), k < N/2, computed once per context into
+// Ctx::fft_tw (k_fft_tw_init) and copied into LDS by each block.
 __device__ void fft_twiddles(double2 *w, int logN, int sign) {
     const int N = 1 << logN;
     for (int k = threadIdx.x; k < N / 2; k += blockDim.x) {
@@ -313,6 +320,16 @@ __device__ void fft_twiddles(double2 *w, int logN, int sign) {
         sincospi((double)(sign * 2 * k) / (double)N, &sn, &cs);
         w[k] = make_double2(cs, sn);
     }
+}
+__global__ void k_fft_tw_init(double2 *tw, int lognx, int logny) {
+    const int nx = 1 << lognx, ny = 1 << logny;
+    fft_twiddles(tw, lognx, -1);
+    fft_twiddles(tw + nx / 2, lognx, 1);
+    fft_twiddles(tw + nx, logny, -1);
+    fft_twiddles(tw + nx + ny / 2, logny, 1);
+}
+__device__ void fft_twiddles_from(double2 *w, const double2 *__restrict__ tab, int N) {
+    for (int k = threadIdx.x; k < N / 2; k += blockDim.x) w[k] = tab[k];
 }
 
 // In-place radix-2 decimation-in-time FFT of x[0, N) in LDS by the block's
@@ -345,11 +362,12 @@ __device__ void fft_lds(double2 *x, const double2 *w, int logN) {
 
 __global__ __launch_bounds__(FFT_T) void k_fft_rows_fwd(const double *__restrict__ rho, int nx,
                                                         int ny, int lognx,
+                                                        const double2 *__restrict__ tw,
                                                         double2 *__restrict__ At) {
     extern __shared__ double2 fsm[];
     double2 *x = fsm, *w = fsm + nx;
     const int j = blockIdx.x;
-    fft_twiddles(w, lognx, -1);
+    fft_twiddles_from(w, tw, nx);
     for (int i = threadIdx.x; i < nx; i += FFT_T)
         x[i] = make_double2(rho[(int64_t)j * nx + i], 0.0);
     __syncthreads();
@@ -359,6 +377,7 @@ __global__ __launch_bounds__(FFT_T) void k_fft_rows_fwd(const double *__restrict
 
 __global__ __launch_bounds__(FFT_T) void k_fft_cols_e(const double2 *__restrict__ At, int nx,
                                                       int ny, int logny,
+                                                      const double2 *__restrict__ tw,
                                                       const double *__restrict__ kxk2,
                                                       const double *__restrict__ kyk2,
                                                       double scale, double2 *__restrict__ B) {
@@ -366,8 +385,8 @@ __global__ __launch_bounds__(FFT_T) void k_fft_cols_e(const double2 *__restrict_
     double2 *S = fsm, *Z0 = fsm + ny, *Z1 = fsm + 2 * ny, *wf = fsm + 3 * ny,
             *wi = fsm + 3 * ny + ny / 2;
     const int ih = blockIdx.x, h = nx / 2 + 1, tid = threadIdx.x;
-    fft_twiddles(wf, logny, -1);
-    fft_twiddles(wi, logny, 1);
+    fft_twiddles_from(wf, tw + nx, ny);
+    fft_twiddles_from(wi, tw + nx + ny / 2, ny);
     for (int j = tid; j < ny; j += FFT_T) S[j] = At[(int64_t)ih * ny + j];
     __syncthreads();
     fft_lds(S, wf, logny);
@@ -409,35 +428,12 @@ __global__ __launch_bounds__(FFT_T) void k_fft_cols_e(const double2 *__restrict_
     }
 }
 
-// Inverse transform along x and k_post's work, row by row; partial sums per
-// block over its rows in a fixed order, combined in block order by the last
-// block to arrive.
-__global__ __launch_bounds__(FFT_T) void k_fft_rows_inv(
-    const double2 *__restrict__ B, int nx, int ny, int lognx, double *__restrict__ ex,
-    double *__restrict__ ey, float2 *__restrict__ e32, double *__restrict__ zero_acc,
-    double *part, unsigned *count, double *hist, int slot, int *counter, double dxdy) {
-    extern __shared__ double2 fsm[];
-    __shared__ double sh[2 * FFT_T];
-    __shared__ bool last;
-    double2 *x = fsm, *w = fsm + nx;
-    fft_twiddles(w, lognx, 1);
-    double sx = 0.0, sy = 0.0;
-    for (int j = blockIdx.x; j < ny; j += gridDim.x) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < nx; i += FFT_T) x[i] = B[(int64_t)j * nx + i];
-        __syncthreads();
-        fft_lds(x, w, lognx);
-        for (int i = threadIdx.x; i < nx; i += FFT_T) {
-            const int64_t k = (int64_t)j * nx + i;
-            const double a = x[i].x, b = x[i].y;
-            ex[k] = a;
-            ey[k] = b;
-            sx += a * a;
-            sy += b * b;
-            if (e32) e32[k] = make_float2((float)a, (float)b);
-            if (zero_acc) zero_acc[k] = 0.0;
-        }
-    }
+// The field energy of a 64-thread block's partial sums (sx, sy) over its
+// rows: each block's sums stored, and the last block to arrive adds them in
+// block order and appends (sx, sy) dx dy to the history (slot -2: the next
+// entry of the step counter; -1: none).
+__device__ void energy_tail(double sx, double sy, double *sh, bool &last, double *part,
+                            unsigned *count, double *hist, int slot, int *counter, double dxdy) {
     block_sum2<FFT_T>(sx, sy, sh);
     if (slot == -1) return;
     if (threadIdx.x == 0) {
@@ -468,6 +464,154 @@ __global__ __launch_bounds__(FFT_T) void k_fft_rows_inv(
     }
 }
 
+// Inverse transform along x and k_post's work, row by row; partial sums per
+// block over its rows in a fixed order, combined in block order by the last
+// block to arrive.
+__global__ __launch_bounds__(FFT_T) void k_fft_rows_inv(
+    const double2 *__restrict__ B, int nx, int ny, int lognx, const double2 *__restrict__ tw,
+    double *__restrict__ ex,
+    double *__restrict__ ey, float2 *__restrict__ e32, double *__restrict__ zero_acc,
+    double *part, unsigned *count, double *hist, int slot, int *counter, double dxdy) {
+    extern __shared__ double2 fsm[];
+    __shared__ double sh[2 * FFT_T];
+    __shared__ bool last;
+    double2 *x = fsm, *w = fsm + nx;
+    fft_twiddles_from(w, tw + nx / 2, nx);
+    double sx = 0.0, sy = 0.0;
+    for (int j = blockIdx.x; j < ny; j += gridDim.x) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < nx; i += FFT_T) x[i] = B[(int64_t)j * nx + i];
+        __syncthreads();
+        fft_lds(x, w, lognx);
+        for (int i = threadIdx.x; i < nx; i += FFT_T) {
+            const int64_t k = (int64_t)j * nx + i;
+            const double a = x[i].x, b = x[i].y;
+            ex[k] = a;
+            ey[k] = b;
+            sx += a * a;
+            sy += b * b;
+            if (e32) e32[k] = make_float2((float)a, (float)b);
+            if (zero_acc) zero_acc[k] = 0.0;
+        }
+    }
+    energy_tail(sx, sy, sh, last, part, count, hist, slot, counter, dxdy);
+}
+
+// The same three kernels with the transforms across the lanes of one wave
+// (pic_fftlanes.h) for 64 <= nx, ny <= 512: no LDS round trip or barrier
+// between the stages.  PX = nx / 64 and PY = ny / 64 points per lane; a
+// transform's output sits at bit-reversed positions and is stored there.
+template <int PX>
+__global__ __launch_bounds__(64) void k_wrows_fwd(const double *__restrict__ rho, int nx, int ny,
+                                                  int lognx, const double2 *__restrict__ tw,
+                                                  double2 *__restrict__ At) {
+    __shared__ double2 w[256];
+    const int l = threadIdx.x, j = blockIdx.x;
+    for (int k = l; k < nx / 2; k += 64) w[k] = tw[k];
+    double2 z[PX];
+#pragma unroll
+    for (int r = 0; r < PX; ++r) z[r] = make_double2(rho[(int64_t)j * nx + l + 64 * r], 0.0);
+    __syncthreads();
+    fft_wave<PX>(z, l, lognx, w, false);
+#pragma unroll
+    for (int r = 0; r < PX; ++r) {
+        const int ik = brev_bits(l + 64 * r, lognx);
+        if (ik <= nx / 2) At[(int64_t)ik * ny + j] = z[r];
+    }
+}
+
+template <int PY>
+__global__ __launch_bounds__(64) void k_wcols_e(const double2 *__restrict__ At, int nx, int ny,
+                                                int logny, const double2 *__restrict__ tw,
+                                                const double *__restrict__ kxk2,
+                                                const double *__restrict__ kyk2, double scale,
+                                                double2 *__restrict__ B) {
+    __shared__ double2 w[256];
+    __shared__ double2 S[512];
+    const int l = threadIdx.x, ih = blockIdx.x, h = nx / 2 + 1;
+    for (int k = l; k < ny / 2; k += 64) w[k] = tw[k];
+    double2 z[PY];
+#pragma unroll
+    for (int r = 0; r < PY; ++r) z[r] = At[(int64_t)ih * ny + l + 64 * r];
+    __syncthreads();
+    fft_wave<PY>(z, l, logny, w, false);
+#pragma unroll
+    for (int r = 0; r < PY; ++r) S[brev_bits(l + 64 * r, logny)] = z[r];
+    __syncthreads();
+    const bool self = ih == 0 || 2 * ih == nx;  // self-conjugate column
+    auto hv = [&](int jh, double2 &f, double2 &g) {
+        const int64_t q = (int64_t)ih + (int64_t)h * jh;
+        if (q == 0) {
+            f = g = make_double2(0.0, 0.0);
+        } else {
+            f = mul_mi(kxk2[q], S[jh], scale);
+            g = mul_mi(kyk2[q], S[jh], scale);
+        }
+    };
+    double2 z0[PY], z1[PY];
+#pragma unroll
+    for (int r = 0; r < PY; ++r) {
+        const int jk = l + 64 * r;
+        double2 f, g;
+        hv(jk, f, g);
+        if (self) {
+            double2 fm, gm;
+            hv((ny - jk) % ny, fm, gm);
+            f = make_double2(0.5 * (f.x + fm.x), 0.5 * (f.y - fm.y));
+            g = make_double2(0.5 * (g.x + gm.x), 0.5 * (g.y - gm.y));
+            z1[r] = make_double2(0.0, 0.0);
+        } else {  // column nx - ih: the conjugate half values at -jk
+            double2 f1, g1;
+            hv((ny - jk) % ny, f1, g1);
+            f1 = cconj(f1);
+            g1 = cconj(g1);
+            z1[r] = make_double2(f1.x - g1.y, f1.y + g1.x);
+        }
+        z0[r] = make_double2(f.x - g.y, f.y + g.x);
+    }
+    fft_wave<PY>(z0, l, logny, w, true);
+    if (!self) fft_wave<PY>(z1, l, logny, w, true);  // uniform over the block
+#pragma unroll
+    for (int r = 0; r < PY; ++r) {
+        const int j = brev_bits(l + 64 * r, logny);
+        B[(int64_t)j * nx + ih] = z0[r];
+        if (!self) B[(int64_t)j * nx + (nx - ih)] = z1[r];
+    }
+}
+
+template <int PX>
+__global__ __launch_bounds__(64) void k_wrows_inv(
+    const double2 *__restrict__ B, int nx, int ny, int lognx, const double2 *__restrict__ tw,
+    double *__restrict__ ex, double *__restrict__ ey, float2 *__restrict__ e32,
+    double *__restrict__ zero_acc, double *part, unsigned *count, double *hist, int slot,
+    int *counter, double dxdy) {
+    __shared__ double2 w[256];
+    __shared__ double sh[2 * FFT_T];
+    __shared__ bool last;
+    const int l = threadIdx.x;
+    for (int k = l; k < nx / 2; k += 64) w[k] = tw[k];
+    __syncthreads();
+    double sx = 0.0, sy = 0.0;
+    for (int j = blockIdx.x; j < ny; j += gridDim.x) {
+        double2 z[PX];
+#pragma unroll
+        for (int r = 0; r < PX; ++r) z[r] = B[(int64_t)j * nx + l + 64 * r];
+        fft_wave<PX>(z, l, lognx, w, true);
+#pragma unroll
+        for (int r = 0; r < PX; ++r) {
+            const int64_t k = (int64_t)j * nx + brev_bits(l + 64 * r, lognx);
+            const double a = z[r].x, b = z[r].y;
+            ex[k] = a;
+            ey[k] = b;
+            sx += a * a;
+            sy += b * b;
+            if (e32) e32[k] = make_float2((float)a, (float)b);
+            if (zero_acc) zero_acc[k] = 0.0;
+        }
+    }
+    energy_tail(sx, sy, sh, last, part, count, hist, slot, counter, dxdy);
+}
+
 static int log2_exact(int n) {
     int l = 0;
     while ((1 << l) < n) ++l;
@@ -480,18 +624,69 @@ static bool fused_solve_ok(const Ctx *c) {
     return lx >= 2 && ly >= 2 && lx <= FFT_MAX_LOG && ly <= FFT_MAX_LOG;
 }
 
+// nx, ny from 64 to 512: the wave-transform kernels, PX = nx / 64 and
+// PY = ny / 64 as template arguments.
+template <class F>
+static void with_points(int n, F &&f) {
+    switch (n / 64) {
+        case 1: f(std::integral_constant<int, 1>()); break;
+        case 2: f(std::integral_constant<int, 2>()); break;
+        case 4: f(std::integral_constant<int, 4>()); break;
+        default: f(std::integral_constant<int, 8>()); break;
+    }
+}
+
+static int solve_e_wave(Ctx *c, const double *rho_in, double scale, int post, int energy_slot) {
+    const int nx = c->nx, ny = c->ny, lx = log2_exact(nx), ly = log2_exact(ny);
+    double2 *At = c->zf, *B = c->zg;  // (nx/2+1) x ny column-major; nx x ny
+    const double2 *twx = (const double2 *)c->fft_tw, *twy = twx + nx;
+    with_points(nx, [&](auto px) {
+        hipLaunchKernelGGL(k_wrows_fwd<decltype(px)::value>, dim3((unsigned)ny), dim3(64), 0,
+                           c->stream, rho_in, nx, ny, lx, twx, At);
+    });
+    with_points(ny, [&](auto py) {
+        hipLaunchKernelGGL(k_wcols_e<decltype(py)::value>, dim3((unsigned)(nx / 2 + 1)), dim3(64),
+                           0, c->stream, (const double2 *)At, nx, ny, ly, twy,
+                           (const double *)c->kx_k2, (const double *)c->ky_k2, scale, B);
+    });
+    const int blocks = std::min(ny, RED_BLOCKS);
+    with_points(nx, [&](auto px) {
+        hipLaunchKernelGGL(k_wrows_inv<decltype(px)::value>, dim3((unsigned)blocks), dim3(64), 0,
+                           c->stream, (const double2 *)B, nx, ny, lx, twx, c->ex, c->ey,
+                           (post & 1) ? (float2 *)c->e32 : nullptr,
+                           (post & 2) ? c->rho_acc : nullptr, c->red_part, c->red_count,
+                           c->energy_hist, energy_slot, c->step_counter, c->dx * c->dy);
+    });
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+static int fft_tw_init(Ctx *c) {
+    if (!fused_solve_ok(c)) return PIC_OK;
+    const int lx = log2_exact(c->nx), ly = log2_exact(c->ny);
+    PIC_HIP(hipMalloc(&c->fft_tw, (size_t)(c->nx + c->ny) * sizeof(double2)));
+    hipLaunchKernelGGL(k_fft_tw_init, dim3(1), dim3(256), 0, c->stream, (double2 *)c->fft_tw, lx,
+                       ly);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
 static int solve_e_fused(Ctx *c, const double *rho_in, double scale, int post, int energy_slot) {
     const int nx = c->nx, ny = c->ny, lx = log2_exact(nx), ly = log2_exact(ny);
     double2 *At = c->zf, *B = c->zg;  // (nx/2+1) x ny column-major; nx x ny
+    if (lx >= 6 && ly >= 6) return solve_e_wave(c, rho_in, scale, post, energy_slot);
     hipLaunchKernelGGL(k_fft_rows_fwd, dim3((unsigned)ny), dim3(FFT_T),
-                       (size_t)(nx + nx / 2) * sizeof(double2), c->stream, rho_in, nx, ny, lx, At);
+                       (size_t)(nx + nx / 2) * sizeof(double2), c->stream, rho_in, nx, ny, lx,
+                       (const double2 *)c->fft_tw, At);
     hipLaunchKernelGGL(k_fft_cols_e, dim3((unsigned)(nx / 2 + 1)), dim3(FFT_T),
                        (size_t)(4 * ny) * sizeof(double2), c->stream, (const double2 *)At, nx, ny,
-                       ly, (const double *)c->kx_k2, (const double *)c->ky_k2, scale, B);
+                       ly, (const double2 *)c->fft_tw, (const double *)c->kx_k2,
+                       (const double *)c->ky_k2, scale, B);
     const int blocks = std::min(ny, RED_BLOCKS);
     hipLaunchKernelGGL(k_fft_rows_inv, dim3((unsigned)blocks), dim3(FFT_T),
                        (size_t)(nx + nx / 2) * sizeof(double2), c->stream, (const double2 *)B,
-                       nx, ny, lx, c->ex, c->ey, (post & 1) ? (float2 *)c->e32 : nullptr,
+                       nx, ny, lx, (const double2 *)c->fft_tw, c->ex, c->ey,
+                       (post & 1) ? (float2 *)c->e32 : nullptr,
                        (post & 2) ? c->rho_acc : nullptr, c->red_part, c->red_count,
                        c->energy_hist, energy_slot, c->step_counter, c->dx * c->dy);
     PIC_HIP(hipGetLastError());
