@@ -96,14 +96,15 @@ LDS_BYTES_PER_S = 256 * 128 * CLOCK_HZ
 
 
 def issue_counts(kernel, profile="sq_counters_dominant_kernels.txt"):
-    """(VALU wave instructions, LDS bytes) per particle of `kernel` from the
-    committed SQ-counter profile's 'issue' lines, or None."""
+    """(VALU wave instructions, LDS bytes) per particle of `kernel` and the
+    shader clock (Hz) measured during it, from the committed SQ-counter
+    profile's 'issue' lines, or None."""
     path = os.path.join(ROOT, "profiles", profile)
     try:
         for line in open(path):
             f = line.split()
-            if len(f) >= 4 and f[0] == "issue" and f[1] == kernel:
-                return float(f[2]), float(f[3])
+            if len(f) >= 5 and f[0] == "issue" and f[1] == kernel:
+                return float(f[2]), float(f[3]), float(f[4]) * 1e9
     except OSError:
         pass
     return None
@@ -948,9 +949,12 @@ def main():
         valu_us = n * ic[0] / VALU_WAVE_INSTS_PER_S * 1e6
         lds_us = n * ic[1] / LDS_BYTES_PER_S * 1e6
         pass_us = r["pass_ms_avg"] * 1e3
+        # the same floor at the shader clock measured during the pass
+        valu_clk_us = valu_us * CLOCK_HZ / ic[2]
         out["roofline"]["issue"] = {
             "valu": {"wave_insts_per_particle": ic[0], "floor_us": valu_us,
-                     "frac": valu_us / pass_us},
+                     "frac": valu_us / pass_us, "clock_ghz": ic[2] / 1e9,
+                     "floor_us_at_clock": valu_clk_us, "frac_at_clock": valu_clk_us / pass_us},
             "lds": {"bytes_per_particle": ic[1], "floor_us": lds_us, "frac": lds_us / pass_us},
             "source": "profiles/sq_counters_dominant_kernels.txt"}
     # the HBM-bound passes against the measured streaming ceiling of their

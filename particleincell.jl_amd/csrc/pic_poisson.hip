@@ -813,6 +813,23 @@ int finalize_rho(Ctx *c, int kernel) {
     return PIC_OK;
 }
 
+// The ghost-node terms of meshfields.jl:105-106's energy sum: column i = 0,
+// row j = 0, then node (0,0) once more, added in that order by one thread
+// (the order of a sequential loop over the nodes, so the sum does not depend
+// on the launch).
+__global__ void k_ghost_energy(const double *__restrict__ ex, const double *__restrict__ ey,
+                               int nx, int ny, double *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double g = 0.0;
+    for (int j = 0; j < ny; ++j) {
+        const int64_t k = (int64_t)nx * j;
+        g += ex[k] * ex[k] + ey[k] * ey[k];
+    }
+    for (int i = 0; i < nx; ++i) g += ex[i] * ex[i] + ey[i] * ey[i];
+    g += ex[0] * ex[0] + ey[0] * ey[0];
+    *out = g;
+}
+
 int field_energy(Ctx *c, double *out2) {
     PIC_TRY(ensure_energy(c, 1));
     const int64_t mm = (int64_t)c->nx * c->ny;
@@ -867,22 +884,24 @@ int pic_solve_fields(pic_ctx *ctx, double *energy) {
     if (energy) {
         // meshfields.jl:105-106 sums over the (nx+1)(ny+1) ghosted nodes: the
         // interior, plus column i = 0 and row j = 0 again, plus node (0,0) a
-        // third time (the ghost corner).
-        double s[2];
-        PIC_TRY(field_energy(c, s));
-        const int nx = c->nx, ny = c->ny;
-        std::vector<double> ex((size_t)nx * ny), ey((size_t)nx * ny);
-        PIC_HIP(hipMemcpyAsync(ex.data(), c->ex, ex.size() * 8, hipMemcpyDeviceToHost, c->stream));
-        PIC_HIP(hipMemcpyAsync(ey.data(), c->ey, ey.size() * 8, hipMemcpyDeviceToHost, c->stream));
+        // third time (the ghost corner).  Interior sums (k_post) and ghost
+        // terms (k_ghost_energy) on the device, one copy of three values.
+        PIC_TRY(ensure_energy(c, 2));
+        const int64_t mm = (int64_t)c->nx * c->ny;
+        int blocks = (int)((mm + RED_THREADS - 1) / RED_THREADS);
+        if (blocks > RED_BLOCKS) blocks = RED_BLOCKS;
+        hipLaunchKernelGGL(k_post, dim3(blocks), dim3(RED_THREADS), 0, c->stream,
+                           (const double2 *)nullptr, c->ex, c->ey, mm, nullptr, nullptr,
+                           c->red_part, c->red_count, c->energy_hist, 0, c->step_counter,
+                           c->dx * c->dy);
+        hipLaunchKernelGGL(k_ghost_energy, dim3(1), dim3(64), 0, c->stream, c->ex, c->ey, c->nx,
+                           c->ny, c->energy_hist + 2);
+        PIC_HIP(hipGetLastError());
+        double s[3];
+        PIC_HIP(hipMemcpyAsync(s, c->energy_hist, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
         PIC_HIP(hipStreamSynchronize(c->stream));
-        double g = 0.0;
-        for (int j = 0; j < ny; ++j) {
-            size_t k = (size_t)nx * j;
-            g += ex[k] * ex[k] + ey[k] * ey[k];
-        }
-        for (int i = 0; i < nx; ++i) g += ex[i] * ex[i] + ey[i] * ey[i];
-        g += ex[0] * ex[0] + ey[0] * ey[0];
-        *energy = s[0] + s[1] + g * c->dx * c->dy;
+        *energy = s[0] + s[1] + s[2] * c->dx * c->dy;
     }
     PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
