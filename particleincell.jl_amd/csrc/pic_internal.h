@@ -210,7 +210,10 @@ int poisson_solve_phi(Ctx *c);
 int poisson_adjoint(Ctx *c, const double *lex, const double *ley, double *lrho, double scale);
 int finalize_rho(Ctx *c, int kernel);
 int field_energy(Ctx *c, double *out2);
-int field_energy_mf(Ctx *c, double *out);
+// Queues meshfields.jl:105-106's energy sums of the current ex, ey: the
+// interior sums times dx dy into energy_hist[2 slot], [2 slot + 1] and the
+// ghost-node terms (without dx dy) into the device double *ghost.  No wait.
+int field_energy_async(Ctx *c, int slot, double *ghost);
 
 // Stable sort by cell (lanes: then the lane-spread order of the segment
 // passes).  apply = false with lanes leaves the permutation pending in

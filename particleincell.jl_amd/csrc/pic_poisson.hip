@@ -830,6 +830,19 @@ __global__ void k_ghost_energy(const double *__restrict__ ex, const double *__re
     *out = g;
 }
 
+int field_energy_async(Ctx *c, int slot, double *ghost) {
+    const int64_t mm = (int64_t)c->nx * c->ny;
+    int blocks = (int)((mm + RED_THREADS - 1) / RED_THREADS);
+    if (blocks > RED_BLOCKS) blocks = RED_BLOCKS;
+    hipLaunchKernelGGL(k_post, dim3(blocks), dim3(RED_THREADS), 0, c->stream,
+                       (const double2 *)nullptr, c->ex, c->ey, mm, nullptr, nullptr, c->red_part,
+                       c->red_count, c->energy_hist, slot, c->step_counter, c->dx * c->dy);
+    hipLaunchKernelGGL(k_ghost_energy, dim3(1), dim3(64), 0, c->stream, c->ex, c->ey, c->nx,
+                       c->ny, ghost);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
 int field_energy(Ctx *c, double *out2) {
     PIC_TRY(ensure_energy(c, 1));
     const int64_t mm = (int64_t)c->nx * c->ny;
@@ -887,16 +900,7 @@ int pic_solve_fields(pic_ctx *ctx, double *energy) {
         // third time (the ghost corner).  Interior sums (k_post) and ghost
         // terms (k_ghost_energy) on the device, one copy of three values.
         PIC_TRY(ensure_energy(c, 2));
-        const int64_t mm = (int64_t)c->nx * c->ny;
-        int blocks = (int)((mm + RED_THREADS - 1) / RED_THREADS);
-        if (blocks > RED_BLOCKS) blocks = RED_BLOCKS;
-        hipLaunchKernelGGL(k_post, dim3(blocks), dim3(RED_THREADS), 0, c->stream,
-                           (const double2 *)nullptr, c->ex, c->ey, mm, nullptr, nullptr,
-                           c->red_part, c->red_count, c->energy_hist, 0, c->step_counter,
-                           c->dx * c->dy);
-        hipLaunchKernelGGL(k_ghost_energy, dim3(1), dim3(64), 0, c->stream, c->ex, c->ey, c->nx,
-                           c->ny, c->energy_hist + 2);
-        PIC_HIP(hipGetLastError());
+        PIC_TRY(field_energy_async(c, 0, c->energy_hist + 2));
         double s[3];
         PIC_HIP(hipMemcpyAsync(s, c->energy_hist, 3 * sizeof(double), hipMemcpyDeviceToHost,
                                c->stream));

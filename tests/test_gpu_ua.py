@@ -3,6 +3,7 @@ src/compute_rho_m6.jl:29-179 and src/interpolation_m6.jl:3-123, the loop of
 test/bupdate.jl) against the oracle.  The GPU's sin, cos and tau-FFTs round
 differently from the oracle's libm and radix-2 transform (and FFTW's), so
 the bars are relative tolerances written in each test."""
+import ctypes
 import math
 
 import numpy as np
@@ -137,7 +138,9 @@ def _bupdate_start(pic, O, n, nx, ny, seed):
 def test_ua_loop_matches_oracle(gpu, O, ntau, nx, ny, n, steps):
     """pic_ua_run against the oracle's loop of test/bupdate.jl from the same
     prologue (eps 0.1, dt = pi/16): positions, velocities, fields and the
-    solve! energies within 1e-9."""
+    solve! energies within 1e-9.  The loop queues its energy sums on the
+    device and reads them once; pic_solve_fields on the final charge gives
+    the last one bit for bit."""
     pic = gpu
     eps, dt = 0.1, math.pi / 16
     mesh = pic.TwoDGrid(0.0, DIMX, nx, 0.0, DIMY, ny)
@@ -146,6 +149,9 @@ def test_ua_loop_matches_oracle(gpu, O, ntau, nx, ny, n, steps):
     en = sim.run(dt, steps)
     x, v = sim.particles()
     ef = sim.fields_e()
+    again = ctypes.c_double()
+    pic._lib.call("pic_solve_fields", sim.ctx.h, ctypes.byref(again))
+    assert again.value == en[1, -1]
     sim.close()
     vo = np.asfortranarray(p.v.copy())
     oen = O.ua_run(ntau, eps, dt, steps, xo, vo, eo, p.w, e_f, 0.0, DIMX, 0.0, DIMY)
