@@ -327,7 +327,8 @@ int pic_set_timing(pic_ctx *ctx, int enable);
 
 /* The E solve of pic_solve_e and pic_run on power-of-two grids up to
  * 512 x 512: fused = 1 (the default) runs it as three kernels with the FFTs
- * in LDS, fused = 0 as rocFFT transforms around the spectral kernel.  Both
+ * across the lanes of a wave (64 to 512 nodes a side; in LDS below 64),
+ * fused = 0 as rocFFT transforms around the spectral kernel.  Both
  * compute the same transforms and spectral values (poisson_2d.jl:97-113);
  * results agree to rounding.  Other grids always use rocFFT.  No reference
  * counterpart. */

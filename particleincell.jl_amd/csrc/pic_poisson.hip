@@ -295,9 +295,10 @@ static inline dim3 grid_of(int64_t m) { return dim3((unsigned)((m + 255) / 256))
 
 // ------------------------------------------- fused solve for small grids
 //
-// For power-of-two nx, ny <= 512 the E solve runs as three kernels with the
-// transforms in LDS instead of the rocFFT sequence (seven launches of a few
-// microseconds each at these sizes):
+// For power-of-two nx, ny <= 512 the E solve runs as three kernels instead of
+// the rocFFT sequence (seven launches of a few microseconds each at these
+// sizes); with 64 <= nx, ny the transforms run across the lanes of a wave
+// (k_wrows_fwd, k_wcols_e, k_wrows_inv below), on smaller grids in LDS:
 //   k_fft_rows_fwd   per row: forward FFT along x of rho (imaginary part 0);
 //                    keeps the half spectrum ik <= nx/2, column-major
 //   k_fft_cols_e     per half-spectrum column ih: forward FFT along y, the
