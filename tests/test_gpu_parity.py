@@ -238,7 +238,7 @@ def test_solve_e_phi(gpu, O):
     pic = gpu
     rng = np.random.default_rng(36)
     for nx, ny, dimx, dimy in ((64, 64, 4 * np.pi, 4 * np.pi), (128, 32, 1.0, 3.0),
-                               (16, 256, 2 * np.pi, 4 * np.pi)):
+                               (16, 256, 2 * np.pi, 4 * np.pi), (256, 128, 3.0, 1.5)):
         grid = pic.TwoDGrid(0.0, dimx, nx, 0.0, dimy, ny)
         ps = pic.TwoDPoissonPeriodic(grid)
         rho = np.asfortranarray(rng.normal(size=(nx, ny)))
@@ -251,12 +251,15 @@ def test_solve_e_phi(gpu, O):
         assert nrel(phi, O.solve_phi(rho, dimx, dimy)) < RTOL
 
 
-@pytest.mark.parametrize("nx,ny", [(8, 8), (32, 16), (128, 128), (512, 64), (64, 512)])
+@pytest.mark.parametrize("nx,ny", [(8, 8), (32, 16), (128, 128), (512, 64), (64, 512),
+                                   (256, 256), (128, 256)])
 def test_solve_e_fused_matches_rocfft(gpu, nx, ny):
-    """The E solve's LDS-FFT kernels (power-of-two grids up to 512) against
-    the rocFFT transforms around the same spectral kernel: fields to 1e-13
-    normwise, and the fused path's field energy as the reduction of those
-    fields (one pic_run step from a random deposit)."""
+    """The three-kernel E solve (power-of-two grids up to 512: transforms
+    across the lanes of a wave from 64 nodes a side, 1, 2, 4 or 8 points per
+    lane; in LDS below) against the rocFFT transforms around the same
+    spectral kernel: fields to 1e-13 normwise, and the fused path's field
+    energy as the reduction of those fields (one pic_run step from a random
+    deposit)."""
     pic = gpu
     rng = np.random.default_rng(nx + 7 * ny)
     grid = pic.TwoDGrid(0.0, 2.0, nx, 0.0, 3.0, ny)
