@@ -574,10 +574,20 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, doub
         c->pending_perm = nullptr;
         return PIC_OK;
     };
+    // The FP32 field the solve writes and the segment passes read is the
+    // context's e32 array; pointed at slot s of Es, the forward solve stores
+    // E_s there and the recomputation reads it there, with no copies.
+    float *const e32_own = c->e32;
+    auto field_at = [&](int s) { c->e32 = reinterpret_cast<float *>(Es.p + (size_t)m * s); };
+    struct FieldBack {
+        Ctx *c;
+        float *p;
+        ~FieldBack() { c->e32 = p; }
+    } field_back{c, e32_own};
     // x'_s from x'_{s-1}: the loop's kick with E_{s-1} and drift (its segment
     // pass without the deposit), then the sort of step s if there is one
     auto recompute = [&](int s, bool *sorted) -> int {
-        PIC_TRY(dcopy(c->e32, Es.p + (size_t)m * (s - 1), (size_t)m * 8));
+        field_at(s - 1);
         PIC_TRY(tm.mark(3));
         PIC_TRY(launch_seg32(c, K, 2, dt, dt));
         PIC_TRY(tm.mark(3));
@@ -612,8 +622,8 @@ int adjoint32_impl(Ctx *c, double dt, int nsteps, int ckpt, int sort_every, doub
             PIC_TRY(sort_by_cell(c, nullptr, true, ck));
             if (ck) PIC_TRY(save_to(ckc.p, cko.p, perms.p, offs.p, s / ckpt));
         }
+        field_at(s);
         PIC_TRY(poisson_solve_e(c, c->rho_acc, scale_rho, last ? 1 : 3, -2));
-        PIC_TRY(dcopy(Es.p + (size_t)m * s, c->e32, (size_t)m * 8));
         if (last) break;
         PIC_TRY(tm.mark(0));
         PIC_TRY(launch_seg32(c, K, 1, dt, dt));
