@@ -252,7 +252,7 @@ def test_solve_e_phi(gpu, O):
 
 
 @pytest.mark.parametrize("nx,ny", [(8, 8), (32, 16), (128, 128), (512, 64), (64, 512),
-                                   (256, 256), (128, 256)])
+                                   (256, 256), (128, 256), (256, 64), (512, 128)])
 def test_solve_e_fused_matches_rocfft(gpu, nx, ny):
     """The three-kernel E solve (power-of-two grids up to 512: transforms
     across the lanes of a wave from 64 nodes a side, 1, 2, 4 or 8 points per
