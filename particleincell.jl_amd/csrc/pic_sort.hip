@@ -82,20 +82,29 @@ __global__ __launch_bounds__(SORT_T) void k_digit_hist(const uint32_t *__restric
 // --- exclusive scan of uint32 (3 phases, 2048 items per block)
 constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_B = SCAN_T * SCAN_I;
 
+// Exclusive scan of v over the SCAN_T threads of the block, and the total:
+// an inclusive scan within each wave by lane shuffles, the wave totals
+// through LDS (two barriers in all).
 __device__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh, uint32_t &total) {
-    const int tid = threadIdx.x;
-    sh[tid] = v;
-    __syncthreads();
-    for (int o = 1; o < SCAN_T; o <<= 1) {
-        uint32_t t = tid >= o ? sh[tid - o] : 0u;
-        __syncthreads();
-        sh[tid] += t;
-        __syncthreads();
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
     }
-    total = sh[SCAN_T - 1];
-    uint32_t excl = sh[tid] - v;
+    if (lane == 63) sh[w] = x;
     __syncthreads();
-    return excl;
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_T / 64; ++k) {
+        const uint32_t t = sh[k];
+        off += k < w ? t : 0u;
+        tot += t;
+    }
+    total = tot;
+    __syncthreads();  // sh is reused by the next call
+    return off + x - v;
 }
 
 __global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const uint32_t *in, int64_t n,
@@ -112,18 +121,28 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const uint32_t *in, int6
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// single block: exclusive scan of nb block sums in place
+// single block: exclusive scan of nb values in place, 16 consecutive
+// values per thread (one block scan per 4096 values)
 __global__ __launch_bounds__(SCAN_T) void k_scan_top(uint32_t *bsum, int nb) {
+    constexpr int PER = 16;
     __shared__ uint32_t sh[SCAN_T];
     uint32_t carry = 0;
-    for (int b0 = 0; b0 < nb; b0 += SCAN_T) {
-        int i = b0 + threadIdx.x;
-        uint32_t v = i < nb ? bsum[i] : 0u;
+    for (int b0 = 0; b0 < nb; b0 += SCAN_T * PER) {
+        const int i0 = b0 + threadIdx.x * PER;
+        uint32_t v[PER], s = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            v[k] = i0 + k < nb ? bsum[i0 + k] : 0u;
+            s += v[k];
+        }
         uint32_t tot;
-        uint32_t e = block_excl_scan(v, sh, tot);
-        if (i < nb) bsum[i] = carry + e;
+        uint32_t e = carry + block_excl_scan(s, sh, tot);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            if (i0 + k < nb) bsum[i0 + k] = e;
+            e += v[k];
+        }
         carry += tot;
-        __syncthreads();
     }
 }
 
