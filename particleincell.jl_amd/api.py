@@ -878,8 +878,9 @@ class Simulation:
         L.call("pic_set_timing", self.ctx.h, (max(1, int(every)) if on else 0))
 
     def set_fused_solve(self, on=True):
-        """E solve as three LDS-FFT kernels (power-of-two grids up to 512) or
-        as rocFFT transforms (pic_set_fused_solve)."""
+        """E solve as three kernels (power-of-two grids up to 512: transforms
+        across the lanes of a wave from 64 nodes a side, in LDS below) or as
+        rocFFT transforms (pic_set_fused_solve)."""
         L.call("pic_set_fused_solve", self.ctx.h, 1 if on else 0)
 
     def set_fused_step(self, on=True):
