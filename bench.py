@@ -297,27 +297,27 @@ def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_ac
 WORKLOADS = {
     "landau_m6": dict(name="landau2d2v-m6-128x128-1e7-dt0.05-400steps", grid=128, n=10_000_000,
                       kernel="m6", alpha=0.01, k=0.5, dt=0.05, steps=400, warmup=20,
-                      sampler="quiet", sort=(12, 4)),
+                      sampler="quiet", sort=(12, 12)),
     "landau_cic": dict(name="landau2d2v-cic-64x64-204800-dt0.1-100steps", grid=64, n=204_800,
                        kernel="cic", alpha=0.05, k=0.5, dt=0.1, steps=100, warmup=10,
-                       sampler="quiet", sort=(8, 4)),
+                       sampler="quiet", sort=(8, 8)),
     "strong_b": dict(name="strongB-cic-256x256-1e8-eps0.05-dt0.0125-1000steps", grid=256,
                      n=100_000_000, kernel="cic", alpha=0.05, k=0.5, dt=0.0125, steps=1000,
-                     warmup=10, sampler="maxwellian_device", bz=20.0, sort=(32, 16)),
+                     warmup=10, sampler="maxwellian_device", bz=20.0, sort=(32, 32)),
     "large_m6": dict(name="landau2d2v-m6-512x512-5e8-dt0.05-200steps", grid=512,
                      n=500_000_000, kernel="m6", alpha=0.05, k=0.5, dt=0.05, steps=200,
-                     warmup=5, sampler="landau_device", sort=(4, 1), tiles=2),
+                     warmup=5, sampler="landau_device", sort=(4, 4), tiles=2),
     # the other shapes of the reference's Landau example
     # (examples/landau_damping_poisson_cic_m6.jl:88-103) on the M6 workload
     "landau_ngp": dict(name="landau2d2v-ngp-128x128-1e7-dt0.05-400steps", grid=128,
                        n=10_000_000, kernel="ngp", alpha=0.01, k=0.5, dt=0.05, steps=400,
-                       warmup=20, sampler="quiet", sort=(8, 4)),
+                       warmup=20, sampler="quiet", sort=(8, 12)),
     "landau_tsc": dict(name="landau2d2v-tsc-128x128-1e7-dt0.05-400steps", grid=128,
                        n=10_000_000, kernel="tsc", alpha=0.01, k=0.5, dt=0.05, steps=400,
-                       warmup=20, sampler="quiet", sort=(12, 4)),
+                       warmup=20, sampler="quiet", sort=(12, 12)),
     "landau_m4": dict(name="landau2d2v-m4-128x128-1e7-dt0.05-400steps", grid=128,
                       n=10_000_000, kernel="m4", alpha=0.01, k=0.5, dt=0.05, steps=400,
-                      warmup=20, sampler="quiet", sort=(12, 4)),
+                      warmup=20, sampler="quiet", sort=(12, 12)),
     # the Vlasov-Maxwell loop of examples/vm2d.jl (FDTD fields + CIC current
     # deposit, FP64): 100 particles per cell (the README's 204,800
     # particles) and 500 per cell (vm2d.jl's timed run), 1000 steps

@@ -501,8 +501,20 @@ __global__ __launch_bounds__(DEFER_T) void k_defer_f32(SegF32 a) {
 // ============================================================ FP64 state
 
 // FP64 counterpart of k_seg_f32; out-of-window particles go to k_defer_f64.
+// The FP64 passes run one 1024-thread block per CU with a 160 KiB window
+// (16 B of field per node, 8 B more with a bz grid, and two copies of the
+// FP64 charge tile, 16 B): an M6 window of a 64-cell row segment keeps a drift
+// margin of 20 cells, so the state can be sorted as seldom as the FP32 state.
+// Measured on MI355X, FP64 state, against 512-thread blocks in 40 KiB windows
+// (a 7-cell margin, one tile): Landau M6, 1e7 particles on 128^2, 1.86e10
+// particle-steps/s sorting every 4 steps -> 2.19e10 every 12 (2.09e10 with
+// one tile copy); 5e8 particles on 512^2, 1.12e10 sorting every step ->
+// 1.81e10 every 4.
+constexpr int SEG64_T = 1024;
+constexpr size_t SEG64_LDS_BYTES = 160 * 1024;
+
 template <int K, bool GATHER, int NDRIFT, bool DEPOSIT, int BZ, bool PERM>
-__global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
+__global__ __launch_bounds__(SEG64_T) void k_seg_f64(SegF64 a) {
     using St = SegStencil<K>;
     constexpr int S = St::S, HL = St::HL;
     extern __shared__ __align__(16) char smem_s64[];
@@ -518,11 +530,13 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
     const int cap = a.cap;
     double2 *te = reinterpret_cast<double2 *>(smem_s64);
     double *tb = reinterpret_cast<double *>(smem_s64 + 16 * (size_t)cap);
+    // two copies of the charge tile, lane l adding into copy l mod 2
     double *dacc = reinterpret_cast<double *>(
         smem_s64 + (GATHER ? (BZ == PIC_BZ_GRID ? 24 : 16) * (size_t)cap : 0));
+    double *dtile = dacc + (tid & 1) * cap;
     if (fits) {
         const int nt = W.tw * W.th;
-        for (int t = tid; t < nt; t += SEG_T) {
+        for (int t = tid; t < nt; t += SEG64_T) {
             const int u = t % W.tw, v = t / W.tw;
             const int64_t g = (int64_t)picd::wrap2(W.x0 + u, nx) +
                               (int64_t)nx * picd::wrap2(W.y0 + v, ny);
@@ -530,12 +544,15 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
                 te[t] = make_double2(a.ex[g], a.ey[g]);
                 if (BZ == PIC_BZ_GRID) tb[t] = a.bz[g];
             }
-            if (DEPOSIT) dacc[t] = 0.0;
+            if (DEPOSIT) {
+                dacc[t] = 0.0;
+                dacc[t + cap] = 0.0;
+            }
         }
     }
     __syncthreads();
 
-    for (uint32_t p = (uint32_t)p0 + tid; p < (uint32_t)p1; p += SEG_T) {
+    for (uint32_t p = (uint32_t)p0 + tid; p < (uint32_t)p1; p += SEG64_T) {
         const uint32_t src = PERM ? a.perm[p] : p;
         double px = a.x_in[src], py = a.y_in[src];
         double pvx = 0.0, pvy = 0.0;
@@ -577,7 +594,7 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
             defer_push(!in, (uint32_t)p | DEFER_DEPOSIT_ONLY, a.dlist, a.dcount);
             if (!in) continue;
             charge64<K>(fx, fy, ci, cj, a.w ? a.w[src] : a.w0, [&](int u, int v, double q) {
-                __hip_atomic_fetch_add(&dacc[(lx + u) + W.tw * (ly + v)], q, __ATOMIC_RELAXED,
+                __hip_atomic_fetch_add(&dtile[(lx + u) + W.tw * (ly + v)], q, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
             });
         }
@@ -585,8 +602,8 @@ __global__ __launch_bounds__(SEG_T) void k_seg_f64(SegF64 a) {
     if (!DEPOSIT || !fits) return;
     __syncthreads();
     const int nt = W.tw * W.th;
-    for (int t = tid; t < nt; t += SEG_T) {
-        const double q = dacc[t];
+    for (int t = tid; t < nt; t += SEG64_T) {
+        const double q = dacc[t] + dacc[t + cap];
         if (q != 0.0) {
             const int u = t % W.tw, v = t / W.tw;
             unsafeAtomicAdd(&a.acc[(int64_t)picd::wrap2(W.x0 + u, nx) +
@@ -806,9 +823,12 @@ int launch_seg32(Ctx *c, int kernel, int mode, double dt, double dtd) {
 
 template <class Kern, class Defer>
 static void seg64_launch(const Ctx *c, SegF64 a, Kern k, Defer kd, size_t sm) {
-    a.bps = seg_bps(c, reinterpret_cast<const void *>(k), sm);
+    if (sm > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    a.bps = seg_bps(c, reinterpret_cast<const void *>(k), sm, SEG64_T);
     dim3 grid((unsigned)(((c->nx * c->ny + 63) / 64) * a.bps));
-    hipLaunchKernelGGL(k, grid, dim3(SEG_T), sm, c->stream, a);
+    hipLaunchKernelGGL(k, grid, dim3(SEG64_T), sm, c->stream, a);
     hipLaunchKernelGGL(kd, dim3(DEFER_BLOCKS), dim3(DEFER_T), 0, c->stream, a);
 }
 
@@ -865,8 +885,8 @@ int launch_seg64(Ctx *c, int kernel, int mode, double dt, double dtd) {
     a.margin = 32;
     const bool gather = mode == 1 || mode == 2;
     const size_t per_node =
-        (gather ? (c->bz_mode == PIC_BZ_GRID ? 24 : 16) : 0) + (mode == 2 ? 0 : 8);
-    a.cap = (int)(SEG_LDS_BYTES / (per_node ? per_node : 8)) & ~15;
+        (gather ? (c->bz_mode == PIC_BZ_GRID ? 24 : 16) : 0) + (mode == 2 ? 0 : 16);
+    a.cap = (int)(SEG64_LDS_BYTES / (per_node ? per_node : 8)) & ~15;
     a.slow = c->slow_chunks;
     PIC_TRY(ensure_defer(c));
     a.dlist = c->defer;

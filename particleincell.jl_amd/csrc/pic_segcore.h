@@ -101,10 +101,12 @@ constexpr uint32_t DEFER_DEPOSIT_ONLY = 0x80000000u;
 
 constexpr int DEFER_T = 256, DEFER_BLOCKS = 256;
 
-// LDS per block for the window: FP32 8 B field (+4 B bz) + 8 B charge
-// (pair or FP64), FP64 16 B field (+8 B bz) + 8 B charge.  40 KiB lets four
+// LDS per block for the window of the FP32 passes other than the
+// fixed-point drift-and-deposit ones (and of the adjoint's deposit pass):
+// 8 B field (+4 B bz) + 8 B charge (pair or FP64).  40 KiB lets four
 // 512-thread blocks share a CU's 160 KiB; an M6 window of a 64-cell row
-// segment then keeps a drift margin of 11 cells (FP32) or 7 cells (FP64).
+// segment then keeps a drift margin of 11 cells.  The FP64 passes and the
+// FP32 fixed-point drift-and-deposit passes run one 160 KiB block per CU.
 constexpr size_t SEG_LDS_BYTES = 40 * 1024;
 
 // Blocks per segment: as many as keep every CU busy in one round of
