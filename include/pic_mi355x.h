@@ -257,7 +257,7 @@ int pic_run(pic_ctx *ctx, int kernel, double dt, int nsteps, int sort_every, dou
  * fused = 0 the separate solve, particle-pass and sort kernels.  Measured
  * on MI355X, Landau CIC 64 x 64 with 204,800 particles, 100 steps: one
  * launch 21.5 us (FP32 state) and 24.0 us (FP64) per step, 24.0 and 26.5 us
- * per step of wall time; the separate kernels 37.7 and 52.1 us.  The FP32 state's
+ * per step of wall time; the separate kernels 37.7 and 38.0 us.  The FP32 state's
  * one-launch charge is int32 fixed point with one quantum for every block,
  * summed as int64 (exact and run-to-run identical).  No reference
  * counterpart. */
