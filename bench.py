@@ -4,9 +4,20 @@ Default workload (--workload landau_m6, the headline): 2D2V Landau damping
 with M6 (quintic spline) deposition and gather, grid 128 x 128 on
 [0, 4pi]^2, 1e7 particles, alpha = 0.01, k = 0.5, dt = 0.05, 400 steps; the
 FP32 cell-relative state and the FP64 state run side by side from the same
-quiet start.  A step is one iteration of the reference's time loop
+quiet start, made on the GPU by the library (pic_sample_landau).  A step is
+one iteration of the reference's time loop
 (examples/landau_damping_poisson_cic_m6.jl:70-83): Poisson solve, kick,
 drift, deposit, on device-resident particles.
+
+Warmup: the W warmup steps run on the workload itself and continue, in
+blocks of W steps, until the device has been busy --settle-ms (default 50)
+milliseconds; the initial state is then made again on the device, sorted,
+and the K timed steps run from it at once.  From idle, MI355X's shader
+clock takes ~15 ms of load to come up to the clock it holds under load, so
+timed steps that follow a host-side reload or a warmup of a millisecond run
+their first ~80 steps ~7 % slower (measured: 207 us per step after the
+bench's former host reload, 209-215 after 100 ms idle, 192-194 after 40 ms
+of load).
 
 The UA workloads ua_bupdate and ua_large time the uniformly accurate loop
 of test/bupdate.jl (FP64, 128 x 64 cells, ntau 16, eps 0.1, dt pi/16, 8
@@ -23,7 +34,8 @@ pass and sort kernels), strong_b (CIC, 256 x 256, 1e8
 particles, density 1 + 0.05 cos(x/2) cos(y/2), Maxwellian v, uniform bz =
 1/eps = 20, dt = eps/4, 1000 steps) and large_m6 (M6, 512 x 512, 5e8
 particles, alpha 0.05, 200 steps); the last two are sampled on the GPU by
-the library (pic_sample_maxwellian, pic_sample_landau).
+the library (pic_sample_maxwellian, pic_sample_landau), like the 1e7
+Landau workloads (landau_m6, landau_ngp, landau_tsc, landau_m4).
 
 Prints one JSON line.  value = particle-steps/s of the FP32 path over the
 timed steps; roofline = the fused FP32 particle pass (the dominant kernel),
@@ -47,6 +59,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Least device time of the warmup phase, milliseconds (see above).
+SETTLE_MS = 50.0
 # The timed runs record HIP events at every 5th step: coprime to the
 # workloads' sort intervals (3, 4, 8, 12, 16, 32), so the passes that apply a
 # sort's permutation are sampled in proportion
@@ -212,7 +226,7 @@ def _reload(sim, p0, state):
 
 
 def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_acc=2, bz=None,
-             tiles=4, acc_steps=0, window=160, fused_step=True):
+             tiles=4, acc_steps=0, window=160, fused_step=True, settle_ms=SETTLE_MS):
     """Warmup, then the timed run from the initial state, then (acc_steps >
     0) an untimed accuracy run of acc_steps from the same initial state,
     whose energy history the damping rate is fitted on. Returns dict."""
@@ -222,8 +236,17 @@ def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_ac
     sim.set_f32_window(window)
     sim.set_fused_step(fused_step)
     sim.sort(lanes=True)
+    # warmup steps in blocks of `warmup` until the device has been busy
+    # settle_ms, so that the timed steps start at the clock it holds under
+    # load (each pic_run call ends with a stream synchronisation)
+    warm_steps = 0
     if warmup:
-        sim.run(dt, warmup, sort_every=sort_every)
+        t_w = time.perf_counter()
+        while True:
+            sim.run(dt, warmup, sort_every=sort_every)
+            warm_steps += warmup
+            if (time.perf_counter() - t_w) * 1e3 >= settle_ms:
+                break
     # restart from the initial state for the timed (and fitted) run; the
     # initial ordering (sort by cell, lane spread) is part of the set-up
     _reload(sim, p0, state)
@@ -241,6 +264,7 @@ def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_ac
     pass_ms, passes, solve_ms = sim.timing()
     out = {
         "seconds": t1 - t0,
+        "warmup_steps": warm_steps,
         "energy": e,
         "pass_ms_avg": pass_ms / max(passes, 1),
         "solve_ms_avg": solve_ms / max(passes, 1),
@@ -297,7 +321,7 @@ def run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every, f32_ac
 WORKLOADS = {
     "landau_m6": dict(name="landau2d2v-m6-128x128-1e7-dt0.05-400steps", grid=128, n=10_000_000,
                       kernel="m6", alpha=0.01, k=0.5, dt=0.05, steps=400, warmup=20,
-                      sampler="quiet", sort=(12, 12)),
+                      sampler="landau_device", sort=(12, 12)),
     "landau_cic": dict(name="landau2d2v-cic-64x64-204800-dt0.1-100steps", grid=64, n=204_800,
                        kernel="cic", alpha=0.05, k=0.5, dt=0.1, steps=100, warmup=10,
                        sampler="quiet", sort=(8, 8)),
@@ -311,13 +335,13 @@ WORKLOADS = {
     # (examples/landau_damping_poisson_cic_m6.jl:88-103) on the M6 workload
     "landau_ngp": dict(name="landau2d2v-ngp-128x128-1e7-dt0.05-400steps", grid=128,
                        n=10_000_000, kernel="ngp", alpha=0.01, k=0.5, dt=0.05, steps=400,
-                       warmup=20, sampler="quiet", sort=(8, 12)),
+                       warmup=20, sampler="landau_device", sort=(8, 12)),
     "landau_tsc": dict(name="landau2d2v-tsc-128x128-1e7-dt0.05-400steps", grid=128,
                        n=10_000_000, kernel="tsc", alpha=0.01, k=0.5, dt=0.05, steps=400,
-                       warmup=20, sampler="quiet", sort=(12, 12)),
+                       warmup=20, sampler="landau_device", sort=(12, 12)),
     "landau_m4": dict(name="landau2d2v-m4-128x128-1e7-dt0.05-400steps", grid=128,
                       n=10_000_000, kernel="m4", alpha=0.01, k=0.5, dt=0.05, steps=400,
-                      warmup=20, sampler="quiet", sort=(12, 12)),
+                      warmup=20, sampler="landau_device", sort=(12, 12)),
     # the Vlasov-Maxwell loop of examples/vm2d.jl (FDTD fields + CIC current
     # deposit, FP64): 100 particles per cell (the README's 204,800
     # particles) and 500 per cell (vm2d.jl's timed run), 1000 steps
@@ -738,6 +762,9 @@ def main():
                          "instead of the one-launch iteration")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the untimed full-length run the damping rate is fitted on")
+    ap.add_argument("--settle-ms", type=float, default=SETTLE_MS, metavar="MS",
+                    help="least device time of the warmup phase: the warmup steps repeat in "
+                         "blocks of --warmup steps until it has passed (0: --warmup steps only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -808,7 +835,8 @@ def main():
         sort_every = args.sort_every_f32 if name == "f32" else args.sort_every_f64
         res[name] = run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every,
                              args.f32_acc, bz, W.get("tiles", 4), acc_steps,
-                             W.get("window", 160), fused_step=not args.no_fused_step)
+                             W.get("window", 160), fused_step=not args.no_fused_step,
+                             settle_ms=args.settle_ms)
         r = res[name]
         log(f"bench: {name} {r['seconds']*1e3:.2f} ms total, pass {r['pass_ms_avg']*1e3:.1f} us,"
             f" solve {r['solve_ms_avg']*1e3:.1f} us, deferred {r['slow_chunks']}")
@@ -853,6 +881,10 @@ def main():
         "n_gpus": 1,
         "steps": steps,
         "warmup": warmup,
+        # the warmup steps repeat in blocks of `warmup` until the device has
+        # been busy settle_ms; these are the untimed steps that ran
+        "warmup_settle": {"settle_ms": args.settle_ms,
+                          "steps_run": {k: v["warmup_steps"] for k, v in res.items()}},
         "ms_per_step": r["seconds"] * 1e3 / steps,
         "higher_is_better": True,
         "scaling": "weak",
