@@ -553,9 +553,17 @@ def bench_maxwell(args, W, pic, entry, steps, warmup):
         sim.set_fdtd(f0)
         sim.ctx.sync()
 
+    # warmup in blocks of `warmup` steps until the device has been busy
+    # settle_ms (as in run_path)
+    warm_steps = 0
     if warmup:
         reset()
-        sim.run_maxwell(dt, warmup)
+        t_w = time.perf_counter()
+        while True:
+            sim.run_maxwell(dt, warmup)
+            warm_steps += warmup
+            if (time.perf_counter() - t_w) * 1e3 >= args.settle_ms:
+                break
     reset()
     log(f"bench: maxwell loop, {steps} steps")
     t0 = time.perf_counter()
@@ -576,7 +584,8 @@ def bench_maxwell(args, W, pic, entry, steps, warmup):
     out = {
         "metric": "particle-steps/s of the Vlasov-Maxwell FDTD + CIC loop",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
-        "warmup": warmup, "ms_per_step": seconds * 1e3 / steps, "higher_is_better": True,
+        "warmup": warmup, "warmup_settle": {"settle_ms": args.settle_ms, "steps_run": warm_steps},
+        "ms_per_step": seconds * 1e3 / steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (Landau quiet start, Sobol)",
         "config": {"workload": W["name"], "grid": [nx, ny], "particles": n, "kernel": "CIC",
@@ -659,11 +668,20 @@ def bench_ua(args, W, pic, entry, steps, warmup):
     def fresh():
         return pic.UASimulation(mesh, p, ntau, eps)
 
+    # the timed simulation is set up first, so that the warmup (on a second
+    # one, in blocks of `warmup` steps until the device has been busy
+    # settle_ms, as in run_path) runs right before the timed steps
+    sim = fresh()
+    warm_steps = 0
     if warmup:
         s0 = fresh()
-        s0.run(dt, warmup)
+        t_w = time.perf_counter()
+        while True:
+            s0.run(dt, warmup)
+            warm_steps += warmup
+            if (time.perf_counter() - t_w) * 1e3 >= args.settle_ms:
+                break
         s0.close()
-    sim = fresh()
     sim.ctx.sync()
     log(f"bench: UA loop, {steps} steps")
     t0 = time.perf_counter()
@@ -693,7 +711,8 @@ def bench_ua(args, W, pic, entry, steps, warmup):
     out = {
         "metric": "particle-steps/s of the UA PIC loop (test/bupdate.jl)",
         "value": n * steps / seconds, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
-        "warmup": warmup, "ms_per_step": seconds * 1e3 / steps, "higher_is_better": True,
+        "warmup": warmup, "warmup_settle": {"settle_ms": args.settle_ms, "steps_run": warm_steps},
+        "ms_per_step": seconds * 1e3 / steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (plasma(mesh, n) densities by rejection, numpy seed 1234)",
         "config": {"workload": W["name"], "grid": [nx, ny], "particles": n, "ntau": ntau,
@@ -950,7 +969,7 @@ def main():
             log(f"bench: {name} path as separate kernels, timed {steps} steps")
             rr = run_path(pic, mesh, p0, kernel, state, dt, steps, warmup, sort_every,
                           args.f32_acc, bz, W.get("tiles", 4), 0, W.get("window", 160),
-                          fused_step=False)
+                          fused_step=False, settle_ms=args.settle_ms)
             sep[name] = {"value": n * steps / rr["seconds"],
                          "ms_per_step": rr["seconds"] * 1e3 / steps,
                          "pass_us": rr["pass_ms_avg"] * 1e3, "solve_us": rr["solve_ms_avg"] * 1e3,
