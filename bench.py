@@ -11,8 +11,9 @@ drift, deposit, on device-resident particles.
 
 Warmup: the W warmup steps run on the workload itself and continue, in
 blocks of W steps, until the device has been busy --settle-ms (default 50)
-milliseconds; the initial state is then made again on the device, sorted,
-and the K timed steps run from it at once.  From idle, MI355X's shader
+milliseconds; the initial state is then made again (on the device, for the
+workloads sampled there), sorted, and the K timed steps run from it at
+once.  From idle, MI355X's shader
 clock takes ~15 ms of load to come up to the clock it holds under load, so
 timed steps that follow a host-side reload or a warmup of a millisecond run
 their first ~80 steps ~7 % slower (measured: 207 us per step after the
